@@ -1,0 +1,180 @@
+/*
+ * lauematch.h - C ABI of the MI355X-native LaueMatching indexing hot path.
+ *
+ * The reference has no library ABI: its hot path lives inside the main()
+ * functions of LaueMatchingCPU / LaueMatchingGPU / LaueMatchingGPUStream.
+ * These entry points replace the in-process seams of those programs:
+ *
+ *   - the OpenMP forward simulation that fills the ForwardFile table
+ *     (src/LaueMatchingCPU.c:591-667, src/LaueMatchingGPU.cu:463-617,
+ *     src/LaueMatchingGPUStream.cu:836-952)        -> lm_db_forward
+ *   - reading / writing the ForwardFile table
+ *     (src/LaueMatchingCPU.c:446-486, 548-576, 693-726)
+ *                                                  -> lm_db_import_table,
+ *                                                     lm_db_export_table
+ *   - the uint8 image quantization (src/LaueMatchingCPU.c:355-377)
+ *                                                  -> lm_quantize_u8
+ *   - the compare/score step with hit collection in ascending orientation
+ *     order (src/LaueMatchingCPU.c:668-689, 772-788; the CUDA kernel
+ *     `compare` and its chunk loop, src/LaueMatchingGPU.cu:45-81, 660-792)
+ *                                                  -> lm_score, lm_score_dev
+ *   - the streaming daemon's per-image GPU work
+ *     (src/LaueMatchingGPUStream.cu:1167-1257)     -> lm_score_batch and the
+ *                                                     lm_server_* request loop
+ *
+ * Conventions: every call returns LM_OK (0) or a negative LM_ERR_* code and
+ * never exits the process; lm_last_error() gives the calling thread's last
+ * message.  Host inputs stay owned by the caller; result arrays inside
+ * lm_hits are owned by the library until lm_hits_free().  Units are those of
+ * the reference parameter file: lattice in nm and degrees, P_Array and pixel
+ * pitch in metres, R_Array a rotation vector in radians, energies in keV.
+ */
+#ifndef LAUEMATCH_H
+#define LAUEMATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LM_OK 0
+#define LM_ERR_INVALID (-1)  /* bad argument or inconsistent sizes            */
+#define LM_ERR_HIP (-2)      /* HIP runtime error                             */
+#define LM_ERR_NOMEM (-3)    /* host or device allocation failed              */
+#define LM_ERR_BUDGET (-4)   /* request exceeds the context's HBM budget      */
+#define LM_ERR_OVERFLOW (-5) /* hit list exceeded its capacity                */
+#define LM_ERR_STATE (-6)    /* object not ready (e.g. no table yet)          */
+
+/* Forward-simulation modes for lm_db_forward. */
+#define LM_FWD_FP64 0       /* float64, exact reference operation order          */
+#define LM_FWD_FP32_GUARD 1 /* float32 with a float64 recompute near boundaries  */
+
+typedef struct lm_ctx lm_ctx;
+typedef struct lm_db lm_db;
+
+/* Geometry of one detector + crystal, computed on the host exactly as the
+ * reference does (src/LaueMatchingCPU.c:224-239, src/LaueMatchingHeaders.h:379-436). */
+typedef struct lm_geom {
+  double recip[3][3]; /* reciprocal basis B (1/nm), reference calcRecipArray  */
+  double rotT[3][3];  /* transpose of the detector rotation (Rodrigues)       */
+  double P[3];        /* P_Array (m)                                          */
+  double pxX, pxY;    /* pixel pitch (m)                                      */
+  int32_t nrPxX, nrPxY;
+  double Elo, Ehi;    /* inclusive energy window (keV)                        */
+} lm_geom;
+
+typedef struct lm_fwd_stats {
+  uint64_t n_orients;    /* orientations simulated                             */
+  uint64_t n_spots;      /* spots in the table                                 */
+  uint64_t n_pairs;      /* (orientation, hkl) pairs evaluated                 */
+  uint64_t n_recomputed; /* pairs recomputed in float64 (FP32_GUARD only)      */
+  double ms;             /* device time of the simulation                      */
+} lm_fwd_stats;
+
+/* Compact result of scoring one image against one DB. */
+typedef struct lm_hits {
+  uint64_t nr_results; /* orientations passing the predicate (reference
+                          nrResults, src/LaueMatchingCPU.c:683-689)           */
+  uint64_t count;      /* collected hits (score != 0), ascending id,
+                          src/LaueMatchingCPU.c:781-788                       */
+  uint64_t *ids;       /* orientation row numbers                            */
+  int32_t *nspots;     /* spots with intensity > 0                           */
+  double *totint;      /* float64 intensity sum, record order                */
+  double *score;       /* totint * sqrt(nspots)                              */
+  double kernel_ms;    /* device time of the scoring kernels                 */
+} lm_hits;
+
+/* ---- context -------------------------------------------------------------- */
+
+/* Bind device `device`; hbm_budget_bytes = 0 means 90% of free device memory.
+ * No reference counterpart (the reference selects device 0 implicitly). */
+int lm_ctx_create(int device, uint64_t hbm_budget_bytes, lm_ctx **out);
+void lm_ctx_destroy(lm_ctx *ctx);
+/* Bytes of device memory currently held by the context's DBs and buffers. */
+uint64_t lm_ctx_bytes_in_use(const lm_ctx *ctx);
+const char *lm_last_error(void);
+const char *lm_version(void);
+
+/* ---- host-side geometry ----------------------------------------------------- */
+
+/* Fill g from parameter-file values (reference main(), src/LaueMatchingCPU.c:
+ * 224-239 and :418-419). */
+int lm_geom_init(const double lattice[6], int32_t space_group, const double P[3],
+                 const double R[3], double pxX, double pxY, int32_t nrPxX, int32_t nrPxY,
+                 double Elo, double Ehi, lm_geom *g);
+
+/* uint8 quantization of a float64 image, src/LaueMatchingCPU.c:355-377.
+ * Writes out[npx] and *im_scale (= max/255, or 1 when max <= 0). */
+int lm_quantize_u8(const double *img, uint64_t npx, uint8_t *out, double *im_scale);
+
+/* ---- orientation DB + spot table ------------------------------------------------ */
+
+/* Upload n orientations (row-major 3x3 float64, the 72-byte records of the
+ * orientation DB file, src/LaueMatchingCPU.c:241-287) and the HKL list
+ * (src/LaueMatchingCPU.c:289-312).  max_spots = MaxNrLaueSpots (1..511). */
+int lm_db_create(lm_ctx *ctx, const double *orients, uint64_t n, const int32_t *hkls,
+                 int32_t nhkls, const lm_geom *geom, int32_t max_spots, lm_db **out);
+void lm_db_destroy(lm_db *db);
+uint64_t lm_db_size(const lm_db *db);
+/* Device bytes of the resident spot table (0 before forward/import). */
+uint64_t lm_db_table_bytes(const lm_db *db);
+/* Spots in the table (sum of per-orientation counts). */
+uint64_t lm_db_table_spots(const lm_db *db);
+
+/* Build the resident spot table on the device (replaces the DoFwd=1 forward
+ * simulation).  mode = LM_FWD_FP64 or LM_FWD_FP32_GUARD; stats may be NULL. */
+int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats);
+
+/* Replace the table from ForwardFile records: count records of (1+2M) uint16
+ * (word 0 = spot count, then x,y pairs), orientation-major, covering rows
+ * [0, count) with count == lm_db_size(db).  Counts > M or pixels outside the
+ * detector are rejected with LM_ERR_INVALID. */
+int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count);
+
+/* Write rows [first, first+count) as ForwardFile records (zero padded),
+ * byte-identical to the reference writer. */
+int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out);
+
+/* ---- scoring -------------------------------------------------------------------- */
+
+/* Score a host uint8 image (nrPxX x nrPxY of the DB geometry, row-major,
+ * X fastest) with the reference predicate nspots >= min_nr_spots &&
+ * totint >= min_intensity, totint = sequential float64 sum of raw*im_scale
+ * in table order.  Results in ascending orientation order. */
+int lm_score(lm_db *db, const uint8_t *img_u8, double im_scale, int32_t min_nr_spots,
+             double min_intensity, lm_hits *out);
+
+/* Same with the image already in device memory (d_img_u8 is a device
+ * pointer); stream is a hipStream_t (NULL = the context's stream). */
+int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t min_nr_spots,
+                 double min_intensity, void *stream, lm_hits *out);
+
+/* Score nimg host images against the DB in one pass over the table.
+ * outs[i] receives image i's hits, identical to lm_score on that image. */
+int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
+                   const double *im_scales, int32_t min_nr_spots, double min_intensity,
+                   lm_hits *outs);
+
+/* Device-resident batch: d_imgs points to nimg device images laid out back to
+ * back (nimg * nrPxX * nrPxY bytes). */
+int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,
+                       int32_t min_nr_spots, double min_intensity, void *stream, lm_hits *outs);
+
+void lm_hits_free(lm_hits *h);
+
+/* ---- diagnostics ------------------------------------------------------------------ */
+
+/* Evaluate on the device, with the forward simulation's arithmetic, q[i] =
+ * a[i] / b[i] and s[i] = sqrt(a[i]) for n host inputs (results to host).  No
+ * reference counterpart: it lets callers verify that the device float64
+ * division and square root round like the host's (the property the
+ * bit-exact ForwardFile relies on). */
+int lm_selftest_fp64(lm_ctx *ctx, const double *a, const double *b, uint64_t n, double *q,
+                     double *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

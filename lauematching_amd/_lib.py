@@ -1,0 +1,109 @@
+"""ctypes binding of liblauematch.so (the C ABI declared in include/lauematch.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+call raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblauematch.so")
+
+LM_OK = 0
+LM_ERR = {-1: "LM_ERR_INVALID", -2: "LM_ERR_HIP", -3: "LM_ERR_NOMEM", -4: "LM_ERR_BUDGET",
+          -5: "LM_ERR_OVERFLOW", -6: "LM_ERR_STATE"}
+LM_FWD_FP64 = 0
+LM_FWD_FP32_GUARD = 1
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class LmGeom(ctypes.Structure):
+    _fields_ = [("recip", (ctypes.c_double * 3) * 3), ("rotT", (ctypes.c_double * 3) * 3),
+                ("P", ctypes.c_double * 3), ("pxX", ctypes.c_double), ("pxY", ctypes.c_double),
+                ("nrPxX", ctypes.c_int32), ("nrPxY", ctypes.c_int32),
+                ("Elo", ctypes.c_double), ("Ehi", ctypes.c_double)]
+
+
+class LmFwdStats(ctypes.Structure):
+    _fields_ = [("n_orients", ctypes.c_uint64), ("n_spots", ctypes.c_uint64),
+                ("n_pairs", ctypes.c_uint64), ("n_recomputed", ctypes.c_uint64),
+                ("ms", ctypes.c_double)]
+
+
+class LmHits(ctypes.Structure):
+    _fields_ = [("nr_results", ctypes.c_uint64), ("count", ctypes.c_uint64),
+                ("ids", ctypes.POINTER(ctypes.c_uint64)), ("nspots", ctypes.POINTER(ctypes.c_int32)),
+                ("totint", c_double_p), ("score", c_double_p), ("kernel_ms", ctypes.c_double)]
+
+
+class LaueMatchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{LM_ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol declared in include/lauematch.h
+SIGNATURES = {
+    "lm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "lm_ctx_destroy": (None, [ctypes.c_void_p]),
+    "lm_ctx_bytes_in_use": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "lm_last_error": (ctypes.c_char_p, []),
+    "lm_version": (ctypes.c_char_p, []),
+    "lm_geom_init": (ctypes.c_int, [c_double_p, ctypes.c_int32, c_double_p, c_double_p,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                    ctypes.POINTER(LmGeom)]),
+    "lm_quantize_u8": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                      c_double_p]),
+    "lm_db_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                    ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(LmGeom),
+                                    ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "lm_db_destroy": (None, [ctypes.c_void_p]),
+    "lm_db_size": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "lm_db_table_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "lm_db_table_spots": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "lm_db_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(LmFwdStats)]),
+    "lm_db_import_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    "lm_db_export_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.c_void_p]),
+    "lm_score": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                                ctypes.c_double, ctypes.POINTER(LmHits)]),
+    "lm_score_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
+                                    ctypes.POINTER(LmHits)]),
+    "lm_score_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_void_p), c_double_p, ctypes.c_int32,
+                                      ctypes.c_double, ctypes.POINTER(LmHits)]),
+    "lm_score_batch_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                          c_double_p, ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_void_p, ctypes.POINTER(LmHits)]),
+    "lm_hits_free": (None, [ctypes.POINTER(LmHits)]),
+    "lm_selftest_fp64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+
+def lib():
+    """Load liblauematch.so (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m lauematching_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != LM_OK:
+        raise LaueMatchError(rc, lib().lm_last_error().decode(errors="replace"))
+    return rc

@@ -1,0 +1,254 @@
+"""Python host interface over the C ABI (include/lauematch.h).
+
+Mirrors the reference hot path's data flow (``src/LaueMatchingCPU.c``
+main(): parameters -> orientation DB + HKLs -> forward table -> quantized
+image -> compare -> ascending hit list) with objects that own the device
+state:
+
+>>> ctx = Context()
+>>> db = OrientationDB(ctx, orients, hkls, Geometry.from_params(p), p.MaxNrLaueSpots)
+>>> db.forward()                                   # DoFwd=1 table build
+>>> u8, scale = quantize_u8(image_f64)
+>>> hits = db.score(u8, scale, p.MinNrSpots, p.MinIntensity)
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import (LM_FWD_FP32_GUARD, LM_FWD_FP64, LmFwdStats, LmGeom, LmHits, check, lib)
+
+__all__ = ["Context", "Geometry", "OrientationDB", "Hits", "quantize_u8", "read_params",
+           "LM_FWD_FP64", "LM_FWD_FP32_GUARD"]
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One HIP device plus an HBM budget (0 = 90% of free memory)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().lm_ctx_create(device, hbm_budget, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bytes_in_use(self) -> int:
+        return int(lib().lm_ctx_bytes_in_use(self._h))
+
+    def close(self):
+        if self._h:
+            lib().lm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Geometry:
+    """Detector + crystal geometry (reference parameter-file units)."""
+
+    def __init__(self, lattice, space_group, P, R, pxX, pxY, nrPxX, nrPxY, Elo, Ehi):
+        g = LmGeom()
+        lat = (ctypes.c_double * 6)(*lattice)
+        PP = (ctypes.c_double * 3)(*P)
+        RR = (ctypes.c_double * 3)(*R)
+        check(lib().lm_geom_init(lat, int(space_group), PP, RR, float(pxX), float(pxY),
+                                 int(nrPxX), int(nrPxY), float(Elo), float(Ehi), ctypes.byref(g)))
+        self.c = g
+        self.nrPxX, self.nrPxY = int(nrPxX), int(nrPxY)
+
+    @classmethod
+    def from_params(cls, p) -> "Geometry":
+        return cls(p.LatticeParameter, p.SpaceGroup, p.P_Array, p.R_Array, p.PxX, p.PxY,
+                   p.NrPxX, p.NrPxY, p.Elo, p.Ehi)
+
+    @property
+    def recip(self) -> np.ndarray:
+        return np.array([[self.c.recip[i][j] for j in range(3)] for i in range(3)])
+
+    @property
+    def rotT(self) -> np.ndarray:
+        return np.array([[self.c.rotT[i][j] for j in range(3)] for i in range(3)])
+
+
+@dataclass
+class Hits:
+    """Ascending hit list of one image (reference src/LaueMatchingCPU.c:772-788)."""
+
+    nr_results: int
+    ids: np.ndarray
+    nspots: np.ndarray
+    totint: np.ndarray
+    score: np.ndarray
+    kernel_ms: float
+
+    @classmethod
+    def _take(cls, h: LmHits) -> "Hits":
+        k = int(h.count)
+        if k:
+            ids = np.ctypeslib.as_array(h.ids, (k,)).copy()
+            ns = np.ctypeslib.as_array(h.nspots, (k,)).copy()
+            tot = np.ctypeslib.as_array(h.totint, (k,)).copy()
+            sc = np.ctypeslib.as_array(h.score, (k,)).copy()
+        else:
+            ids = np.zeros(0, np.uint64)
+            ns = np.zeros(0, np.int32)
+            tot = np.zeros(0)
+            sc = np.zeros(0)
+        out = cls(int(h.nr_results), ids, ns, tot, sc, float(h.kernel_ms))
+        lib().lm_hits_free(ctypes.byref(h))
+        return out
+
+
+class OrientationDB:
+    """Orientation DB + HKL list resident on the device, with its spot table."""
+
+    def __init__(self, ctx: Context, orients: np.ndarray, hkls: np.ndarray, geom: Geometry,
+                 max_spots: int):
+        o = np.ascontiguousarray(orients, dtype=np.float64).reshape(-1, 9)
+        h = np.ascontiguousarray(hkls, dtype=np.int32).reshape(-1, 3)
+        hd = ctypes.c_void_p()
+        check(lib().lm_db_create(ctx.handle, _ptr(o), o.shape[0], _ptr(h), h.shape[0],
+                                 ctypes.byref(geom.c), int(max_spots), ctypes.byref(hd)))
+        self._h = hd
+        self.ctx = ctx
+        self.geom = geom
+        self.n = o.shape[0]
+        self.max_spots = int(max_spots)
+
+    def close(self):
+        if self._h:
+            lib().lm_db_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def table_bytes(self) -> int:
+        return int(lib().lm_db_table_bytes(self._h))
+
+    @property
+    def table_spots(self) -> int:
+        return int(lib().lm_db_table_spots(self._h))
+
+    def forward(self, mode: int = LM_FWD_FP64) -> LmFwdStats:
+        st = LmFwdStats()
+        check(lib().lm_db_forward(self._h, int(mode), ctypes.byref(st)))
+        return st
+
+    def import_table(self, records: np.ndarray) -> None:
+        r = np.ascontiguousarray(records, dtype=np.uint16).reshape(-1, 1 + 2 * self.max_spots)
+        check(lib().lm_db_import_table(self._h, _ptr(r), r.shape[0]))
+
+    def export_table(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.n - first if count is None else count
+        out = np.zeros((count, 1 + 2 * self.max_spots), dtype=np.uint16)
+        check(lib().lm_db_export_table(self._h, first, count, _ptr(out)))
+        return out
+
+    def score(self, img_u8: np.ndarray, im_scale: float, min_nr_spots: int,
+              min_intensity: float) -> Hits:
+        im = np.ascontiguousarray(img_u8, dtype=np.uint8).ravel()
+        if im.size != self.geom.nrPxX * self.geom.nrPxY:
+            raise ValueError("image size does not match the detector")
+        h = LmHits()
+        check(lib().lm_score(self._h, _ptr(im), float(im_scale), int(min_nr_spots),
+                             float(min_intensity), ctypes.byref(h)))
+        return Hits._take(h)
+
+    def score_dev(self, dev_ptr: int, im_scale: float, min_nr_spots: int, min_intensity: float,
+                  stream: int = 0) -> Hits:
+        h = LmHits()
+        check(lib().lm_score_dev(self._h, ctypes.c_void_p(dev_ptr), float(im_scale),
+                                 int(min_nr_spots), float(min_intensity),
+                                 ctypes.c_void_p(stream), ctypes.byref(h)))
+        return Hits._take(h)
+
+    def score_batch(self, imgs_u8, im_scales, min_nr_spots: int, min_intensity: float):
+        ims = [np.ascontiguousarray(i, dtype=np.uint8).ravel() for i in imgs_u8]
+        npx = self.geom.nrPxX * self.geom.nrPxY
+        if any(i.size != npx for i in ims):
+            raise ValueError("image size does not match the detector")
+        n = len(ims)
+        ptrs = (ctypes.c_void_p * n)(*[i.ctypes.data for i in ims])
+        sc = (ctypes.c_double * n)(*[float(s) for s in im_scales])
+        hs = (LmHits * n)()
+        check(lib().lm_score_batch(self._h, n, ptrs, sc, int(min_nr_spots), float(min_intensity),
+                                   hs))
+        return [Hits._take(hs[i]) for i in range(n)]
+
+    def score_batch_dev(self, dev_ptr: int, nimg: int, im_scales, min_nr_spots: int,
+                        min_intensity: float, stream: int = 0):
+        sc = (ctypes.c_double * nimg)(*[float(s) for s in im_scales])
+        hs = (LmHits * nimg)()
+        check(lib().lm_score_batch_dev(self._h, nimg, ctypes.c_void_p(dev_ptr), sc,
+                                       int(min_nr_spots), float(min_intensity),
+                                       ctypes.c_void_p(stream), hs))
+        return [Hits._take(hs[i]) for i in range(nimg)]
+
+
+def quantize_u8(img: np.ndarray):
+    """uint8 quantization + imScale (reference src/LaueMatchingCPU.c:355-377)."""
+    im = np.ascontiguousarray(img, dtype=np.float64).ravel()
+    out = np.zeros(im.size, dtype=np.uint8)
+    s = ctypes.c_double()
+    check(lib().lm_quantize_u8(_ptr(im), im.size, _ptr(out), ctypes.byref(s)))
+    return out, s.value
+
+
+_PARAM_KEYS = [  # (key, kind, count) in the reference's matching order
+    ("LatticeParameter", float, 6), ("P_Array", float, 3), ("R_Array", float, 3),
+    ("tol_c_over_a", float, 1), ("PxX", float, 1), ("PxY", float, 1), ("Elo", float, 1),
+    ("Ehi", float, 1), ("DoFwd", int, 1), ("NrPxX", int, 1), ("NrPxY", int, 1),
+    ("MaxNrLaueSpots", int, 1), ("BatchSize", int, 1), ("MinNrSpots", int, 1),
+    ("SpaceGroup", int, 1), ("MinIntensity", float, 1), ("MaxAngle", float, 1),
+    ("ForwardFile", str, 1), ("tol_LatC", float, 6), ("Optimizer", str, 1),
+    ("ResultDir", str, 1), ("MinGoodSpots", int, 1),
+]
+
+
+def read_params(path: str):
+    """Parameter file -> synth.Params, with the reference's prefix matching
+    (src/LaueMatchingCPU.c:75-204) and C defaults (:60-70)."""
+    from .synth import Params
+
+    p = Params(MaxNrLaueSpots=500, MinNrSpots=5, MinIntensity=1000.0, DoFwd=1, SpaceGroup=225,
+               Elo=5.0, Ehi=30.0, MaxAngle=2.0, NrPxX=0, NrPxY=0, PxX=0.0, PxY=0.0,
+               P_Array=(0.0, 0.0, 0.0), R_Array=(0.0, 0.0, 0.0),
+               LatticeParameter=(0.0,) * 6, ForwardFile="")
+    with open(path) as f:
+        for line in f:
+            for key, kind, cnt in _PARAM_KEYS:
+                if not line.startswith(key):
+                    continue
+                vals = line.split()[1:1 + cnt]
+                try:
+                    conv = [kind(v) if kind is not int else int(float(v)) for v in vals]
+                except ValueError:
+                    conv = []
+                if conv:
+                    if cnt == 1:
+                        v = conv[0]
+                    else:
+                        v = tuple(conv + [0.0] * (cnt - len(conv)))
+                    if hasattr(p, key):
+                        setattr(p, key, v)
+                    else:
+                        p.extra[key] = v
+                break
+    return p

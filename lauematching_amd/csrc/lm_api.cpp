@@ -1,0 +1,442 @@
+// lm_api.cpp - the extern "C" entry points of liblauematch.so.
+//
+// Host-side pieces that must be bit-identical to the reference host code are
+// computed here in plain double arithmetic in the reference's association
+// order: the detector rotation (src/LaueMatchingCPU.c:224-239), the
+// reciprocal basis (src/LaueMatchingHeaders.h:379-436) and the uint8
+// quantization (src/LaueMatchingCPU.c:355-377).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "lm_internal.h"
+
+namespace lm {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return LM_ERR_HIP;
+}
+
+int pix_bits_for(uint64_t npx) {
+  int b = 0;
+  while ((1ull << b) < npx) ++b;
+  return b;
+}
+
+namespace {
+
+constexpr double kDeg2Rad = 0.0174532925199433;  // src/LaueMatchingHeaders.h:31
+constexpr double kEps = 1E-12;                   // src/LaueMatchingHeaders.h:42
+
+double zero_small(double v) { return std::fabs(v) < kEps ? 0 : v; }
+
+// Reciprocal basis: real-space basis vectors, then b_i x b_j * 2pi/V
+// (rhombohedral setting for space groups 146/148/155/160/161/166/167).
+void recip_basis(const double lat[6], int sgnum, double out[9]) {
+  const double a = lat[0], b = lat[1], c = lat[2];
+  const bool rh = sgnum == 146 || sgnum == 148 || sgnum == 155 || sgnum == 160 || sgnum == 161 ||
+                  sgnum == 166 || sgnum == 167;
+  const double ca = std::cos(lat[3] * kDeg2Rad), cb = std::cos(lat[4] * kDeg2Rad);
+  const double cg = std::cos(lat[5] * kDeg2Rad), sg = std::sin(lat[5] * kDeg2Rad);
+  const double phi = std::sqrt(1.0 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg);
+  const double vol = a * b * c * phi;
+  const double pv = (2 * M_PI) / vol;
+  double A[3], B[3], C[3];
+  if (!rh) {
+    A[0] = zero_small(a);
+    A[1] = 0.0;
+    A[2] = 0.0;
+    B[0] = b * cg;
+    B[1] = zero_small(b * sg);
+    B[2] = 0;
+    C[0] = c * cb;
+    C[1] = c * (ca - cb * cg) / sg;
+    C[2] = zero_small(c * phi / sg);
+  } else {
+    const double p = std::sqrt(1.0 + 2 * ca), q = std::sqrt(1.0 - ca);
+    const double u = (a / 3.0) * (p - q), w = (a / 3.0) * (p + 2 * q);
+    A[0] = w; A[1] = u; A[2] = u;
+    B[0] = u; B[1] = w; B[2] = u;
+    C[0] = u; C[1] = u; C[2] = w;
+  }
+  out[0] = zero_small((B[1] * C[2] - B[2] * C[1]) * pv);
+  out[3] = zero_small((B[2] * C[0] - B[0] * C[2]) * pv);
+  out[6] = zero_small((B[0] * C[1] - B[1] * C[0]) * pv);
+  out[1] = zero_small((C[1] * A[2] - C[2] * A[1]) * pv);
+  out[4] = zero_small((C[2] * A[0] - C[0] * A[2]) * pv);
+  out[7] = zero_small((C[0] * A[1] - C[1] * A[0]) * pv);
+  out[2] = zero_small((A[1] * B[2] - A[2] * B[1]) * pv);
+  out[5] = zero_small((A[2] * B[0] - A[0] * B[2]) * pv);
+  out[8] = zero_small((A[0] * B[1] - A[1] * B[0]) * pv);
+}
+
+// Rodrigues rotation for the rotation vector r, returned transposed.
+void detector_rotT(const double r[3], double rotT[9]) {
+  const double ang = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  const double v0 = r[0] / ang, v1 = r[1] / ang, v2 = r[2] / ang;
+  const double c = std::cos(ang), s = std::sin(ang), omc = 1 - std::cos(ang);
+  double R[9];
+  R[0] = c + omc * (v0 * v0);
+  R[1] = omc * v0 * v1 - s * v2;
+  R[2] = omc * v0 * v2 + s * v1;
+  R[3] = omc * v1 * v0 + s * v2;
+  R[4] = c + omc * (v1 * v1);
+  R[5] = omc * v1 * v2 - s * v0;
+  R[6] = omc * v2 * v0 - s * v1;
+  R[7] = omc * v2 * v1 + s * v0;
+  R[8] = c + omc * (v2 * v2);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) rotT[i * 3 + j] = R[j * 3 + i];
+}
+
+int check_ctx(lm_ctx *ctx) {
+  if (!ctx) {
+    set_error("null context");
+    return LM_ERR_INVALID;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return LM_OK;
+}
+
+int stage_host_images(lm_db *db, int nimg, const uint8_t *const *imgs, uint8_t **dptr) {
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  const uint64_t need = npx * (uint64_t)nimg;
+  if (db->img_cap < need) {
+    if (db->d_img) (void)hipFree(db->d_img);
+    db->d_img = nullptr;
+    db->img_cap = 0;
+    hipError_t e = hipMalloc(&db->d_img, need);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(image staging)");
+    db->img_cap = need;
+  }
+  for (int b = 0; b < nimg; ++b) {
+    if (!imgs[b]) {
+      set_error("null image pointer");
+      return LM_ERR_INVALID;
+    }
+    hipError_t e = hipMemcpyAsync(db->d_img + (uint64_t)b * npx, imgs[b], npx,
+                                  hipMemcpyHostToDevice, db->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "image upload");
+  }
+  *dptr = db->d_img;
+  return LM_OK;
+}
+
+}  // namespace
+}  // namespace lm
+
+using namespace lm;
+
+extern "C" {
+
+const char *lm_last_error(void) { return g_err.c_str(); }
+
+const char *lm_version(void) { return "lauematch-mi355x 0.1 (gfx950)"; }
+
+int lm_ctx_create(int device, uint64_t budget, lm_ctx **out) {
+  if (!out) {
+    set_error("null output pointer");
+    return LM_ERR_INVALID;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+  if (device < 0 || device >= ndev) {
+    set_error("device index out of range");
+    return LM_ERR_INVALID;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  size_t freeb = 0, totalb = 0;
+  e = hipMemGetInfo(&freeb, &totalb);
+  if (e != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+  lm_ctx *c = new (std::nothrow) lm_ctx;
+  if (!c) {
+    set_error("out of host memory");
+    return LM_ERR_NOMEM;
+  }
+  c->device = device;
+  c->budget = budget ? budget : (uint64_t)(0.9 * (double)freeb);
+  e = hipStreamCreate(&c->stream);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = c;
+  return LM_OK;
+}
+
+void lm_ctx_destroy(lm_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+uint64_t lm_ctx_bytes_in_use(const lm_ctx *ctx) { return ctx ? ctx->in_use : 0; }
+
+int lm_geom_init(const double lattice[6], int32_t space_group, const double P[3],
+                 const double R[3], double pxX, double pxY, int32_t nrPxX, int32_t nrPxY,
+                 double Elo, double Ehi, lm_geom *g) {
+  if (!lattice || !P || !R || !g) {
+    set_error("lm_geom_init: null argument");
+    return LM_ERR_INVALID;
+  }
+  if (nrPxX <= 0 || nrPxY <= 0 || nrPxX > 65535 || nrPxY > 65535 || lattice[0] == 0.0) {
+    set_error("lm_geom_init: detector size must be 1..65535 and lattice a nonzero");
+    return LM_ERR_INVALID;
+  }
+  double rb[9], rt[9];
+  recip_basis(lattice, space_group, rb);
+  detector_rotT(R, rt);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      g->recip[i][j] = rb[i * 3 + j];
+      g->rotT[i][j] = rt[i * 3 + j];
+    }
+  for (int i = 0; i < 3; ++i) g->P[i] = P[i];
+  g->pxX = pxX;
+  g->pxY = pxY;
+  g->nrPxX = nrPxX;
+  g->nrPxY = nrPxY;
+  g->Elo = Elo;
+  g->Ehi = Ehi;
+  return LM_OK;
+}
+
+int lm_quantize_u8(const double *img, uint64_t npx, uint8_t *out, double *im_scale) {
+  if ((!img || !out) && npx) {
+    set_error("lm_quantize_u8: null buffer");
+    return LM_ERR_INVALID;
+  }
+  double mx = 0;
+  for (uint64_t i = 0; i < npx; ++i)
+    if (img[i] > mx) mx = img[i];
+  const double scale = (mx > 0) ? mx / 255.0 : 1.0;
+  const double inv = (mx > 0) ? 255.0 / mx : 0.0;
+  for (uint64_t i = 0; i < npx; ++i) {
+    if (img[i] > 0) {
+      const double v = img[i] * inv;
+      out[i] = (v >= 255.0) ? 255 : (v < 1.0) ? 1 : (uint8_t)v;
+    } else {
+      out[i] = 0;
+    }
+  }
+  if (im_scale) *im_scale = scale;
+  return LM_OK;
+}
+
+int lm_db_create(lm_ctx *ctx, const double *orients, uint64_t n, const int32_t *hkls,
+                 int32_t nhkls, const lm_geom *geom, int32_t max_spots, lm_db **out) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!out || !geom || (n && !orients) || nhkls < 0 || (nhkls && !hkls)) {
+    set_error("lm_db_create: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  if (max_spots < 1 || max_spots > kMaxSpots) {
+    set_error("lm_db_create: MaxNrLaueSpots must be 1..511");
+    return LM_ERR_INVALID;
+  }
+  if (n >= (1ull << 32)) {
+    set_error("lm_db_create: at most 2^32-1 orientations per DB");
+    return LM_ERR_INVALID;
+  }
+  if (geom->nrPxX <= 0 || geom->nrPxY <= 0 || geom->nrPxX > 65535 || geom->nrPxY > 65535 ||
+      (uint64_t)geom->nrPxX * geom->nrPxY > (1ull << 31)) {
+    set_error("lm_db_create: detector size out of range");
+    return LM_ERR_INVALID;
+  }
+  *out = nullptr;
+  const uint64_t bytes = n * 72 + (uint64_t)nhkls * 12 + 64;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->in_use + bytes > ctx->budget) {
+      set_error("lm_db_create: HBM budget exceeded");
+      return LM_ERR_BUDGET;
+    }
+    ctx->in_use += bytes;
+  }
+  lm_db *db = new (std::nothrow) lm_db;
+  if (!db) {
+    set_error("out of host memory");
+    return LM_ERR_NOMEM;
+  }
+  db->ctx = ctx;
+  db->n = n;
+  db->nhkls = nhkls;
+  db->max_spots = max_spots;
+  Geom &g = db->geom;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      g.recip[i * 3 + j] = geom->recip[i][j];
+      g.rotT[i * 3 + j] = geom->rotT[i][j];
+    }
+  for (int i = 0; i < 3; ++i) g.P[i] = geom->P[i];
+  g.pxX = geom->pxX;
+  g.pxY = geom->pxY;
+  g.nrPxX = geom->nrPxX;
+  g.nrPxY = geom->nrPxY;
+  g.Elo = geom->Elo;
+  g.Ehi = geom->Ehi;
+  hipError_t e = hipMalloc(&db->d_orients, n * 72 + 8);
+  if (e == hipSuccess) e = hipMalloc(&db->d_hkls, (uint64_t)nhkls * 12 + 16);
+  if (e == hipSuccess && n)
+    e = hipMemcpyAsync(db->d_orients, orients, n * 72, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && nhkls)
+    e = hipMemcpyAsync(db->d_hkls, hkls, (uint64_t)nhkls * 12, hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "lm_db_create upload");
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->in_use -= bytes;
+    }
+    if (db->d_orients) (void)hipFree(db->d_orients);
+    if (db->d_hkls) (void)hipFree(db->d_hkls);
+    delete db;
+    return rc;
+  }
+  *out = db;
+  return LM_OK;
+}
+
+void lm_db_destroy(lm_db *db) {
+  if (!db) return;
+  (void)hipSetDevice(db->ctx->device);
+  (void)hipStreamSynchronize(db->ctx->stream);
+  release_table(db);
+  if (db->d_orients) (void)hipFree(db->d_orients);
+  if (db->d_hkls) (void)hipFree(db->d_hkls);
+  if (db->d_hits) (void)hipFree(db->d_hits);
+  if (db->d_counters) (void)hipFree(db->d_counters);
+  if (db->d_batch) (void)hipFree(db->d_batch);
+  if (db->d_img) (void)hipFree(db->d_img);
+  {
+    std::lock_guard<std::mutex> lk(db->ctx->mu);
+    db->ctx->in_use -= db->n * 72 + (uint64_t)db->nhkls * 12 + 64;
+  }
+  delete db;
+}
+
+uint64_t lm_db_size(const lm_db *db) { return db ? db->n : 0; }
+uint64_t lm_db_table_bytes(const lm_db *db) { return db && db->table.ready ? db->table.bytes : 0; }
+uint64_t lm_db_table_spots(const lm_db *db) { return db && db->table.ready ? db->table.total : 0; }
+
+int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats) {
+  if (!db) {
+    set_error("null db");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  return forward_build(db, mode, stats);
+}
+
+int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count) {
+  if (!db) {
+    set_error("null db");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  return import_table(db, records, count);
+}
+
+int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out) {
+  if (!db) {
+    set_error("null db");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  return export_table(db, first, count, out);
+}
+
+int lm_score(lm_db *db, const uint8_t *img_u8, double im_scale, int32_t min_nr_spots,
+             double min_intensity, lm_hits *out) {
+  return lm_score_batch(db, 1, &img_u8, &im_scale, min_nr_spots, min_intensity, out);
+}
+
+int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t min_nr_spots,
+                 double min_intensity, void *stream, lm_hits *out) {
+  return lm_score_batch_dev(db, 1, d_img_u8, &im_scale, min_nr_spots, min_intensity, stream,
+                            out);
+}
+
+int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
+                   const double *im_scales, int32_t min_nr_spots, double min_intensity,
+                   lm_hits *outs) {
+  if (!db || !imgs_u8 || nimg < 1) {
+    set_error("lm_score_batch: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  if (nimg > 64) {
+    set_error("lm_score_batch: at most 64 images per call");
+    return LM_ERR_INVALID;
+  }
+  uint8_t *d = nullptr;
+  rc = stage_host_images(db, nimg, imgs_u8, &d);
+  if (rc) return rc;
+  return score_images(db, nimg, d, im_scales, min_nr_spots, min_intensity, db->ctx->stream,
+                      outs);
+}
+
+int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,
+                       int32_t min_nr_spots, double min_intensity, void *stream, lm_hits *outs) {
+  if (!db || !d_imgs || nimg < 1) {
+    set_error("lm_score_batch_dev: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  hipStream_t st = stream ? (hipStream_t)stream : db->ctx->stream;
+  return score_images(db, nimg, d_imgs, im_scales, min_nr_spots, min_intensity, st, outs);
+}
+
+int lm_selftest_fp64(lm_ctx *ctx, const double *a, const double *b, uint64_t n, double *q,
+                     double *s) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (n && (!a || !b || !q || !s)) {
+    set_error("lm_selftest_fp64: null buffer");
+    return LM_ERR_INVALID;
+  }
+  return selftest_fp64(ctx->stream, a, b, n, q, s);
+}
+
+void lm_hits_free(lm_hits *h) {
+  if (!h) return;
+  free(h->ids);
+  free(h->nspots);
+  free(h->totint);
+  free(h->score);
+  h->ids = nullptr;
+  h->nspots = nullptr;
+  h->totint = nullptr;
+  h->score = nullptr;
+  h->count = 0;
+}
+
+}  // extern "C"

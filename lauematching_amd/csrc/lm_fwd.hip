@@ -1,0 +1,562 @@
+// lm_fwd.hip - forward simulation of the orientation DB into the resident
+// spot table, and ForwardFile import / export.
+//
+// Reference behaviour (src/LaueMatchingCPU.c:591-667): for each orientation,
+// thisOrient = orientation x recip; each hkl in file order is projected onto
+// the detector; accepted reflections whose pixel is not yet taken by this
+// orientation are appended until MaxNrLaueSpots spots are listed.
+//
+// Device design: one lane per orientation, the hkl loop is wave-uniform so the
+// hkl triplets are scalar loads; each lane keeps its accepted pixels in an
+// LDS column (slot-major, bank-conflict free) for the duplicate test.  The
+// table is built in two passes over the DB: a count pass (spot counts only)
+// then, once the host has turned the per-tile totals into offsets, a write
+// pass that places every spot at its final position.  Both passes evaluate
+// the same projection, so the counts of the two passes agree by construction.
+//
+// The float64 path is compiled with FP contraction off and keeps the
+// reference's association order; HIP's double division and sqrt are the
+// IEEE correctly rounded operations, as are the x86 SSE2 ones of the
+// reference build.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "lm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace lm {
+namespace {
+
+constexpr int kFwdBlock = 128;
+constexpr int kLdsSlots = 64;  // per-lane duplicate list kept in LDS when M <= 64
+constexpr double kHc = 1.2398419739;  // src/LaueMatchingHeaders.h:41
+constexpr double kFourPi = 4 * 3.14159265358979323846;
+
+struct DevGeom {
+  double rotT[9];
+  double P[3];
+  double pxX, pxY;
+  double halfX, halfY;  // 0.5 * (N - 1)
+  double W, H;
+  double Elo, Ehi;
+  int nrPxX;
+};
+
+// Pixel from a detector coordinate: the reference keeps (int)t within
+// [0, N-1] (src/LaueMatchingCPU.c:627-632), i.e. exactly -1 < t < N.
+__device__ __forceinline__ bool to_pixel(double t, double n, int *ip) {
+  if (!(t > -1.0 && t < n)) return false;
+  *ip = (int)t;
+  return true;
+}
+
+// src/LaueMatchingCPU.c:604-636 for one (orientation, hkl).
+__device__ __forceinline__ bool project64(const double *M, double h, double k, double l,
+                                          const DevGeom &g, uint32_t *pix) {
+  double q0 = 0, q1 = 0, q2 = 0;
+  q0 += M[0] * h; q0 += M[1] * k; q0 += M[2] * l;
+  q1 += M[3] * h; q1 += M[4] * k; q1 += M[5] * l;
+  q2 += M[6] * h; q2 += M[7] * k; q2 += M[8] * l;
+  double qlen = __builtin_sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+  if (qlen == 0) return false;
+  double u0 = q0 / qlen, u1 = q1 / qlen, u2 = q2 / qlen;
+  double td = 2 * u2;
+  double k0 = 0 - td * u0, k1 = 0 - td * u1, k2 = 1.0 - td * u2;
+  double x = 0, y = 0, z = 0;
+  x += g.rotT[0] * k0; x += g.rotT[1] * k1; x += g.rotT[2] * k2;
+  y += g.rotT[3] * k0; y += g.rotT[4] * k1; y += g.rotT[5] * k2;
+  z += g.rotT[6] * k0; z += g.rotT[7] * k1; z += g.rotT[8] * k2;
+  if (z <= 0) return false;
+  double X = x * g.P[2] / z;
+  double Y = y * g.P[2] / z;
+  int ipx, ipy;
+  if (!to_pixel((X - g.P[0]) / g.pxX + g.halfX, g.W, &ipx)) return false;
+  if (!to_pixel((Y - g.P[1]) / g.pxY + g.halfY, g.H, &ipy)) return false;
+  double E = kHc * qlen / (kFourPi * (-u2));
+  if (E < g.Elo || E > g.Ehi) return false;
+  *pix = (uint32_t)ipy * (uint32_t)g.nrPxX + (uint32_t)ipx;
+  return true;
+}
+
+// One lane per orientation.  WRITE=false: counts[o] = number of spots.
+// WRITE=true: spots of orientation o go to entries[tile_off[tile] + in-tile
+// prefix of counts], in spot order.
+template <bool WRITE, bool LDS_LIST>
+__global__ void __launch_bounds__(kFwdBlock)
+fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__restrict__ hkls,
+           int nhkls, double r00, double r01, double r02, double r10, double r11, double r12,
+           double r20, double r21, double r22, DevGeom g, int maxSpots, int tileBits,
+           uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+           uint32_t *__restrict__ entries, uint32_t *__restrict__ spill) {
+  __shared__ uint32_t lst[LDS_LIST ? kLdsSlots * kFwdBlock : 1];
+  __shared__ uint32_t scan[kFwdBlock];
+  const uint64_t o = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
+  const bool live = o < n;
+  double M[9];
+  if (live) {
+    double t[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t[i] = orients[o * 9 + i];
+    // MatrixMultF33 (src/LaueMatchingHeaders.h:349-357): (a*b + c*d) + e*f
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      M[r * 3 + 0] = t[r * 3 + 0] * r00 + t[r * 3 + 1] * r10 + t[r * 3 + 2] * r20;
+      M[r * 3 + 1] = t[r * 3 + 0] * r01 + t[r * 3 + 1] * r11 + t[r * 3 + 2] * r21;
+      M[r * 3 + 2] = t[r * 3 + 0] * r02 + t[r * 3 + 1] * r12 + t[r * 3 + 2] * r22;
+    }
+  }
+  uint32_t *col = LDS_LIST ? lst + threadIdx.x : spill + o;
+  const uint64_t colStride = LDS_LIST ? (uint64_t)kFwdBlock : (uint64_t)gridDim.x * kFwdBlock;
+  int ns = live ? 0 : maxSpots;
+  for (int hi = 0; hi < nhkls; ++hi) {
+    if (__all(ns >= maxSpots)) break;
+    const double h = (double)hkls[3 * hi + 0];
+    const double k = (double)hkls[3 * hi + 1];
+    const double l = (double)hkls[3 * hi + 2];
+    if (ns < maxSpots) {
+      uint32_t pix;
+      if (project64(M, h, k, l, g, &pix)) {
+        bool dup = false;
+        for (int s = 0; s < ns; ++s)
+          if (col[(uint64_t)s * colStride] == pix) { dup = true; break; }
+        if (!dup) {
+          col[(uint64_t)ns * colStride] = pix;
+          ++ns;
+        }
+      }
+    }
+  }
+  if (!WRITE) {
+    if (live) counts[o] = (uint16_t)ns;
+    return;
+  }
+  // In-tile offset of this orientation: counts of the tile's rows before this
+  // block, then an exclusive scan over the block.
+  const int c = live ? ns : 0;
+  const uint64_t tile = o >> tileBits;
+  const uint64_t tileFirst = tile << tileBits;
+  const uint64_t blockFirst = (uint64_t)blockIdx.x * kFwdBlock;
+  uint32_t before = 0;
+  for (uint64_t r = tileFirst + threadIdx.x; r < blockFirst; r += kFwdBlock) before += counts[r];
+  scan[threadIdx.x] = before;
+  __syncthreads();
+  for (int s = kFwdBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) scan[threadIdx.x] += scan[threadIdx.x + s];
+    __syncthreads();
+  }
+  const uint32_t blockBase = scan[0];
+  __syncthreads();
+  scan[threadIdx.x] = (uint32_t)c;
+  __syncthreads();
+  for (int s = 1; s < kFwdBlock; s <<= 1) {
+    uint32_t v = ((int)threadIdx.x >= s) ? scan[threadIdx.x - s] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const uint32_t excl = scan[threadIdx.x] - (uint32_t)c;
+  if (live) {
+    uint32_t *dst = entries + tileOff[tile] + blockBase + excl;
+    for (int s = 0; s < c; ++s) dst[s] = col[(uint64_t)s * colStride];
+  }
+}
+
+// Per-tile sums of the counts.
+__global__ void tile_sum_kernel(const uint16_t *__restrict__ counts, uint64_t n, int tileBits,
+                                uint64_t *__restrict__ tileSum) {
+  __shared__ uint64_t red[256];
+  const uint64_t t = blockIdx.x;
+  const uint64_t first = t << tileBits;
+  uint64_t last = first + (1ull << tileBits);
+  if (last > n) last = n;
+  uint64_t s = 0;
+  for (uint64_t r = first + threadIdx.x; r < last; r += 256) s += counts[r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tileSum[t] = red[0];
+}
+
+// ForwardFile records (chunk of rows [row0, row0+rows)) -> counts, with
+// validation (count <= M, pixel inside the detector).
+__global__ void import_count_kernel(const uint16_t *__restrict__ rec, uint64_t rows,
+                                    uint64_t row0, int M, int W, int H,
+                                    uint16_t *__restrict__ counts, uint32_t *__restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const uint16_t *r = rec + i * (uint64_t)(1 + 2 * M);
+  int c = r[0];
+  bool ok = c <= M;
+  if (ok)
+    for (int s = 0; s < c; ++s)
+      if (r[1 + 2 * s] >= W || r[2 + 2 * s] >= H) { ok = false; break; }
+  if (!ok) atomicOr(bad, 1u);
+  counts[row0 + i] = (uint16_t)(ok ? c : 0);
+}
+
+__global__ void import_write_kernel(const uint16_t *__restrict__ rec, uint64_t rows,
+                                    uint64_t row0, int M, int W, int tileBits,
+                                    const uint16_t *__restrict__ counts,
+                                    const uint64_t *__restrict__ tileOff,
+                                    uint32_t *__restrict__ entries) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const uint64_t o = row0 + i;
+  const uint64_t tile = o >> tileBits;
+  uint64_t off = tileOff[tile];
+  for (uint64_t r = tile << tileBits; r < o; ++r) off += counts[r];
+  const uint16_t *src = rec + i * (uint64_t)(1 + 2 * M);
+  const int c = counts[o];
+  for (int s = 0; s < c; ++s) entries[off + s] = (uint32_t)src[2 + 2 * s] * W + src[1 + 2 * s];
+}
+
+__global__ void export_kernel(uint16_t *__restrict__ rec, uint64_t rows, uint64_t row0, int M,
+                              int W, int tileBits, const uint16_t *__restrict__ counts,
+                              const uint64_t *__restrict__ tileOff,
+                              const uint32_t *__restrict__ entries) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const uint64_t o = row0 + i;
+  const uint64_t tile = o >> tileBits;
+  uint64_t off = tileOff[tile];
+  for (uint64_t r = tile << tileBits; r < o; ++r) off += counts[r];
+  uint16_t *dst = rec + i * (uint64_t)(1 + 2 * M);
+  const int c = counts[o];
+  dst[0] = (uint16_t)c;
+  for (int s = 0; s < M; ++s) {
+    uint16_t x = 0, y = 0;
+    if (s < c) {
+      const uint32_t p = entries[off + s];
+      x = (uint16_t)(p % (uint32_t)W);
+      y = (uint16_t)(p / (uint32_t)W);
+    }
+    dst[1 + 2 * s] = x;
+    dst[2 + 2 * s] = y;
+  }
+}
+
+__global__ void selftest_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                uint64_t n, double *__restrict__ q, double *__restrict__ s) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  q[i] = a[i] / b[i];
+  s[i] = __builtin_sqrt(a[i]);
+}
+
+DevGeom dev_geom(const Geom &g) {
+  DevGeom d{};
+  for (int i = 0; i < 9; ++i) d.rotT[i] = g.rotT[i];
+  for (int i = 0; i < 3; ++i) d.P[i] = g.P[i];
+  d.pxX = g.pxX;
+  d.pxY = g.pxY;
+  d.halfX = 0.5 * (g.nrPxX - 1);
+  d.halfY = 0.5 * (g.nrPxY - 1);
+  d.W = (double)g.nrPxX;
+  d.H = (double)g.nrPxY;
+  d.Elo = g.Elo;
+  d.Ehi = g.Ehi;
+  d.nrPxX = g.nrPxX;
+  return d;
+}
+
+void free_table(lm_db *db) {
+  Table &t = db->table;
+  if (t.entries) (void)hipFree(t.entries);
+  if (t.tile_off) (void)hipFree(t.tile_off);
+  if (t.counts) (void)hipFree(t.counts);
+  {
+    std::lock_guard<std::mutex> lk(db->ctx->mu);
+    db->ctx->in_use -= t.bytes;
+  }
+  t = Table{};
+}
+
+int reserve(lm_db *db, uint64_t bytes) {
+  std::lock_guard<std::mutex> lk(db->ctx->mu);
+  if (db->ctx->in_use + bytes > db->ctx->budget) {
+    char msg[256];
+    snprintf(msg, sizeof msg, "HBM budget exceeded: need %llu more bytes, %llu of %llu in use",
+             (unsigned long long)bytes, (unsigned long long)db->ctx->in_use,
+             (unsigned long long)db->ctx->budget);
+    set_error(msg);
+    return LM_ERR_BUDGET;
+  }
+  db->ctx->in_use += bytes;
+  return LM_OK;
+}
+
+// Allocate counts + tile offsets for the DB's geometry.
+int alloc_table_index(lm_db *db) {
+  Table &t = db->table;
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  t.pix_bits = pix_bits_for(npx);
+  t.tile_bits = kTileBits;
+  t.ntiles = (db->n + (1ull << t.tile_bits) - 1) >> t.tile_bits;
+  const uint64_t bytes = (db->n + 64) * 2 + (t.ntiles + 1) * 8;
+  int rc = reserve(db, bytes);
+  if (rc) return rc;
+  t.bytes = bytes;
+  hipError_t e = hipMalloc(&t.counts, (db->n + 64) * 2);
+  if (e == hipSuccess) e = hipMalloc(&t.tile_off, (t.ntiles + 1) * 8);
+  if (e != hipSuccess) {
+    free_table(db);
+    return hip_fail(e, "hipMalloc(table index)");
+  }
+  return LM_OK;
+}
+
+// Per-tile totals -> offsets (host scan), then allocate the entry array.
+int finish_offsets(lm_db *db, hipStream_t st) {
+  Table &t = db->table;
+  uint64_t *d_sum = nullptr;
+  hipError_t e = hipMalloc(&d_sum, t.ntiles * 8);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(tile sums)");
+  hipLaunchKernelGGL(tile_sum_kernel, dim3((unsigned)t.ntiles), dim3(256), 0, st, t.counts, db->n,
+                     t.tile_bits, d_sum);
+  std::vector<uint64_t> h(t.ntiles + 1);
+  e = hipMemcpyAsync(h.data(), d_sum, t.ntiles * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d_sum);
+  if (e != hipSuccess) return hip_fail(e, "tile sums");
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < t.ntiles; ++i) {
+    uint64_t v = h[i];
+    h[i] = acc;
+    acc += v;
+  }
+  h[t.ntiles] = acc;
+  t.total = acc;
+  e = hipMemcpyAsync(t.tile_off, h.data(), (t.ntiles + 1) * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "tile offsets");
+  const uint64_t ebytes = (acc + 64) * 4;
+  int rc = reserve(db, ebytes);
+  if (rc) return rc;
+  t.bytes += ebytes;
+  e = hipMalloc(&t.entries, ebytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(table entries)");
+  e = hipMemsetAsync(t.entries + acc, 0, 64 * 4, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemset(table slack)");
+  return LM_OK;
+}
+
+}  // namespace
+
+int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
+  if (mode != LM_FWD_FP64 && mode != LM_FWD_FP32_GUARD) {
+    set_error("unknown forward mode");
+    return LM_ERR_INVALID;
+  }
+  if (mode == LM_FWD_FP32_GUARD) {
+    set_error("LM_FWD_FP32_GUARD is not available in this build");
+    return LM_ERR_INVALID;
+  }
+  hipStream_t st = db->ctx->stream;
+  if (db->table.counts) free_table(db);
+  int rc = alloc_table_index(db);
+  if (rc) return rc;
+  Table &t = db->table;
+  const DevGeom g = dev_geom(db->geom);
+  const double *R = db->geom.recip;
+  const bool ldsList = db->max_spots <= kLdsSlots;
+  // Grid chunks keep each launch under 2^31 threads and bound the spill list.
+  const uint64_t chunkBlocks = 1u << 18;  // 32M orientations per launch
+  uint32_t *spill = nullptr;
+  if (!ldsList) {
+    hipError_t e = hipMalloc(&spill, chunkBlocks * kFwdBlock * (uint64_t)db->max_spots * 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(forward spill list)");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, st);
+  for (int pass = 0; pass < 2 && rc == LM_OK; ++pass) {
+    if (pass == 1) {
+      rc = finish_offsets(db, st);
+      if (rc) break;
+    }
+    for (uint64_t b0 = 0; b0 * kFwdBlock < db->n; b0 += chunkBlocks) {
+      const uint64_t first = b0 * kFwdBlock;
+      const uint64_t rows = std::min<uint64_t>(db->n - first, chunkBlocks * kFwdBlock);
+      const unsigned blocks = (unsigned)((rows + kFwdBlock - 1) / kFwdBlock);
+      // Chunks start on tile boundaries (chunkBlocks*kFwdBlock is a multiple
+      // of the tile size), so the in-tile scan never crosses a chunk.
+      const double *op = db->d_orients + first * 9;
+      uint16_t *cp = t.counts + first;
+      const uint64_t tileBase = first >> t.tile_bits;
+      const uint64_t *tp = t.tile_off + tileBase;
+      if (pass == 0) {
+        if (ldsList)
+          hipLaunchKernelGGL((fwd_kernel<false, true>), dim3(blocks), dim3(kFwdBlock), 0, st, op,
+                             rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6],
+                             R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries, spill);
+        else
+          hipLaunchKernelGGL((fwd_kernel<false, false>), dim3(blocks), dim3(kFwdBlock), 0, st,
+                             op, rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5],
+                             R[6], R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries,
+                             spill);
+      } else {
+        if (ldsList)
+          hipLaunchKernelGGL((fwd_kernel<true, true>), dim3(blocks), dim3(kFwdBlock), 0, st, op,
+                             rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6],
+                             R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries, spill);
+        else
+          hipLaunchKernelGGL((fwd_kernel<true, false>), dim3(blocks), dim3(kFwdBlock), 0, st, op,
+                             rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6],
+                             R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries, spill);
+      }
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "forward kernel launch");
+        break;
+      }
+    }
+  }
+  (void)hipEventRecord(e1, st);
+  hipError_t e = hipStreamSynchronize(st);
+  if (spill) (void)hipFree(spill);
+  if (rc == LM_OK && e != hipSuccess) rc = hip_fail(e, "forward simulation");
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    free_table(db);
+    return rc;
+  }
+  t.ready = true;
+  if (stats) {
+    stats->n_orients = db->n;
+    stats->n_spots = t.total;
+    stats->n_pairs = 2ull * db->n * (uint64_t)db->nhkls;
+    stats->n_recomputed = 0;
+    stats->ms = ms;
+  }
+  return LM_OK;
+}
+
+int import_table(lm_db *db, const uint16_t *records, uint64_t count) {
+  if (count != db->n || !records) {
+    set_error("import_table: record count must equal the DB size");
+    return LM_ERR_INVALID;
+  }
+  hipStream_t st = db->ctx->stream;
+  if (db->table.counts) free_table(db);
+  int rc = alloc_table_index(db);
+  if (rc) return rc;
+  Table &t = db->table;
+  const int M = db->max_spots;
+  const uint64_t rec = (uint64_t)(1 + 2 * M);
+  const uint64_t chunk = std::max<uint64_t>(1, (256ull << 20) / (rec * 2));
+  uint16_t *d_rec = nullptr;
+  uint32_t *d_bad = nullptr;
+  hipError_t e = hipMalloc(&d_rec, chunk * rec * 2);
+  if (e == hipSuccess) e = hipMalloc(&d_bad, 4);
+  if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, 4, st);
+  for (int pass = 0; pass < 2 && e == hipSuccess && rc == LM_OK; ++pass) {
+    if (pass == 1) {
+      uint32_t bad = 0;
+      e = hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) break;
+      if (bad) {
+        set_error("import_table: a record has count > MaxNrLaueSpots or a pixel outside the "
+                  "detector");
+        rc = LM_ERR_INVALID;
+        break;
+      }
+      rc = finish_offsets(db, st);
+      if (rc) break;
+    }
+    for (uint64_t r0 = 0; r0 < count && e == hipSuccess; r0 += chunk) {
+      const uint64_t rows = std::min(chunk, count - r0);
+      e = hipMemcpyAsync(d_rec, records + r0 * rec, rows * rec * 2, hipMemcpyHostToDevice, st);
+      if (e != hipSuccess) break;
+      const unsigned blocks = (unsigned)((rows + 255) / 256);
+      if (pass == 0)
+        hipLaunchKernelGGL(import_count_kernel, dim3(blocks), dim3(256), 0, st, d_rec, rows, r0, M,
+                           db->geom.nrPxX, db->geom.nrPxY, t.counts, d_bad);
+      else
+        hipLaunchKernelGGL(import_write_kernel, dim3(blocks), dim3(256), 0, st, d_rec, rows, r0,
+                           M, db->geom.nrPxX, t.tile_bits, t.counts, t.tile_off, t.entries);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+  }
+  if (d_rec) (void)hipFree(d_rec);
+  if (d_bad) (void)hipFree(d_bad);
+  if (rc == LM_OK && e != hipSuccess) rc = hip_fail(e, "import_table");
+  if (rc) {
+    free_table(db);
+    return rc;
+  }
+  t.ready = true;
+  return LM_OK;
+}
+
+int export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out) {
+  Table &t = db->table;
+  if (!t.ready) {
+    set_error("export_table: no table (run lm_db_forward or lm_db_import_table)");
+    return LM_ERR_STATE;
+  }
+  if (first > db->n || count > db->n - first || (!out && count)) {
+    set_error("export_table: row range outside the DB");
+    return LM_ERR_INVALID;
+  }
+  hipStream_t st = db->ctx->stream;
+  const int M = db->max_spots;
+  const uint64_t rec = (uint64_t)(1 + 2 * M);
+  const uint64_t chunk = std::max<uint64_t>(1, (256ull << 20) / (rec * 2));
+  uint16_t *d_rec = nullptr;
+  hipError_t e = hipMalloc(&d_rec, std::min(chunk, std::max<uint64_t>(count, 1)) * rec * 2);
+  for (uint64_t r0 = 0; r0 < count && e == hipSuccess; r0 += chunk) {
+    const uint64_t rows = std::min(chunk, count - r0);
+    hipLaunchKernelGGL(export_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st,
+                       d_rec, rows, first + r0, M, db->geom.nrPxX, t.tile_bits, t.counts,
+                       t.tile_off, t.entries);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + r0 * rec, d_rec, rows * rec * 2, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  if (d_rec) (void)hipFree(d_rec);
+  if (e != hipSuccess) return hip_fail(e, "export_table");
+  return LM_OK;
+}
+
+void release_table(lm_db *db) {
+  if (db->table.counts) free_table(db);
+}
+
+int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, double *q,
+                  double *s) {
+  double *d = nullptr;
+  hipError_t e = hipMalloc(&d, std::max<uint64_t>(n, 1) * 32);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(selftest)");
+  double *da = d, *db = d + n, *dq = d + 2 * n, *ds = d + 3 * n;
+  e = hipMemcpyAsync(da, a, n * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(db, b, n * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && n) {
+    hipLaunchKernelGGL(selftest_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, da,
+                       db, n, dq, ds);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(q, dq, n * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s, ds, n * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "selftest_fp64");
+  return LM_OK;
+}
+
+}  // namespace lm

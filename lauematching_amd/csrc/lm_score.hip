@@ -1,0 +1,456 @@
+// lm_score.hip - compare/score of uint8 images against the resident spot
+// table, with hit compaction.
+//
+// Reference behaviour (src/LaueMatchingCPU.c:668-689, 772-788): for each
+// orientation, over its listed spots: raw = image_u8[y*W + x]; if raw > 0,
+// totInt += raw*imScale (float64, table order) and nSpots++.  A hit is
+// nSpots >= MinNrSpots && totInt >= MinIntensity, with score
+// totInt*sqrt(nSpots); hits are collected in ascending orientation order and
+// a zero score is not collected.
+//
+// Device design.  One workgroup per tile of 256 orientations, one lane per
+// orientation: the tile's spot entries are streamed into LDS with coalesced
+// loads, then every lane walks its own spots from LDS, gathers the image byte
+// (the image stays hot in L2 / Infinity Cache) and keeps the packed integer
+// accumulator (nspots << 17 | sum of raw) in a register.  B images can be
+// scored in the same pass: they are interleaved pixel-major so one gather of
+// B bytes serves all of them, and each lane keeps B accumulators.
+//
+// Exactness.  The integer sum is exact.  When imScale == 1 (the quantizer's
+// scale for any image whose maximum is 255) totInt equals that sum and the
+// predicate is decided on the device.  Otherwise the device keeps a
+// conservative superset of candidates and a second kernel recomputes their
+// float64 sums in table order, which the host then tests exactly.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "lm_internal.h"
+
+namespace lm {
+namespace {
+
+constexpr int kTile = 256;          // orientations per workgroup (= 1 << tile bits)
+constexpr int kStageWords = 12288;  // 48 KB of staged entries per workgroup
+constexpr int kMaxBatch = 64;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct Pred {
+  double min_int;
+  int min_spots;
+  int nimg;
+  uint32_t exact_mask_lo, exact_mask_hi;  // bit b: image b has imScale == 1
+  double cand_thr[kMaxBatch];             // candidate threshold on the integer sum
+};
+
+__host__ __device__ __forceinline__ bool bit(const Pred &p, int b) {
+  return b < 32 ? (p.exact_mask_lo >> b) & 1u : (p.exact_mask_hi >> (b - 32)) & 1u;
+}
+
+template <int B>
+__device__ __forceinline__ void gather(const uint8_t *__restrict__ img, uint32_t pix,
+                                       uint32_t (&acc)[B]) {
+  if constexpr (B == 1) {
+    const uint32_t raw = img[pix];
+    acc[0] += raw ? (1u << kSumBits) + raw : 0u;
+  } else if constexpr (B == 4) {
+    const uint32_t v = *(const uint32_t *)(img + (uint64_t)pix * 4);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t raw = (v >> (8 * b)) & 0xffu;
+      acc[b] += raw ? (1u << kSumBits) + raw : 0u;
+    }
+  } else {
+    static_assert(B % 8 == 0, "batch must be 1, 4 or a multiple of 8");
+    const uint2 *p = (const uint2 *)(img + (uint64_t)pix * B);
+#pragma unroll
+    for (int q = 0; q < B / 8; ++q) {
+      const uint2 v = p[q];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t r0 = (v.x >> (8 * b)) & 0xffu;
+        const uint32_t r1 = (v.y >> (8 * b)) & 0xffu;
+        acc[q * 8 + b] += r0 ? (1u << kSumBits) + r0 : 0u;
+        acc[q * 8 + 4 + b] += r1 ? (1u << kSumBits) + r1 : 0u;
+      }
+    }
+  }
+}
+
+// Wave-aggregated append: one atomic per wave, lanes write at base + rank.
+__device__ __forceinline__ void append(bool flag, uint64_t rec, uint64_t *__restrict__ out,
+                                       uint32_t *__restrict__ counter, uint64_t cap) {
+  const uint64_t m = __ballot(flag);
+  if (m == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (flag) {
+    const uint64_t below = m & ((1ull << lane) - 1ull);
+    const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(below);
+    if (pos < cap) out[pos] = rec;
+  }
+}
+
+// counters layout: [2*b] = hits written for image b, [2*b+1] = zero-spot rows
+// passing the predicate for image b.
+template <int B>
+__global__ void __launch_bounds__(kTile)
+score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+             const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
+             const uint8_t *__restrict__ img, Pred pred, uint64_t *__restrict__ hits,
+             uint64_t capPer, uint32_t *__restrict__ counters) {
+  __shared__ uint32_t stage[kStageWords];
+  __shared__ uint32_t wsum[kTile / 64];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t o = t * kTile + tid;
+    const uint32_t c = o < n ? counts[o] : 0u;
+    // exclusive scan of c over the workgroup
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    const uint64_t base = tileOff[t];
+    const uint64_t tot = tileOff[t + 1] - base;
+    __syncthreads();
+    uint32_t excl = incl - c;
+    for (int w = 0; w < wave; ++w) excl += wsum[w];
+    const bool staged = tot <= (uint64_t)kStageWords;
+    if (staged) {
+      // 16-byte loads from an aligned start; words outside [base, base+tot) dropped.
+      const uint64_t a0 = base & ~3ull;
+      const uint64_t nvec = (base + tot - a0 + 3) >> 2;
+      const u32x4 *src = (const u32x4 *)(entries + a0);
+      for (uint64_t i = tid; i < nvec; i += kTile) {
+        const u32x4 v = __builtin_nontemporal_load(src + i);
+        const int64_t j = (int64_t)(a0 + 4 * i) - (int64_t)base;
+        if (j >= 0 && j < (int64_t)tot) stage[j] = v.x;
+        if (j + 1 >= 0 && j + 1 < (int64_t)tot) stage[j + 1] = v.y;
+        if (j + 2 >= 0 && j + 2 < (int64_t)tot) stage[j + 2] = v.z;
+        if (j + 3 >= 0 && j + 3 < (int64_t)tot) stage[j + 3] = v.w;
+      }
+    }
+    __syncthreads();
+    uint32_t acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    if (staged) {
+      for (uint32_t s = 0; s < c; ++s) gather<B>(img, stage[excl + s], acc);
+    } else {
+      const uint32_t *src = entries + base + excl;
+      for (uint32_t s = 0; s < c; ++s) gather<B>(img, src[s], acc);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const uint32_t ns = acc[b] >> kSumBits;
+      const uint32_t sum = acc[b] & kSumMask;
+      const bool live = o < n;
+      bool cand = false, zero_pass = false;
+      if (live) {
+        if (ns == 0) {
+          zero_pass = (0 >= pred.min_spots) && (0.0 >= pred.min_int);
+        } else {
+          cand = ((int)ns >= pred.min_spots) && ((double)sum >= pred.cand_thr[b]);
+        }
+      }
+      const uint64_t zm = __ballot(zero_pass);
+      if (zm && lane == 0) atomicAdd(&counters[2 * b + 1], (uint32_t)__popcll(zm));
+      const uint64_t rec = ((uint64_t)b << 58) | (o << 26) | (uint64_t)acc[b];
+      append(cand, rec, hits + (uint64_t)b * capPer, &counters[2 * b], capPer);
+    }
+    __syncthreads();
+  }
+}
+
+// [nimg][npx] -> [npx][B] (B >= nimg; missing images read as zero).
+template <int B>
+__global__ void interleave_kernel(const uint8_t *__restrict__ src, int nimg, uint64_t npx,
+                                  uint8_t *__restrict__ dst) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npx) return;
+  uint8_t v[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) v[b] = b < nimg ? src[(uint64_t)b * npx + p] : 0;
+  if constexpr (B == 4) {
+    *(uint32_t *)(dst + p * 4) = v[0] | (v[1] << 8) | (v[2] << 16) | ((uint32_t)v[3] << 24);
+  } else {
+#pragma unroll
+    for (int q = 0; q < B / 8; ++q) {
+      uint2 w;
+      w.x = v[8 * q + 0] | (v[8 * q + 1] << 8) | (v[8 * q + 2] << 16) |
+            ((uint32_t)v[8 * q + 3] << 24);
+      w.y = v[8 * q + 4] | (v[8 * q + 5] << 8) | (v[8 * q + 6] << 16) |
+            ((uint32_t)v[8 * q + 7] << 24);
+      *(uint2 *)(dst + p * B + 8 * q) = w;
+    }
+  }
+}
+
+// Exact float64 sums of candidates (src/LaueMatchingCPU.c:674-681 order).
+// img points at image b's bytes with pixel stride `stride`.
+__global__ void exact_kernel(const uint16_t *__restrict__ counts,
+                             const uint64_t *__restrict__ tileOff,
+                             const uint32_t *__restrict__ entries, const uint32_t *__restrict__ ids,
+                             uint64_t ncand, const uint8_t *__restrict__ img, uint32_t stride,
+                             double scale, double *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncand) return;
+  const uint64_t o = ids[i];
+  const uint64_t t = o / kTile;
+  uint64_t off = tileOff[t];
+  for (uint64_t r = t * kTile; r < o; ++r) off += counts[r];
+  const int c = counts[o];
+  double tot = 0;
+  for (int s = 0; s < c; ++s) {
+    const uint8_t raw = img[(uint64_t)entries[off + s] * stride];
+    if (raw > 0) tot += (double)raw * scale;
+  }
+  out[i] = tot;
+}
+
+int ensure_scratch(lm_db *db, int nimg, uint64_t capPer) {
+  const uint64_t need = capPer * (uint64_t)nimg;
+  if (db->hit_cap < need) {
+    if (db->d_hits) (void)hipFree(db->d_hits);
+    db->d_hits = nullptr;
+    db->hit_cap = 0;
+    hipError_t e = hipMalloc(&db->d_hits, need * 8);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit buffer)");
+    db->hit_cap = need;
+  }
+  if (db->counter_slots < (uint64_t)2 * kMaxBatch) {
+    hipError_t e = hipMalloc(&db->d_counters, 2 * kMaxBatch * 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
+    db->counter_slots = 2 * kMaxBatch;
+  }
+  return LM_OK;
+}
+
+int launch_score(lm_db *db, int B, const uint8_t *img, const Pred &pred, uint64_t capPer,
+                 hipStream_t st) {
+  const Table &t = db->table;
+  const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
+#define LM_LAUNCH(BB)                                                                           \
+  hipLaunchKernelGGL(score_kernel<BB>, dim3(grid), dim3(kTile), 0, st, t.counts, t.tile_off,   \
+                     t.entries, db->n, t.ntiles, img, pred, db->d_hits, capPer, db->d_counters)
+  switch (B) {
+    case 1: LM_LAUNCH(1); break;
+    case 4: LM_LAUNCH(4); break;
+    case 8: LM_LAUNCH(8); break;
+    case 16: LM_LAUNCH(16); break;
+    case 32: LM_LAUNCH(32); break;
+    case 64: LM_LAUNCH(64); break;
+    default: set_error("unsupported batch width"); return LM_ERR_INVALID;
+  }
+#undef LM_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "score kernel launch");
+  return LM_OK;
+}
+
+int launch_interleave(int B, const uint8_t *src, int nimg, uint64_t npx, uint8_t *dst,
+                      hipStream_t st) {
+  const unsigned grid = (unsigned)((npx + 255) / 256);
+#define LM_IL(BB) \
+  hipLaunchKernelGGL(interleave_kernel<BB>, dim3(grid), dim3(256), 0, st, src, nimg, npx, dst)
+  switch (B) {
+    case 4: LM_IL(4); break;
+    case 8: LM_IL(8); break;
+    case 16: LM_IL(16); break;
+    case 32: LM_IL(32); break;
+    case 64: LM_IL(64); break;
+    default: set_error("unsupported batch width"); return LM_ERR_INVALID;
+  }
+#undef LM_IL
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "interleave kernel launch");
+  return LM_OK;
+}
+
+int batch_width(int nimg) {
+  if (nimg <= 1) return 1;
+  if (nimg <= 4) return 4;
+  int B = 8;
+  while (B < nimg) B <<= 1;
+  return B;
+}
+
+}  // namespace
+
+int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_scales,
+                 int min_nr_spots, double min_intensity, hipStream_t st, lm_hits *outs) {
+  const Table &t = db->table;
+  if (!t.ready) {
+    set_error("score: no table (run lm_db_forward or lm_db_import_table)");
+    return LM_ERR_STATE;
+  }
+  if (nimg < 1 || nimg > kMaxBatch || !d_imgs || !im_scales || !outs) {
+    set_error("score: need 1..64 images");
+    return LM_ERR_INVALID;
+  }
+  for (int b = 0; b < nimg; ++b)
+    if (!(im_scales[b] > 0) || !std::isfinite(im_scales[b])) {
+      set_error("score: imScale must be positive and finite");
+      return LM_ERR_INVALID;
+    }
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  const int B = batch_width(nimg);
+  Pred pred{};
+  pred.min_int = min_intensity;
+  pred.min_spots = min_nr_spots;
+  pred.nimg = nimg;
+  for (int b = 0; b < kMaxBatch; ++b) pred.cand_thr[b] = INFINITY;
+  for (int b = 0; b < nimg; ++b) {
+    const double s = im_scales[b];
+    if (s == 1.0) {
+      // totInt == sum exactly: sum >= min_int decided on the device
+      pred.cand_thr[b] = min_intensity;
+      if (b < 32) pred.exact_mask_lo |= 1u << b; else pred.exact_mask_hi |= 1u << (b - 32);
+    } else {
+      // superset: the sequential float64 sum is within n*2^-53 relative of sum*s
+      pred.cand_thr[b] = min_intensity / s * (1.0 - 1e-12);
+      if (min_intensity <= 0) pred.cand_thr[b] = -INFINITY;
+    }
+  }
+  uint64_t capPer = std::max<uint64_t>(db->hit_cap / (uint64_t)std::max(nimg, 1), 1u << 16);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  uint8_t *d_il = nullptr;
+  int rc = LM_OK;
+  std::vector<uint32_t> cnt(2 * kMaxBatch);
+  const uint8_t *img = d_imgs;
+  if (B > 1) {
+    if (db->batch_cap < npx * B) {
+      if (db->d_batch) (void)hipFree(db->d_batch);
+      db->d_batch = nullptr;
+      db->batch_cap = 0;
+      hipError_t e = hipMalloc(&db->d_batch, npx * B);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc(interleaved batch)");
+      db->batch_cap = npx * B;
+    }
+    d_il = db->d_batch;
+    rc = launch_interleave(B, d_imgs, nimg, npx, d_il, st);
+    img = d_il;
+  }
+  float ms = 0;
+  for (int attempt = 0; attempt < 2 && rc == LM_OK; ++attempt) {
+    rc = ensure_scratch(db, nimg, capPer);
+    if (rc) break;
+    hipError_t e = hipMemsetAsync(db->d_counters, 0, 2 * kMaxBatch * 4, st);
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(counters)"); break; }
+    (void)hipEventRecord(e0, st);
+    rc = launch_score(db, B, img, pred, capPer, st);
+    if (rc) break;
+    (void)hipEventRecord(e1, st);
+    e = hipMemcpyAsync(cnt.data(), db->d_counters, 2 * kMaxBatch * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { rc = hip_fail(e, "score kernel"); break; }
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    uint64_t need = 0;
+    for (int b = 0; b < nimg; ++b) need = std::max<uint64_t>(need, cnt[2 * b]);
+    if (need <= capPer) break;
+    if (attempt == 1) {
+      set_error("score: hit list overflow");
+      rc = LM_ERR_OVERFLOW;
+      break;
+    }
+    capPer = need + need / 4 + 1024;  // rerun once with a buffer that fits
+  }
+  std::vector<uint64_t> rec;
+  uint32_t *d_ids = nullptr;
+  double *d_exact = nullptr;
+  for (int b = 0; b < nimg && rc == LM_OK; ++b) {
+    lm_hits &h = outs[b];
+    memset(&h, 0, sizeof h);
+    h.kernel_ms = ms;
+    const uint64_t k = cnt[2 * b];
+    rec.resize(k);
+    if (k) {
+      hipError_t e = hipMemcpyAsync(rec.data(), db->d_hits + (uint64_t)b * capPer, k * 8,
+                                    hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) { rc = hip_fail(e, "hit copy"); break; }
+      std::sort(rec.begin(), rec.end());
+    }
+    std::vector<double> tot(k);
+    const bool exact = bit(pred, b);
+    if (exact) {
+      for (uint64_t i = 0; i < k; ++i) tot[i] = (double)(rec[i] & kSumMask);
+    } else if (k) {
+      std::vector<uint32_t> ids(k);
+      for (uint64_t i = 0; i < k; ++i) ids[i] = (uint32_t)((rec[i] >> 26) & 0xffffffffull);
+      if (d_ids) (void)hipFree(d_ids);
+      if (d_exact) (void)hipFree(d_exact);
+      d_ids = nullptr;
+      d_exact = nullptr;
+      hipError_t e = hipMalloc(&d_ids, k * 4);
+      if (e == hipSuccess) e = hipMalloc(&d_exact, k * 8);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(d_ids, ids.data(), k * 4, hipMemcpyHostToDevice, st);
+      if (e != hipSuccess) { rc = hip_fail(e, "exact pass setup"); break; }
+      const uint8_t *ib = (B > 1) ? img + b : img;
+      hipLaunchKernelGGL(exact_kernel, dim3((unsigned)((k + 127) / 128)), dim3(128), 0, st,
+                         t.counts, t.tile_off, t.entries, d_ids, k, ib, (uint32_t)B,
+                         im_scales[b], d_exact);
+      e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(tot.data(), d_exact, k * 8, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) { rc = hip_fail(e, "exact pass"); break; }
+    }
+    // final predicate and arrays (ascending id)
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < k; ++i)
+      if (tot[i] >= min_intensity) ++m;
+    h.nr_results = (uint64_t)cnt[2 * b + 1] + m;
+    h.count = m;
+    if (m) {
+      h.ids = (uint64_t *)malloc(m * 8);
+      h.nspots = (int32_t *)malloc(m * 4);
+      h.totint = (double *)malloc(m * 8);
+      h.score = (double *)malloc(m * 8);
+      if (!h.ids || !h.nspots || !h.totint || !h.score) {
+        lm_hits_free(&h);
+        set_error("out of host memory for hits");
+        rc = LM_ERR_NOMEM;
+        break;
+      }
+      uint64_t j = 0;
+      for (uint64_t i = 0; i < k; ++i) {
+        if (!(tot[i] >= min_intensity)) continue;
+        const uint32_t packed = (uint32_t)(rec[i] & 0x3ffffffull);
+        const int ns = (int)(packed >> kSumBits);
+        h.ids[j] = (rec[i] >> 26) & 0xffffffffull;
+        h.nspots[j] = ns;
+        h.totint[j] = tot[i];
+        h.score[j] = tot[i] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
+        ++j;
+      }
+    }
+  }
+  if (rc) {
+    for (int b = 0; b < nimg; ++b) lm_hits_free(&outs[b]);
+  }
+  if (d_ids) (void)hipFree(d_ids);
+  if (d_exact) (void)hipFree(d_exact);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+}  // namespace lm

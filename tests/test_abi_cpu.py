@@ -1,0 +1,94 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol that
+include/*.h declares, and its host-only functions match the oracle bit for
+bit (no GPU needed)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from lauematching_amd import _lib, api, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(lm_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in sorted(names):
+        assert hasattr(L, n), n
+    assert set(_lib.SIGNATURES) >= names
+
+
+def test_version_and_errors():
+    assert b"gfx950" in _lib.lib().lm_version()
+    h = ctypes.c_void_p()
+    rc = _lib.lib().lm_ctx_create(0, 0, ctypes.byref(h))
+    try:
+        import torch
+        gpu = torch.cuda.is_available()
+    except Exception:
+        gpu = False
+    if not gpu:
+        assert rc != 0 and _lib.lib().lm_last_error()
+    elif rc == 0:
+        _lib.lib().lm_ctx_destroy(h)
+
+
+@pytest.mark.parametrize("lat,sg,R", [
+    ((0.36, 0.36, 0.36, 90, 90, 90), 225, (synth.R_ARRAY_90DEG,) * 3),
+    ((0.295, 0.295, 0.468165, 90, 90, 120), 194, (-1.20161887, -1.21404493, -1.21852276)),
+    ((0.51, 0.51, 0.51, 60, 60, 60), 166, (0.2, -0.4, 1.1)),
+    ((0.3, 0.41, 0.52, 88, 95, 101), 2, (0.0, 1e-3, 0.0)),
+])
+def test_geometry_bit_identical_to_oracle(lat, sg, R):
+    p = synth.baseline_params(LatticeParameter=lat, SpaceGroup=sg, R_Array=R)
+    g = api.Geometry.from_params(p)
+    og = ob.geom(p)
+    assert g.recip.tobytes() == np.array(og.recip).reshape(3, 3).tobytes()
+    assert g.rotT.tobytes() == np.array(og.rotT).reshape(3, 3).tobytes()
+
+
+def test_quantize_bit_identical_to_oracle():
+    rng = np.random.default_rng(4)
+    for scale in (1.0, 0.37, 1234.5):
+        img = rng.gamma(0.3, 20.0, 100000) * scale
+        img[rng.random(img.size) < 0.2] = 0
+        img[rng.random(img.size) < 0.01] = -1
+        a, sa = api.quantize_u8(img)
+        b, sb = ob.quantize(img)
+        assert sa == sb
+        np.testing.assert_array_equal(a, b)
+
+
+def test_invalid_geometry_rejected():
+    with pytest.raises(_lib.LaueMatchError):
+        api.Geometry((0, 0, 0, 90, 90, 90), 225, (0, 0, .5), (0.1, 0, 0), 1e-4, 1e-4, 64, 64, 5, 30)
+    with pytest.raises(_lib.LaueMatchError):
+        api.Geometry((.3, .3, .3, 90, 90, 90), 225, (0, 0, .5), (0.1, 0, 0), 1e-4, 1e-4, 0, 64, 5, 30)
+
+
+def test_read_params_prefix_matching(tmp_path):
+    f = tmp_path / "p.txt"
+    f.write_text("LatticeParameter 0.35 0.35 0.35 90 90 90\nNrPxX 1024 # comment\nNrPxY 512\n"
+                 "MaxNrLaueSpotsXYZ 30\nElo 6\nEhi 25.5\nMinIntensity 12.5\nDoFwd 0\n"
+                 "ForwardFile /dev/shm/x.bin\nP_Array 0 0 0.4\n")
+    p = api.read_params(str(f))
+    assert p.LatticeParameter[0] == 0.35 and p.NrPxX == 1024 and p.NrPxY == 512
+    assert p.MaxNrLaueSpots == 30  # prefix match, as the C parser does
+    assert p.Elo == 6 and p.Ehi == 25.5 and p.MinIntensity == 12.5 and p.DoFwd == 0
+    assert p.ForwardFile == "/dev/shm/x.bin" and p.P_Array == (0.0, 0.0, 0.4)
+    assert p.MinNrSpots == 5  # C default

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.  Integer tables, counts, ids and sums must be bit-exact;
+float64 scores and intensity sums must be bit-identical too (the oracle is the
+reference's float64 arithmetic)."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from lauematching_amd import _lib, api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return api.Context(0)
+
+
+def _db(ctx, p, O, hk):
+    return api.OrientationDB(ctx, O, hk, api.Geometry.from_params(p), p.MaxNrLaueSpots)
+
+
+def test_selftest_fp64_division_and_sqrt(ctx):
+    """Device float64 division and sqrt are correctly rounded (bit-equal to
+    the host's IEEE results) over random and hard inputs."""
+    rng = np.random.default_rng(1)
+    n = 1 << 22
+    a = np.exp(rng.uniform(-40, 40, n)) * rng.choice([-1.0, 1.0], n)
+    b = np.exp(rng.uniform(-40, 40, n)) * rng.choice([-1.0, 1.0], n)
+    a[:1000] = np.nextafter(1.0, 2.0) ** np.arange(1000)  # near 1
+    b[:1000] = 3.0
+    a = np.abs(a) * np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    q = np.zeros(n)
+    s = np.zeros(n)
+    _say("selftest_fp64")
+    _lib.check(_lib.lib().lm_selftest_fp64(ctx.handle, a.ctypes.data, b.ctypes.data, n,
+                                           q.ctypes.data, s.ctypes.data))
+    np.testing.assert_array_equal(q.view(np.uint64), (a / b).view(np.uint64))
+    pos = a > 0
+    np.testing.assert_array_equal(s[pos].view(np.uint64), np.sqrt(a[pos]).view(np.uint64))
+    assert np.isnan(s[~pos]).all()
+
+
+GEOMS = {
+    "baseline": dict(),
+    "small_rect": dict(NrPxX=300, NrPxY=200, PxX=0.0008, PxY=0.0011, P_Array=(0.003, -0.002, 0.3)),
+    "ni_fixture": dict(LatticeParameter=(0.35238,) * 3 + (90.0,) * 3,
+                       P_Array=(0.028828, 0.002715, 0.512993),
+                       R_Array=(-1.20161887, -1.21404493, -1.21852276)),
+}
+
+
+@pytest.mark.parametrize("geo", list(GEOMS))
+@pytest.mark.parametrize("M", [64, 7, 100])
+def test_forward_table_bit_identical(ctx, geo, M):
+    p = synth.baseline_params(MaxNrLaueSpots=M, **GEOMS[geo])
+    hk = synth.fcc_hkls(8, "box")
+    n = 20000 if M != 100 else 3000
+    O = synth.random_orientations(n, seed=11)
+    _say(f"forward geo={geo} M={M} n={n}")
+    db = _db(ctx, p, O, hk)
+    st = db.forward()
+    got = db.export_table()
+    want, _, _ = ob.forward(O, hk, ob.geom(p), M)
+    np.testing.assert_array_equal(got, want)
+    assert st.n_spots == int(want[:, 0].astype(np.int64).sum()) == db.table_spots
+    # partial export of a range that starts mid-tile
+    np.testing.assert_array_equal(db.export_table(1234, 999), want[1234:1234 + 999])
+
+
+def test_forward_hexagonal(ctx):
+    p = synth.baseline_params(LatticeParameter=(0.295, 0.295, 0.468165, 90, 90, 120),
+                              SpaceGroup=194, MaxNrLaueSpots=64)
+    hk = synth.hcp_hkls(6)
+    O = synth.random_orientations(8000, seed=12)
+    _say("forward hexagonal")
+    db = _db(ctx, p, O, hk)
+    db.forward()
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    np.testing.assert_array_equal(db.export_table(), want)
+
+
+@pytest.fixture(scope="module")
+def scored_case(ctx):
+    p = synth.baseline_params(MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(8, "box")
+    n = 60000
+    O = synth.random_orientations(n, seed=21)
+    grains = [123, 40000, 59999]
+    img = synth.synth_image(p, hk, O[grains], seed=5)
+    table, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    db = _db(ctx, p, O, hk)
+    db.forward()
+    return p, hk, O, grains, img, table, db
+
+
+@pytest.mark.parametrize("min_sp,min_int", [(5, 1000.0), (5, 300.0), (8, 0.0), (0, 0.0),
+                                             (3, 2500.0)])
+def test_score_matches_oracle_unit_scale(scored_case, min_sp, min_int):
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img)
+    assert s == 1.0
+    _say(f"score unit scale min_sp={min_sp} min_int={min_int}")
+    hits = db.score(u8, s, min_sp, min_int)
+    matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, min_sp, min_int)
+    rows, mA = ob.collect(matched)
+    assert hits.nr_results == nres
+    np.testing.assert_array_equal(hits.ids, rows)
+    np.testing.assert_array_equal(hits.score, mA)
+    np.testing.assert_array_equal(hits.nspots, ns[rows.astype(np.int64)])
+    np.testing.assert_array_equal(hits.totint, tot[rows.astype(np.int64)])
+    if min_int <= 1000:
+        for g in grains:
+            assert g in set(hits.ids.tolist())
+
+
+@pytest.mark.parametrize("factor", [0.37, 3.3])
+def test_score_matches_oracle_nonunit_scale(scored_case, factor):
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img * factor + (img > 0) * 0.01)
+    assert s != 1.0
+    for min_sp, min_int in [(5, 1000.0 * factor), (4, 200.0 * factor)]:
+        _say(f"score scale={s} min_int={min_int}")
+        hits = db.score(u8, s, min_sp, min_int)
+        matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, min_sp, min_int)
+        rows, mA = ob.collect(matched)
+        assert hits.nr_results == nres
+        np.testing.assert_array_equal(hits.ids, rows)
+        np.testing.assert_array_equal(hits.totint, tot[rows.astype(np.int64)])
+        np.testing.assert_array_equal(hits.score, mA)
+
+
+def test_import_matches_forward_and_roundtrips(ctx, scored_case):
+    p, hk, O, grains, img, table, db = scored_case
+    _say("import/export roundtrip")
+    db2 = _db(ctx, p, O, hk)
+    db2.import_table(table)
+    np.testing.assert_array_equal(db2.export_table(), table)
+    u8, s = api.quantize_u8(img)
+    a = db.score(u8, s, 5, 500.0)
+    b = db2.score(u8, s, 5, 500.0)
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.score, b.score)
+
+
+def test_import_rejects_bad_records(ctx):
+    p = synth.baseline_params(NrPxX=64, NrPxY=32, MaxNrLaueSpots=4)
+    O = synth.random_orientations(10, seed=1)
+    db = _db(ctx, p, O, synth.fcc_hkls(4))
+    rec = np.zeros((10, 9), np.uint16)
+    rec[3, 0] = 5  # count > M
+    with pytest.raises(_lib.LaueMatchError):
+        db.import_table(rec)
+    rec[3, 0] = 1
+    rec[3, 1], rec[3, 2] = 10, 40  # y outside the detector
+    with pytest.raises(_lib.LaueMatchError):
+        db.import_table(rec)
+    rec[3, 2] = 31
+    db.import_table(rec)
+    np.testing.assert_array_equal(db.export_table(), rec)
+
+
+@pytest.mark.parametrize("nimg", [2, 5, 8, 16, 64])
+def test_batch_equals_single(scored_case, nimg):
+    p, hk, O, grains, img, table, db = scored_case
+    rng = np.random.default_rng(nimg)
+    imgs, scales = [], []
+    base, _ = api.quantize_u8(img)
+    for i in range(nimg):
+        perm = rng.permutation(base.size) if i % 3 == 1 else None
+        u8 = base if perm is None else base[perm]
+        if i % 4 == 3:
+            u8, s = api.quantize_u8(img * 0.5 + 0.01 * (img > 0))
+        else:
+            s = 1.0
+        imgs.append(u8)
+        scales.append(s)
+    _say(f"batch nimg={nimg}")
+    got = db.score_batch(imgs, scales, 5, 600.0)
+    for i in (0, nimg // 2, nimg - 1):
+        single = db.score(imgs[i], scales[i], 5, 600.0)
+        assert got[i].nr_results == single.nr_results
+        np.testing.assert_array_equal(got[i].ids, single.ids)
+        np.testing.assert_array_equal(got[i].score, single.score)
+        np.testing.assert_array_equal(got[i].nspots, single.nspots)
+
+
+def test_budget_admission(ctx):
+    small = api.Context(0, hbm_budget=1 << 20)
+    p = synth.baseline_params()
+    O = synth.random_orientations(20000, seed=2)  # 1.44 MB of orientations
+    with pytest.raises(_lib.LaueMatchError) as e:
+        api.OrientationDB(small, O, synth.fcc_hkls(8), api.Geometry.from_params(p), 64)
+    assert e.value.code == -4
