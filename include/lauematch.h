@@ -146,6 +146,13 @@ int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out)
 int lm_score(lm_db *db, const uint8_t *img_u8, double im_scale, int32_t min_nr_spots,
              double min_intensity, lm_hits *out);
 
+/* Score a float64 image the way the reference's DoFwd=1 branch does:
+ * nspots counts pixels > 0, totint is their sequential float64 sum in table
+ * order (src/LaueMatchingCPU.c:647-653, 683-689; the GPU programs reach it
+ * through src/LaueMatchingGPU.cu:463-617). */
+int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double min_intensity,
+                 lm_hits *out);
+
 /* Same with the image already in device memory (d_img_u8 is a device
  * pointer); stream is a hipStream_t (NULL = the context's stream). */
 int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t min_nr_spots,

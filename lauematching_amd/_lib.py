@@ -73,6 +73,8 @@ SIGNATURES = {
                                           ctypes.c_void_p]),
     "lm_score": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
                                 ctypes.c_double, ctypes.POINTER(LmHits)]),
+    "lm_score_f64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                    ctypes.c_double, ctypes.POINTER(LmHits)]),
     "lm_score_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                     ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
                                     ctypes.POINTER(LmHits)]),

@@ -171,6 +171,16 @@ class OrientationDB:
                              float(min_intensity), ctypes.byref(h)))
         return Hits._take(h)
 
+    def score_f64(self, img: np.ndarray, min_nr_spots: int, min_intensity: float) -> Hits:
+        """Float64-image scoring of the reference DoFwd=1 branch."""
+        im = np.ascontiguousarray(img, dtype=np.float64).ravel()
+        if im.size != self.geom.nrPxX * self.geom.nrPxY:
+            raise ValueError("image size does not match the detector")
+        h = LmHits()
+        check(lib().lm_score_f64(self._h, _ptr(im), int(min_nr_spots), float(min_intensity),
+                                 ctypes.byref(h)))
+        return Hits._take(h)
+
     def score_dev(self, dev_ptr: int, im_scale: float, min_nr_spots: int, min_intensity: float,
                   stream: int = 0) -> Hits:
         h = LmHits()

@@ -326,6 +326,7 @@ void lm_db_destroy(lm_db *db) {
   if (db->d_counters) (void)hipFree(db->d_counters);
   if (db->d_batch) (void)hipFree(db->d_batch);
   if (db->d_img) (void)hipFree(db->d_img);
+  if (db->d_vals) (void)hipFree(db->d_vals);
   {
     std::lock_guard<std::mutex> lk(db->ctx->mu);
     db->ctx->in_use -= db->n * 72 + (uint64_t)db->nhkls * 12 + 64;
@@ -398,8 +399,35 @@ int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
   uint8_t *d = nullptr;
   rc = stage_host_images(db, nimg, imgs_u8, &d);
   if (rc) return rc;
+  if (nimg == 1 && im_scales && im_scales[0] != 1.0)
+    return score_seq(db, d, false, im_scales[0], min_nr_spots, min_intensity, db->ctx->stream,
+                     outs);
   return score_images(db, nimg, d, im_scales, min_nr_spots, min_intensity, db->ctx->stream,
                       outs);
+}
+
+int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double min_intensity,
+                 lm_hits *out) {
+  if (!db || !img_f64 || !out) {
+    set_error("lm_score_f64: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  const uint64_t bytes = (uint64_t)db->geom.nrPxX * db->geom.nrPxY * 8;
+  if (db->img_cap < bytes) {
+    if (db->d_img) (void)hipFree(db->d_img);
+    db->d_img = nullptr;
+    db->img_cap = 0;
+    hipError_t e = hipMalloc(&db->d_img, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(image staging)");
+    db->img_cap = bytes;
+  }
+  hipError_t e = hipMemcpyAsync(db->d_img, img_f64, bytes, hipMemcpyHostToDevice,
+                                db->ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "image upload");
+  return score_seq(db, db->d_img, true, 1.0, min_nr_spots, min_intensity, db->ctx->stream, out);
 }
 
 int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,
@@ -412,6 +440,8 @@ int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const dou
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   hipStream_t st = stream ? (hipStream_t)stream : db->ctx->stream;
+  if (nimg == 1 && im_scales && im_scales[0] != 1.0)
+    return score_seq(db, d_imgs, false, im_scales[0], min_nr_spots, min_intensity, st, outs);
   return score_images(db, nimg, d_imgs, im_scales, min_nr_spots, min_intensity, st, outs);
 }
 

@@ -71,6 +71,8 @@ struct lm_db {
   uint64_t batch_cap = 0;
   uint8_t *d_img = nullptr;       // staging for host images
   uint64_t img_cap = 0;
+  double *d_vals = nullptr;       // float64 sums of in-order hits
+  uint64_t val_cap = 0;
   std::mutex mu;
 };
 
@@ -99,5 +101,9 @@ int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, 
 // lm_score.hip
 int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_scales,
                  int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *outs);
+// One image, float64 sums accumulated in table order on the device
+// (f64 = image of doubles; otherwise uint8 values times scale).
+int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_spots,
+              double min_intensity, hipStream_t stream, lm_hits *out);
 
 }  // namespace lm

@@ -176,6 +176,78 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
   }
 }
 
+// In-order float64 scoring, one lane per orientation: totInt accumulates the
+// pixel values > 0 in table order exactly like the reference loops
+// (src/LaueMatchingCPU.c:647-653 for the float64 image of the DoFwd=1
+// branch, :674-681 for raw*imScale of the uint8 image).  Hits with a
+// nonzero score go to hits[] (id << 26 | nspots) and vals[] (totInt).
+template <typename T>
+__global__ void __launch_bounds__(kTile)
+score_seq_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+                 const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
+                 const T *__restrict__ img, double scale, int min_spots, double min_int,
+                 uint64_t *__restrict__ hits, double *__restrict__ vals, uint64_t cap,
+                 uint32_t *__restrict__ counters) {
+  __shared__ uint32_t stage[kStageWords];
+  __shared__ uint32_t wsum[kTile / 64];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t o = t * kTile + tid;
+    const uint32_t c = o < n ? counts[o] : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    const uint64_t base = tileOff[t];
+    const uint64_t tot = tileOff[t + 1] - base;
+    __syncthreads();
+    uint32_t excl = incl - c;
+    for (int w = 0; w < wave; ++w) excl += wsum[w];
+    const bool staged = tot <= (uint64_t)kStageWords;
+    if (staged)
+      for (uint64_t i = tid; i < tot; i += kTile) stage[i] = entries[base + i];
+    __syncthreads();
+    double acc = 0;
+    int ns = 0;
+    for (uint32_t s = 0; s < c; ++s) {
+      const uint32_t pix = staged ? stage[excl + s] : entries[base + excl + s];
+      const T v = img[pix];
+      if (v > 0) {
+        if constexpr (sizeof(T) == 1)
+          acc += (double)v * scale;
+        else
+          acc += (double)v;
+        ++ns;
+      }
+    }
+    const bool live = o < n;
+    const bool pass = live && ns >= min_spots && acc >= min_int;
+    const uint64_t zm = __ballot(pass && ns == 0);
+    if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
+    const bool keep = pass && ns > 0;
+    const uint64_t m = __ballot(keep);
+    if (m) {
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      uint32_t b0 = 0;
+      if (lane == leader) b0 = atomicAdd(&counters[0], (uint32_t)__popcll(m));
+      b0 = __shfl(b0, leader);
+      if (keep) {
+        const uint64_t pos = (uint64_t)b0 + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (pos < cap) {
+          hits[pos] = (o << 26) | (uint64_t)ns;
+          vals[pos] = acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // [nimg][npx] -> [npx][B] (B >= nimg; missing images read as zero).
 template <int B>
 __global__ void interleave_kernel(const uint8_t *__restrict__ src, int nimg, uint64_t npx,
@@ -289,7 +361,109 @@ int batch_width(int nimg) {
   return B;
 }
 
+int ensure_vals(lm_db *db, uint64_t cap) {
+  if (db->val_cap < cap) {
+    if (db->d_vals) (void)hipFree(db->d_vals);
+    db->d_vals = nullptr;
+    db->val_cap = 0;
+    hipError_t e = hipMalloc(&db->d_vals, cap * 8);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit values)");
+    db->val_cap = cap;
+  }
+  return LM_OK;
+}
+
 }  // namespace
+
+int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_spots,
+              double min_intensity, hipStream_t st, lm_hits *out) {
+  const Table &t = db->table;
+  if (!t.ready) {
+    set_error("score: no table (run lm_db_forward or lm_db_import_table)");
+    return LM_ERR_STATE;
+  }
+  if (!d_img || !out) {
+    set_error("score: null argument");
+    return LM_ERR_INVALID;
+  }
+  memset(out, 0, sizeof *out);
+  uint64_t cap = std::max<uint64_t>(db->hit_cap, 1u << 16);
+  uint32_t cnt[2] = {0, 0};
+  float ms = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int rc = LM_OK;
+  const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    rc = ensure_scratch(db, 1, cap);
+    if (rc == LM_OK) rc = ensure_vals(db, cap);
+    if (rc) break;
+    hipError_t e = hipMemsetAsync(db->d_counters, 0, 2 * kMaxBatch * 4, st);
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(counters)"); break; }
+    (void)hipEventRecord(e0, st);
+    if (f64)
+      hipLaunchKernelGGL(score_seq_kernel<double>, dim3(grid), dim3(kTile), 0, st, t.counts,
+                         t.tile_off, t.entries, db->n, t.ntiles, (const double *)d_img, scale,
+                         min_nr_spots, min_intensity, db->d_hits, db->d_vals, cap,
+                         db->d_counters);
+    else
+      hipLaunchKernelGGL(score_seq_kernel<uint8_t>, dim3(grid), dim3(kTile), 0, st, t.counts,
+                         t.tile_off, t.entries, db->n, t.ntiles, (const uint8_t *)d_img, scale,
+                         min_nr_spots, min_intensity, db->d_hits, db->d_vals, cap,
+                         db->d_counters);
+    e = hipGetLastError();
+    if (e != hipSuccess) { rc = hip_fail(e, "score kernel launch"); break; }
+    (void)hipEventRecord(e1, st);
+    e = hipMemcpyAsync(cnt, db->d_counters, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { rc = hip_fail(e, "score kernel"); break; }
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (cnt[0] <= cap) break;
+    if (attempt == 1) { set_error("score: hit list overflow"); rc = LM_ERR_OVERFLOW; break; }
+    cap = cnt[0] + cnt[0] / 4 + 1024;
+  }
+  const uint64_t k = cnt[0];
+  std::vector<uint64_t> rec(k);
+  std::vector<double> val(k);
+  if (rc == LM_OK && k) {
+    hipError_t e = hipMemcpyAsync(rec.data(), db->d_hits, k * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(val.data(), db->d_vals, k * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_fail(e, "hit copy");
+  }
+  if (rc == LM_OK) {
+    std::vector<uint32_t> idx(k);
+    for (uint64_t i = 0; i < k; ++i) idx[i] = (uint32_t)i;
+    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return rec[a] < rec[b]; });
+    out->nr_results = (uint64_t)cnt[1] + k;
+    out->count = k;
+    out->kernel_ms = ms;
+    if (k) {
+      out->ids = (uint64_t *)malloc(k * 8);
+      out->nspots = (int32_t *)malloc(k * 4);
+      out->totint = (double *)malloc(k * 8);
+      out->score = (double *)malloc(k * 8);
+      if (!out->ids || !out->nspots || !out->totint || !out->score) {
+        lm_hits_free(out);
+        set_error("out of host memory for hits");
+        rc = LM_ERR_NOMEM;
+      } else {
+        for (uint64_t j = 0; j < k; ++j) {
+          const uint64_t r = rec[idx[j]];
+          const int ns = (int)(r & 0x3ffffffull);
+          out->ids[j] = r >> 26;
+          out->nspots[j] = ns;
+          out->totint[j] = val[idx[j]];
+          out->score[j] = val[idx[j]] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
 
 int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_scales,
                  int min_nr_spots, double min_intensity, hipStream_t st, lm_hits *outs) {

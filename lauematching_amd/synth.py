@@ -281,7 +281,9 @@ def quantize_to_255(img: np.ndarray) -> np.ndarray:
         return np.zeros_like(img, dtype=np.float64)
     v = img * (255.0 / mx)
     out = np.where(v >= 255.0, 255.0, np.where(v < 1.0, 1.0, np.floor(v)))
-    return np.where(img > 0, out, 0.0)
+    out = np.where(img > 0, out, 0.0)
+    out[img == mx] = 255.0  # the maximum maps to 255 exactly (imScale = 255/255)
+    return out
 
 
 def synth_image(p: Params, hkls: np.ndarray, grains: np.ndarray, seed: int,

@@ -139,6 +139,24 @@ def test_score_matches_oracle_nonunit_scale(scored_case, factor):
         np.testing.assert_array_equal(hits.score, mA)
 
 
+@pytest.mark.parametrize("min_sp,min_int", [(5, 300.0), (0, 0.0), (6, 4000.0)])
+def test_score_f64_matches_dofwd1_oracle(scored_case, min_sp, min_int):
+    """Float64 image (non-integer values): the DoFwd=1 scoring of the oracle's
+    forward pass (src/LaueMatchingCPU.c:647-653, 683-689)."""
+    p, hk, O, grains, img, table, db = scored_case
+    rng = np.random.default_rng(9)
+    im = img * rng.uniform(0.3, 3.0, img.shape)
+    im[rng.random(img.shape) < 0.05] = -0.5
+    _say(f"score f64 min_sp={min_sp} min_int={min_int}")
+    hits = db.score_f64(im, min_sp, min_int)
+    _, matched, nres = ob.forward(O, hk, ob.geom(p), 64, img=im.ravel(), min_spots=min_sp,
+                                  min_int=min_int)
+    rows, mA = ob.collect(matched)
+    assert hits.nr_results == nres
+    np.testing.assert_array_equal(hits.ids, rows)
+    np.testing.assert_array_equal(hits.score, mA)
+
+
 def test_import_matches_forward_and_roundtrips(ctx, scored_case):
     p, hk, O, grains, img, table, db = scored_case
     _say("import/export roundtrip")
