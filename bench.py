@@ -45,7 +45,7 @@ def cpu_baseline(db, u8, im_scale, min_sp, min_int, sample, gpu_hits, budget_s=1
     table = db.export_table(0, sample)
     cores = len(os.sched_getaffinity(0))
     passes, t_cpu = 0, 0.0
-    while t_cpu < budget_s and passes < 50:
+    while t_cpu < budget_s and passes < 2000:
         t0 = time.perf_counter()
         matched, ns, tot, nres = ob.compare(table, db.max_spots, u8, db.geom.nrPxX, im_scale,
                                             min_sp, min_int, threads=cores)
@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tile-bits", type=int, default=14,
+                    help="super-tile size (log2) of the pixel-banded table; 0 = none")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -97,6 +99,7 @@ def main():
     t0 = time.time()
     db = api.OrientationDB(ctx, O, hk, api.Geometry.from_params(p), p.MaxNrLaueSpots)
     log(f"DB upload {time.time() - t0:.1f} s")
+    db.set_banded_tile_bits(args.tile_bits)
     st = db.forward()
     log(f"forward simulation: {st.n_spots} spots ({st.n_spots / n:.2f}/orientation), "
         f"{st.ms:.1f} ms device time")
@@ -165,7 +168,8 @@ def main():
                    "min_intensity": p.MinIntensity, "spots_per_orientation": st.n_spots / n,
                    "hits": int(len(h.ids)), "nr_results": int(h.nr_results),
                    "grains_recovered": bool(found), "prefix_parity_vs_oracle": parity_ok,
-                   "forward_ms": st.ms, "kernel_ms": kernel_ms},
+                   "forward_ms": st.ms, "kernel_ms": kernel_ms,
+                   "banded_tile_bits": args.tile_bits},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
         "cpu_baseline": cpu,

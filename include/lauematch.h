@@ -123,6 +123,14 @@ uint64_t lm_db_table_bytes(const lm_db *db);
 /* Spots in the table (sum of per-orientation counts). */
 uint64_t lm_db_table_spots(const lm_db *db);
 
+/* Layout options (no reference counterpart; used by the autotuner).
+ * LM_OPT_BANDED_TILE_BITS: orientations per super-tile of the pixel-banded
+ * table copy used by single-image scoring, as log2 (13 or 14); 0 drops the
+ * copy (scoring then reads the orientation-major table only).  Applies at
+ * once if a table exists. */
+#define LM_OPT_BANDED_TILE_BITS 1
+int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
+
 /* Build the resident spot table on the device (replaces the DoFwd=1 forward
  * simulation).  mode = LM_FWD_FP64 or LM_FWD_FP32_GUARD; stats may be NULL. */
 int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats);

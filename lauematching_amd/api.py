@@ -146,6 +146,10 @@ class OrientationDB:
     def table_spots(self) -> int:
         return int(lib().lm_db_table_spots(self._h))
 
+    def set_banded_tile_bits(self, bits: int) -> None:
+        """Super-tile size (log2) of the pixel-banded copy; 0 drops it."""
+        check(lib().lm_db_set_option(self._h, 1, int(bits)))
+
     def forward(self, mode: int = LM_FWD_FP64) -> LmFwdStats:
         st = LmFwdStats()
         check(lib().lm_db_forward(self._h, int(mode), ctypes.byref(st)))

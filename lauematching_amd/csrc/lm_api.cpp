@@ -349,6 +349,27 @@ int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats) {
   return forward_build(db, mode, stats);
 }
 
+int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
+  if (!db) {
+    set_error("null db");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  if (option == LM_OPT_BANDED_TILE_BITS) {
+    if (value != 0 && value != 13 && value != 14) {
+      set_error("LM_OPT_BANDED_TILE_BITS must be 0, 13 or 14");
+      return LM_ERR_INVALID;
+    }
+    db->banded_tb = (int)value;
+    if (db->table.ready) return build_banded(db);
+    return LM_OK;
+  }
+  set_error("unknown option");
+  return LM_ERR_INVALID;
+}
+
 int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count) {
   if (!db) {
     set_error("null db");

@@ -1,0 +1,274 @@
+// lm_band.hip - pixel-banded copy of the spot table and the single-image
+// compare kernel that reads it.
+//
+// Why: the compare step is one byte gather per spot.  On MI355X, 64 lanes
+// gathering from random addresses of a 4 MiB image sustain about 260 G
+// gathers/s, while 64 lanes confined to a 2 KiB window sustain about 1000 G/s
+// (the L1/TA unit serves each distinct cache line of a wave instruction
+// separately).  Reordering each super-tile's spots by pixel bucket makes
+// consecutive lanes read neighbouring pixels, so a wave's gathers fall in a few
+// cache lines; the per-orientation sums are then accumulated in LDS counters
+// (one ds_add per nonzero spot) instead of per-lane registers.
+//
+// Exactness: the accumulator (nspots << 17 | sum of raw) is an integer, so
+// the accumulation order does not matter; with imScale == 1 the reference
+// totInt is that sum (src/LaueMatchingCPU.c:674-681).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "lm_internal.h"
+
+namespace lm {
+namespace {
+
+constexpr int kBuildThreads = 1024;
+constexpr int kMaxBuckets = 16384;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// One workgroup per super-tile: counting sort of the super-tile's entries by
+// pixel bucket (two traversals of the orientation-major table: histogram,
+// then scatter).
+__global__ void __launch_bounds__(kBuildThreads)
+band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+                  const uint32_t *__restrict__ entries, uint64_t n, int tb, int pb, int bb,
+                  int nbuckets, int nbands, uint32_t *__restrict__ bentries,
+                  uint32_t *__restrict__ bandOff) {
+  __shared__ uint32_t hist[kMaxBuckets];
+  __shared__ uint32_t wsum[kBuildThreads / 64];
+  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint64_t t = blockIdx.x;
+  const uint64_t first = t << tb;
+  const uint64_t rows = std::min<uint64_t>(1ull << tb, n - first);
+  const uint64_t base = tileOff[first >> kTileBits];
+  const uint32_t pmask = (pb >= 32) ? 0xffffffffu : ((1u << pb) - 1u);
+  for (int i = tid; i < nbuckets; i += kBuildThreads) hist[i] = 0;
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    uint32_t run = 0;
+    for (uint64_t k = 0; k < rows; k += kBuildThreads) {
+      const uint64_t r = k + tid;
+      const uint32_t c = r < rows ? counts[first + r] : 0u;
+      const uint32_t incl = wave_incl_scan(c, lane);
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t before = 0, tot = 0;
+      for (int w = 0; w < kBuildThreads / 64; ++w) {
+        const uint32_t v = wsum[w];
+        if (w < wave) before += v;
+        tot += v;
+      }
+      const uint32_t excl = run + before + incl - c;
+      for (uint32_t s = 0; s < c; ++s) {
+        const uint32_t pix = entries[base + excl + s];
+        const uint32_t b = pix >> bb;
+        if (pass == 0) {
+          atomicAdd(&hist[b], 1u);
+        } else {
+          const uint32_t pos = atomicAdd(&hist[b], 1u);
+          bentries[base + pos] = ((uint32_t)r << pb) | (pix & pmask);
+        }
+      }
+      run += tot;
+      __syncthreads();
+    }
+    if (pass == 0) {
+      // exclusive scan of hist (contiguous chunk per thread)
+      const int per = (nbuckets + kBuildThreads - 1) / kBuildThreads;
+      const int lo = std::min(nbuckets, tid * per), hi = std::min(nbuckets, lo + per);
+      uint32_t s = 0;
+      for (int i = lo; i < hi; ++i) s += hist[i];
+      const uint32_t incl = wave_incl_scan(s, lane);
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t acc = incl - s;
+      for (int w = 0; w < wave; ++w) acc += wsum[w];
+      __syncthreads();
+      for (int i = lo; i < hi; ++i) {
+        const uint32_t v = hist[i];
+        hist[i] = acc;
+        acc += v;
+      }
+      __syncthreads();
+      const int shift = pb - bb;
+      uint32_t *bo = bandOff + t * (uint64_t)(nbands + 1);
+      for (int j = tid; j <= nbands; j += kBuildThreads) {
+        const int64_t bk = (int64_t)j << shift;
+        bo[j] = (j == nbands || bk >= nbuckets) ? run : hist[bk];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Single-image compare over the banded copy; one workgroup per super-tile
+// (grid-stride).  cnt[r] = (nspots << 17) | sum of raw for row r.
+template <int TB, int THREADS>
+__global__ void __launch_bounds__(THREADS)
+score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
+                  const uint32_t *__restrict__ bandOff, int nbands, uint64_t n, uint64_t nsuper,
+                  const uint8_t *__restrict__ img, int min_spots, double min_int,
+                  uint64_t *__restrict__ hits, uint64_t cap, uint32_t *__restrict__ counters) {
+  constexpr int T = 1 << TB;
+  constexpr int PB = 32 - TB;
+  constexpr uint32_t PMASK = (1u << PB) - 1u;
+  __shared__ uint32_t cnt[T];
+  const int tid = threadIdx.x, lane = __lane_id();
+  for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
+    for (int i = tid; i < T; i += THREADS) cnt[i] = 0;
+    __syncthreads();
+    const uint64_t first = t << TB;
+    const uint32_t *src = bentries + tileOff[first >> kTileBits];
+    const uint32_t *bo = bandOff + t * (uint64_t)(nbands + 1);
+    for (int j = 0; j < nbands; ++j) {
+      const uint32_t b0 = bo[j], b1 = bo[j + 1];
+      const uint8_t *bimg = img + ((uint64_t)j << PB);
+      uint32_t i = b0 + tid;
+      for (; i + 3 * THREADS < b1; i += 4 * THREADS) {
+        const uint32_t e0 = __builtin_nontemporal_load(src + i);
+        const uint32_t e1 = __builtin_nontemporal_load(src + i + THREADS);
+        const uint32_t e2 = __builtin_nontemporal_load(src + i + 2 * THREADS);
+        const uint32_t e3 = __builtin_nontemporal_load(src + i + 3 * THREADS);
+        const uint32_t r0 = bimg[e0 & PMASK], r1 = bimg[e1 & PMASK];
+        const uint32_t r2 = bimg[e2 & PMASK], r3 = bimg[e3 & PMASK];
+        if (r0) atomicAdd(&cnt[e0 >> PB], (1u << kSumBits) + r0);
+        if (r1) atomicAdd(&cnt[e1 >> PB], (1u << kSumBits) + r1);
+        if (r2) atomicAdd(&cnt[e2 >> PB], (1u << kSumBits) + r2);
+        if (r3) atomicAdd(&cnt[e3 >> PB], (1u << kSumBits) + r3);
+      }
+      for (; i < b1; i += THREADS) {
+        const uint32_t e = __builtin_nontemporal_load(src + i);
+        const uint32_t r = bimg[e & PMASK];
+        if (r) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + r);
+      }
+    }
+    __syncthreads();
+    for (int rr = tid; rr < T; rr += THREADS) {
+      const uint64_t o = first + rr;
+      const uint32_t acc = cnt[rr];
+      const uint32_t ns = acc >> kSumBits, sum = acc & kSumMask;
+      const bool live = o < n;
+      const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= min_int);
+      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int;
+      const uint64_t zm = __ballot(zero_pass);
+      if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
+      const uint64_t m = __ballot(cand);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t b0 = 0;
+        if (lane == leader) b0 = atomicAdd(&counters[0], (uint32_t)__popcll(m));
+        b0 = __shfl(b0, leader);
+        if (cand) {
+          const uint64_t pos = (uint64_t)b0 + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+          if (pos < cap) hits[pos] = (o << 26) | (uint64_t)acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int reserve_bytes(lm_db *db, uint64_t bytes) {
+  std::lock_guard<std::mutex> lk(db->ctx->mu);
+  if (db->ctx->in_use + bytes > db->ctx->budget) {
+    set_error("HBM budget exceeded by the banded table copy");
+    return LM_ERR_BUDGET;
+  }
+  db->ctx->in_use += bytes;
+  return LM_OK;
+}
+
+}  // namespace
+
+void release_banded(lm_db *db) {
+  Banded &b = db->banded;
+  if (b.entries) (void)hipFree(b.entries);
+  if (b.band_off) (void)hipFree(b.band_off);
+  if (b.bytes) {
+    std::lock_guard<std::mutex> lk(db->ctx->mu);
+    db->ctx->in_use -= b.bytes;
+  }
+  b = Banded{};
+}
+
+int build_banded(lm_db *db) {
+  release_banded(db);
+  const int tb = db->banded_tb;
+  if (tb == 0) return LM_OK;
+  if (tb != 13 && tb != 14) {
+    set_error("banded super-tile bits must be 13 or 14");
+    return LM_ERR_INVALID;
+  }
+  const Table &t = db->table;
+  Banded &b = db->banded;
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  b.tb = tb;
+  b.pb = 32 - tb;
+  b.nbands = (int)((npx + (1ull << b.pb) - 1) >> b.pb);
+  int bb = kBucketBits;
+  while (((npx + (1ull << bb) - 1) >> bb) > (uint64_t)kMaxBuckets) ++bb;
+  if (bb > b.pb) {
+    set_error("detector too large for the banded table");
+    return LM_ERR_INVALID;
+  }
+  const int nbuckets = (int)((npx + (1ull << bb) - 1) >> bb);
+  b.nsuper = (db->n + (1ull << tb) - 1) >> tb;
+  const uint64_t ebytes = (t.total + 64) * 4;
+  const uint64_t obytes = b.nsuper * (uint64_t)(b.nbands + 1) * 4;
+  int rc = reserve_bytes(db, ebytes + obytes);
+  if (rc) {
+    b = Banded{};
+    return rc;
+  }
+  b.bytes = ebytes + obytes;
+  hipError_t e = hipMalloc(&b.entries, ebytes);
+  if (e == hipSuccess) e = hipMalloc(&b.band_off, obytes);
+  hipStream_t st = db->ctx->stream;
+  if (e == hipSuccess) e = hipMemsetAsync(b.entries + t.total, 0, 64 * 4, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(band_build_kernel, dim3((unsigned)b.nsuper), dim3(kBuildThreads), 0, st,
+                       t.counts, t.tile_off, t.entries, db->n, tb, b.pb, bb, nbuckets, b.nbands,
+                       b.entries, b.band_off);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    release_banded(db);
+    return hip_fail(e, "build_banded");
+  }
+  b.ready = true;
+  return LM_OK;
+}
+
+int launch_banded(lm_db *db, const uint8_t *img, int min_spots, double min_int, uint64_t cap,
+                  hipStream_t st) {
+  const Banded &b = db->banded;
+  const Table &t = db->table;
+  hipError_t e;
+  if (b.tb == 14) {
+    const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+    hipLaunchKernelGGL((score_band_kernel<14, 1024>), dim3(grid), dim3(1024), 0, st, b.entries,
+                       t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, min_spots, min_int,
+                       db->d_hits, cap, db->d_counters);
+  } else {
+    const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+    hipLaunchKernelGGL((score_band_kernel<13, 512>), dim3(grid), dim3(512), 0, st, b.entries,
+                       t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, min_spots, min_int,
+                       db->d_hits, cap, db->d_counters);
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
+  return LM_OK;
+}
+
+}  // namespace lm

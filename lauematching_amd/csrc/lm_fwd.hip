@@ -269,6 +269,7 @@ DevGeom dev_geom(const Geom &g) {
 }
 
 void free_table(lm_db *db) {
+  release_banded(db);
   Table &t = db->table;
   if (t.entries) (void)hipFree(t.entries);
   if (t.tile_off) (void)hipFree(t.tile_off);
@@ -434,6 +435,11 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
     return rc;
   }
   t.ready = true;
+  rc = build_banded(db);
+  if (rc) {
+    free_table(db);
+    return rc;
+  }
   if (stats) {
     stats->n_orients = db->n;
     stats->n_spots = t.total;
@@ -500,6 +506,11 @@ int import_table(lm_db *db, const uint16_t *records, uint64_t count) {
     return rc;
   }
   t.ready = true;
+  rc = build_banded(db);
+  if (rc) {
+    free_table(db);
+    return rc;
+  }
   return LM_OK;
 }
 

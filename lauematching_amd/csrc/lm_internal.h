@@ -23,7 +23,7 @@ namespace lm {
 // of the first entry of tile t (tile_off[ntiles] = total).  counts[o] is the
 // per-orientation spot count (the ForwardFile's word 0).
 struct Table {
-  uint32_t *entries = nullptr;  // total + 4 words of slack for 16-byte loads
+  uint32_t *entries = nullptr;  // total + 64 words of slack for 16-byte loads
   uint64_t *tile_off = nullptr; // ntiles + 1
   uint16_t *counts = nullptr;   // n
   uint64_t total = 0;
@@ -33,6 +33,28 @@ struct Table {
   int pix_bits = 0;
   bool ready = false;
 };
+
+// Pixel-banded copy of the table used by the single-image compare kernel.
+//
+// Orientations are grouped in super-tiles of 2^tb rows.  A super-tile's spots
+// (the same spots as its rows in Table, same total) are reordered by pixel
+// bucket (2^kBucketBits consecutive pixels per bucket), so consecutive entries
+// hit neighbouring pixels, and encoded as
+//     entry = (row_in_super_tile << pb) | (pixel & (2^pb - 1)),  pb = 32 - tb
+// band j of a super-tile holds the pixels [j << pb, (j+1) << pb);
+// band_off[t * (nbands+1) + j] is the offset of band j from the super-tile's
+// first entry, which is Table::tile_off at the super-tile's first 256-row tile.
+struct Banded {
+  uint32_t *entries = nullptr;   // same length as Table::entries
+  uint32_t *band_off = nullptr;  // nsuper * (nbands + 1)
+  uint64_t nsuper = 0;
+  int tb = 0;
+  int pb = 0;
+  int nbands = 0;
+  uint64_t bytes = 0;
+  bool ready = false;
+};
+constexpr int kBucketBits = 10;
 
 struct Geom {
   double recip[9];
@@ -62,6 +84,8 @@ struct lm_db {
   double *d_orients = nullptr;  // n * 9
   int32_t *d_hkls = nullptr;    // nhkls * 3
   lm::Table table;
+  lm::Banded banded;
+  int banded_tb = 14;             // super-tile bits for the banded copy (0 = no copy)
   // scratch for scoring
   uint64_t *d_hits = nullptr;     // packed hit records
   uint64_t hit_cap = 0;
@@ -101,6 +125,14 @@ int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, 
 // lm_score.hip
 int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_scales,
                  int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *outs);
+// Build / drop the pixel-banded copy (after forward or import).
+int build_banded(lm_db *db);
+void release_banded(lm_db *db);
+int launch_banded(lm_db *db, const uint8_t *img, int min_spots, double min_int, uint64_t cap,
+                  hipStream_t st);
+// One image against the banded copy; imScale must be 1 (integer sums).
+int score_banded(lm_db *db, const uint8_t *d_img, int min_nr_spots, double min_intensity,
+                 hipStream_t stream, lm_hits *out);
 // One image, float64 sums accumulated in table order on the device
 // (f64 = image of doubles; otherwise uint8 values times scale).
 int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_spots,

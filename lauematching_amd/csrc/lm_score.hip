@@ -528,7 +528,10 @@ int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_sc
     hipError_t e = hipMemsetAsync(db->d_counters, 0, 2 * kMaxBatch * 4, st);
     if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(counters)"); break; }
     (void)hipEventRecord(e0, st);
-    rc = launch_score(db, B, img, pred, capPer, st);
+    if (B == 1 && db->banded.ready && bit(pred, 0))
+      rc = launch_banded(db, img, min_nr_spots, min_intensity, capPer, st);
+    else
+      rc = launch_score(db, B, img, pred, capPer, st);
     if (rc) break;
     (void)hipEventRecord(e1, st);
     e = hipMemcpyAsync(cnt.data(), db->d_counters, 2 * kMaxBatch * 4, hipMemcpyDeviceToHost, st);

@@ -123,6 +123,52 @@ def test_score_matches_oracle_unit_scale(scored_case, min_sp, min_int):
             assert g in set(hits.ids.tolist())
 
 
+@pytest.mark.parametrize("bits", [0, 13, 14])
+def test_score_layouts_agree(scored_case, bits):
+    """Banded copy with 8K or 16K super-tiles, and no banded copy (the
+    orientation-major kernel): identical results to the oracle."""
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img)
+    _say(f"score banded bits={bits}")
+    db.set_banded_tile_bits(bits)
+    try:
+        for min_sp, min_int in [(5, 250.0), (0, 0.0)]:
+            hits = db.score(u8, s, min_sp, min_int)
+            matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, min_sp, min_int)
+            rows, mA = ob.collect(matched)
+            assert hits.nr_results == nres
+            np.testing.assert_array_equal(hits.ids, rows)
+            np.testing.assert_array_equal(hits.score, mA)
+            np.testing.assert_array_equal(hits.nspots, ns[rows.astype(np.int64)])
+    finally:
+        db.set_banded_tile_bits(14)
+
+
+@pytest.mark.parametrize("geo", ["small_rect", "det1024"])
+def test_score_other_detectors(ctx, geo):
+    over = dict(GEOMS.get(geo, {}))
+    if geo == "det1024":
+        over = dict(NrPxX=1024, NrPxY=1024, PxX=0.0004, PxY=0.0004)
+    p = synth.baseline_params(MaxNrLaueSpots=64, **over)
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.random_orientations(40000, seed=31)
+    img = synth.synth_image(p, hk, O[[3, 33333]], seed=8)
+    _say(f"score detector {geo}")
+    db = _db(ctx, p, O, hk)
+    db.forward()
+    table = db.export_table()
+    u8, s = api.quantize_u8(img)
+    assert s == 1.0
+    for bits in (14, 13, 0):
+        db.set_banded_tile_bits(bits)
+        hits = db.score(u8, s, 4, 150.0)
+        matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, 4, 150.0)
+        rows, mA = ob.collect(matched)
+        assert hits.nr_results == nres
+        np.testing.assert_array_equal(hits.ids, rows)
+        np.testing.assert_array_equal(hits.score, mA)
+
+
 @pytest.mark.parametrize("factor", [0.37, 3.3])
 def test_score_matches_oracle_nonunit_scale(scored_case, factor):
     p, hk, O, grains, img, table, db = scored_case
