@@ -179,6 +179,28 @@ int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const dou
 
 void lm_hits_free(lm_hits *h);
 
+/* ---- merge of duplicate candidates (CPU) -------------------------------------------- */
+
+/* Cluster the nr candidates (orientation rows + coarse scores, in hit-list
+ * order) by misorientation <= max_angle degrees under the point group of
+ * space_group, greedily in list order; each cluster reports its best-scored
+ * row and score and its size (mergeDuplicateOrientations,
+ * src/LaueMatchingHeaders.h:969-1065, including the cap at the 50000
+ * best-scored candidates).  Output arrays hold min(nr, 50000) entries;
+ * *n_clusters receives the cluster count.  threads <= 1 runs serially. */
+int lm_merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows,
+                        const double *scores, int32_t nr, double max_angle, int32_t space_group,
+                        int32_t threads, uint64_t *best_row, double *best_score,
+                        int32_t *cluster_size, int32_t *n_clusters);
+
+/* Unit quaternion (w >= 0) of a row-major rotation matrix
+ * (OrientMat2Quat, src/LaueMatchingHeaders.h:262-296). */
+void lm_quat_from_matrix(const double m[9], double q[4]);
+
+/* Misorientation angle (degrees) between two unit quaternions under the
+ * point group of space_group (GetMisOrientation, src/LaueMatchingHeaders.h:247-260). */
+double lm_misorientation_deg(const double q1[4], const double q2[4], int32_t space_group);
+
 /* ---- diagnostics ------------------------------------------------------------------ */
 
 /* Evaluate on the device, with the forward simulation's arithmetic, q[i] =

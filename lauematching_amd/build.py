@@ -29,11 +29,11 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
           "-I", INCLUDE, "-I", CSRC]
 
-LIB_SOURCES = ["lm_api.cpp", "lm_fwd.hip", "lm_score.hip", "lm_band.hip", "lm_serve.cpp"]
-HOST_PROGRAMS = {"LaueMatchingGPU": ["host/lm_host_gpu.cpp", "host/lm_params.cpp",
-                                     "host/lm_postproc.cpp"],
-                 "LaueMatchingGPUStream": ["host/lm_host_stream.cpp", "host/lm_params.cpp",
-                                           "host/lm_postproc.cpp"]}
+LIB_SOURCES = ["lm_api.cpp", "lm_fwd.hip", "lm_score.hip", "lm_band.hip", "lm_merge.cpp",
+               "lm_serve.cpp"]
+HOST_COMMON = ["host/lm_params.cpp", "host/lm_hostio.cpp", "host/lm_postproc.cpp"]
+HOST_PROGRAMS = {"LaueMatchingGPU": ["host/lm_host_gpu.cpp", *HOST_COMMON],
+                 "LaueMatchingGPUStream": ["host/lm_host_stream.cpp", *HOST_COMMON]}
 
 
 def _run(cmd):

@@ -182,6 +182,7 @@ int main(int argc, char **argv) {
   }
   PostInput in;
   in.orients = (const double *)orient.data;
+  in.n_orients = n;
   in.hits = &hits;
   in.nr_results = hits.nr_results;
   in.params = &p;

@@ -14,6 +14,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
+#include <algorithm>
 
 #include "lm_internal.h"
 
@@ -475,6 +477,30 @@ int lm_selftest_fp64(lm_ctx *ctx, const double *a, const double *b, uint64_t n, 
     return LM_ERR_INVALID;
   }
   return selftest_fp64(ctx->stream, a, b, n, q, s);
+}
+
+int lm_merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows,
+                        const double *scores, int32_t nr, double max_angle, int32_t space_group,
+                        int32_t threads, uint64_t *best_row, double *best_score,
+                        int32_t *cluster_size, int32_t *n_clusters) {
+  if (nr < 0 || (nr && (!orients || !rows || !scores || !best_row || !best_score ||
+                        !cluster_size)) || !n_clusters) {
+    set_error("lm_merge_candidates: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  std::vector<int> sizes((size_t)std::min(nr, 50000));
+  const int k = merge_candidates(orients, n_orients, rows, scores, nr, max_angle, space_group,
+                                 threads, best_row, best_score, sizes.data());
+  if (k < 0) return LM_ERR_INVALID;
+  for (int i = 0; i < k; ++i) cluster_size[i] = sizes[i];
+  *n_clusters = k;
+  return LM_OK;
+}
+
+void lm_quat_from_matrix(const double m[9], double q[4]) { quat_from_matrix(m, q); }
+
+double lm_misorientation_deg(const double q1[4], const double q2[4], int32_t space_group) {
+  return misorientation_deg(q1, q2, space_group);
 }
 
 void lm_hits_free(lm_hits *h) {

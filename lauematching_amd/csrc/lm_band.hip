@@ -113,7 +113,15 @@ band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restric
 
 // Single-image compare over the banded copy; one workgroup per super-tile
 // (grid-stride).  cnt[r] = (nspots << 17) | sum of raw for row r.
-template <int TB, int THREADS>
+//
+// The super-tile's entries are walked as one flat range; lane l of group g
+// takes entries g*THREADS*U + k*THREADS + l (k < U), so the 64 lanes of a wave
+// read consecutive, pixel-sorted entries.  Each lane tracks the band of its
+// entries incrementally (entries are sorted by pixel, so the band never
+// decreases).  The next group's entry loads are issued after the current
+// group's gathers, so they stay in flight across the gather wait and the LDS
+// adds.
+template <int TB, int THREADS, int U>
 __global__ void __launch_bounds__(THREADS)
 score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                   const uint32_t *__restrict__ bandOff, int nbands, uint64_t n, uint64_t nsuper,
@@ -122,6 +130,7 @@ score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restr
   constexpr int T = 1 << TB;
   constexpr int PB = 32 - TB;
   constexpr uint32_t PMASK = (1u << PB) - 1u;
+  constexpr uint32_t STEP = THREADS * U;
   __shared__ uint32_t cnt[T];
   const int tid = threadIdx.x, lane = __lane_id();
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
@@ -130,27 +139,39 @@ score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restr
     const uint64_t first = t << TB;
     const uint32_t *src = bentries + tileOff[first >> kTileBits];
     const uint32_t *bo = bandOff + t * (uint64_t)(nbands + 1);
-    for (int j = 0; j < nbands; ++j) {
-      const uint32_t b0 = bo[j], b1 = bo[j + 1];
-      const uint8_t *bimg = img + ((uint64_t)j << PB);
-      uint32_t i = b0 + tid;
-      for (; i + 3 * THREADS < b1; i += 4 * THREADS) {
-        const uint32_t e0 = __builtin_nontemporal_load(src + i);
-        const uint32_t e1 = __builtin_nontemporal_load(src + i + THREADS);
-        const uint32_t e2 = __builtin_nontemporal_load(src + i + 2 * THREADS);
-        const uint32_t e3 = __builtin_nontemporal_load(src + i + 3 * THREADS);
-        const uint32_t r0 = bimg[e0 & PMASK], r1 = bimg[e1 & PMASK];
-        const uint32_t r2 = bimg[e2 & PMASK], r3 = bimg[e3 & PMASK];
-        if (r0) atomicAdd(&cnt[e0 >> PB], (1u << kSumBits) + r0);
-        if (r1) atomicAdd(&cnt[e1 >> PB], (1u << kSumBits) + r1);
-        if (r2) atomicAdd(&cnt[e2 >> PB], (1u << kSumBits) + r2);
-        if (r3) atomicAdd(&cnt[e3 >> PB], (1u << kSumBits) + r3);
+    const uint32_t total = bo[nbands];
+    int band = 0;
+    uint32_t bend = bo[1];
+    uint32_t e[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      e[k] = i < total ? __builtin_nontemporal_load(src + i) : 0u;
+    }
+    for (uint32_t g0 = 0; g0 < total; g0 += STEP) {
+      uint32_t raw[U];
+      uint32_t row[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = g0 + k * THREADS + tid;
+        raw[k] = 0;
+        row[k] = e[k] >> PB;
+        if (i < total) {
+          while (i >= bend) {
+            ++band;
+            bend = bo[band + 1];
+          }
+          raw[k] = img[((uint64_t)band << PB) | (e[k] & PMASK)];
+        }
       }
-      for (; i < b1; i += THREADS) {
-        const uint32_t e = __builtin_nontemporal_load(src + i);
-        const uint32_t r = bimg[e & PMASK];
-        if (r) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + r);
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = g0 + STEP + k * THREADS + tid;
+        e[k] = i < total ? __builtin_nontemporal_load(src + i) : 0u;
       }
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (raw[k]) atomicAdd(&cnt[row[k]], (1u << kSumBits) + raw[k]);
     }
     __syncthreads();
     for (int rr = tid; rr < T; rr += THREADS) {
@@ -257,12 +278,12 @@ int launch_banded(lm_db *db, const uint8_t *img, int min_spots, double min_int, 
   hipError_t e;
   if (b.tb == 14) {
     const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
-    hipLaunchKernelGGL((score_band_kernel<14, 1024>), dim3(grid), dim3(1024), 0, st, b.entries,
+    hipLaunchKernelGGL((score_band_kernel<14, 1024, 8>), dim3(grid), dim3(1024), 0, st, b.entries,
                        t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, min_spots, min_int,
                        db->d_hits, cap, db->d_counters);
   } else {
     const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
-    hipLaunchKernelGGL((score_band_kernel<13, 512>), dim3(grid), dim3(512), 0, st, b.entries,
+    hipLaunchKernelGGL((score_band_kernel<13, 512, 8>), dim3(grid), dim3(512), 0, st, b.entries,
                        t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, min_spots, min_int,
                        db->d_hits, cap, db->d_counters);
   }

@@ -133,6 +133,12 @@ int launch_banded(lm_db *db, const uint8_t *img, int min_spots, double min_int, 
 // One image against the banded copy; imScale must be 1 (integer sums).
 int score_banded(lm_db *db, const uint8_t *d_img, int min_nr_spots, double min_intensity,
                  hipStream_t stream, lm_hits *out);
+// lm_merge.cpp
+void quat_from_matrix(const double m[9], double q[4]);
+double misorientation_deg(const double a[4], const double b[4], int space_group);
+int merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows,
+                     const double *scores, int nr, double max_angle, int space_group,
+                     int threads, uint64_t *best_row, double *best_score, int *cluster_size);
 // One image, float64 sums accumulated in table order on the device
 // (f64 = image of doubles; otherwise uint8 values times scale).
 int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_spots,
