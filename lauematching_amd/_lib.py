@@ -86,6 +86,14 @@ SIGNATURES = {
                                           c_double_p, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.POINTER(LmHits)]),
     "lm_hits_free": (None, [ctypes.POINTER(LmHits)]),
+    "lm_merge_candidates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                           ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_int32)]),
+    "lm_quat_from_matrix": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "lm_misorientation_deg": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_int32]),
     "lm_selftest_fp64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
 }
