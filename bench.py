@@ -62,6 +62,72 @@ def cpu_baseline(db, u8, im_scale, min_sp, min_int, sample, gpu_hits, budget_s=1
                       f"{t_cpu:.1f} s"}, ok
 
 
+def _batch_image(args):
+    """One synthetic batch image: seeded grain count U{1..20}, grains drawn
+    from the DB rows (regenerated from the DB seed)."""
+    seed, n, db_seed, i = args
+    from lauematching_amd import synth
+
+    rng = np.random.default_rng([seed, i])
+    k = int(rng.integers(1, 21))
+    rows = np.sort(rng.choice(n, k, replace=False))
+    p = synth.baseline_params()
+    img = synth.synth_image(p, synth.fcc_hkls(8, "box"), synth.orientation_rows(db_seed, rows),
+                            seed=int(rng.integers(1 << 30)))
+    return img, rows
+
+
+def make_batch_images(count, n, db_seed, seed=1234, workers=16):
+    """Generate batch images in worker processes (before any GPU use)."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    jobs = [(seed, n, db_seed, i) for i in range(count)]
+    with ProcessPoolExecutor(min(workers, count)) as ex:
+        return list(ex.map(_batch_image, jobs))
+
+
+def run_batches(db, batch_imgs, widths, p, steps, warmup):
+    """Scores batches of B images in one pass over the table; returns per-B
+    rates and checks each batched image against scoring it alone."""
+    import torch
+    from lauematching_amd import api
+
+    u8s, scales = [], []
+    for img, _ in batch_imgs:
+        u, s = api.quantize_u8(img)
+        u8s.append(u)
+        scales.append(s)
+    out = []
+    for B in widths:
+        dev = torch.from_numpy(np.stack(u8s[:B])).to("cuda:0")
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            hs = db.score_batch_dev(dev.data_ptr(), B, scales[:B], p.MinNrSpots, p.MinIntensity)
+        torch.cuda.synchronize()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            hs = db.score_batch_dev(dev.data_ptr(), B, scales[:B], p.MinNrSpots, p.MinIntensity)
+            kms.append(hs[0].kernel_ms)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        ok = True
+        for i in sorted({0, B // 2, B - 1}):
+            one = db.score_dev(dev[i].data_ptr(), scales[i], p.MinNrSpots, p.MinIntensity)
+            ok &= (np.array_equal(one.ids, hs[i].ids) and np.array_equal(one.score, hs[i].score)
+                   and one.nr_results == hs[i].nr_results)
+        found = sum(set(rows.tolist()) <= set(hs[i].ids.tolist())
+                    for i, (_, rows) in enumerate(batch_imgs[:B]))
+        out.append({"images": B, "ms_per_batch": ms, "kernel_ms": float(np.mean(kms)),
+                    "pairs_per_s": db.n * B / (ms / 1e3), "bit_exact_vs_single": bool(ok),
+                    "images_with_all_grains_recovered": int(found),
+                    "hits_per_image": [int(h.count) for h in hs]})
+        log(f"batch B={B}: {ms:.2f} ms/batch, {out[-1]['pairs_per_s']:.3e} pairs/s, "
+            f"exact={ok}, grains recovered in {found}/{B}")
+        del dev
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,11 +142,19 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tile-bits", type=int, default=14,
                     help="super-tile size (log2) of the pixel-banded table; 0 = none")
+    ap.add_argument("--batch", type=str, default="",
+                    help="comma-separated batch widths to measure as well, e.g. 8,16,32,64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by orientation range across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    widths = [int(b) for b in args.batch.split(",") if b]
+    batch_imgs = []
+    if widths:
+        t0 = time.time()
+        batch_imgs = make_batch_images(max(widths), args.orients, args.seed)
+        log(f"generated {len(batch_imgs)} batch images in {time.time() - t0:.1f} s")
 
     import torch
     from lauematching_amd import api, synth
@@ -141,6 +215,8 @@ def main():
         except Exception:
             traffic = None
 
+    batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1) if widths else None
+
     cpu, parity_ok = None, None
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
@@ -174,6 +250,10 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if batched:
+        for b in batched:
+            b["speedup_vs_plain"] = b["pairs_per_s"] / value
+        line["batched"] = batched
     print(json.dumps(line), flush=True)
 
 

@@ -179,6 +179,46 @@ int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const dou
 
 void lm_hits_free(lm_hits *h);
 
+/* ---- serving: continuous batching over resident DBs ---------------------------------- */
+
+/* Replaces the streaming daemon's GPU side (src/LaueMatchingGPUStream.cu:103-292
+ * bounded queue, :1040-1075 per-stream buffers, :1167-1257 request loop with its
+ * serialising "GPU gate").  Requests wait in pinned admission slots; each worker
+ * owns a HIP stream and takes the oldest request's DB with up to max_batch of that
+ * DB's pending requests, so requests arriving while others run join the next
+ * batch and batches of different workers overlap on the device.  Each request's
+ * result equals lm_score on that image alone.  DBs must be added before the first
+ * submit and must not be rebuilt while the server runs. */
+typedef struct lm_server lm_server;
+typedef struct lm_server_config {
+  int32_t max_batch;    /* images per launch, 1..64 (0 = 32)                      */
+  int32_t workers;      /* concurrent batches / HIP streams (0 = 2)              */
+  int32_t max_inflight; /* admitted, not yet completed requests (0 = 2*workers*max_batch) */
+  int32_t use_graphs;   /* replay captured launch sequences (hipGraph)           */
+} lm_server_config;
+typedef struct lm_server_stats {
+  uint64_t completed;
+  uint64_t batches;
+  double mean_batch;    /* images per launch                                     */
+  double p50_ms, p99_ms, max_ms; /* submit -> result latency                      */
+  double images_per_s;  /* completed / (last completion - first submit)           */
+  uint64_t graph_launches;
+} lm_server_stats;
+
+int lm_server_create(lm_ctx *ctx, const lm_server_config *cfg, lm_server **out);
+int lm_server_add_db(lm_server *s, lm_db *db, int32_t *db_id);
+/* Copies the image (db geometry, uint8) into an admission slot, blocking while
+ * all slots are in use; reserves the serving buffers from the context's HBM
+ * budget on the first call (LM_ERR_BUDGET if they do not fit). */
+int lm_server_submit(lm_server *s, int32_t db_id, const uint8_t *img_u8, double im_scale,
+                     int32_t min_nr_spots, double min_intensity, uint64_t *ticket);
+/* Blocks until the ticket's result is ready; the caller frees it with lm_hits_free. */
+int lm_server_wait(lm_server *s, uint64_t ticket, lm_hits *out, double *latency_ms);
+int lm_server_get_stats(lm_server *s, lm_server_stats *stats);
+void lm_server_reset_stats(lm_server *s);
+/* Finishes every admitted request, then releases the server. */
+void lm_server_destroy(lm_server *s);
+
 /* ---- merge of duplicate candidates (CPU) -------------------------------------------- */
 
 /* Cluster the nr candidates (orientation rows + coarse scores, in hit-list

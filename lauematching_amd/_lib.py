@@ -45,6 +45,18 @@ class LaueMatchError(RuntimeError):
         self.code = code
 
 
+class LmServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_int32), ("workers", ctypes.c_int32),
+                ("max_inflight", ctypes.c_int32), ("use_graphs", ctypes.c_int32)]
+
+
+class LmServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("images_per_s", ctypes.c_double), ("graph_launches", ctypes.c_uint64)]
+
+
 _lib = None
 
 # name -> (restype, argtypes); every symbol declared in include/lauematch.h
@@ -94,6 +106,18 @@ SIGNATURES = {
     "lm_quat_from_matrix": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "lm_misorientation_deg": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_int32]),
+    "lm_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LmServerConfig),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "lm_server_add_db": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_int32)]),
+    "lm_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                        ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "lm_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(LmHits),
+                                      ctypes.POINTER(ctypes.c_double)]),
+    "lm_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LmServerStats)]),
+    "lm_server_reset_stats": (None, [ctypes.c_void_p]),
+    "lm_server_destroy": (None, [ctypes.c_void_p]),
     "lm_selftest_fp64": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
 }

@@ -114,26 +114,33 @@ int check_ctx(lm_ctx *ctx) {
 
 int stage_host_images(lm_db *db, int nimg, const uint8_t *const *imgs, uint8_t **dptr) {
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
-  const uint64_t need = npx * (uint64_t)nimg;
-  if (db->img_cap < need) {
-    if (db->d_img) (void)hipFree(db->d_img);
-    db->d_img = nullptr;
-    db->img_cap = 0;
-    hipError_t e = hipMalloc(&db->d_img, need);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(image staging)");
-    db->img_cap = need;
-  }
+  int rc = scratch_images(db->scratch, npx * (uint64_t)nimg);
+  if (rc) return rc;
   for (int b = 0; b < nimg; ++b) {
     if (!imgs[b]) {
       set_error("null image pointer");
       return LM_ERR_INVALID;
     }
-    hipError_t e = hipMemcpyAsync(db->d_img + (uint64_t)b * npx, imgs[b], npx,
+    hipError_t e = hipMemcpyAsync(db->scratch.img + (uint64_t)b * npx, imgs[b], npx,
                                   hipMemcpyHostToDevice, db->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "image upload");
   }
-  *dptr = db->d_img;
+  *dptr = db->scratch.img;
   return LM_OK;
+}
+
+// Direct scoring call on the DB's own buffers with one threshold pair.
+int score_uniform(lm_db *db, int nimg, const uint8_t *d, const double *scales, int min_sp,
+                  double min_int, hipStream_t st, lm_hits *outs) {
+  if (nimg == 1 && scales && scales[0] != 1.0)
+    return score_seq(db, db->scratch, d, false, scales[0], min_sp, min_int, st, outs);
+  int sp[kMaxBatch];
+  double mi[kMaxBatch];
+  for (int b = 0; b < kMaxBatch; ++b) {
+    sp[b] = min_sp;
+    mi[b] = min_int;
+  }
+  return score_images(db, db->scratch, nimg, d, scales, sp, mi, st, outs);
 }
 
 }  // namespace
@@ -324,11 +331,7 @@ void lm_db_destroy(lm_db *db) {
   release_table(db);
   if (db->d_orients) (void)hipFree(db->d_orients);
   if (db->d_hkls) (void)hipFree(db->d_hkls);
-  if (db->d_hits) (void)hipFree(db->d_hits);
-  if (db->d_counters) (void)hipFree(db->d_counters);
-  if (db->d_batch) (void)hipFree(db->d_batch);
-  if (db->d_img) (void)hipFree(db->d_img);
-  if (db->d_vals) (void)hipFree(db->d_vals);
+  scratch_release(db->scratch);
   {
     std::lock_guard<std::mutex> lk(db->ctx->mu);
     db->ctx->in_use -= db->n * 72 + (uint64_t)db->nhkls * 12 + 64;
@@ -422,11 +425,8 @@ int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
   uint8_t *d = nullptr;
   rc = stage_host_images(db, nimg, imgs_u8, &d);
   if (rc) return rc;
-  if (nimg == 1 && im_scales && im_scales[0] != 1.0)
-    return score_seq(db, d, false, im_scales[0], min_nr_spots, min_intensity, db->ctx->stream,
-                     outs);
-  return score_images(db, nimg, d, im_scales, min_nr_spots, min_intensity, db->ctx->stream,
-                      outs);
+  return score_uniform(db, nimg, d, im_scales, min_nr_spots, min_intensity, db->ctx->stream,
+                       outs);
 }
 
 int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double min_intensity,
@@ -439,18 +439,13 @@ int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double 
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   const uint64_t bytes = (uint64_t)db->geom.nrPxX * db->geom.nrPxY * 8;
-  if (db->img_cap < bytes) {
-    if (db->d_img) (void)hipFree(db->d_img);
-    db->d_img = nullptr;
-    db->img_cap = 0;
-    hipError_t e = hipMalloc(&db->d_img, bytes);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(image staging)");
-    db->img_cap = bytes;
-  }
-  hipError_t e = hipMemcpyAsync(db->d_img, img_f64, bytes, hipMemcpyHostToDevice,
+  rc = scratch_images(db->scratch, bytes);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(db->scratch.img, img_f64, bytes, hipMemcpyHostToDevice,
                                 db->ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "image upload");
-  return score_seq(db, db->d_img, true, 1.0, min_nr_spots, min_intensity, db->ctx->stream, out);
+  return score_seq(db, db->scratch, db->scratch.img, true, 1.0, min_nr_spots, min_intensity,
+                   db->ctx->stream, out);
 }
 
 int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,
@@ -463,9 +458,7 @@ int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const dou
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   hipStream_t st = stream ? (hipStream_t)stream : db->ctx->stream;
-  if (nimg == 1 && im_scales && im_scales[0] != 1.0)
-    return score_seq(db, d_imgs, false, im_scales[0], min_nr_spots, min_intensity, st, outs);
-  return score_images(db, nimg, d_imgs, im_scales, min_nr_spots, min_intensity, st, outs);
+  return score_uniform(db, nimg, d_imgs, im_scales, min_nr_spots, min_intensity, st, outs);
 }
 
 int lm_selftest_fp64(lm_ctx *ctx, const double *a, const double *b, uint64_t n, double *q,

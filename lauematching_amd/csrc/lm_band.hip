@@ -125,7 +125,7 @@ template <int TB, int THREADS, int U>
 __global__ void __launch_bounds__(THREADS)
 score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                   const uint32_t *__restrict__ bandOff, int nbands, uint64_t n, uint64_t nsuper,
-                  const uint8_t *__restrict__ img, int min_spots, double min_int,
+                  const uint8_t *__restrict__ img, const Pred *__restrict__ pred,
                   uint64_t *__restrict__ hits, uint64_t cap, uint32_t *__restrict__ counters) {
   constexpr int T = 1 << TB;
   constexpr int PB = 32 - TB;
@@ -179,6 +179,8 @@ score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restr
       const uint32_t acc = cnt[rr];
       const uint32_t ns = acc >> kSumBits, sum = acc & kSumMask;
       const bool live = o < n;
+      const int min_spots = pred->min_spots[0];
+      const double min_int = pred->min_int[0];
       const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= min_int);
       const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int;
       const uint64_t zm = __ballot(zero_pass);
@@ -212,6 +214,7 @@ int reserve_bytes(lm_db *db, uint64_t bytes) {
 }  // namespace
 
 void release_banded(lm_db *db) {
+  ++db->generation;  // any captured launch that read the old table is stale
   Banded &b = db->banded;
   if (b.entries) (void)hipFree(b.entries);
   if (b.band_off) (void)hipFree(b.band_off);
@@ -271,22 +274,19 @@ int build_banded(lm_db *db) {
   return LM_OK;
 }
 
-int launch_banded(lm_db *db, const uint8_t *img, int min_spots, double min_int, uint64_t cap,
-                  hipStream_t st) {
+int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st) {
   const Banded &b = db->banded;
   const Table &t = db->table;
   hipError_t e;
-  if (b.tb == 14) {
-    const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+  const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+  if (b.tb == 14)
     hipLaunchKernelGGL((score_band_kernel<14, 1024, 8>), dim3(grid), dim3(1024), 0, st, b.entries,
-                       t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, min_spots, min_int,
-                       db->d_hits, cap, db->d_counters);
-  } else {
-    const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+                       t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, sc.pred, sc.hits,
+                       cap, sc.counters);
+  else
     hipLaunchKernelGGL((score_band_kernel<13, 512, 8>), dim3(grid), dim3(512), 0, st, b.entries,
-                       t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, min_spots, min_int,
-                       db->d_hits, cap, db->d_counters);
-  }
+                       t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, sc.pred, sc.hits,
+                       cap, sc.counters);
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
   return LM_OK;

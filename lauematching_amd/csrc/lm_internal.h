@@ -15,13 +15,10 @@ namespace lm {
 // Spot table resident in HBM.
 //
 // Orientations are grouped in tiles of T = 2^tile_bits consecutive rows.  A
-// tile's spots are stored contiguously as 32-bit entries
-//     entry = (row_in_tile << pix_bits) | (y * nrPxX + x)
-// in table order (row-major within the tile, spot order within a row), so a
-// single streaming read of the entry array feeds the compare kernel and the
-// owner of each entry is known without a search.  tile_off[t] is the index
-// of the first entry of tile t (tile_off[ntiles] = total).  counts[o] is the
-// per-orientation spot count (the ForwardFile's word 0).
+// tile's spots are stored contiguously as 32-bit pixel indices y*nrPxX + x in
+// table order (row after row, spot order within a row).  tile_off[t] is the
+// index of the first entry of tile t (tile_off[ntiles] = total); counts[o] is
+// the per-orientation spot count (the ForwardFile's word 0).
 struct Table {
   uint32_t *entries = nullptr;  // total + 64 words of slack for 16-byte loads
   uint64_t *tile_off = nullptr; // ntiles + 1
@@ -65,6 +62,53 @@ struct Geom {
   double Elo, Ehi;
 };
 
+constexpr int kMaxBatch = 64;
+
+// Per-image predicate of one scoring launch, read by the kernels from device
+// memory (so a captured launch sequence stays valid for new thresholds).
+struct Pred {
+  double min_int[kMaxBatch];
+  double cand_thr[kMaxBatch];  // candidate threshold on the integer sum
+  int min_spots[kMaxBatch];
+  uint32_t exact_lo, exact_hi; // bit b: image b has imScale == 1
+  int nimg;
+};
+
+// Device buffers of one scoring stream.  A DB owns one for direct calls;
+// each serving worker owns its own so batches on different streams never
+// share buffers.
+struct Scratch {
+  uint64_t *hits = nullptr;  // packed hit records, cap_per * nimg
+  uint64_t hit_cap = 0;
+  uint32_t *counters = nullptr;  // 2 * kMaxBatch
+  double *vals = nullptr;        // float64 sums of in-order hits
+  uint64_t val_cap = 0;
+  uint8_t *batch = nullptr;      // interleaved images [npx][B]
+  uint64_t batch_cap = 0;
+  uint8_t *img = nullptr;        // staging of host images
+  uint64_t img_cap = 0;
+  Pred *pred = nullptr;          // device copy of the predicate
+  Pred *pred_host = nullptr;     // pinned host copy
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Captured launch sequences (interleave + counter fill + score kernel),
+  // keyed by every pointer and size their kernel arguments depend on.
+  bool use_graphs = false;
+  struct Graph {
+    const void *db, *src, *hits;
+    uint64_t generation;
+    int nimg;
+    uint64_t cap_per;
+    bool banded;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  uint64_t graph_launches = 0;
+};
+
+int scratch_init(Scratch &s);
+void scratch_release(Scratch &s);
+int scratch_images(Scratch &s, uint64_t bytes);
+
 }  // namespace lm
 
 struct lm_ctx {
@@ -85,18 +129,9 @@ struct lm_db {
   int32_t *d_hkls = nullptr;    // nhkls * 3
   lm::Table table;
   lm::Banded banded;
-  int banded_tb = 14;             // super-tile bits for the banded copy (0 = no copy)
-  // scratch for scoring
-  uint64_t *d_hits = nullptr;     // packed hit records
-  uint64_t hit_cap = 0;
-  uint32_t *d_counters = nullptr; // [0] = hits written, [1] = nr_results, [2..] per-image
-  uint64_t counter_slots = 0;
-  uint8_t *d_batch = nullptr;     // interleaved image batch [npx][B]
-  uint64_t batch_cap = 0;
-  uint8_t *d_img = nullptr;       // staging for host images
-  uint64_t img_cap = 0;
-  double *d_vals = nullptr;       // float64 sums of in-order hits
-  uint64_t val_cap = 0;
+  int banded_tb = 14;           // super-tile bits for the banded copy (0 = no copy)
+  uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
+  lm::Scratch scratch;          // buffers of direct (non-serving) calls
   std::mutex mu;
 };
 
@@ -122,26 +157,27 @@ void release_table(lm_db *db);
 int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, double *q,
                   double *s);
 
-// lm_score.hip
-int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_scales,
-                 int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *outs);
-// Build / drop the pixel-banded copy (after forward or import).
+// lm_band.hip
 int build_banded(lm_db *db);
 void release_banded(lm_db *db);
-int launch_banded(lm_db *db, const uint8_t *img, int min_spots, double min_int, uint64_t cap,
-                  hipStream_t st);
-// One image against the banded copy; imScale must be 1 (integer sums).
-int score_banded(lm_db *db, const uint8_t *d_img, int min_nr_spots, double min_intensity,
-                 hipStream_t stream, lm_hits *out);
+int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st);
+
+// lm_score.hip
+// Score nimg device images laid out back to back ([nimg][npx]) with
+// per-image thresholds; outs[b] receives image b's ascending hit list.
+int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
+                 const double *im_scales, const int *min_spots, const double *min_int,
+                 hipStream_t stream, lm_hits *outs);
+// One image, float64 sums accumulated in table order on the device
+// (f64 = image of doubles; otherwise uint8 values times scale).
+int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
+              int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *out);
+
 // lm_merge.cpp
 void quat_from_matrix(const double m[9], double q[4]);
 double misorientation_deg(const double a[4], const double b[4], int space_group);
 int merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows,
                      const double *scores, int nr, double max_angle, int space_group,
                      int threads, uint64_t *best_row, double *best_score, int *cluster_size);
-// One image, float64 sums accumulated in table order on the device
-// (f64 = image of doubles; otherwise uint8 values times scale).
-int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_spots,
-              double min_intensity, hipStream_t stream, lm_hits *out);
 
 }  // namespace lm

@@ -37,49 +37,76 @@ namespace {
 
 constexpr int kTile = 256;          // orientations per workgroup (= 1 << tile bits)
 constexpr int kStageWords = 12288;  // 48 KB of staged entries per workgroup
-constexpr int kMaxBatch = 64;
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-struct Pred {
-  double min_int;
-  int min_spots;
-  int nimg;
-  uint32_t exact_mask_lo, exact_mask_hi;  // bit b: image b has imScale == 1
-  double cand_thr[kMaxBatch];             // candidate threshold on the integer sum
+__host__ __device__ __forceinline__ bool bit(const Pred &p, int b) {
+  return b < 32 ? (p.exact_lo >> b) & 1u : (p.exact_hi >> (b - 32)) & 1u;
+}
+
+// Words holding one pixel of B interleaved images.
+template <int B>
+struct PxW {
+  static constexpr int W = B < 4 ? 1 : B / 4;
 };
 
-__host__ __device__ __forceinline__ bool bit(const Pred &p, int b) {
-  return b < 32 ? (p.exact_mask_lo >> b) & 1u : (p.exact_mask_hi >> (b - 32)) & 1u;
+// One gather per spot serves all B images: B bytes at img + pix*B
+// (1 byte, 4 bytes, 8 bytes or B/16 16-byte loads).
+template <int B>
+__device__ __forceinline__ void load_px(const uint8_t *__restrict__ img, uint32_t pix,
+                                        uint32_t (&w)[PxW<B>::W]) {
+  if constexpr (B == 1) {
+    w[0] = img[pix];
+  } else if constexpr (B == 4) {
+    w[0] = *(const uint32_t *)(img + (uint64_t)pix * 4);
+  } else if constexpr (B == 8) {
+    const uint2 v = *(const uint2 *)(img + (uint64_t)pix * 8);
+    w[0] = v.x;
+    w[1] = v.y;
+  } else {
+    static_assert(B % 16 == 0, "batch width must be 1, 4, 8 or a multiple of 16");
+    const u32x4 *p = (const u32x4 *)(img + (uint64_t)pix * B);
+#pragma unroll
+    for (int q = 0; q < B / 16; ++q) {
+      const u32x4 v = p[q];
+      w[4 * q + 0] = v.x;
+      w[4 * q + 1] = v.y;
+      w[4 * q + 2] = v.z;
+      w[4 * q + 3] = v.w;
+    }
+  }
 }
 
 template <int B>
-__device__ __forceinline__ void gather(const uint8_t *__restrict__ img, uint32_t pix,
-                                       uint32_t (&acc)[B]) {
-  if constexpr (B == 1) {
-    const uint32_t raw = img[pix];
-    acc[0] += raw ? (1u << kSumBits) + raw : 0u;
-  } else if constexpr (B == 4) {
-    const uint32_t v = *(const uint32_t *)(img + (uint64_t)pix * 4);
+__device__ __forceinline__ void accum_px(const uint32_t (&w)[PxW<B>::W], uint32_t (&acc)[B]) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const uint32_t raw = (v >> (8 * b)) & 0xffu;
-      acc[b] += raw ? (1u << kSumBits) + raw : 0u;
-    }
-  } else {
-    static_assert(B % 8 == 0, "batch must be 1, 4 or a multiple of 8");
-    const uint2 *p = (const uint2 *)(img + (uint64_t)pix * B);
-#pragma unroll
-    for (int q = 0; q < B / 8; ++q) {
-      const uint2 v = p[q];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint32_t r0 = (v.x >> (8 * b)) & 0xffu;
-        const uint32_t r1 = (v.y >> (8 * b)) & 0xffu;
-        acc[q * 8 + b] += r0 ? (1u << kSumBits) + r0 : 0u;
-        acc[q * 8 + 4 + b] += r1 ? (1u << kSumBits) + r1 : 0u;
-      }
-    }
+  for (int b = 0; b < B; ++b) {
+    const uint32_t raw = B == 1 ? w[0] : (w[b >> 2] >> (8 * (b & 3))) & 0xffu;
+    acc[b] += raw ? (1u << kSumBits) + raw : 0u;
+  }
+}
+
+// Accumulate the c spots at src[0..c) (LDS or global), four gathers in flight.
+template <int B>
+__device__ __forceinline__ void accum_spots(const uint8_t *__restrict__ img, const uint32_t *src,
+                                            uint32_t c, uint32_t (&acc)[B]) {
+  uint32_t s = 0;
+  for (; s + 4 <= c; s += 4) {
+    uint32_t w0[PxW<B>::W], w1[PxW<B>::W], w2[PxW<B>::W], w3[PxW<B>::W];
+    const uint32_t p0 = src[s], p1 = src[s + 1], p2 = src[s + 2], p3 = src[s + 3];
+    load_px<B>(img, p0, w0);
+    load_px<B>(img, p1, w1);
+    load_px<B>(img, p2, w2);
+    load_px<B>(img, p3, w3);
+    accum_px<B>(w0, acc);
+    accum_px<B>(w1, acc);
+    accum_px<B>(w2, acc);
+    accum_px<B>(w3, acc);
+  }
+  for (; s < c; ++s) {
+    uint32_t w[PxW<B>::W];
+    load_px<B>(img, src[s], w);
+    accum_px<B>(w, acc);
   }
 }
 
@@ -106,8 +133,8 @@ template <int B>
 __global__ void __launch_bounds__(kTile)
 score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
              const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
-             const uint8_t *__restrict__ img, Pred pred, uint64_t *__restrict__ hits,
-             uint64_t capPer, uint32_t *__restrict__ counters) {
+             const uint8_t *__restrict__ img, const Pred *__restrict__ pred,
+             uint64_t *__restrict__ hits, uint64_t capPer, uint32_t *__restrict__ counters) {
   __shared__ uint32_t stage[kStageWords];
   __shared__ uint32_t wsum[kTile / 64];
   const int tid = threadIdx.x;
@@ -148,23 +175,21 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
     uint32_t acc[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
-    if (staged) {
-      for (uint32_t s = 0; s < c; ++s) gather<B>(img, stage[excl + s], acc);
-    } else {
-      const uint32_t *src = entries + base + excl;
-      for (uint32_t s = 0; s < c; ++s) gather<B>(img, src[s], acc);
-    }
+    if (staged)
+      accum_spots<B>(img, stage + excl, c, acc);
+    else
+      accum_spots<B>(img, entries + base + excl, c, acc);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const uint32_t ns = acc[b] >> kSumBits;
       const uint32_t sum = acc[b] & kSumMask;
       const bool live = o < n;
       bool cand = false, zero_pass = false;
-      if (live) {
+      if (live && b < pred->nimg) {
         if (ns == 0) {
-          zero_pass = (0 >= pred.min_spots) && (0.0 >= pred.min_int);
+          zero_pass = (0 >= pred->min_spots[b]) && (0.0 >= pred->min_int[b]);
         } else {
-          cand = ((int)ns >= pred.min_spots) && ((double)sum >= pred.cand_thr[b]);
+          cand = ((int)ns >= pred->min_spots[b]) && ((double)sum >= pred->cand_thr[b]);
         }
       }
       const uint64_t zm = __ballot(zero_pass);
@@ -294,31 +319,62 @@ __global__ void exact_kernel(const uint16_t *__restrict__ counts,
   out[i] = tot;
 }
 
-int ensure_scratch(lm_db *db, int nimg, uint64_t capPer) {
-  const uint64_t need = capPer * (uint64_t)nimg;
-  if (db->hit_cap < need) {
-    if (db->d_hits) (void)hipFree(db->d_hits);
-    db->d_hits = nullptr;
-    db->hit_cap = 0;
-    hipError_t e = hipMalloc(&db->d_hits, need * 8);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit buffer)");
-    db->hit_cap = need;
-  }
-  if (db->counter_slots < (uint64_t)2 * kMaxBatch) {
-    hipError_t e = hipMalloc(&db->d_counters, 2 * kMaxBatch * 4);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
-    db->counter_slots = 2 * kMaxBatch;
-  }
+// Fill kernel for the launch counters (kept as a kernel so that a captured
+// launch sequence holds no memset node).
+__global__ void zero_kernel(uint32_t *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Captured sequences reference the scratch buffers; drop them before any of
+// those buffers is reallocated.
+void drop_graphs(Scratch &sc) {
+  for (auto &g : sc.graphs) (void)hipGraphExecDestroy(g.exec);
+  sc.graphs.clear();
+}
+
+int ensure_hits(Scratch &sc, uint64_t need) {
+  if (sc.hit_cap >= need) return LM_OK;
+  drop_graphs(sc);
+  if (sc.hits) (void)hipFree(sc.hits);
+  sc.hits = nullptr;
+  sc.hit_cap = 0;
+  hipError_t e = hipMalloc(&sc.hits, need * 8);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit buffer)");
+  sc.hit_cap = need;
   return LM_OK;
 }
 
-int launch_score(lm_db *db, int B, const uint8_t *img, const Pred &pred, uint64_t capPer,
+int ensure_vals(Scratch &sc, uint64_t need) {
+  if (sc.val_cap >= need) return LM_OK;
+  if (sc.vals) (void)hipFree(sc.vals);
+  sc.vals = nullptr;
+  sc.val_cap = 0;
+  hipError_t e = hipMalloc(&sc.vals, need * 8);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit values)");
+  sc.val_cap = need;
+  return LM_OK;
+}
+
+int ensure_batch(Scratch &sc, uint64_t need) {
+  if (sc.batch_cap >= need) return LM_OK;
+  drop_graphs(sc);
+  if (sc.batch) (void)hipFree(sc.batch);
+  sc.batch = nullptr;
+  sc.batch_cap = 0;
+  hipError_t e = hipMalloc(&sc.batch, need);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(interleaved batch)");
+  sc.batch_cap = need;
+  return LM_OK;
+}
+
+int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
                  hipStream_t st) {
   const Table &t = db->table;
   const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
 #define LM_LAUNCH(BB)                                                                           \
   hipLaunchKernelGGL(score_kernel<BB>, dim3(grid), dim3(kTile), 0, st, t.counts, t.tile_off,   \
-                     t.entries, db->n, t.ntiles, img, pred, db->d_hits, capPer, db->d_counters)
+                     t.entries, db->n, t.ntiles, img, sc.pred, sc.hits, capPer, sc.counters)
   switch (B) {
     case 1: LM_LAUNCH(1); break;
     case 4: LM_LAUNCH(4); break;
@@ -361,22 +417,63 @@ int batch_width(int nimg) {
   return B;
 }
 
-int ensure_vals(lm_db *db, uint64_t cap) {
-  if (db->val_cap < cap) {
-    if (db->d_vals) (void)hipFree(db->d_vals);
-    db->d_vals = nullptr;
-    db->val_cap = 0;
-    hipError_t e = hipMalloc(&db->d_vals, cap * 8);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit values)");
-    db->val_cap = cap;
+int fill_hits(lm_hits &h, uint64_t m) {
+  if (!m) return LM_OK;
+  h.ids = (uint64_t *)malloc(m * 8);
+  h.nspots = (int32_t *)malloc(m * 4);
+  h.totint = (double *)malloc(m * 8);
+  h.score = (double *)malloc(m * 8);
+  if (!h.ids || !h.nspots || !h.totint || !h.score) {
+    lm_hits_free(&h);
+    set_error("out of host memory for hits");
+    return LM_ERR_NOMEM;
   }
   return LM_OK;
 }
 
 }  // namespace
 
-int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_spots,
-              double min_intensity, hipStream_t st, lm_hits *out) {
+int scratch_init(Scratch &s) {
+  if (s.pred) return LM_OK;
+  hipError_t e = hipMalloc(&s.counters, 2 * kMaxBatch * 4);
+  if (e == hipSuccess) e = hipMalloc(&s.pred, sizeof(Pred));
+  if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&s.ev0);
+  if (e == hipSuccess) e = hipEventCreate(&s.ev1);
+  if (e != hipSuccess) {
+    scratch_release(s);
+    return hip_fail(e, "scratch buffers");
+  }
+  return LM_OK;
+}
+
+void scratch_release(Scratch &s) {
+  for (auto &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
+  if (s.hits) (void)hipFree(s.hits);
+  if (s.counters) (void)hipFree(s.counters);
+  if (s.vals) (void)hipFree(s.vals);
+  if (s.batch) (void)hipFree(s.batch);
+  if (s.img) (void)hipFree(s.img);
+  if (s.pred) (void)hipFree(s.pred);
+  if (s.pred_host) (void)hipHostFree(s.pred_host);
+  if (s.ev0) (void)hipEventDestroy(s.ev0);
+  if (s.ev1) (void)hipEventDestroy(s.ev1);
+  s = Scratch{};
+}
+
+int scratch_images(Scratch &s, uint64_t bytes) {
+  if (s.img_cap >= bytes) return LM_OK;
+  if (s.img) (void)hipFree(s.img);
+  s.img = nullptr;
+  s.img_cap = 0;
+  hipError_t e = hipMalloc(&s.img, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(image staging)");
+  s.img_cap = bytes;
+  return LM_OK;
+}
+
+int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
+              int min_nr_spots, double min_intensity, hipStream_t st, lm_hits *out) {
   const Table &t = db->table;
   if (!t.ready) {
     set_error("score: no table (run lm_db_forward or lm_db_import_table)");
@@ -387,86 +484,70 @@ int score_seq(lm_db *db, const void *d_img, bool f64, double scale, int min_nr_s
     return LM_ERR_INVALID;
   }
   memset(out, 0, sizeof *out);
-  uint64_t cap = std::max<uint64_t>(db->hit_cap, 1u << 16);
+  int rc = scratch_init(sc);
+  if (rc) return rc;
+  uint64_t cap = std::max<uint64_t>(sc.hit_cap, 1u << 16);
   uint32_t cnt[2] = {0, 0};
   float ms = 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  int rc = LM_OK;
   const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
   for (int attempt = 0; attempt < 2; ++attempt) {
-    rc = ensure_scratch(db, 1, cap);
-    if (rc == LM_OK) rc = ensure_vals(db, cap);
+    rc = ensure_hits(sc, cap);
+    if (rc == LM_OK) rc = ensure_vals(sc, cap);
     if (rc) break;
-    hipError_t e = hipMemsetAsync(db->d_counters, 0, 2 * kMaxBatch * 4, st);
-    if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(counters)"); break; }
-    (void)hipEventRecord(e0, st);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(2 * kMaxBatch), 0, st, sc.counters,
+                       2 * kMaxBatch);
+    (void)hipEventRecord(sc.ev0, st);
     if (f64)
       hipLaunchKernelGGL(score_seq_kernel<double>, dim3(grid), dim3(kTile), 0, st, t.counts,
                          t.tile_off, t.entries, db->n, t.ntiles, (const double *)d_img, scale,
-                         min_nr_spots, min_intensity, db->d_hits, db->d_vals, cap,
-                         db->d_counters);
+                         min_nr_spots, min_intensity, sc.hits, sc.vals, cap, sc.counters);
     else
       hipLaunchKernelGGL(score_seq_kernel<uint8_t>, dim3(grid), dim3(kTile), 0, st, t.counts,
                          t.tile_off, t.entries, db->n, t.ntiles, (const uint8_t *)d_img, scale,
-                         min_nr_spots, min_intensity, db->d_hits, db->d_vals, cap,
-                         db->d_counters);
-    e = hipGetLastError();
+                         min_nr_spots, min_intensity, sc.hits, sc.vals, cap, sc.counters);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) { rc = hip_fail(e, "score kernel launch"); break; }
-    (void)hipEventRecord(e1, st);
-    e = hipMemcpyAsync(cnt, db->d_counters, 8, hipMemcpyDeviceToHost, st);
+    (void)hipEventRecord(sc.ev1, st);
+    e = hipMemcpyAsync(cnt, sc.counters, 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { rc = hip_fail(e, "score kernel"); break; }
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms, sc.ev0, sc.ev1);
     if (cnt[0] <= cap) break;
     if (attempt == 1) { set_error("score: hit list overflow"); rc = LM_ERR_OVERFLOW; break; }
     cap = cnt[0] + cnt[0] / 4 + 1024;
   }
+  if (rc) return rc;
   const uint64_t k = cnt[0];
   std::vector<uint64_t> rec(k);
   std::vector<double> val(k);
-  if (rc == LM_OK && k) {
-    hipError_t e = hipMemcpyAsync(rec.data(), db->d_hits, k * 8, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(val.data(), db->d_vals, k * 8, hipMemcpyDeviceToHost, st);
+  if (k) {
+    hipError_t e = hipMemcpyAsync(rec.data(), sc.hits, k * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(val.data(), sc.vals, k * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) rc = hip_fail(e, "hit copy");
+    if (e != hipSuccess) return hip_fail(e, "hit copy");
   }
-  if (rc == LM_OK) {
-    std::vector<uint32_t> idx(k);
-    for (uint64_t i = 0; i < k; ++i) idx[i] = (uint32_t)i;
-    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return rec[a] < rec[b]; });
-    out->nr_results = (uint64_t)cnt[1] + k;
-    out->count = k;
-    out->kernel_ms = ms;
-    if (k) {
-      out->ids = (uint64_t *)malloc(k * 8);
-      out->nspots = (int32_t *)malloc(k * 4);
-      out->totint = (double *)malloc(k * 8);
-      out->score = (double *)malloc(k * 8);
-      if (!out->ids || !out->nspots || !out->totint || !out->score) {
-        lm_hits_free(out);
-        set_error("out of host memory for hits");
-        rc = LM_ERR_NOMEM;
-      } else {
-        for (uint64_t j = 0; j < k; ++j) {
-          const uint64_t r = rec[idx[j]];
-          const int ns = (int)(r & 0x3ffffffull);
-          out->ids[j] = r >> 26;
-          out->nspots[j] = ns;
-          out->totint[j] = val[idx[j]];
-          out->score[j] = val[idx[j]] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
-        }
-      }
-    }
+  std::vector<uint32_t> idx(k);
+  for (uint64_t i = 0; i < k; ++i) idx[i] = (uint32_t)i;
+  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return rec[a] < rec[b]; });
+  out->nr_results = (uint64_t)cnt[1] + k;
+  out->count = k;
+  out->kernel_ms = ms;
+  rc = fill_hits(*out, k);
+  if (rc) return rc;
+  for (uint64_t j = 0; j < k; ++j) {
+    const uint64_t r = rec[idx[j]];
+    const int ns = (int)(r & 0x3ffffffull);
+    out->ids[j] = r >> 26;
+    out->nspots[j] = ns;
+    out->totint[j] = val[idx[j]];
+    out->score[j] = val[idx[j]] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return rc;
+  return LM_OK;
 }
 
-int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_scales,
-                 int min_nr_spots, double min_intensity, hipStream_t st, lm_hits *outs) {
+int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
+                 const double *im_scales, const int *min_spots, const double *min_int,
+                 hipStream_t st, lm_hits *outs) {
   const Table &t = db->table;
   if (!t.ready) {
     set_error("score: no table (run lm_db_forward or lm_db_import_table)");
@@ -481,92 +562,113 @@ int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_sc
       set_error("score: imScale must be positive and finite");
       return LM_ERR_INVALID;
     }
+  int rc = scratch_init(sc);
+  if (rc) return rc;
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
   const int B = batch_width(nimg);
-  Pred pred{};
-  pred.min_int = min_intensity;
-  pred.min_spots = min_nr_spots;
+  Pred &pred = *sc.pred_host;
+  memset(&pred, 0, sizeof pred);
   pred.nimg = nimg;
   for (int b = 0; b < kMaxBatch; ++b) pred.cand_thr[b] = INFINITY;
   for (int b = 0; b < nimg; ++b) {
     const double s = im_scales[b];
+    pred.min_spots[b] = min_spots[b];
+    pred.min_int[b] = min_int[b];
     if (s == 1.0) {
       // totInt == sum exactly: sum >= min_int decided on the device
-      pred.cand_thr[b] = min_intensity;
-      if (b < 32) pred.exact_mask_lo |= 1u << b; else pred.exact_mask_hi |= 1u << (b - 32);
+      pred.cand_thr[b] = min_int[b];
+      if (b < 32) pred.exact_lo |= 1u << b; else pred.exact_hi |= 1u << (b - 32);
     } else {
       // superset: the sequential float64 sum is within n*2^-53 relative of sum*s
-      pred.cand_thr[b] = min_intensity / s * (1.0 - 1e-12);
-      if (min_intensity <= 0) pred.cand_thr[b] = -INFINITY;
+      pred.cand_thr[b] = min_int[b] <= 0 ? -INFINITY : min_int[b] / s * (1.0 - 1e-12);
     }
   }
-  uint64_t capPer = std::max<uint64_t>(db->hit_cap / (uint64_t)std::max(nimg, 1), 1u << 16);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  uint8_t *d_il = nullptr;
-  int rc = LM_OK;
-  std::vector<uint32_t> cnt(2 * kMaxBatch);
-  const uint8_t *img = d_imgs;
+  hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "predicate upload");
+  uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)nimg, 1u << 16);
+  const bool banded = B == 1 && db->banded.ready && bit(pred, 0);
   if (B > 1) {
-    if (db->batch_cap < npx * B) {
-      if (db->d_batch) (void)hipFree(db->d_batch);
-      db->d_batch = nullptr;
-      db->batch_cap = 0;
-      hipError_t e = hipMalloc(&db->d_batch, npx * B);
-      if (e != hipSuccess) return hip_fail(e, "hipMalloc(interleaved batch)");
-      db->batch_cap = npx * B;
-    }
-    d_il = db->d_batch;
-    rc = launch_interleave(B, d_imgs, nimg, npx, d_il, st);
-    img = d_il;
+    rc = ensure_batch(sc, npx * B);
+    if (rc) return rc;
   }
+  const uint8_t *img = B > 1 ? sc.batch : d_imgs;
+  // interleave (B > 1), counter fill, score kernel: enqueued directly, or
+  // replayed from a captured graph keyed by everything the arguments use
+  auto sequence = [&](uint64_t cp) -> int {
+    if (B > 1) {
+      const int r = launch_interleave(B, d_imgs, nimg, npx, sc.batch, st);
+      if (r) return r;
+    }
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(2 * kMaxBatch), 0, st, sc.counters,
+                       2 * kMaxBatch);
+    return banded ? launch_banded(db, sc, img, cp, st) : launch_score(db, sc, B, img, cp, st);
+  };
+  auto run = [&](uint64_t cp) -> int {
+    if (!sc.use_graphs) return sequence(cp);
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : sc.graphs)
+      if (g.db == db && g.generation == db->generation && g.src == d_imgs &&
+          g.hits == sc.hits && g.nimg == nimg && g.cap_per == cp && g.banded == banded)
+        exec = g.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      hipError_t ge = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+      if (ge != hipSuccess) return hip_fail(ge, "hipStreamBeginCapture");
+      const int r = sequence(cp);
+      ge = hipStreamEndCapture(st, &graph);
+      if (r) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      if (ge == hipSuccess) ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (ge != hipSuccess) return hip_fail(ge, "graph capture");
+      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, nimg, cp, banded, exec});
+    }
+    ++sc.graph_launches;
+    const hipError_t le = hipGraphLaunch(exec, st);
+    return le == hipSuccess ? LM_OK : hip_fail(le, "hipGraphLaunch");
+  };
   float ms = 0;
-  for (int attempt = 0; attempt < 2 && rc == LM_OK; ++attempt) {
-    rc = ensure_scratch(db, nimg, capPer);
-    if (rc) break;
-    hipError_t e = hipMemsetAsync(db->d_counters, 0, 2 * kMaxBatch * 4, st);
-    if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(counters)"); break; }
-    (void)hipEventRecord(e0, st);
-    if (B == 1 && db->banded.ready && bit(pred, 0))
-      rc = launch_banded(db, img, min_nr_spots, min_intensity, capPer, st);
-    else
-      rc = launch_score(db, B, img, pred, capPer, st);
-    if (rc) break;
-    (void)hipEventRecord(e1, st);
-    e = hipMemcpyAsync(cnt.data(), db->d_counters, 2 * kMaxBatch * 4, hipMemcpyDeviceToHost, st);
+  std::vector<uint32_t> cnt(2 * kMaxBatch);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    rc = ensure_hits(sc, capPer * (uint64_t)nimg);
+    if (rc) return rc;
+    (void)hipEventRecord(sc.ev0, st);
+    rc = run(capPer);
+    if (rc) return rc;
+    (void)hipEventRecord(sc.ev1, st);
+    e = hipMemcpyAsync(cnt.data(), sc.counters, 2 * kMaxBatch * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) { rc = hip_fail(e, "score kernel"); break; }
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) return hip_fail(e, "score kernel");
+    (void)hipEventElapsedTime(&ms, sc.ev0, sc.ev1);
     uint64_t need = 0;
     for (int b = 0; b < nimg; ++b) need = std::max<uint64_t>(need, cnt[2 * b]);
     if (need <= capPer) break;
     if (attempt == 1) {
       set_error("score: hit list overflow");
-      rc = LM_ERR_OVERFLOW;
-      break;
+      return LM_ERR_OVERFLOW;
     }
     capPer = need + need / 4 + 1024;  // rerun once with a buffer that fits
   }
   std::vector<uint64_t> rec;
   uint32_t *d_ids = nullptr;
   double *d_exact = nullptr;
+  for (int b = 0; b < nimg; ++b) memset(&outs[b], 0, sizeof(lm_hits));
   for (int b = 0; b < nimg && rc == LM_OK; ++b) {
     lm_hits &h = outs[b];
-    memset(&h, 0, sizeof h);
     h.kernel_ms = ms;
     const uint64_t k = cnt[2 * b];
     rec.resize(k);
     if (k) {
-      hipError_t e = hipMemcpyAsync(rec.data(), db->d_hits + (uint64_t)b * capPer, k * 8,
-                                    hipMemcpyDeviceToHost, st);
+      e = hipMemcpyAsync(rec.data(), sc.hits + (uint64_t)b * capPer, k * 8,
+                         hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) { rc = hip_fail(e, "hit copy"); break; }
       std::sort(rec.begin(), rec.end());
     }
     std::vector<double> tot(k);
-    const bool exact = bit(pred, b);
-    if (exact) {
+    if (bit(pred, b)) {
       for (uint64_t i = 0; i < k; ++i) tot[i] = (double)(rec[i] & kSumMask);
     } else if (k) {
       std::vector<uint32_t> ids(k);
@@ -575,58 +677,43 @@ int score_images(lm_db *db, int nimg, const uint8_t *d_imgs, const double *im_sc
       if (d_exact) (void)hipFree(d_exact);
       d_ids = nullptr;
       d_exact = nullptr;
-      hipError_t e = hipMalloc(&d_ids, k * 4);
+      e = hipMalloc(&d_ids, k * 4);
       if (e == hipSuccess) e = hipMalloc(&d_exact, k * 8);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(d_ids, ids.data(), k * 4, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(d_ids, ids.data(), k * 4, hipMemcpyHostToDevice, st);
       if (e != hipSuccess) { rc = hip_fail(e, "exact pass setup"); break; }
       const uint8_t *ib = (B > 1) ? img + b : img;
       hipLaunchKernelGGL(exact_kernel, dim3((unsigned)((k + 127) / 128)), dim3(128), 0, st,
                          t.counts, t.tile_off, t.entries, d_ids, k, ib, (uint32_t)B,
                          im_scales[b], d_exact);
       e = hipGetLastError();
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(tot.data(), d_exact, k * 8, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(tot.data(), d_exact, k * 8, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) { rc = hip_fail(e, "exact pass"); break; }
     }
     // final predicate and arrays (ascending id)
     uint64_t m = 0;
     for (uint64_t i = 0; i < k; ++i)
-      if (tot[i] >= min_intensity) ++m;
+      if (tot[i] >= min_int[b]) ++m;
     h.nr_results = (uint64_t)cnt[2 * b + 1] + m;
     h.count = m;
-    if (m) {
-      h.ids = (uint64_t *)malloc(m * 8);
-      h.nspots = (int32_t *)malloc(m * 4);
-      h.totint = (double *)malloc(m * 8);
-      h.score = (double *)malloc(m * 8);
-      if (!h.ids || !h.nspots || !h.totint || !h.score) {
-        lm_hits_free(&h);
-        set_error("out of host memory for hits");
-        rc = LM_ERR_NOMEM;
-        break;
-      }
-      uint64_t j = 0;
-      for (uint64_t i = 0; i < k; ++i) {
-        if (!(tot[i] >= min_intensity)) continue;
-        const uint32_t packed = (uint32_t)(rec[i] & 0x3ffffffull);
-        const int ns = (int)(packed >> kSumBits);
-        h.ids[j] = (rec[i] >> 26) & 0xffffffffull;
-        h.nspots[j] = ns;
-        h.totint[j] = tot[i];
-        h.score[j] = tot[i] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
-        ++j;
-      }
+    rc = fill_hits(h, m);
+    if (rc) break;
+    uint64_t j = 0;
+    for (uint64_t i = 0; i < k; ++i) {
+      if (!(tot[i] >= min_int[b])) continue;
+      const uint32_t packed = (uint32_t)(rec[i] & 0x3ffffffull);
+      const int ns = (int)(packed >> kSumBits);
+      h.ids[j] = (rec[i] >> 26) & 0xffffffffull;
+      h.nspots[j] = ns;
+      h.totint[j] = tot[i];
+      h.score[j] = tot[i] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
+      ++j;
     }
   }
-  if (rc) {
+  if (rc)
     for (int b = 0; b < nimg; ++b) lm_hits_free(&outs[b]);
-  }
   if (d_ids) (void)hipFree(d_ids);
   if (d_exact) (void)hipFree(d_exact);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return rc;
 }
 

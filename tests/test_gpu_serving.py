@@ -1,0 +1,89 @@
+"""Continuous-batching server: each result, computed in a batch with other
+requests (different DBs, detectors, thresholds and scales, submitted from
+several threads), equals scoring that image alone; captured launch sequences
+replay correctly across many batches interleaved with eager work."""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+from lauematching_amd import api, synth
+from lauematching_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def two_dbs():
+    ctx = api.Context(0)
+    pc = synth.baseline_params(NrPxX=1024, NrPxY=1024, PxX=0.0004, PxY=0.0004)
+    hc = synth.fcc_hkls(8, "box")
+    Oc = synth.random_orientations(40000, seed=41)
+    ph = synth.baseline_params(LatticeParameter=(0.295, 0.295, 0.468165, 90, 90, 120),
+                               SpaceGroup=194)
+    hh = synth.hcp_hkls(6)
+    Oh = synth.random_orientations(30000, seed=42)
+    dbc = api.OrientationDB(ctx, Oc, hc, api.Geometry.from_params(pc), 64)
+    dbh = api.OrientationDB(ctx, Oh, hh, api.Geometry.from_params(ph), 64)
+    dbc.forward()
+    dbh.forward()
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i in range(48):
+        cubic = i % 10 < 7
+        p, hk, O, db = (pc, hc, Oc, dbc) if cubic else (ph, hh, Oh, dbh)
+        grains = O[rng.choice(len(O), int(rng.integers(1, 6)), replace=False)]
+        img = synth.synth_image(p, hk, grains, seed=int(rng.integers(1 << 30)))
+        if i % 7 == 3:
+            img = img * 0.61 + 0.01 * (img > 0)  # imScale != 1
+        u8, s = api.quantize_u8(img)
+        reqs.append((0 if cubic else 1, u8, s, int(rng.integers(3, 7)),
+                     float(rng.choice([150.0, 400.0, 900.0]) * s)))
+    return ctx, dbc, dbh, reqs
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_results_alone_equal_in_the_mix(two_dbs, graphs):
+    ctx, dbc, dbh, reqs = two_dbs
+    dbs = [dbc, dbh]
+    alone = [dbs[d].score(u8, s, ms, mi) for d, u8, s, ms, mi in reqs]
+    _say(f"server graphs={graphs}")
+    srv = Server(ctx, max_batch=8, workers=2, max_inflight=24, use_graphs=graphs)
+    ids = [srv.add_db(dbc), srv.add_db(dbh)]
+    assert ids == [0, 1]
+    results = [None] * len(reqs)
+
+    def client(k):
+        for i in range(k, len(reqs), 3):
+            d, u8, s, ms, mi = reqs[i]
+            t = srv.submit(d, u8, s, ms, mi)
+            results[i] = srv.wait(t)
+
+    for rnd in range(3):  # replays with eager work between them
+        th = [threading.Thread(target=client, args=(k,)) for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for i, (h, lat) in enumerate(results):
+            a = alone[i]
+            assert h.nr_results == a.nr_results, (rnd, i)
+            np.testing.assert_array_equal(h.ids, a.ids)
+            np.testing.assert_array_equal(h.score, a.score)
+            np.testing.assert_array_equal(h.nspots, a.nspots)
+            assert lat > 0
+        extra = dbc.score(reqs[0][1], reqs[0][2], 5, 100.0)  # eager work between rounds
+        assert extra.nr_results >= 0
+    st = srv.stats()
+    assert st["completed"] == 3 * len(reqs)
+    assert st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert st["mean_batch"] >= 1.0
+    if graphs:
+        assert st["graph_launches"] > 0
+    srv.close()
