@@ -128,6 +128,69 @@ def run_batches(db, batch_imgs, widths, p, steps, warmup):
     return out
 
 
+def precision_check(db, fp64_ms):
+    """Rebuild the table with the FP32 projection + FP64 guard band and check
+    it is the FP64 table (64-bit position-tagged checksum over counts and
+    spots); report the recompute fraction."""
+    from lauematching_amd import api
+
+    c64 = db.checksum()
+    st = db.forward(api.LM_FWD_FP32_GUARD)
+    c32 = db.checksum()
+    out = {"fp64_forward_ms": fp64_ms, "fp32_guard_forward_ms": st.ms,
+           "speedup": fp64_ms / st.ms if st.ms else None,
+           "pairs": int(st.n_pairs), "recomputed": int(st.n_recomputed),
+           "recompute_fraction": st.n_recomputed / st.n_pairs,
+           "table_checksum_equal": bool(c64 == c32)}
+    log(f"precision: FP32+guard forward {st.ms:.1f} ms vs FP64 {fp64_ms:.1f} ms, "
+        f"recomputed {st.n_recomputed / st.n_pairs:.4%}, tables equal={c64 == c32}")
+    return out
+
+
+def config4(ctx, O, n, steps, batch_imgs):
+    """Config 4: FCC |hkl| <= 10 (1066 reflections), 5-40 keV; FP64 vs
+    FP32+guard forward; scoring a batch of 32 with the table resident versus
+    regenerated (FP32+guard) per batch."""
+    from lauematching_amd import api, synth
+
+    p = synth.baseline_params(Ehi=40.0, MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(10, "sphere")
+    db = api.OrientationDB(ctx, O, hk, api.Geometry.from_params(p), 64)
+    s64 = db.forward(api.LM_FWD_FP64)
+    c64 = db.checksum()
+    s32 = db.forward(api.LM_FWD_FP32_GUARD)
+    c32 = db.checksum()
+    res = {"hkls": int(len(hk)), "fp64_forward_ms": s64.ms, "fp32_guard_forward_ms": s32.ms,
+           "recompute_fraction": s32.n_recomputed / s32.n_pairs,
+           "table_checksum_equal": bool(c64 == c32),
+           "spots_per_orientation": s32.n_spots / n}
+    if batch_imgs:
+        import torch
+        B = min(32, len(batch_imgs))
+        u8s, scales = zip(*[api.quantize_u8(im) for im, _ in batch_imgs[:B]])
+        dev = torch.from_numpy(np.stack(u8s)).to("cuda:0")
+        torch.cuda.synchronize()
+        db.score_batch_dev(dev.data_ptr(), B, scales, 5, 1000.0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            db.score_batch_dev(dev.data_ptr(), B, scales, 5, 1000.0)
+        torch.cuda.synchronize()
+        ms_res = (time.perf_counter() - t0) / steps * 1e3
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps // 3)):
+            db.forward(api.LM_FWD_FP32_GUARD)
+            db.score_batch_dev(dev.data_ptr(), B, scales, 5, 1000.0)
+        torch.cuda.synchronize()
+        ms_regen = (time.perf_counter() - t0) / max(1, steps // 3) * 1e3
+        res.update({"batch": B, "resident_ms_per_batch": ms_res,
+                    "resident_pairs_per_s": n * B / (ms_res / 1e3),
+                    "regenerated_ms_per_batch": ms_regen,
+                    "regenerated_pairs_per_s": n * B / (ms_regen / 1e3)})
+    log(f"config4: {res}")
+    db.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +207,10 @@ def main():
                     help="super-tile size (log2) of the pixel-banded table; 0 = none")
     ap.add_argument("--batch", type=str, default="",
                     help="comma-separated batch widths to measure as well, e.g. 8,16,32,64")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32+guard rebuild of the table")
+    ap.add_argument("--config4", action="store_true",
+                    help="also run the |hkl|<=10, 5-40 keV precision configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -151,9 +218,11 @@ def main():
         sys.exit(2)
     widths = [int(b) for b in args.batch.split(",") if b]
     batch_imgs = []
-    if widths:
+    need = max(widths + ([32] if args.config4 else []), default=0)
+    if need:
+        # worker processes are started before anything touches the GPU
         t0 = time.time()
-        batch_imgs = make_batch_images(max(widths), args.orients, args.seed)
+        batch_imgs = make_batch_images(need, args.orients, args.seed)
         log(f"generated {len(batch_imgs)} batch images in {time.time() - t0:.1f} s")
 
     import torch
@@ -216,6 +285,10 @@ def main():
             traffic = None
 
     batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1) if widths else None
+    precision = None if args.no_precision else precision_check(db, st.ms)
+    c4 = None
+    if args.config4:
+        c4 = config4(ctx, O, n, max(3, args.steps // 2), batch_imgs)
 
     cpu, parity_ok = None, None
     if not args.no_cpu:
@@ -254,6 +327,10 @@ def main():
         for b in batched:
             b["speedup_vs_plain"] = b["pairs_per_s"] / value
         line["batched"] = batched
+    if precision:
+        line["precision"] = precision
+    if c4:
+        line["config4"] = c4
     print(json.dumps(line), flush=True)
 
 

@@ -141,6 +141,12 @@ int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats);
  * detector are rejected with LM_ERR_INVALID. */
 int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count);
 
+/* 64-bit position-tagged checksum of the table (counts and spot lists in
+ * table order), for comparing tables built different ways (e.g. LM_FWD_FP64
+ * vs LM_FWD_FP32_GUARD) at sizes where exporting them is impractical.  No
+ * reference counterpart. */
+int lm_db_table_checksum(lm_db *db, uint64_t *sum);
+
 /* Write rows [first, first+count) as ForwardFile records (zero padded),
  * byte-identical to the reference writer. */
 int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out);

@@ -155,6 +155,12 @@ class OrientationDB:
         check(lib().lm_db_forward(self._h, int(mode), ctypes.byref(st)))
         return st
 
+    def checksum(self) -> int:
+        """64-bit position-tagged checksum of the resident table."""
+        s = ctypes.c_uint64()
+        check(lib().lm_db_table_checksum(self._h, ctypes.byref(s)))
+        return s.value
+
     def import_table(self, records: np.ndarray) -> None:
         r = np.ascontiguousarray(records, dtype=np.uint16).reshape(-1, 1 + 2 * self.max_spots)
         check(lib().lm_db_import_table(self._h, _ptr(r), r.shape[0]))

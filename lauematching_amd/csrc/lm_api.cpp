@@ -375,6 +375,17 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
   return LM_ERR_INVALID;
 }
 
+int lm_db_table_checksum(lm_db *db, uint64_t *sum) {
+  if (!db || !sum) {
+    set_error("lm_db_table_checksum: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  return table_checksum(db, sum);
+}
+
 int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count) {
   if (!db) {
     set_error("null db");

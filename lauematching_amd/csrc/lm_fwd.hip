@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -84,16 +85,111 @@ __device__ __forceinline__ bool project64(const double *M, double h, double k, d
   return true;
 }
 
+// Float32 geometry for the guarded path.
+struct DevGeom32 {
+  float rotT[9];
+  float P[3];
+  float inv_pxX, inv_pxY;  // 1/pitch (the bound covers the reciprocal's rounding)
+  float halfX, halfY, W, H;
+  float Elo, Ehi;
+  float dist_px;           // max(P[2]/pxX, P[2]/pxY): pixels per unit of x/z
+  float P0px, P1px;        // |P[0]|/pxX, |P[1]|/pxY
+};
+
+// Classification of one detector coordinate t against the reference's
+// decisions: trunc(t) for -1 < t < N.  The decision points are -1, the
+// integers k >= 1 (where trunc changes; trunc is 0 on (-1, 1)) and N.
+// Returns 0 = clearly rejected, 1 = clearly accepted (*ip set), 2 = within e
+// of a decision point.
+__device__ __forceinline__ int classify_px(float t, float n, float e, int *ip) {
+  if (t <= -1.0f - e || t >= n + e) return 0;
+  float d;
+  if (t >= 1.0f)
+    d = fminf(t - floorf(t), ceilf(t) - t);
+  else
+    d = fminf(fabsf(t + 1.0f), 1.0f - t);
+  if (d <= e) return 2;
+  *ip = (int)t;
+  return 1;
+}
+
+// Float32 evaluation of one (orientation, hkl) with a forward-error bound
+// (Higham-style, constants carrying a 2x slack over the derivation below).
+// u = 2^-24.  With S = max row of sum |M_ij||h_j| and r = S/|q|:
+//   q-hat components: <= 8u(1 + r) absolute      (q: gamma_4 S; /|q|)
+//   kf = e_z - 2 qz q-hat: <= 32u(1 + r)
+//   lab xyz = rotT kf: e_xyz <= 2 e_kf + 8u       (row sums of |rotT| <= sqrt 3)
+//   t = (x P2/z - P0)/px + (N-1)/2:
+//     e_t <= 2 [ (P2/px) e_xyz (1/|z| + |x|/z^2) + 8u (|t| + (N-1)/2 + |x P2/z|/px + |P0|/px) ]
+//   E = hc |q| / (4 pi (-qz_hat)): relative e_E <= 2 [ 8u(1 + r) + e_kf/|qz_hat| + 8u ]
+// Returns 0 = reject, 1 = accept (*pix set), 2 = undecided (recompute in FP64).
+__device__ __forceinline__ int project32(const float *M, int hi, int ki, int li,
+                                         const DevGeom32 &g, uint32_t nrPxX, uint32_t *pix) {
+  constexpr float U = 5.9604645e-8f;
+  const float h = (float)hi, k = (float)ki, l = (float)li;
+  const float q0 = M[0] * h + M[1] * k + M[2] * l;
+  const float q1 = M[3] * h + M[4] * k + M[5] * l;
+  const float q2 = M[6] * h + M[7] * k + M[8] * l;
+  const float s0 = fabsf(M[0] * h) + fabsf(M[1] * k) + fabsf(M[2] * l);
+  const float s1 = fabsf(M[3] * h) + fabsf(M[4] * k) + fabsf(M[5] * l);
+  const float s2 = fabsf(M[6] * h) + fabsf(M[7] * k) + fabsf(M[8] * l);
+  const float ql2 = q0 * q0 + q1 * q1 + q2 * q2;
+  if (hi == 0 && ki == 0 && li == 0) return 0;  // qlen == 0 exactly
+  if (!(ql2 > 0.0f)) return 2;
+  const float ql = sqrtf(ql2);
+  const float r = fmaxf(s0, fmaxf(s1, s2)) / ql;
+  const float u0 = q0 / ql, u1 = q1 / ql, u2 = q2 / ql;
+  const float td = 2.0f * u2;
+  const float k0 = -td * u0, k1 = -td * u1, k2 = 1.0f - td * u2;
+  const float x = g.rotT[0] * k0 + g.rotT[1] * k1 + g.rotT[2] * k2;
+  const float y = g.rotT[3] * k0 + g.rotT[4] * k1 + g.rotT[5] * k2;
+  const float z = g.rotT[6] * k0 + g.rotT[7] * k1 + g.rotT[8] * k2;
+  const float ekf = 32.0f * U * (1.0f + r);
+  const float exyz = 2.0f * ekf + 8.0f * U;
+  if (z < -exyz) return 0;
+  if (z <= exyz) return 2;
+  int verdict = 1;
+  // energy window (inclusive)
+  const float su = -u2;
+  if (fabsf(su) <= ekf) {
+    verdict = 2;
+  } else {
+    constexpr float kHc4Pi = (float)(kHc / kFourPi);  // hc / (4 pi), keV nm
+    const float E = kHc4Pi * ql / su;
+    const float eE = 2.0f * (8.0f * U * (1.0f + r) + ekf / fabsf(su) + 8.0f * U) * fabsf(E) +
+                     1e-6f * fabsf(E);
+    if (E < g.Elo - eE || E > g.Ehi + eE) return 0;
+    if (fabsf(E - g.Elo) <= eE || fabsf(E - g.Ehi) <= eE) verdict = 2;
+  }
+  const float iz = 1.0f / z;
+  const float X = x * g.P[2] * iz, Y = y * g.P[2] * iz;
+  const float tx = (X - g.P[0]) * g.inv_pxX + g.halfX;
+  const float ty = (Y - g.P[1]) * g.inv_pxY + g.halfY;
+  const float amp = g.dist_px * exyz * (iz + fmaxf(fabsf(x), fabsf(y)) * iz * iz);
+  const float etx = 2.0f * (amp + 8.0f * U * (fabsf(tx) + g.halfX + fabsf(X) * g.inv_pxX + g.P0px));
+  const float ety = 2.0f * (amp + 8.0f * U * (fabsf(ty) + g.halfY + fabsf(Y) * g.inv_pxY + g.P1px));
+  int ipx = 0, ipy = 0;
+  const int cx = classify_px(tx, g.W, etx, &ipx);
+  if (cx == 0) return 0;
+  const int cy = classify_px(ty, g.H, ety, &ipy);
+  if (cy == 0) return 0;
+  if (cx == 2 || cy == 2 || verdict == 2) return 2;
+  *pix = (uint32_t)ipy * nrPxX + (uint32_t)ipx;
+  return 1;
+}
+
 // One lane per orientation.  WRITE=false: counts[o] = number of spots.
 // WRITE=true: spots of orientation o go to entries[tile_off[tile] + in-tile
-// prefix of counts], in spot order.
-template <bool WRITE, bool LDS_LIST>
+// prefix of counts], in spot order.  GUARD: float32 projection, float64
+// recompute of undecided pairs (counted in *recomputed by the count pass).
+template <bool WRITE, bool LDS_LIST, bool GUARD>
 __global__ void __launch_bounds__(kFwdBlock)
 fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__restrict__ hkls,
            int nhkls, double r00, double r01, double r02, double r10, double r11, double r12,
-           double r20, double r21, double r22, DevGeom g, int maxSpots, int tileBits,
-           uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
-           uint32_t *__restrict__ entries, uint32_t *__restrict__ spill) {
+           double r20, double r21, double r22, DevGeom g, DevGeom32 g32, int maxSpots,
+           int tileBits, uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+           uint32_t *__restrict__ entries, uint32_t *__restrict__ spill,
+           unsigned long long *__restrict__ recomputed) {
   __shared__ uint32_t lst[LDS_LIST ? kLdsSlots * kFwdBlock : 1];
   __shared__ uint32_t scan[kFwdBlock];
   const uint64_t o = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
@@ -111,17 +207,33 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
       M[r * 3 + 2] = t[r * 3 + 0] * r02 + t[r * 3 + 1] * r12 + t[r * 3 + 2] * r22;
     }
   }
+  float M32[9];
+  if (GUARD) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M32[i] = (float)M[i];
+  }
+  uint32_t nrec = 0;
   uint32_t *col = LDS_LIST ? lst + threadIdx.x : spill + o;
   const uint64_t colStride = LDS_LIST ? (uint64_t)kFwdBlock : (uint64_t)gridDim.x * kFwdBlock;
   int ns = live ? 0 : maxSpots;
   for (int hi = 0; hi < nhkls; ++hi) {
     if (__all(ns >= maxSpots)) break;
-    const double h = (double)hkls[3 * hi + 0];
-    const double k = (double)hkls[3 * hi + 1];
-    const double l = (double)hkls[3 * hi + 2];
+    const int hI = hkls[3 * hi + 0], kI = hkls[3 * hi + 1], lI = hkls[3 * hi + 2];
     if (ns < maxSpots) {
-      uint32_t pix;
-      if (project64(M, h, k, l, g, &pix)) {
+      uint32_t pix = 0;
+      bool ok;
+      if (GUARD) {
+        const int v = project32(M32, hI, kI, lI, g32, (uint32_t)g.nrPxX, &pix);
+        if (v == 2) {
+          ok = project64(M, (double)hI, (double)kI, (double)lI, g, &pix);
+          ++nrec;
+        } else {
+          ok = v == 1;
+        }
+      } else {
+        ok = project64(M, (double)hI, (double)kI, (double)lI, g, &pix);
+      }
+      if (ok) {
         bool dup = false;
         for (int s = 0; s < ns; ++s)
           if (col[(uint64_t)s * colStride] == pix) { dup = true; break; }
@@ -134,6 +246,12 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   }
   if (!WRITE) {
     if (live) counts[o] = (uint16_t)ns;
+    if (GUARD) {
+      uint32_t w = nrec;
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) w += __shfl_xor(w, d);
+      if (__lane_id() == 0 && w) atomicAdd(recomputed, (unsigned long long)w);
+    }
     return;
   }
   // In-tile offset of this orientation: counts of the tile's rows before this
@@ -244,6 +362,30 @@ __global__ void export_kernel(uint16_t *__restrict__ rec, uint64_t rows, uint64_
   }
 }
 
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Position-tagged sum (mod 2^64) over the counts and the orientation-major
+// entries: equal tables give equal sums, and any changed, moved or missing
+// value changes the sum except with probability ~2^-64.
+__global__ void checksum_kernel(const uint16_t *__restrict__ counts, uint64_t n,
+                                const uint32_t *__restrict__ entries, uint64_t total,
+                                unsigned long long *__restrict__ out) {
+  uint64_t acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc += mix64((i << 16) ^ counts[i]);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride)
+    acc += mix64(~(i << 32) ^ entries[i]);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d);
+  if (__lane_id() == 0) atomicAdd(out, (unsigned long long)acc);
+}
+
 __global__ void selftest_kernel(const double *__restrict__ a, const double *__restrict__ b,
                                 uint64_t n, double *__restrict__ q, double *__restrict__ s) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -265,6 +407,24 @@ DevGeom dev_geom(const Geom &g) {
   d.Elo = g.Elo;
   d.Ehi = g.Ehi;
   d.nrPxX = g.nrPxX;
+  return d;
+}
+
+DevGeom32 dev_geom32(const Geom &g) {
+  DevGeom32 d{};
+  for (int i = 0; i < 9; ++i) d.rotT[i] = (float)g.rotT[i];
+  for (int i = 0; i < 3; ++i) d.P[i] = (float)g.P[i];
+  d.inv_pxX = (float)(1.0 / g.pxX);
+  d.inv_pxY = (float)(1.0 / g.pxY);
+  d.halfX = (float)(0.5 * (g.nrPxX - 1));
+  d.halfY = (float)(0.5 * (g.nrPxY - 1));
+  d.W = (float)g.nrPxX;
+  d.H = (float)g.nrPxY;
+  d.Elo = (float)g.Elo;
+  d.Ehi = (float)g.Ehi;
+  d.dist_px = (float)std::max(std::fabs(g.P[2]) / g.pxX, std::fabs(g.P[2]) / g.pxY);
+  d.P0px = (float)(std::fabs(g.P[0]) / g.pxX);
+  d.P1px = (float)(std::fabs(g.P[1]) / g.pxY);
   return d;
 }
 
@@ -357,24 +517,28 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
     set_error("unknown forward mode");
     return LM_ERR_INVALID;
   }
-  if (mode == LM_FWD_FP32_GUARD) {
-    set_error("LM_FWD_FP32_GUARD is not available in this build");
-    return LM_ERR_INVALID;
-  }
+  const bool guard = mode == LM_FWD_FP32_GUARD;
   hipStream_t st = db->ctx->stream;
   if (db->table.counts) free_table(db);
   int rc = alloc_table_index(db);
   if (rc) return rc;
   Table &t = db->table;
   const DevGeom g = dev_geom(db->geom);
+  const DevGeom32 g32 = dev_geom32(db->geom);
   const double *R = db->geom.recip;
   const bool ldsList = db->max_spots <= kLdsSlots;
   // Grid chunks keep each launch under 2^31 threads and bound the spill list.
   const uint64_t chunkBlocks = 1u << 18;  // 32M orientations per launch
   uint32_t *spill = nullptr;
-  if (!ldsList) {
-    hipError_t e = hipMalloc(&spill, chunkBlocks * kFwdBlock * (uint64_t)db->max_spots * 4);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(forward spill list)");
+  unsigned long long *d_rec = nullptr;
+  hipError_t e = hipMalloc(&d_rec, 8);
+  if (e == hipSuccess) e = hipMemsetAsync(d_rec, 0, 8, st);
+  if (e == hipSuccess && !ldsList)
+    e = hipMalloc(&spill, chunkBlocks * kFwdBlock * (uint64_t)db->max_spots * 4);
+  if (e != hipSuccess) {
+    if (d_rec) (void)hipFree(d_rec);
+    free_table(db);
+    return hip_fail(e, "hipMalloc(forward scratch)");
   }
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
@@ -393,29 +557,24 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
       // of the tile size), so the in-tile scan never crosses a chunk.
       const double *op = db->d_orients + first * 9;
       uint16_t *cp = t.counts + first;
-      const uint64_t tileBase = first >> t.tile_bits;
-      const uint64_t *tp = t.tile_off + tileBase;
-      if (pass == 0) {
-        if (ldsList)
-          hipLaunchKernelGGL((fwd_kernel<false, true>), dim3(blocks), dim3(kFwdBlock), 0, st, op,
-                             rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6],
-                             R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries, spill);
-        else
-          hipLaunchKernelGGL((fwd_kernel<false, false>), dim3(blocks), dim3(kFwdBlock), 0, st,
-                             op, rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5],
-                             R[6], R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries,
-                             spill);
-      } else {
-        if (ldsList)
-          hipLaunchKernelGGL((fwd_kernel<true, true>), dim3(blocks), dim3(kFwdBlock), 0, st, op,
-                             rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6],
-                             R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries, spill);
-        else
-          hipLaunchKernelGGL((fwd_kernel<true, false>), dim3(blocks), dim3(kFwdBlock), 0, st, op,
-                             rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6],
-                             R[7], R[8], g, db->max_spots, t.tile_bits, cp, tp, t.entries, spill);
+      const uint64_t *tp = t.tile_off + (first >> t.tile_bits);
+#define LM_FWD(W, L, G)                                                                         \
+  hipLaunchKernelGGL((fwd_kernel<W, L, G>), dim3(blocks), dim3(kFwdBlock), 0, st, op, rows,      \
+                     db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], \
+                     g, g32, db->max_spots, t.tile_bits, cp, tp, t.entries, spill, d_rec)
+      const int sel = (pass << 2) | (ldsList << 1) | (guard ? 1 : 0);
+      switch (sel) {
+        case 0: LM_FWD(false, false, false); break;
+        case 1: LM_FWD(false, false, true); break;
+        case 2: LM_FWD(false, true, false); break;
+        case 3: LM_FWD(false, true, true); break;
+        case 4: LM_FWD(true, false, false); break;
+        case 5: LM_FWD(true, false, true); break;
+        case 6: LM_FWD(true, true, false); break;
+        default: LM_FWD(true, true, true); break;
       }
-      hipError_t e = hipGetLastError();
+#undef LM_FWD
+      e = hipGetLastError();
       if (e != hipSuccess) {
         rc = hip_fail(e, "forward kernel launch");
         break;
@@ -423,8 +582,11 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
     }
   }
   (void)hipEventRecord(e1, st);
-  hipError_t e = hipStreamSynchronize(st);
+  unsigned long long nrec = 0;
+  e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipMemcpy(&nrec, d_rec, 8, hipMemcpyDeviceToHost);
   if (spill) (void)hipFree(spill);
+  (void)hipFree(d_rec);
   if (rc == LM_OK && e != hipSuccess) rc = hip_fail(e, "forward simulation");
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
@@ -443,8 +605,8 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
   if (stats) {
     stats->n_orients = db->n;
     stats->n_spots = t.total;
-    stats->n_pairs = 2ull * db->n * (uint64_t)db->nhkls;
-    stats->n_recomputed = 0;
+    stats->n_pairs = db->n * (uint64_t)db->nhkls;
+    stats->n_recomputed = nrec;
     stats->ms = ms;
   }
   return LM_OK;
@@ -547,6 +709,30 @@ int export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out) {
 
 void release_table(lm_db *db) {
   if (db->table.counts) free_table(db);
+}
+
+int table_checksum(lm_db *db, uint64_t *sum) {
+  const Table &t = db->table;
+  if (!t.ready) {
+    set_error("checksum: no table");
+    return LM_ERR_STATE;
+  }
+  hipStream_t st = db->ctx->stream;
+  unsigned long long *d = nullptr;
+  hipError_t e = hipMalloc(&d, 8);
+  if (e == hipSuccess) e = hipMemsetAsync(d, 0, 8, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(checksum_kernel, dim3(4096), dim3(256), 0, st, t.counts, db->n, t.entries,
+                       t.total, d);
+    e = hipGetLastError();
+  }
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (d) (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "table checksum");
+  *sum = h;
+  return LM_OK;
 }
 
 int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, double *q,

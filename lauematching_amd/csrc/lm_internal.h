@@ -156,6 +156,7 @@ int export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out);
 void release_table(lm_db *db);
 int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, double *q,
                   double *s);
+int table_checksum(lm_db *db, uint64_t *sum);
 
 // lm_band.hip
 int build_banded(lm_db *db);

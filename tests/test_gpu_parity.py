@@ -77,6 +77,59 @@ def test_forward_table_bit_identical(ctx, geo, M):
     np.testing.assert_array_equal(db.export_table(1234, 999), want[1234:1234 + 999])
 
 
+@pytest.mark.parametrize("geo", list(GEOMS))
+def test_forward_fp32_guard_bit_identical(ctx, geo):
+    """FP32 projection with the FP64 guard band: the same table as FP64 and as
+    the oracle; a small fraction of pairs is recomputed."""
+    p = synth.baseline_params(MaxNrLaueSpots=64, **GEOMS[geo])
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.random_orientations(30000, seed=13)
+    _say(f"forward fp32-guard geo={geo}")
+    db = _db(ctx, p, O, hk)
+    st = db.forward(api.LM_FWD_FP32_GUARD)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    np.testing.assert_array_equal(db.export_table(), want)
+    frac = st.n_recomputed / st.n_pairs
+    _say(f"  recomputed {st.n_recomputed} of {st.n_pairs} pairs ({frac:.4%})")
+    assert 0 < st.n_recomputed and frac < 0.05
+    c32 = db.checksum()
+    db.forward(api.LM_FWD_FP64)
+    assert db.checksum() == c32
+
+
+def test_forward_fp32_guard_config4_hkls(ctx):
+    """400-reflection-class list (FCC |hkl| <= 10, 1066 rows), 5-40 keV."""
+    p = synth.baseline_params(Ehi=40.0, MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(10, "sphere")
+    O = synth.random_orientations(30000, seed=14)
+    _say("forward fp32-guard |hkl|<=10, 5-40 keV")
+    db = _db(ctx, p, O, hk)
+    db.forward(api.LM_FWD_FP32_GUARD)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    np.testing.assert_array_equal(db.export_table(), want)
+
+
+def test_checksum_detects_a_changed_spot(ctx):
+    p = synth.baseline_params(NrPxX=256, NrPxY=256, PxX=0.0016, PxY=0.0016, MaxNrLaueSpots=16)
+    O = synth.random_orientations(5000, seed=15)
+    hk = synth.fcc_hkls(6)
+    db = _db(ctx, p, O, hk)
+    db.forward()
+    rec = db.export_table()
+    c0 = db.checksum()
+    i = int(np.nonzero(rec[:, 0] > 1)[0][0])
+    rec2 = rec.copy()
+    rec2[i, 1] = (int(rec2[i, 1]) + 1) % 256
+    db.import_table(rec2)
+    assert db.checksum() != c0
+    rec3 = rec.copy()  # swap two spots of one orientation: same set, other order
+    rec3[i, 1:3], rec3[i, 3:5] = rec[i, 3:5].copy(), rec[i, 1:3].copy()
+    db.import_table(rec3)
+    assert db.checksum() != c0
+    db.import_table(rec)
+    assert db.checksum() == c0
+
+
 def test_forward_hexagonal(ctx):
     p = synth.baseline_params(LatticeParameter=(0.295, 0.295, 0.468165, 90, 90, 120),
                               SpaceGroup=194, MaxNrLaueSpots=64)
