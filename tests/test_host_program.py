@@ -59,6 +59,75 @@ def _hits_file(path):
             np.array([float(r[3]) for r in rows]))
 
 
+def _free_port():
+    import socket
+    s = socket.socket()
+    s.bind(("127.0.0.1", 0))
+    port = s.getsockname()[1]
+    s.close()
+    return port
+
+
+@pytest.mark.gpu
+def test_stream_daemon_matches_single_image_program(tmp_path):
+    """bin/LaueMatchingGPUStream: wire v1 frames (uint16 LE image number +
+    float32 pixels), SIGTERM drain, ImageNr-prefixed rows equal to the
+    single-image program's rows for the same images."""
+    import signal
+    import socket
+    import struct
+    import time
+
+    p, hk, O, img = _case(tmp_path)
+    p.extra = {"MinGoodSpots": p.MinNrSpots, "ResultDir": str(tmp_path / "res")}
+    p.DoFwd = 1
+    p.write(str(tmp_path / "params.txt"))
+    rng = np.random.default_rng(2)
+    frames = {1: img}
+    for num in (2, 7):
+        g = O[rng.choice(len(O), 3, replace=False)]
+        frames[num] = synth.synth_image(p, hk, g, seed=num)
+    port = _free_port()
+    env = dict(os.environ, LM_STREAM_PORT=str(port))
+    exe = os.path.join(ROOT, "bin", "LaueMatchingGPUStream")
+    proc = subprocess.Popen([exe, str(tmp_path / "params.txt"), str(tmp_path / "orients.bin"),
+                             str(tmp_path / "hkls.csv"), "4"], stdout=subprocess.PIPE,
+                            stderr=subprocess.STDOUT, text=True, env=env)
+    try:
+        deadline = time.time() + 120
+        sock = None
+        while time.time() < deadline:
+            try:
+                sock = socket.create_connection(("127.0.0.1", port), timeout=2)
+                break
+            except OSError:
+                assert proc.poll() is None, proc.stdout.read()
+                time.sleep(0.3)
+        assert sock is not None
+        for num, im in frames.items():
+            sock.sendall(struct.pack("<H", num) + im.astype(np.float32).tobytes())
+        sock.close()
+        time.sleep(2.0)
+        proc.send_signal(signal.SIGTERM)
+        out, _ = proc.communicate(timeout=120)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+    assert proc.returncode == 0, out
+    assert "exited cleanly" in out
+    rows = [l.rstrip("\n").split("\t") for l in open(tmp_path / "res" / "solutions.txt")]
+    assert rows[0][0] == "%ImageNr"
+    got = {}
+    for r in rows[1:]:
+        got.setdefault(int(r[0]), []).append(r[1:])
+    p.DoFwd = 0
+    for num, im in frames.items():
+        synth.write_image(str(tmp_path / "image.bin"), im)
+        _run(tmp_path, p)
+        want = [l.rstrip("\n").split("\t") for l in open(tmp_path / "image.bin.solutions.txt")][1:]
+        assert got.get(num, []) == want, num
+
+
 @pytest.mark.gpu
 def test_end_to_end_dofwd1_then_dofwd0(tmp_path):
     p, hk, O, img = _case(tmp_path)
