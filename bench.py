@@ -128,6 +128,126 @@ def run_batches(db, batch_imgs, widths, p, steps, warmup):
     return out
 
 
+HEX_LAT = (0.295, 0.295, 0.468165, 90.0, 90.0, 120.0)
+
+
+def serve_params(crystal, det):
+    from lauematching_amd import synth
+
+    over = {}
+    if det == 1024:
+        over.update(NrPxX=1024, NrPxY=1024, PxX=0.0004, PxY=0.0004)
+    if crystal == "hex":
+        over.update(LatticeParameter=HEX_LAT, SpaceGroup=194)
+    return synth.baseline_params(**over)
+
+
+def _serve_image(args):
+    crystal, det, n, db_seed, i = args
+    from lauematching_amd import synth
+
+    p = serve_params(crystal, det)
+    hk = synth.fcc_hkls(8, "box") if crystal == "fcc" else synth.hcp_hkls(6)
+    rng = np.random.default_rng([11, i, det, crystal == "hex"])
+    rows = rng.choice(n, int(rng.integers(1, 11)), replace=False)
+    ops = None if crystal == "fcc" else synth.hexagonal_ops()
+    img = synth.synth_image(p, hk, synth.orientation_rows(db_seed, rows, ops),
+                            seed=int(rng.integers(1 << 30)))
+    return crystal, det, img
+
+
+def make_serve_pool(combos, per, workers=16):
+    """Images per (crystal, detector, DB size, DB seed), made in worker
+    processes before anything touches the GPU."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    jobs = [(c, d, n, s, i) for (c, d, n, s) in combos for i in range(per)]
+    with ProcessPoolExecutor(min(workers, len(jobs))) as ex:
+        return list(ex.map(_serve_image, jobs))
+
+
+def poisson_run(srv, reqs, rate, duration, seed):
+    """Submit requests at Poisson arrival times (rate per second) from one
+    thread while another collects results; returns the server's latency
+    statistics for the run."""
+    import queue
+    import threading
+
+    srv.reset_stats()
+    rng = np.random.default_rng(seed)
+    nreq = max(20, int(rate * duration))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    picks = rng.integers(0, len(reqs), nreq)
+    tickets = queue.Queue()
+
+    def collect():
+        for _ in range(nreq):
+            h, _ = srv.wait(tickets.get())
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for k in range(nreq):
+        dt = t0 + arrivals[k] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        db_id, u8, s = reqs[picks[k]]
+        tickets.put(srv.submit(db_id, u8, s, 5, 1000.0))
+    th.join()
+    st = srv.stats()
+    return {"rate": rate, "requests": nreq, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "max_ms": st["max_ms"], "images_per_s": st["images_per_s"],
+            "mean_batch": st["mean_batch"]}
+
+
+def closed_loop(srv, reqs, count):
+    """As many requests as the admission slots allow, back to back."""
+    import threading
+
+    srv.reset_stats()
+    out = []
+
+    def collect(ts):
+        for t in ts:
+            out.append(srv.wait(t))
+
+    tickets = []
+    t0 = time.perf_counter()
+    for k in range(count):
+        db_id, u8, s = reqs[k % len(reqs)]
+        tickets.append(srv.submit(db_id, u8, s, 5, 1000.0))
+        if len(tickets) == 64:
+            collect(tickets)
+            tickets = []
+    collect(tickets)
+    dt = time.perf_counter() - t0
+    st = srv.stats()
+    return {"images": count, "images_per_s": count / dt, "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]}
+
+
+def run_serving(ctx, dbs, reqs, rates, duration, nmax):
+    """Continuous batching versus a plain server (one image per launch, one
+    stream: the reference daemon's serialised GPU gate)."""
+    from lauematching_amd.serve import Server
+
+    res = {}
+    for name, kw in [("batched", dict(max_batch=32, workers=2, use_graphs=True)),
+                     ("plain", dict(max_batch=1, workers=1, use_graphs=False))]:
+        srv = Server(ctx, **kw)
+        ids = [srv.add_db(db) for db in dbs]
+        rq = [(ids[i], u8, s) for i, u8, s in reqs]
+        runs = [poisson_run(srv, rq, r, duration, 100 + j) for j, r in enumerate(rates)]
+        cl = closed_loop(srv, rq, nmax)
+        st = srv.stats()
+        srv.close()
+        res[name] = {"poisson": runs, "closed_loop": cl, "graph_launches": st["graph_launches"]}
+        log(f"serving {name}: " + ", ".join(
+            f"{r['rate']}/s p50 {r['p50_ms']:.1f} p99 {r['p99_ms']:.1f} ms" for r in runs) +
+            f"; closed loop {cl['images_per_s']:.0f} img/s (batch {cl['mean_batch']:.1f})")
+    return res
+
+
 def precision_check(db, fp64_ms):
     """Rebuild the table with the FP32 projection + FP64 guard band and check
     it is the FP64 table (64-bit position-tagged checksum over counts and
@@ -211,6 +331,11 @@ def main():
                     help="skip the FP32+guard rebuild of the table")
     ap.add_argument("--config4", action="store_true",
                     help="also run the |hkl|<=10, 5-40 keV precision configuration")
+    ap.add_argument("--serve", action="store_true",
+                    help="also run the full serving configuration (two crystals x two "
+                         "detectors resident, Poisson rates 5..200 images/s)")
+    ap.add_argument("--no-latency", action="store_true",
+                    help="skip the default single-DB serving latency measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -224,6 +349,16 @@ def main():
         t0 = time.time()
         batch_imgs = make_batch_images(need, args.orients, args.seed)
         log(f"generated {len(batch_imgs)} batch images in {time.time() - t0:.1f} s")
+    n_hex = args.orients // 2
+    serve_pool = []
+    if args.serve or not args.no_latency:
+        combos = [("fcc", 2048, args.orients, args.seed)]
+        if args.serve:
+            combos += [("fcc", 1024, args.orients, args.seed),
+                       ("hex", 2048, n_hex, args.seed + 7), ("hex", 1024, n_hex, args.seed + 7)]
+        t0 = time.time()
+        serve_pool = make_serve_pool(combos, 8)
+        log(f"generated {len(serve_pool)} serving images in {time.time() - t0:.1f} s")
 
     import torch
     from lauematching_amd import api, synth
@@ -285,6 +420,40 @@ def main():
             traffic = None
 
     batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1) if widths else None
+
+    latency = None
+    if not args.no_latency:
+        reqs = [(0, *api.quantize_u8(img)) for c, d, img in serve_pool if (c, d) == ("fcc", 2048)]
+        latency = run_serving(ctx, [db], reqs, [50, 200], 3.0, 128)
+    serving = None
+    if args.serve:
+        from lauematching_amd import synth as sy
+
+        t0 = time.time()
+        dbs = {("fcc", 2048): db}
+        Ohex = sy.random_orientations_reduced(n_hex, args.seed + 7, sy.hexagonal_ops())
+        for cr, det in [("fcc", 1024), ("hex", 2048), ("hex", 1024)]:
+            pp = serve_params(cr, det)
+            hk2 = sy.fcc_hkls(8, "box") if cr == "fcc" else sy.hcp_hkls(6)
+            d2 = api.OrientationDB(ctx, O if cr == "fcc" else Ohex, hk2,
+                                   api.Geometry.from_params(pp), 64)
+            d2.forward(api.LM_FWD_FP32_GUARD)
+            dbs[(cr, det)] = d2
+        keys = list(dbs)
+        log(f"serving DBs ready in {time.time() - t0:.1f} s; HBM in use "
+            f"{ctx.bytes_in_use() / 1e9:.1f} GB")
+        # 70% 2048x2048, 30% 1024x1024; crystals alternate
+        reqs = []
+        for c, d, img in serve_pool:
+            u8, s = api.quantize_u8(img)
+            reps = 7 if d == 2048 else 3
+            reqs += [(keys.index((c, d)), u8, s)] * reps
+        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [5, 20, 50, 100, 200], 4.0, 256)
+        serving["dbs"] = [{"crystal": c, "detector": d, "orientations": dbs[(c, d)].n}
+                          for c, d in keys]
+        serving["hbm_in_use_gb"] = ctx.bytes_in_use() / 1e9
+        for k in keys[1:]:
+            dbs[k].close()
     precision = None if args.no_precision else precision_check(db, st.ms)
     c4 = None
     if args.config4:
@@ -327,6 +496,13 @@ def main():
         for b in batched:
             b["speedup_vs_plain"] = b["pairs_per_s"] / value
         line["batched"] = batched
+    if latency:
+        b, pl = latency["batched"]["poisson"][-1], latency["plain"]["poisson"][-1]
+        line["latency"] = {"rate_images_per_s": b["rate"], "p50_ms": b["p50_ms"],
+                           "p99_ms": b["p99_ms"], "plain_p50_ms": pl["p50_ms"],
+                           "plain_p99_ms": pl["p99_ms"], "detail": latency}
+    if serving:
+        line["serving"] = serving
     if precision:
         line["precision"] = precision
     if c4:

@@ -138,15 +138,71 @@ def random_orientations(n: int, seed: int = 0, out: np.ndarray | None = None,
     return out
 
 
-def orientation_rows(seed: int, rows) -> np.ndarray:
-    """Regenerate selected DB rows without building the whole DB."""
+def _qmul(a, b):
+    """Hamilton product of quaternion arrays (w, x, y, z)."""
+    w1, x1, y1, z1 = np.moveaxis(a, -1, 0)
+    w2, x2, y2, z2 = np.moveaxis(b, -1, 0)
+    return np.stack([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2,
+                     w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                     w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2], axis=-1)
+
+
+def hexagonal_ops() -> np.ndarray:
+    """The 12 proper rotations of point group 622 (c along z): six turns about
+    z by k*60 deg and six 2-fold axes in the basal plane."""
+    ops = []
+    for k in range(6):
+        a = math.radians(60 * k)
+        ops.append([math.cos(a / 2), 0, 0, math.sin(a / 2)])
+    for k in range(6):
+        a = math.radians(30 * k)
+        ops.append([0, math.cos(a), math.sin(a), 0])
+    return np.array(ops)
+
+
+def _orient_chunk_reduced(seed: int, k: int, m: int, ops: np.ndarray) -> np.ndarray:
+    rng = np.random.default_rng([seed, k])
+    q = rng.normal(size=(m, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    cand = _qmul(q[:, None, :], ops[None, :, :])  # q * s for every symmetry s
+    best = np.argmax(np.abs(cand[..., 0]), axis=1)
+    q = cand[np.arange(m), best]
+    q *= np.sign(q[:, :1] + (q[:, :1] == 0))
+    return quat_to_matrices(q)
+
+
+def random_orientations_reduced(n: int, seed: int, ops: np.ndarray,
+                                out: np.ndarray | None = None, workers: int | None = None):
+    """Uniform random rotations brought into the fundamental zone of the
+    given point-group rotations (the equivalent q*s with the largest |w|):
+    a symmetry-reduced DB.  Same chunked seeding as random_orientations."""
+    if out is None:
+        out = np.empty((n, 9), dtype=np.float64)
+    nchunks = (n + ORIENT_CHUNK - 1) // ORIENT_CHUNK
+
+    def job(k):
+        lo = k * ORIENT_CHUNK
+        hi = min(n, lo + ORIENT_CHUNK)
+        out[lo:hi] = _orient_chunk_reduced(seed, k, ORIENT_CHUNK, ops)[: hi - lo]
+
+    workers = workers or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        list(ex.map(job, range(nchunks)))
+    return out
+
+
+def orientation_rows(seed: int, rows, ops: np.ndarray | None = None) -> np.ndarray:
+    """Regenerate selected DB rows without building the whole DB (ops: the
+    point-group rotations of a symmetry-reduced DB, or None)."""
     rows = np.asarray(rows, dtype=np.int64)
     res = np.empty((rows.size, 9))
     cache = {}
     for i, r in enumerate(rows):
         k = int(r) // ORIENT_CHUNK
         if k not in cache:
-            cache[k] = _orient_chunk(seed, k, ORIENT_CHUNK)
+            cache[k] = (_orient_chunk(seed, k, ORIENT_CHUNK) if ops is None
+                        else _orient_chunk_reduced(seed, k, ORIENT_CHUNK, ops))
         res[i] = cache[k][int(r) % ORIENT_CHUNK]
     return res
 
