@@ -327,6 +327,8 @@ def main():
                     help="super-tile size (log2) of the pixel-banded table; 0 = none")
     ap.add_argument("--batch", type=str, default="",
                     help="comma-separated batch widths to measure as well, e.g. 8,16,32,64")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="score batches with the default layouts instead of the autotuned ones")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32+guard rebuild of the table")
     ap.add_argument("--config4", action="store_true",
@@ -339,7 +341,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by orientation range across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(every box here has one GPU; across GPUs the path would shard by orientation range)", file=sys.stderr)
         sys.exit(2)
     widths = [int(b) for b in args.batch.split(",") if b]
     batch_imgs = []
@@ -419,7 +421,19 @@ def main():
         except Exception:
             traffic = None
 
-    batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1) if widths else None
+    batched, tune = None, None
+    if widths:
+        if not args.no_autotune:
+            samples = [api.quantize_u8(im)[0] for im, _ in batch_imgs[:4]]
+            t0 = time.time()
+            tune = db.autotune(samples, max_batch=max(widths), reps=2)
+            tune["seconds"] = time.time() - t0
+            log(f"autotune ({tune['seconds']:.1f} s): single {tune['ms_single']}, "
+                f"batch tile bits {tune['batch_tile_bits']}, widths mask "
+                f"{tune['batch_widths_mask']:#x}; per width {tune['ms_batch']}")
+        db.set_use_graphs(True)
+        batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1)
+        db.set_use_graphs(False)
 
     latency = None
     if not args.no_latency:
@@ -496,6 +510,8 @@ def main():
         for b in batched:
             b["speedup_vs_plain"] = b["pairs_per_s"] / value
         line["batched"] = batched
+    if tune:
+        line["autotune"] = tune
     if latency:
         b, pl = latency["batched"]["poisson"][-1], latency["plain"]["poisson"][-1]
         line["latency"] = {"rate_images_per_s": b["rate"], "p50_ms": b["p50_ms"],

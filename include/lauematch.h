@@ -127,9 +127,40 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * LM_OPT_BANDED_TILE_BITS: orientations per super-tile of the pixel-banded
  * table copy used by single-image scoring, as log2 (13 or 14); 0 drops the
  * copy (scoring then reads the orientation-major table only).  Applies at
- * once if a table exists. */
+ * once if a table exists.
+ * LM_OPT_BATCH_TILE_BITS: super-tile size (log2, 9..13) of a second banded
+ * copy used by batches of width B (4..64) with B * 2^bits <= 32768 (the
+ * per-(image, orientation) counters of a super-tile fill at most 128 KiB of
+ * LDS); 0 (default) drops it.  Applies at once if a table exists.
+ * LM_OPT_BATCH_WIDTHS: bit k set = batches of width 2^k read the batch copy
+ * (when it fits); clear = they read the orientation-major table.  Default
+ * 0x7f. */
 #define LM_OPT_BANDED_TILE_BITS 1
+#define LM_OPT_BATCH_TILE_BITS 2
+#define LM_OPT_BATCH_WIDTHS 3
+/* LM_OPT_USE_GRAPHS: 1 = direct scoring calls on this DB replay captured
+ * launch sequences (hipGraph), keyed by every pointer and size they use. */
+#define LM_OPT_USE_GRAPHS 4
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
+
+/* Autotuner (no reference counterpart): times every scoring layout on the
+ * sample images (nimg >= 1 host uint8 images of the DB's detector, repeated
+ * to fill a batch) for one image and for batch widths 4, 8, ..., max_batch,
+ * best of `reps` runs each, then sets LM_OPT_BANDED_TILE_BITS,
+ * LM_OPT_BATCH_TILE_BITS and LM_OPT_BATCH_WIDTHS to the fastest.  Needs a
+ * table; the timings are device milliseconds per launch sequence. */
+typedef struct lm_tune_report {
+  int32_t n_widths;
+  int32_t widths[5];          /* batch widths timed (4 .. max_batch)             */
+  double ms_single[3];        /* one image: table, banded 2^13, banded 2^14       */
+  double ms_lane[5];          /* per width: orientation-major table               */
+  double ms_banded[5][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
+  int32_t banded_tile_bits;   /* chosen LM_OPT_BANDED_TILE_BITS                   */
+  int32_t batch_tile_bits;    /* chosen LM_OPT_BATCH_TILE_BITS                    */
+  uint32_t batch_widths_mask; /* chosen LM_OPT_BATCH_WIDTHS                       */
+} lm_tune_report;
+int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
+                   int32_t reps, lm_tune_report *out);
 
 /* Build the resident spot table on the device (replaces the DoFwd=1 forward
  * simulation).  mode = LM_FWD_FP64 or LM_FWD_FP32_GUARD; stats may be NULL. */

@@ -57,6 +57,13 @@ class LmServerStats(ctypes.Structure):
                 ("images_per_s", ctypes.c_double), ("graph_launches", ctypes.c_uint64)]
 
 
+class LmTuneReport(ctypes.Structure):
+    _fields_ = [("n_widths", ctypes.c_int32), ("widths", ctypes.c_int32 * 5),
+                ("ms_single", ctypes.c_double * 3), ("ms_lane", ctypes.c_double * 5),
+                ("ms_banded", (ctypes.c_double * 5) * 5), ("banded_tile_bits", ctypes.c_int32),
+                ("batch_tile_bits", ctypes.c_int32), ("batch_widths_mask", ctypes.c_uint32)]
+
+
 _lib = None
 
 # name -> (restype, argtypes); every symbol declared in include/lauematch.h
@@ -82,6 +89,8 @@ SIGNATURES = {
     "lm_db_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(LmFwdStats)]),
     "lm_db_table_checksum": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "lm_db_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64]),
+    "lm_db_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(LmTuneReport)]),
     "lm_db_import_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
     "lm_db_export_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                           ctypes.c_void_p]),

@@ -18,6 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import _lib
 from ._lib import (LM_FWD_FP32_GUARD, LM_FWD_FP64, LmFwdStats, LmGeom, LmHits, check, lib)
 
 __all__ = ["Context", "Geometry", "OrientationDB", "Hits", "quantize_u8", "read_params",
@@ -149,6 +150,41 @@ class OrientationDB:
     def set_banded_tile_bits(self, bits: int) -> None:
         """Super-tile size (log2) of the pixel-banded copy; 0 drops it."""
         check(lib().lm_db_set_option(self._h, 1, int(bits)))
+
+    def set_batch_tile_bits(self, bits: int) -> None:
+        """Super-tile size (log2, 9..13) of the batch banded copy; 0 drops it."""
+        check(lib().lm_db_set_option(self._h, 2, int(bits)))
+
+    def set_batch_widths(self, mask: int) -> None:
+        """Bit k set: batches of width 2**k read the batch banded copy."""
+        check(lib().lm_db_set_option(self._h, 3, int(mask)))
+
+    def set_use_graphs(self, on: bool) -> None:
+        """Replay captured launch sequences (hipGraph) in direct scoring calls."""
+        check(lib().lm_db_set_option(self._h, 4, int(bool(on))))
+
+    def autotune(self, sample_imgs_u8, max_batch: int = 64, reps: int = 3) -> dict:
+        """Time every scoring layout on the sample images and keep the fastest
+        (single-image banded copy, batch copy super-tile, widths using it)."""
+        ims = [np.ascontiguousarray(i, dtype=np.uint8).ravel() for i in sample_imgs_u8]
+        npx = self.geom.nrPxX * self.geom.nrPxY
+        if not ims or any(i.size != npx for i in ims):
+            raise ValueError("sample images must match the detector")
+        ptrs = (ctypes.c_void_p * len(ims))(*[i.ctypes.data for i in ims])
+        r = _lib.LmTuneReport()
+        check(lib().lm_db_autotune(self._h, ptrs, len(ims), int(max_batch), int(reps),
+                                   ctypes.byref(r)))
+        widths = [r.widths[i] for i in range(r.n_widths)]
+        return {
+            "ms_single": {"table": r.ms_single[0], "banded13": r.ms_single[1],
+                          "banded14": r.ms_single[2]},
+            "ms_batch": {B: {"table": r.ms_lane[i],
+                             **{f"banded{tb}": r.ms_banded[i][tb - 9]
+                                for tb in range(9, 14) if r.ms_banded[i][tb - 9] > 0}}
+                         for i, B in enumerate(widths)},
+            "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
+            "batch_widths_mask": r.batch_widths_mask,
+        }
 
     def forward(self, mode: int = LM_FWD_FP64) -> LmFwdStats:
         st = LmFwdStats()

@@ -371,8 +371,50 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     if (db->table.ready) return build_banded(db);
     return LM_OK;
   }
+  if (option == LM_OPT_BATCH_TILE_BITS) {
+    if (value != 0 && (value < 9 || value > 13)) {
+      set_error("LM_OPT_BATCH_TILE_BITS must be 0 or 9..13");
+      return LM_ERR_INVALID;
+    }
+    db->batch_tb = (int)value;
+    if (db->table.ready) return build_banded(db);
+    return LM_OK;
+  }
+  if (option == LM_OPT_BATCH_WIDTHS) {
+    if (value < 0 || value > 0x7f) {
+      set_error("LM_OPT_BATCH_WIDTHS is a mask of bits 0..6");
+      return LM_ERR_INVALID;
+    }
+    db->batch_widths = (uint32_t)value;
+    return LM_OK;
+  }
+  if (option == LM_OPT_USE_GRAPHS) {
+    db->scratch.use_graphs = value != 0;
+    return LM_OK;
+  }
   set_error("unknown option");
   return LM_ERR_INVALID;
+}
+
+int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
+                   int32_t reps, lm_tune_report *out) {
+  if (!db || !images || nimg < 1 || !out || max_batch < 1 || reps < 1) {
+    set_error("lm_db_autotune: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  for (int i = 0; i < nimg; ++i)
+    if (!images[i]) {
+      set_error("lm_db_autotune: null image");
+      return LM_ERR_INVALID;
+    }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  if (!db->table.ready) {
+    set_error("lm_db_autotune: no table (run lm_db_forward or lm_db_import_table first)");
+    return LM_ERR_STATE;
+  }
+  return autotune(db, images, nimg, max_batch, reps, out);
 }
 
 int lm_db_table_checksum(lm_db *db, uint64_t *sum) {

@@ -201,6 +201,135 @@ score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restr
   }
 }
 
+// Batch compare over a banded copy: B images interleaved pixel-major
+// ([pixel][B] bytes), one B-byte gather per spot, LDS counters per
+// (image, row) in image-major planes cnt[b*T + row].  Same flat, pipelined
+// walk of the super-tile's pixel-sorted entries as the single-image kernel;
+// since every workgroup sweeps its bands in pixel order, the interleaved
+// rows being gathered are shared through L2 by the workgroups in flight.
+typedef uint32_t u32x4b __attribute__((ext_vector_type(4)));
+
+template <int B>
+__device__ __forceinline__ void gather_b(const uint8_t *__restrict__ img, uint64_t pix,
+                                         uint32_t (&w)[B / 4]) {
+  if constexpr (B == 4) {
+    w[0] = *(const uint32_t *)(img + pix * 4);
+  } else if constexpr (B == 8) {
+    const uint2 v = *(const uint2 *)(img + pix * 8);
+    w[0] = v.x;
+    w[1] = v.y;
+  } else {
+    const u32x4b *q = (const u32x4b *)(img + pix * B);
+#pragma unroll
+    for (int j = 0; j < B / 16; ++j) {
+      const u32x4b v = q[j];
+      w[4 * j] = v.x;
+      w[4 * j + 1] = v.y;
+      w[4 * j + 2] = v.z;
+      w[4 * j + 3] = v.w;
+    }
+  }
+}
+
+template <int TB, int B, int U>
+__global__ void __launch_bounds__((1 << TB) < 1024 ? (1 << TB) : 1024)
+score_band_batch_kernel(const uint32_t *__restrict__ bentries,
+                        const uint64_t *__restrict__ tileOff,
+                        const uint32_t *__restrict__ bandOff, int nbands, uint64_t n,
+                        uint64_t nsuper, const uint8_t *__restrict__ img,
+                        const Pred *__restrict__ pred, uint64_t *__restrict__ hits,
+                        uint64_t capPer, uint32_t *__restrict__ counters) {
+  constexpr int T = 1 << TB;
+  constexpr int THREADS = T < 1024 ? T : 1024;
+  constexpr int PB = 32 - TB;
+  constexpr uint32_t PMASK = (1u << PB) - 1u;
+  constexpr uint32_t STEP = THREADS * U;
+  static_assert(B * T * 4 <= 131072, "LDS counters exceed 128 KiB");
+  __shared__ uint32_t cnt[B * T];
+  const int tid = threadIdx.x, lane = __lane_id();
+  const int nimg = pred->nimg;
+  for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
+    for (int i = tid; i < B * T; i += THREADS) cnt[i] = 0;
+    __syncthreads();
+    const uint64_t first = t << TB;
+    const uint32_t *src = bentries + tileOff[first >> kTileBits];
+    const uint32_t *bo = bandOff + t * (uint64_t)(nbands + 1);
+    const uint32_t total = bo[nbands];
+    int band = 0;
+    uint32_t bend = bo[1];
+    uint32_t e[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      e[k] = i < total ? __builtin_nontemporal_load(src + i) : 0u;
+    }
+    for (uint32_t g0 = 0; g0 < total; g0 += STEP) {
+      uint32_t w[U][B / 4];
+      uint32_t row[U];
+      bool ok[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = g0 + k * THREADS + tid;
+        row[k] = e[k] >> PB;
+        ok[k] = i < total;
+        if (ok[k]) {
+          while (i >= bend) {
+            ++band;
+            bend = bo[band + 1];
+          }
+          gather_b<B>(img, ((uint64_t)band << PB) | (e[k] & PMASK), w[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = g0 + STEP + k * THREADS + tid;
+        e[k] = i < total ? __builtin_nontemporal_load(src + i) : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        if (!ok[k]) continue;
+        // most pixels are zero: test four images at a time
+#pragma unroll
+        for (int j = 0; j < B / 4; ++j) {
+          const uint32_t v = w[k][j];
+          if (!v) continue;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t raw = (v >> (8 * q)) & 0xffu;
+            if (raw) atomicAdd(&cnt[(4 * j + q) * T + row[k]], (1u << kSumBits) + raw);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int rr = tid; rr < T; rr += THREADS) {
+      const uint64_t o = first + rr;
+      const bool live = o < n;
+      for (int b = 0; b < nimg; ++b) {
+        const uint32_t acc = cnt[b * T + rr];
+        const uint32_t ns = acc >> kSumBits, sum = acc & kSumMask;
+        const int min_spots = pred->min_spots[b];
+        const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= pred->min_int[b]);
+        const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= pred->cand_thr[b];
+        const uint64_t zm = __ballot(zero_pass);
+        if (zm && lane == 0) atomicAdd(&counters[2 * b + 1], (uint32_t)__popcll(zm));
+        const uint64_t m = __ballot(cand);
+        if (m) {
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          uint32_t base = 0;
+          if (lane == leader) base = atomicAdd(&counters[2 * b], (uint32_t)__popcll(m));
+          base = __shfl(base, leader);
+          if (cand) {
+            const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (pos < capPer) hits[(uint64_t)b * capPer + pos] = ((uint64_t)b << 58) | (o << 26) | acc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 int reserve_bytes(lm_db *db, uint64_t bytes) {
   std::lock_guard<std::mutex> lk(db->ctx->mu);
   if (db->ctx->in_use + bytes > db->ctx->budget) {
@@ -211,11 +340,7 @@ int reserve_bytes(lm_db *db, uint64_t bytes) {
   return LM_OK;
 }
 
-}  // namespace
-
-void release_banded(lm_db *db) {
-  ++db->generation;  // any captured launch that read the old table is stale
-  Banded &b = db->banded;
+void release_one(lm_db *db, Banded &b) {
   if (b.entries) (void)hipFree(b.entries);
   if (b.band_off) (void)hipFree(b.band_off);
   if (b.bytes) {
@@ -225,16 +350,10 @@ void release_banded(lm_db *db) {
   b = Banded{};
 }
 
-int build_banded(lm_db *db) {
-  release_banded(db);
-  const int tb = db->banded_tb;
+int build_one(lm_db *db, Banded &b, int tb) {
+  release_one(db, b);
   if (tb == 0) return LM_OK;
-  if (tb != 13 && tb != 14) {
-    set_error("banded super-tile bits must be 13 or 14");
-    return LM_ERR_INVALID;
-  }
   const Table &t = db->table;
-  Banded &b = db->banded;
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
   b.tb = tb;
   b.pb = 32 - tb;
@@ -267,11 +386,35 @@ int build_banded(lm_db *db) {
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
-    release_banded(db);
+    release_one(db, b);
     return hip_fail(e, "build_banded");
   }
   b.ready = true;
   return LM_OK;
+}
+
+}  // namespace
+
+void release_banded(lm_db *db) {
+  ++db->generation;  // any captured launch that read the old table is stale
+  release_one(db, db->banded);
+  release_one(db, db->banded_b);
+}
+
+int build_banded(lm_db *db) {
+  release_banded(db);
+  if (db->banded_tb != 0 && db->banded_tb != 13 && db->banded_tb != 14) {
+    set_error("banded super-tile bits must be 0, 13 or 14");
+    return LM_ERR_INVALID;
+  }
+  if (db->batch_tb != 0 && (db->batch_tb < 9 || db->batch_tb > 13)) {
+    set_error("batch super-tile bits must be 0 or 9..13");
+    return LM_ERR_INVALID;
+  }
+  int rc = build_one(db, db->banded, db->banded_tb);
+  if (rc == LM_OK) rc = build_one(db, db->banded_b, db->batch_tb);
+  if (rc) release_banded(db);
+  return rc;
 }
 
 int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st) {
@@ -289,6 +432,48 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
                        cap, sc.counters);
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
+  return LM_OK;
+}
+
+bool banded_batch_fits(const lm_db *db, int B) {
+  const Banded &b = db->banded_b;
+  return b.ready && B >= 4 && B <= 64 && (uint64_t)B * (1ull << b.tb) <= 32768;
+}
+
+int launch_banded_batch(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
+                        hipStream_t st) {
+  const Banded &b = db->banded_b;
+  const Table &t = db->table;
+  const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+  const unsigned threads = (1u << b.tb) < 1024 ? (1u << b.tb) : 1024;
+#define LM_BB(TB, BB)                                                                            \
+  hipLaunchKernelGGL((score_band_batch_kernel<TB, BB, 4>), dim3(grid), dim3(threads), 0, st,     \
+                     b.entries, t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, sc.pred, \
+                     sc.hits, capPer, sc.counters)
+  const int key = b.tb * 100 + B;
+  switch (key) {
+    case 1304: LM_BB(13, 4); break;
+    case 1204: LM_BB(12, 4); break;
+    case 1208: LM_BB(12, 8); break;
+    case 1104: LM_BB(11, 4); break;
+    case 1108: LM_BB(11, 8); break;
+    case 1116: LM_BB(11, 16); break;
+    case 1004: LM_BB(10, 4); break;
+    case 1008: LM_BB(10, 8); break;
+    case 1016: LM_BB(10, 16); break;
+    case 1032: LM_BB(10, 32); break;
+    case 904: LM_BB(9, 4); break;
+    case 908: LM_BB(9, 8); break;
+    case 916: LM_BB(9, 16); break;
+    case 932: LM_BB(9, 32); break;
+    case 964: LM_BB(9, 64); break;
+    default:
+      set_error("no banded batch kernel for this batch width and super-tile size");
+      return LM_ERR_INVALID;
+  }
+#undef LM_BB
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "banded batch kernel launch");
   return LM_OK;
 }
 

@@ -98,7 +98,7 @@ struct Scratch {
     uint64_t generation;
     int nimg;
     uint64_t cap_per;
-    bool banded;
+    int kind;
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;
@@ -128,8 +128,11 @@ struct lm_db {
   double *d_orients = nullptr;  // n * 9
   int32_t *d_hkls = nullptr;    // nhkls * 3
   lm::Table table;
-  lm::Banded banded;
+  lm::Banded banded;            // single-image copy
+  lm::Banded banded_b;          // batch copy (smaller super-tiles, B x T LDS counters)
   int banded_tb = 14;           // super-tile bits for the banded copy (0 = no copy)
+  int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
+  uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   std::mutex mu;
@@ -162,6 +165,9 @@ int table_checksum(lm_db *db, uint64_t *sum);
 int build_banded(lm_db *db);
 void release_banded(lm_db *db);
 int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st);
+bool banded_batch_fits(const lm_db *db, int B);
+int launch_banded_batch(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
+                        hipStream_t st);
 
 // lm_score.hip
 // Score nimg device images laid out back to back ([nimg][npx]) with
@@ -173,6 +179,12 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
 // (f64 = image of doubles; otherwise uint8 values times scale).
 int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
               int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *out);
+
+int width_log2(int B);  // smallest k with 2^k >= B
+
+// lm_tune.cpp
+int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int reps,
+             lm_tune_report *rep);
 
 // lm_merge.cpp
 void quat_from_matrix(const double m[9], double q[4]);

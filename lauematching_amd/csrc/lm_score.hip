@@ -433,6 +433,12 @@ int fill_hits(lm_hits &h, uint64_t m) {
 
 }  // namespace
 
+int width_log2(int B) {
+  int k = 0;
+  while ((1 << k) < B) ++k;
+  return k;
+}
+
 int scratch_init(Scratch &s) {
   if (s.pred) return LM_OK;
   hipError_t e = hipMalloc(&s.counters, 2 * kMaxBatch * 4);
@@ -586,7 +592,11 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "predicate upload");
   uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)nimg, 1u << 16);
-  const bool banded = B == 1 && db->banded.ready && bit(pred, 0);
+  // kernel: 0 lane-per-orientation over the table, 1 single-image banded
+  // copy, 2 batch banded copy
+  int kind = 0;
+  if (B == 1 && db->banded.ready && bit(pred, 0)) kind = 1;
+  if (B > 1 && banded_batch_fits(db, B) && ((db->batch_widths >> width_log2(B)) & 1)) kind = 2;
   if (B > 1) {
     rc = ensure_batch(sc, npx * B);
     if (rc) return rc;
@@ -601,14 +611,16 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     }
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(2 * kMaxBatch), 0, st, sc.counters,
                        2 * kMaxBatch);
-    return banded ? launch_banded(db, sc, img, cp, st) : launch_score(db, sc, B, img, cp, st);
+    if (kind == 1) return launch_banded(db, sc, img, cp, st);
+    if (kind == 2) return launch_banded_batch(db, sc, B, img, cp, st);
+    return launch_score(db, sc, B, img, cp, st);
   };
   auto run = [&](uint64_t cp) -> int {
     if (!sc.use_graphs) return sequence(cp);
     hipGraphExec_t exec = nullptr;
     for (auto &g : sc.graphs)
       if (g.db == db && g.generation == db->generation && g.src == d_imgs &&
-          g.hits == sc.hits && g.nimg == nimg && g.cap_per == cp && g.banded == banded)
+          g.hits == sc.hits && g.nimg == nimg && g.cap_per == cp && g.kind == kind)
         exec = g.exec;
     if (!exec) {
       hipGraph_t graph = nullptr;
@@ -623,7 +635,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       if (ge == hipSuccess) ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (graph) (void)hipGraphDestroy(graph);
       if (ge != hipSuccess) return hip_fail(ge, "graph capture");
-      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, nimg, cp, banded, exec});
+      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, nimg, cp, kind, exec});
     }
     ++sc.graph_launches;
     const hipError_t le = hipGraphLaunch(exec, st);

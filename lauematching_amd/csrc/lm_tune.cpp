@@ -1,0 +1,140 @@
+// lm_tune.cpp - autotuner for the scoring layouts of one DB.
+//
+// The compare step can read three layouts of the same spot table:
+//   - the orientation-major table, one lane per orientation (any batch width),
+//   - the single-image banded copy (super-tiles of 2^13 or 2^14 rows),
+//   - the batch banded copy (super-tiles of 2^9..2^13 rows; a batch of width
+//     B fits when B * 2^tb <= 32768, its LDS counters filling <= 128 KiB).
+// Which one is fastest depends on the image density, the table's spots per
+// orientation and the batch width, so the tuner times each candidate on the
+// caller's sample images (device time of the whole launch sequence, best of
+// `reps`) and keeps the winners: the single-image super-tile size, the batch
+// copy's super-tile size (best at the widest width asked for) and, per batch
+// width, whether it reads the batch copy or the orientation-major table.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lm_internal.h"
+
+namespace lm {
+namespace {
+
+int time_batch(lm_db *db, const uint8_t *d_imgs, int B, int reps, double *best_ms) {
+  std::vector<double> ones(B, 1.0), thr(B, INFINITY);
+  std::vector<int> sp(B, 1 << 20);  // no hits: time the scan, not the collection
+  std::vector<lm_hits> outs(B);
+  double best = INFINITY;
+  for (int r = 0; r < reps; ++r) {
+    const int rc = score_images(db, db->scratch, B, d_imgs, ones.data(), sp.data(), thr.data(),
+                                db->ctx->stream, outs.data());
+    if (rc) return rc;
+    best = std::min(best, outs[0].kernel_ms);
+    for (auto &h : outs) lm_hits_free(&h);
+  }
+  *best_ms = best;
+  return LM_OK;
+}
+
+}  // namespace
+
+int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int reps,
+             lm_tune_report *rep) {
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  memset(rep, 0, sizeof *rep);
+  // stage kMaxBatch images: the samples, repeated
+  int rc = scratch_images(db->scratch, npx * kMaxBatch);
+  if (rc) return rc;
+  uint8_t *d = db->scratch.img;
+  hipStream_t st = db->ctx->stream;
+  for (int i = 0; i < kMaxBatch; ++i) {
+    const hipError_t e = hipMemcpyAsync(d + (uint64_t)i * npx, imgs[i % nimg], npx,
+                                        hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "autotune staging");
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
+  const int saved_tb = db->banded_tb, saved_btb = db->batch_tb;
+  const uint32_t saved_mask = db->batch_widths;
+  auto restore = [&](int code) {
+    db->banded_tb = saved_tb;
+    db->batch_tb = saved_btb;
+    db->batch_widths = saved_mask;
+    const int r = build_banded(db);
+    return code ? code : r;
+  };
+  // single image: no copy, 2^13, 2^14
+  const int single_tb[3] = {0, 13, 14};
+  int best_single = 0;
+  for (int i = 0; i < 3; ++i) {
+    db->banded_tb = single_tb[i];
+    db->batch_tb = 0;
+    rc = build_banded(db);
+    if (rc) return restore(rc);
+    rc = time_batch(db, d, 1, reps, &rep->ms_single[i]);
+    if (rc) return restore(rc);
+    if (rep->ms_single[i] < rep->ms_single[best_single]) best_single = i;
+  }
+  // batch widths 4..max_batch
+  int nw = 0;
+  for (int B = 4; B <= std::min(max_batch, kMaxBatch); B <<= 1) rep->widths[nw++] = B;
+  rep->n_widths = nw;
+  db->banded_tb = 0;  // room for the batch copies
+  db->batch_tb = 0;
+  rc = build_banded(db);
+  if (rc) return restore(rc);
+  for (int w = 0; w < nw; ++w) {
+    rc = time_batch(db, d, rep->widths[w], reps, &rep->ms_lane[w]);
+    if (rc) return restore(rc);
+  }
+  db->batch_widths = 0x7f;
+  for (int tb = 9; tb <= 13; ++tb) {
+    db->batch_tb = tb;
+    rc = build_banded(db);
+    if (rc) return restore(rc);
+    for (int w = 0; w < nw; ++w) {
+      const int B = rep->widths[w];
+      if (!banded_batch_fits(db, B)) continue;
+      rc = time_batch(db, d, B, reps, &rep->ms_banded[w][tb - 9]);
+      if (rc) return restore(rc);
+    }
+  }
+  // batch copy: the size fastest at the widest width that any size fits;
+  // kept only if it beats the orientation-major table there
+  int best_btb = 0;
+  for (int w = nw - 1; w >= 0 && !best_btb; --w) {
+    double best = rep->ms_lane[w];
+    for (int tb = 9; tb <= 13; ++tb) {
+      const double ms = rep->ms_banded[w][tb - 9];
+      if (ms > 0 && ms < best) {
+        best = ms;
+        best_btb = tb;
+      }
+    }
+    if (!best_btb) {
+      bool any = false;
+      for (int tb = 9; tb <= 13; ++tb) any |= rep->ms_banded[w][tb - 9] > 0;
+      if (any) break;  // fits, but the table wins at this width
+    }
+  }
+  uint32_t mask = 0;
+  if (best_btb)
+    for (int w = 0; w < nw; ++w) {
+      const double ms = rep->ms_banded[w][best_btb - 9];
+      if (ms > 0 && ms < rep->ms_lane[w]) mask |= 1u << width_log2(rep->widths[w]);
+    }
+  rep->banded_tile_bits = single_tb[best_single];
+  rep->batch_tile_bits = best_btb;
+  rep->batch_widths_mask = mask;
+  db->banded_tb = rep->banded_tile_bits;
+  db->batch_tb = best_btb;
+  db->batch_widths = mask;
+  rc = build_banded(db);
+  if (rc) return restore(rc);
+  return LM_OK;
+}
+
+}  // namespace lm

@@ -311,6 +311,97 @@ def test_batch_equals_single(scored_case, nimg):
         np.testing.assert_array_equal(got[i].nspots, single.nspots)
 
 
+@pytest.mark.parametrize("tb,nimg", [(9, 64), (9, 20), (10, 32), (10, 12), (11, 16), (11, 5),
+                                     (12, 8), (13, 4), (13, 3), (0, 16)])
+def test_batch_banded_copy_matches_oracle(scored_case, tb, nimg):
+    """Batches scored from the batch banded copy (super-tiles of 2^tb rows,
+    per-(image, row) LDS counters), and from the orientation-major table
+    (tb=0): every image bit-exact against the oracle."""
+    p, hk, O, grains, img, table, db = scored_case
+    rng = np.random.default_rng(100 + nimg)
+    base, _ = api.quantize_u8(img)
+    imgs, scales = [], []
+    for i in range(nimg):
+        if i % 5 == 4:
+            u8, s = api.quantize_u8(img * 0.37 + 0.02 * (img > 0))
+        elif i % 5 == 2:
+            u8, s = np.zeros_like(base), 1.0
+        else:
+            u8, s = (base if i == 0 else base[rng.permutation(base.size)]), 1.0
+        imgs.append(u8)
+        scales.append(s)
+    _say(f"batch banded tb={tb} nimg={nimg}")
+    db.set_batch_tile_bits(tb)
+    try:
+        got = db.score_batch(imgs, scales, 5, 500.0)
+    finally:
+        db.set_batch_tile_bits(0)
+    for i in range(nimg):
+        matched, ns, tot, nres = ob.compare(table, 64, imgs[i], p.NrPxX, scales[i], 5, 500.0)
+        rows, mA = ob.collect(matched)
+        assert got[i].nr_results == nres, i
+        np.testing.assert_array_equal(got[i].ids, rows)
+        np.testing.assert_array_equal(got[i].score, mA)
+        np.testing.assert_array_equal(got[i].nspots, ns[rows.astype(np.int64)])
+
+
+def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
+    """The autotuner times every layout, keeps the fastest and leaves the DB
+    scoring bit-exactly (single image and batches) with what it chose."""
+    p, hk, O, grains, img, table, db = scored_case
+    base, _ = api.quantize_u8(img)
+    rng = np.random.default_rng(9)
+    samples = [base, base[rng.permutation(base.size)]]
+    _say("autotune")
+    rep = db.autotune(samples, max_batch=32, reps=2)
+    try:
+        assert set(rep["ms_batch"]) == {4, 8, 16, 32}
+        singles = rep["ms_single"]
+        assert all(v > 0 for v in singles.values())
+        chosen = {0: "table", 13: "banded13", 14: "banded14"}[rep["banded_tile_bits"]]
+        assert singles[chosen] == min(singles.values())
+        for B, row in rep["ms_batch"].items():
+            assert row["table"] > 0
+            use = (rep["batch_widths_mask"] >> (B.bit_length() - 1)) & 1
+            if use:
+                assert row[f"banded{rep['batch_tile_bits']}"] < row["table"]
+        got = db.score_batch(samples * 8, [1.0] * 16, 5, 500.0)
+        for i in (0, 1):
+            matched, ns, tot, nres = ob.compare(table, 64, samples[i], p.NrPxX, 1.0, 5, 500.0)
+            rows, mA = ob.collect(matched)
+            for g in got[i::2]:
+                assert g.nr_results == nres
+                np.testing.assert_array_equal(g.ids, rows)
+                np.testing.assert_array_equal(g.score, mA)
+    finally:
+        db.set_batch_tile_bits(0)
+        db.set_batch_widths(0x7F)
+        db.set_banded_tile_bits(14)
+
+
+def test_graph_replay_of_direct_calls(scored_case):
+    """Direct calls with LM_OPT_USE_GRAPHS replay a captured launch sequence;
+    several replays with eager work between them all equal the eager result."""
+    p, hk, O, grains, img, table, db = scored_case
+    base, _ = api.quantize_u8(img)
+    rng = np.random.default_rng(3)
+    imgs = [base] + [base[rng.permutation(base.size)] for _ in range(7)]
+    eager = db.score_batch(imgs, [1.0] * 8, 5, 400.0)
+    db.set_use_graphs(True)
+    try:
+        for rep in range(4):
+            _say(f"graph replay {rep}")
+            got = db.score_batch(imgs, [1.0] * 8, 5, 400.0)
+            single = db.score(imgs[rep], 1.0, 5, 400.0)  # eager-sized work in between
+            np.testing.assert_array_equal(single.ids, eager[rep].ids)
+            for a, b in zip(got, eager):
+                assert a.nr_results == b.nr_results
+                np.testing.assert_array_equal(a.ids, b.ids)
+                np.testing.assert_array_equal(a.score, b.score)
+    finally:
+        db.set_use_graphs(False)
+
+
 def test_budget_admission(ctx):
     small = api.Context(0, hbm_budget=1 << 20)
     p = synth.baseline_params()
