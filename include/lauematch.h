@@ -141,21 +141,28 @@ uint64_t lm_db_table_spots(const lm_db *db);
 /* LM_OPT_USE_GRAPHS: 1 = direct scoring calls on this DB replay captured
  * launch sequences (hipGraph), keyed by every pointer and size they use. */
 #define LM_OPT_USE_GRAPHS 4
+/* LM_OPT_BANDED_LOADS: entry loads of the single-image banded walk: 0 = eight
+ * 4-byte loads per lane in flight, 1 = two 16-byte loads (8 entries), 2 = four
+ * 16-byte loads (16 entries).  Default 0. */
+#define LM_OPT_BANDED_LOADS 5
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 
 /* Autotuner (no reference counterpart): times every scoring layout on the
  * sample images (nimg >= 1 host uint8 images of the DB's detector, repeated
  * to fill a batch) for one image and for batch widths 4, 8, ..., max_batch,
  * best of `reps` runs each, then sets LM_OPT_BANDED_TILE_BITS,
- * LM_OPT_BATCH_TILE_BITS and LM_OPT_BATCH_WIDTHS to the fastest.  Needs a
+ * LM_OPT_BANDED_LOADS, LM_OPT_BATCH_TILE_BITS and LM_OPT_BATCH_WIDTHS to the
+ * fastest.  Needs a
  * table; the timings are device milliseconds per launch sequence. */
 typedef struct lm_tune_report {
   int32_t n_widths;
   int32_t widths[5];          /* batch widths timed (4 .. max_batch)             */
-  double ms_single[3];        /* one image: table, banded 2^13, banded 2^14       */
+  double ms_single_table;     /* one image: orientation-major table               */
+  double ms_single[2][3];     /* one image: banded 2^13 / 2^14 x loads 0, 1, 2    */
   double ms_lane[5];          /* per width: orientation-major table               */
   double ms_banded[5][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
   int32_t banded_tile_bits;   /* chosen LM_OPT_BANDED_TILE_BITS                   */
+  int32_t banded_loads;       /* chosen LM_OPT_BANDED_LOADS                       */
   int32_t batch_tile_bits;    /* chosen LM_OPT_BATCH_TILE_BITS                    */
   uint32_t batch_widths_mask; /* chosen LM_OPT_BATCH_WIDTHS                       */
 } lm_tune_report;
@@ -251,6 +258,11 @@ int lm_server_submit(lm_server *s, int32_t db_id, const uint8_t *img_u8, double 
                      int32_t min_nr_spots, double min_intensity, uint64_t *ticket);
 /* Blocks until the ticket's result is ready; the caller frees it with lm_hits_free. */
 int lm_server_wait(lm_server *s, uint64_t ticket, lm_hits *out, double *latency_ms);
+/* Non-blocking form: *ready = 1 and the result in *out if it is done (the
+ * ticket is then collected), else *ready = 0.  Unknown or already collected
+ * tickets are LM_ERR_INVALID for both calls. */
+int lm_server_poll(lm_server *s, uint64_t ticket, lm_hits *out, double *latency_ms,
+                   int32_t *ready);
 int lm_server_get_stats(lm_server *s, lm_server_stats *stats);
 void lm_server_reset_stats(lm_server *s);
 /* Finishes every admitted request, then releases the server. */

@@ -59,8 +59,9 @@ class LmServerStats(ctypes.Structure):
 
 class LmTuneReport(ctypes.Structure):
     _fields_ = [("n_widths", ctypes.c_int32), ("widths", ctypes.c_int32 * 5),
-                ("ms_single", ctypes.c_double * 3), ("ms_lane", ctypes.c_double * 5),
-                ("ms_banded", (ctypes.c_double * 5) * 5), ("banded_tile_bits", ctypes.c_int32),
+                ("ms_single_table", ctypes.c_double), ("ms_single", (ctypes.c_double * 3) * 2),
+                ("ms_lane", ctypes.c_double * 5), ("ms_banded", (ctypes.c_double * 5) * 5),
+                ("banded_tile_bits", ctypes.c_int32), ("banded_loads", ctypes.c_int32),
                 ("batch_tile_bits", ctypes.c_int32), ("batch_widths_mask", ctypes.c_uint32)]
 
 
@@ -125,6 +126,9 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_uint64)]),
     "lm_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(LmHits),
                                       ctypes.POINTER(ctypes.c_double)]),
+    "lm_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(LmHits),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int32)]),
     "lm_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LmServerStats)]),
     "lm_server_reset_stats": (None, [ctypes.c_void_p]),
     "lm_server_destroy": (None, [ctypes.c_void_p]),
@@ -139,6 +143,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m lauematching_amd.build`")
+        # One HIP runtime per process: PyTorch's ROCm build bundles its own
+        # libamdhip64, and two runtimes loaded side by side cannot both open
+        # the device (whichever initialises second finds no GPU).  Importing
+        # torch first makes this library bind to the runtime already loaded.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

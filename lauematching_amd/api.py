@@ -159,6 +159,10 @@ class OrientationDB:
         """Bit k set: batches of width 2**k read the batch banded copy."""
         check(lib().lm_db_set_option(self._h, 3, int(mask)))
 
+    def set_banded_loads(self, loads: int) -> None:
+        """Single-image banded walk: 0 = 4-byte loads x8, 1 = 16-byte x2, 2 = 16-byte x4."""
+        check(lib().lm_db_set_option(self._h, 5, int(loads)))
+
     def set_use_graphs(self, on: bool) -> None:
         """Replay captured launch sequences (hipGraph) in direct scoring calls."""
         check(lib().lm_db_set_option(self._h, 4, int(bool(on))))
@@ -175,9 +179,12 @@ class OrientationDB:
         check(lib().lm_db_autotune(self._h, ptrs, len(ims), int(max_batch), int(reps),
                                    ctypes.byref(r)))
         widths = [r.widths[i] for i in range(r.n_widths)]
+        single = {"table": r.ms_single_table}
+        for i, tb in enumerate((13, 14)):
+            for loads in range(3):
+                single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]
         return {
-            "ms_single": {"table": r.ms_single[0], "banded13": r.ms_single[1],
-                          "banded14": r.ms_single[2]},
+            "ms_single": single, "banded_loads": r.banded_loads,
             "ms_batch": {B: {"table": r.ms_lane[i],
                              **{f"banded{tb}": r.ms_banded[i][tb - 9]
                                 for tb in range(9, 14) if r.ms_banded[i][tb - 9] > 0}}

@@ -131,6 +131,7 @@ struct lm_db {
   lm::Banded banded;            // single-image copy
   lm::Banded banded_b;          // batch copy (smaller super-tiles, B x T LDS counters)
   int banded_tb = 14;           // super-tile bits for the banded copy (0 = no copy)
+  int banded_loads = 0;         // single-image walk: 0 scalar x8, 1 16-byte x2, 2 16-byte x4
   int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes

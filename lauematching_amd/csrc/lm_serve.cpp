@@ -22,6 +22,7 @@
 #include <deque>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "lm_internal.h"
@@ -63,6 +64,7 @@ struct lm_server {
   std::deque<lm::Request> pending;
   std::unordered_map<uint64_t, lm::Done> done;
   uint64_t next_ticket = 1;
+  std::unordered_set<uint64_t> open_tickets;  // submitted, result not yet collected
   bool stop = false;
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_slot;
@@ -267,6 +269,7 @@ int lm_server_submit(lm_server *s, int32_t db_id, const uint8_t *img_u8, double 
   lk.lock();
   Request r;
   r.ticket = s->next_ticket++;
+  s->open_tickets.insert(r.ticket);
   r.db = db_id;
   r.slot = slot;
   r.scale = im_scale;
@@ -290,13 +293,40 @@ int lm_server_wait(lm_server *s, uint64_t ticket, lm_hits *out, double *latency_
     return LM_ERR_INVALID;
   }
   std::unique_lock<std::mutex> lk(s->mu);
-  if (ticket == 0 || ticket >= s->next_ticket) {
-    set_error("lm_server_wait: unknown ticket");
+  if (!s->open_tickets.count(ticket)) {
+    set_error("lm_server_wait: unknown or already collected ticket");
     return LM_ERR_INVALID;
   }
   s->cv_done.wait(lk, [&] { return s->done.count(ticket) > 0; });
   Done d = s->done[ticket];
   s->done.erase(ticket);
+  s->open_tickets.erase(ticket);
+  *out = d.hits;
+  if (latency_ms) *latency_ms = d.latency_ms;
+  if (d.status != LM_OK) set_error("lm_server: batch failed");
+  return d.status;
+}
+
+int lm_server_poll(lm_server *s, uint64_t ticket, lm_hits *out, double *latency_ms,
+                   int32_t *ready) {
+  if (!s || !out || !ready) {
+    set_error("lm_server_poll: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (!s->open_tickets.count(ticket)) {
+    set_error("lm_server_poll: unknown or already collected ticket");
+    return LM_ERR_INVALID;
+  }
+  auto it = s->done.find(ticket);
+  if (it == s->done.end()) {
+    *ready = 0;
+    return LM_OK;
+  }
+  Done d = it->second;
+  s->done.erase(it);
+  s->open_tickets.erase(ticket);
+  *ready = 1;
   *out = d.hits;
   if (latency_ms) *latency_ms = d.latency_ms;
   if (d.status != LM_OK) set_error("lm_server: batch failed");

@@ -57,26 +57,40 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     if (e != hipSuccess) return hip_fail(e, "autotune staging");
   }
   if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
-  const int saved_tb = db->banded_tb, saved_btb = db->batch_tb;
+  const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
   const uint32_t saved_mask = db->batch_widths;
   auto restore = [&](int code) {
     db->banded_tb = saved_tb;
     db->batch_tb = saved_btb;
     db->batch_widths = saved_mask;
+    db->banded_loads = saved_loads;
     const int r = build_banded(db);
     return code ? code : r;
   };
-  // single image: no copy, 2^13, 2^14
-  const int single_tb[3] = {0, 13, 14};
-  int best_single = 0;
-  for (int i = 0; i < 3; ++i) {
-    db->banded_tb = single_tb[i];
-    db->batch_tb = 0;
+  // single image: the table, then 2^13 and 2^14 super-tiles x entry loads
+  db->batch_tb = 0;
+  db->banded_tb = 0;
+  rc = build_banded(db);
+  if (rc) return restore(rc);
+  rc = time_batch(db, d, 1, reps, &rep->ms_single_table);
+  if (rc) return restore(rc);
+  double best1 = rep->ms_single_table;
+  int best_tb = 0, best_loads = 0;
+  for (int i = 0; i < 2; ++i) {
+    db->banded_tb = 13 + i;
     rc = build_banded(db);
     if (rc) return restore(rc);
-    rc = time_batch(db, d, 1, reps, &rep->ms_single[i]);
-    if (rc) return restore(rc);
-    if (rep->ms_single[i] < rep->ms_single[best_single]) best_single = i;
+    for (int loads = 0; loads < 3; ++loads) {
+      db->banded_loads = loads;
+      ++db->generation;
+      rc = time_batch(db, d, 1, reps, &rep->ms_single[i][loads]);
+      if (rc) return restore(rc);
+      if (rep->ms_single[i][loads] < best1) {
+        best1 = rep->ms_single[i][loads];
+        best_tb = 13 + i;
+        best_loads = loads;
+      }
+    }
   }
   // batch widths 4..max_batch
   int nw = 0;
@@ -126,13 +140,15 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
       const double ms = rep->ms_banded[w][best_btb - 9];
       if (ms > 0 && ms < rep->ms_lane[w]) mask |= 1u << width_log2(rep->widths[w]);
     }
-  rep->banded_tile_bits = single_tb[best_single];
+  rep->banded_tile_bits = best_tb;
+  rep->banded_loads = best_loads;
   rep->batch_tile_bits = best_btb;
   rep->batch_widths_mask = mask;
-  db->banded_tb = rep->banded_tile_bits;
+  db->banded_tb = best_tb;
+  db->banded_loads = best_loads;
   db->batch_tb = best_btb;
   db->batch_widths = mask;
-  rc = build_banded(db);
+  rc = build_banded(db);  // also bumps the generation: no stale captured launch
   if (rc) return restore(rc);
   return LM_OK;
 }

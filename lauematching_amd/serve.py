@@ -47,6 +47,15 @@ class Server:
         check(lib().lm_server_wait(self._h, ticket, ctypes.byref(h), ctypes.byref(lat)))
         return Hits._take(h), lat.value
 
+    def poll(self, ticket: int):
+        """(hits, latency_ms) if the ticket's result is ready, else None."""
+        h = LmHits()
+        lat = ctypes.c_double()
+        ready = ctypes.c_int32()
+        check(lib().lm_server_poll(self._h, ticket, ctypes.byref(h), ctypes.byref(lat),
+                                   ctypes.byref(ready)))
+        return (Hits._take(h), lat.value) if ready.value else None
+
     def stats(self) -> dict:
         s = LmServerStats()
         check(lib().lm_server_get_stats(self._h, ctypes.byref(s)))

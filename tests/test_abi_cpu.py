@@ -92,3 +92,18 @@ def test_read_params_prefix_matching(tmp_path):
     assert p.Elo == 6 and p.Ehi == 25.5 and p.MinIntensity == 12.5 and p.DoFwd == 0
     assert p.ForwardFile == "/dev/shm/x.bin" and p.P_Array == (0.0, 0.0, 0.4)
     assert p.MinNrSpots == 5  # C default
+
+
+def test_one_hip_runtime_per_process():
+    """With PyTorch installed, the library binds to the HIP runtime torch has
+    loaded: two runtimes side by side cannot both open the device."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from lauematching_amd import _lib; _lib.lib()\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(libs))\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "1"

@@ -176,14 +176,17 @@ def test_score_matches_oracle_unit_scale(scored_case, min_sp, min_int):
             assert g in set(hits.ids.tolist())
 
 
-@pytest.mark.parametrize("bits", [0, 13, 14])
-def test_score_layouts_agree(scored_case, bits):
-    """Banded copy with 8K or 16K super-tiles, and no banded copy (the
-    orientation-major kernel): identical results to the oracle."""
+@pytest.mark.parametrize("bits,loads", [(0, 0), (13, 0), (14, 0), (13, 1), (14, 1), (13, 2),
+                                        (14, 2)])
+def test_score_layouts_agree(scored_case, bits, loads):
+    """Banded copy with 8K or 16K super-tiles walked with 4-byte or 16-byte
+    entry loads, and no banded copy (the orientation-major kernel): identical
+    results to the oracle."""
     p, hk, O, grains, img, table, db = scored_case
     u8, s = api.quantize_u8(img)
-    _say(f"score banded bits={bits}")
+    _say(f"score banded bits={bits} loads={loads}")
     db.set_banded_tile_bits(bits)
+    db.set_banded_loads(loads)
     try:
         for min_sp, min_int in [(5, 250.0), (0, 0.0)]:
             hits = db.score(u8, s, min_sp, min_int)
@@ -195,6 +198,7 @@ def test_score_layouts_agree(scored_case, bits):
             np.testing.assert_array_equal(hits.nspots, ns[rows.astype(np.int64)])
     finally:
         db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
 
 
 @pytest.mark.parametrize("geo", ["small_rect", "det1024"])
@@ -357,9 +361,16 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
     try:
         assert set(rep["ms_batch"]) == {4, 8, 16, 32}
         singles = rep["ms_single"]
-        assert all(v > 0 for v in singles.values())
-        chosen = {0: "table", 13: "banded13", 14: "banded14"}[rep["banded_tile_bits"]]
+        assert len(singles) == 7 and all(v > 0 for v in singles.values())
+        tb = rep["banded_tile_bits"]
+        chosen = "table" if tb == 0 else f"banded{tb}_loads{rep['banded_loads']}"
         assert singles[chosen] == min(singles.values())
+        u8 = samples[0]
+        one = db.score(u8, 1.0, 5, 500.0)
+        matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, 1.0, 5, 500.0)
+        rows, mA = ob.collect(matched)
+        np.testing.assert_array_equal(one.ids, rows)
+        np.testing.assert_array_equal(one.score, mA)
         for B, row in rep["ms_batch"].items():
             assert row["table"] > 0
             use = (rep["batch_widths_mask"] >> (B.bit_length() - 1)) & 1
@@ -377,6 +388,7 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
         db.set_batch_tile_bits(0)
         db.set_batch_widths(0x7F)
         db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
 
 
 def test_graph_replay_of_direct_calls(scored_case):

@@ -48,12 +48,13 @@ def two_dbs():
     return ctx, dbc, dbh, reqs
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_results_alone_equal_in_the_mix(two_dbs, graphs):
+@pytest.mark.parametrize("graphs,batch_tb", [(False, 0), (True, 0), (True, 10)])
+def test_results_alone_equal_in_the_mix(two_dbs, graphs, batch_tb):
     ctx, dbc, dbh, reqs = two_dbs
     dbs = [dbc, dbh]
     alone = [dbs[d].score(u8, s, ms, mi) for d, u8, s, ms, mi in reqs]
-    _say(f"server graphs={graphs}")
+    dbc.set_batch_tile_bits(batch_tb)  # batches of the cubic DB read the batch copy
+    _say(f"server graphs={graphs} batch_tb={batch_tb}")
     srv = Server(ctx, max_batch=8, workers=2, max_inflight=24, use_graphs=graphs)
     ids = [srv.add_db(dbc), srv.add_db(dbh)]
     assert ids == [0, 1]
@@ -86,4 +87,40 @@ def test_results_alone_equal_in_the_mix(two_dbs, graphs):
     assert st["mean_batch"] >= 1.0
     if graphs:
         assert st["graph_launches"] > 0
+    srv.close()
+    dbc.set_batch_tile_bits(0)
+
+
+def test_poll_and_ticket_errors(two_dbs):
+    from lauematching_amd._lib import LaueMatchError
+
+    ctx, dbc, dbh, reqs = two_dbs
+    srv = Server(ctx, max_batch=4, workers=1, max_inflight=8, use_graphs=False)
+    srv.add_db(dbc)
+    srv.add_db(dbh)
+    sub = [(i, srv.submit(*reqs[i])) for i in range(6)]
+    got = {}
+    deadline = 600
+    while len(got) < len(sub) and deadline:
+        for i, t in sub:
+            if i not in got:
+                r = srv.poll(t)
+                if r is not None:
+                    got[i] = r
+        deadline -= 1
+        if len(got) < len(sub):
+            import time
+            time.sleep(0.05)
+    assert len(got) == len(sub)
+    for i, (h, lat) in got.items():
+        d, u8, s, ms, mi = reqs[i]
+        a = [dbc, dbh][d].score(u8, s, ms, mi)
+        np.testing.assert_array_equal(h.ids, a.ids)
+        np.testing.assert_array_equal(h.score, a.score)
+    with pytest.raises(LaueMatchError):
+        srv.poll(sub[0][1])  # already collected
+    with pytest.raises(LaueMatchError):
+        srv.wait(sub[0][1])
+    with pytest.raises(LaueMatchError):
+        srv.wait(10 ** 9)
     srv.close()
