@@ -121,7 +121,7 @@ def run_batches(db, batch_imgs, widths, p, steps, warmup):
         out.append({"images": B, "ms_per_batch": ms, "kernel_ms": float(np.mean(kms)),
                     "pairs_per_s": db.n * B / (ms / 1e3), "bit_exact_vs_single": bool(ok),
                     "images_with_all_grains_recovered": int(found),
-                    "hits_per_image": [int(h.count) for h in hs]})
+                    "hits_per_image": [int(len(h.ids)) for h in hs]})
         log(f"batch B={B}: {ms:.2f} ms/batch, {out[-1]['pairs_per_s']:.3e} pairs/s, "
             f"exact={ok}, grains recovered in {found}/{B}")
         del dev
@@ -407,6 +407,22 @@ def main():
         f"(nrResults {h.nr_results})")
     found = all(g in set(h.ids.tolist()) for g in grains)
 
+    # a plain device copy of known size: the HBM rate reachable on this box,
+    # and the reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters
+    csrc = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0")
+    cdst = torch.empty_like(csrc)
+    cdst.copy_(csrc)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        cdst.copy_(csrc)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_ms = e0.elapsed_time(e1) / 5
+    copy_gbps = 2.0 * (1 << 30) / (copy_ms / 1e3) / 1e9
+    del csrc, cdst
+    log(f"plain 1 GiB device copy: {copy_ms:.3f} ms, {copy_gbps:.0f} GB/s (read + write)")
+
     # algorithmic bytes of one compare launch (per orientation 2 + 4 n_o bytes,
     # the image once, 16 bytes per hit)
     b_alg = 2.0 * n + 4.0 * st.n_spots + float(u8.size) + 16.0 * h.nr_results
@@ -501,6 +517,7 @@ def main():
                    "hits": int(len(h.ids)), "nr_results": int(h.nr_results),
                    "grains_recovered": bool(found), "prefix_parity_vs_oracle": parity_ok,
                    "forward_ms": st.ms, "kernel_ms": kernel_ms,
+                   "hbm_copy_GBps": copy_gbps,
                    "banded_tile_bits": args.tile_bits},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},

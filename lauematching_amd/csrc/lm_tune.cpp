@@ -8,9 +8,10 @@
 // Which one is fastest depends on the image density, the table's spots per
 // orientation and the batch width, so the tuner times each candidate on the
 // caller's sample images (device time of the whole launch sequence, best of
-// `reps`) and keeps the winners: the single-image super-tile size, the batch
-// copy's super-tile size (best at the widest width asked for) and, per batch
-// width, whether it reads the batch copy or the orientation-major table.
+// `reps`) and keeps the winners: the single-image super-tile size and entry
+// loads, the batch copy's super-tile size (least per-image time summed over
+// the batch widths) and, per batch width, whether it reads the batch copy or
+// the orientation-major table.
 
 #include <hip/hip_runtime.h>
 
@@ -116,22 +117,21 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
       if (rc) return restore(rc);
     }
   }
-  // batch copy: the size fastest at the widest width that any size fits;
-  // kept only if it beats the orientation-major table there
+  // batch copy: the size with the least per-image time summed over the
+  // widths (each width taking the faster of the copy and the table); none if
+  // the table alone is as fast
+  double best_cost = 0;
+  for (int w = 0; w < nw; ++w) best_cost += rep->ms_lane[w] / rep->widths[w];
   int best_btb = 0;
-  for (int w = nw - 1; w >= 0 && !best_btb; --w) {
-    double best = rep->ms_lane[w];
-    for (int tb = 9; tb <= 13; ++tb) {
+  for (int tb = 9; tb <= 13; ++tb) {
+    double cost = 0;
+    for (int w = 0; w < nw; ++w) {
       const double ms = rep->ms_banded[w][tb - 9];
-      if (ms > 0 && ms < best) {
-        best = ms;
-        best_btb = tb;
-      }
+      cost += (ms > 0 ? std::min(ms, rep->ms_lane[w]) : rep->ms_lane[w]) / rep->widths[w];
     }
-    if (!best_btb) {
-      bool any = false;
-      for (int tb = 9; tb <= 13; ++tb) any |= rep->ms_banded[w][tb - 9] > 0;
-      if (any) break;  // fits, but the table wins at this width
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_btb = tb;
     }
   }
   uint32_t mask = 0;
