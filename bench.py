@@ -200,30 +200,45 @@ def poisson_run(srv, reqs, rate, duration, seed):
             "mean_batch": st["mean_batch"]}
 
 
-def closed_loop(srv, reqs, count):
-    """As many requests as the admission slots allow, back to back."""
+def closed_loop(srv, reqs, count, window=128):
+    """Sustained throughput: keep `window` requests outstanding (submit one
+    as soon as the oldest completes), `count` requests in all."""
+    import collections
     import threading
 
     srv.reset_stats()
-    out = []
+    tickets = collections.deque()
+    lock = threading.Condition()
+    done = [0]
 
-    def collect(ts):
-        for t in ts:
-            out.append(srv.wait(t))
+    def collect():
+        while done[0] < count:
+            with lock:
+                while not tickets:
+                    lock.wait()
+                t = tickets.popleft()
+            srv.wait(t)
+            with lock:
+                done[0] += 1
+                lock.notify_all()
 
-    tickets = []
+    th = threading.Thread(target=collect)
+    th.start()
     t0 = time.perf_counter()
     for k in range(count):
+        with lock:
+            while k - done[0] >= window:
+                lock.wait()
         db_id, u8, s = reqs[k % len(reqs)]
-        tickets.append(srv.submit(db_id, u8, s, 5, 1000.0))
-        if len(tickets) == 64:
-            collect(tickets)
-            tickets = []
-    collect(tickets)
+        t = srv.submit(db_id, u8, s, 5, 1000.0)
+        with lock:
+            tickets.append(t)
+            lock.notify_all()
+    th.join()
     dt = time.perf_counter() - t0
     st = srv.stats()
-    return {"images": count, "images_per_s": count / dt, "p50_ms": st["p50_ms"],
-            "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]}
+    return {"images": count, "window": window, "images_per_s": count / dt,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]}
 
 
 def run_serving(ctx, dbs, reqs, rates, duration, nmax):
@@ -232,7 +247,7 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
     from lauematching_amd.serve import Server
 
     res = {}
-    for name, kw in [("batched", dict(max_batch=32, workers=2, use_graphs=True)),
+    for name, kw in [("batched", dict(max_batch=64, workers=2, use_graphs=True)),
                      ("plain", dict(max_batch=1, workers=1, use_graphs=False))]:
         srv = Server(ctx, **kw)
         ids = [srv.add_db(db) for db in dbs]
@@ -454,7 +469,7 @@ def main():
     latency = None
     if not args.no_latency:
         reqs = [(0, *api.quantize_u8(img)) for c, d, img in serve_pool if (c, d) == ("fcc", 2048)]
-        latency = run_serving(ctx, [db], reqs, [50, 200], 3.0, 128)
+        latency = run_serving(ctx, [db], reqs, [50, 200], 3.0, 512)
     serving = None
     if args.serve:
         from lauematching_amd import synth as sy
@@ -478,7 +493,7 @@ def main():
             u8, s = api.quantize_u8(img)
             reps = 7 if d == 2048 else 3
             reqs += [(keys.index((c, d)), u8, s)] * reps
-        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [5, 20, 50, 100, 200], 4.0, 256)
+        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [5, 20, 50, 100, 200], 4.0, 1024)
         serving["dbs"] = [{"crystal": c, "detector": d, "orientations": dbs[(c, d)].n}
                           for c, d in keys]
         serving["hbm_in_use_gb"] = ctx.bytes_in_use() / 1e9

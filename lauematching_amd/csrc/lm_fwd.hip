@@ -100,14 +100,21 @@ struct DevGeom32 {
 // decisions: trunc(t) for -1 < t < N.  The decision points are -1, the
 // integers k >= 1 (where trunc changes; trunc is 0 on (-1, 1)) and N.
 // Returns 0 = clearly rejected, 1 = clearly accepted (*ip set), 2 = within e
-// of a decision point.
+// of a decision point (or not finite).  Every test compares one difference
+// with e; the differences t - N, -1 - t, t - k, k + 1 - t are exact in float
+// wherever they are small (Sterbenz), so no value of t falls between the
+// reject, undecided and accept tests.
 __device__ __forceinline__ int classify_px(float t, float n, float e, int *ip) {
-  if (t <= -1.0f - e || t >= n + e) return 0;
+  if (!(fabsf(t) < 1e30f)) return 2;
+  if (t >= n) return (t - n <= e) ? 2 : 0;
+  if (t <= -1.0f) return (-1.0f - t <= e) ? 2 : 0;
   float d;
-  if (t >= 1.0f)
-    d = fminf(t - floorf(t), ceilf(t) - t);
-  else
-    d = fminf(fabsf(t + 1.0f), 1.0f - t);
+  if (t >= 1.0f) {
+    const float k = floorf(t);
+    d = fminf(t - k, (k + 1.0f) - t);
+  } else {
+    d = fminf(t + 1.0f, 1.0f - t);
+  }
   if (d <= e) return 2;
   *ip = (int)t;
   return 1;
@@ -149,7 +156,8 @@ __device__ __forceinline__ int project32(const float *M, int hi, int ki, int li,
   if (z < -exyz) return 0;
   if (z <= exyz) return 2;
   int verdict = 1;
-  // energy window (inclusive)
+  // energy window (inclusive): as for the pixel coordinates, each test
+  // compares one difference (exact near the window edges) with eE
   const float su = -u2;
   if (fabsf(su) <= ekf) {
     verdict = 2;
@@ -158,8 +166,13 @@ __device__ __forceinline__ int project32(const float *M, int hi, int ki, int li,
     const float E = kHc4Pi * ql / su;
     const float eE = 2.0f * (8.0f * U * (1.0f + r) + ekf / fabsf(su) + 8.0f * U) * fabsf(E) +
                      1e-6f * fabsf(E);
-    if (E < g.Elo - eE || E > g.Ehi + eE) return 0;
-    if (fabsf(E - g.Elo) <= eE || fabsf(E - g.Ehi) <= eE) verdict = 2;
+    const float dlo = E - g.Elo, dhi = g.Ehi - E;
+    if (!(fabsf(E) < 1e30f)) {
+      verdict = 2;
+    } else {
+      if (dlo < -eE || dhi < -eE) return 0;
+      if (dlo <= eE || dhi <= eE) verdict = 2;
+    }
   }
   const float iz = 1.0f / z;
   const float X = x * g.P[2] * iz, Y = y * g.P[2] * iz;

@@ -96,7 +96,7 @@ struct Scratch {
   struct Graph {
     const void *db, *src, *hits;
     uint64_t generation;
-    int nimg;
+    int width;  // batch width B (the images themselves come from the predicate)
     uint64_t cap_per;
     int kind;
     hipGraphExec_t exec;
@@ -134,6 +134,7 @@ struct lm_db {
   int banded_loads = 0;         // single-image walk: 0 scalar x8, 1 16-byte x2, 2 16-byte x4
   int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
+  double width_ms[7] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   std::mutex mu;
@@ -181,7 +182,8 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
 int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
               int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *out);
 
-int width_log2(int B);  // smallest k with 2^k >= B
+int width_log2(int B);       // smallest k with 2^k >= B
+int batch_width(int nimg);   // kernel width for nimg images: 1, 4, 8, 16, 32, 64
 
 // lm_tune.cpp
 int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int reps,

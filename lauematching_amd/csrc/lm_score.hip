@@ -273,12 +273,15 @@ score_seq_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict
   }
 }
 
-// [nimg][npx] -> [npx][B] (B >= nimg; missing images read as zero).
+// [nimg][npx] -> [npx][B] (B >= nimg; missing images read as zero).  nimg
+// comes from the device predicate, so one captured launch serves every batch
+// of width B.
 template <int B>
-__global__ void interleave_kernel(const uint8_t *__restrict__ src, int nimg, uint64_t npx,
-                                  uint8_t *__restrict__ dst) {
+__global__ void interleave_kernel(const uint8_t *__restrict__ src, const Pred *__restrict__ pred,
+                                  uint64_t npx, uint8_t *__restrict__ dst) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npx) return;
+  const int nimg = pred->nimg;
   uint8_t v[B];
 #pragma unroll
   for (int b = 0; b < B; ++b) v[b] = b < nimg ? src[(uint64_t)b * npx + p] : 0;
@@ -390,11 +393,11 @@ int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t cap
   return LM_OK;
 }
 
-int launch_interleave(int B, const uint8_t *src, int nimg, uint64_t npx, uint8_t *dst,
+int launch_interleave(int B, const uint8_t *src, const Pred *pred, uint64_t npx, uint8_t *dst,
                       hipStream_t st) {
   const unsigned grid = (unsigned)((npx + 255) / 256);
 #define LM_IL(BB) \
-  hipLaunchKernelGGL(interleave_kernel<BB>, dim3(grid), dim3(256), 0, st, src, nimg, npx, dst)
+  hipLaunchKernelGGL(interleave_kernel<BB>, dim3(grid), dim3(256), 0, st, src, pred, npx, dst)
   switch (B) {
     case 4: LM_IL(4); break;
     case 8: LM_IL(8); break;
@@ -407,14 +410,6 @@ int launch_interleave(int B, const uint8_t *src, int nimg, uint64_t npx, uint8_t
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "interleave kernel launch");
   return LM_OK;
-}
-
-int batch_width(int nimg) {
-  if (nimg <= 1) return 1;
-  if (nimg <= 4) return 4;
-  int B = 8;
-  while (B < nimg) B <<= 1;
-  return B;
 }
 
 int fill_hits(lm_hits &h, uint64_t m) {
@@ -432,6 +427,14 @@ int fill_hits(lm_hits &h, uint64_t m) {
 }
 
 }  // namespace
+
+int batch_width(int nimg) {
+  if (nimg <= 1) return 1;
+  if (nimg <= 4) return 4;
+  int B = 8;
+  while (B < nimg) B <<= 1;
+  return B;
+}
 
 int width_log2(int B) {
   int k = 0;
@@ -591,7 +594,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   }
   hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "predicate upload");
-  uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)nimg, 1u << 16);
+  uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)B, 1u << 16);
   // kernel: 0 lane-per-orientation over the table, 1 single-image banded
   // copy, 2 batch banded copy
   int kind = 0;
@@ -606,7 +609,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   // replayed from a captured graph keyed by everything the arguments use
   auto sequence = [&](uint64_t cp) -> int {
     if (B > 1) {
-      const int r = launch_interleave(B, d_imgs, nimg, npx, sc.batch, st);
+      const int r = launch_interleave(B, d_imgs, sc.pred, npx, sc.batch, st);
       if (r) return r;
     }
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(2 * kMaxBatch), 0, st, sc.counters,
@@ -620,7 +623,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     hipGraphExec_t exec = nullptr;
     for (auto &g : sc.graphs)
       if (g.db == db && g.generation == db->generation && g.src == d_imgs &&
-          g.hits == sc.hits && g.nimg == nimg && g.cap_per == cp && g.kind == kind)
+          g.hits == sc.hits && g.width == B && g.cap_per == cp && g.kind == kind)
         exec = g.exec;
     if (!exec) {
       hipGraph_t graph = nullptr;
@@ -635,7 +638,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       if (ge == hipSuccess) ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (graph) (void)hipGraphDestroy(graph);
       if (ge != hipSuccess) return hip_fail(ge, "graph capture");
-      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, nimg, cp, kind, exec});
+      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, B, cp, kind, exec});
     }
     ++sc.graph_launches;
     const hipError_t le = hipGraphLaunch(exec, st);
@@ -644,7 +647,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   float ms = 0;
   std::vector<uint32_t> cnt(2 * kMaxBatch);
   for (int attempt = 0; attempt < 2; ++attempt) {
-    rc = ensure_hits(sc, capPer * (uint64_t)nimg);
+    rc = ensure_hits(sc, capPer * (uint64_t)B);  // a width-B launch serves any nimg <= B
     if (rc) return rc;
     (void)hipEventRecord(sc.ev0, st);
     rc = run(capPer);

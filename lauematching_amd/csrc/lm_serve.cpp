@@ -80,6 +80,32 @@ struct lm_server {
 namespace lm {
 namespace {
 
+// How many of the c pending requests of one DB to score in one launch.
+// A batch of nimg images runs at width batch_width(nimg), so its cost is
+// that width's; with the DB's autotuned per-width device times, take the
+// count with the best images per millisecond (fewer than c when padding a
+// partial batch to the next width costs more than it gains; the rest go to
+// the next free worker).  Without timings, take all c.
+int pick_batch(const lm_db *db, int c) {
+  if (c <= 1 || db->width_ms[0] <= 0) return c;
+  auto cost = [&](int n) { return db->width_ms[width_log2(batch_width(n))]; };
+  int best = 1;
+  double best_rate = 1.0 / db->width_ms[0];
+  std::vector<int> cand;
+  for (int w = 4; w <= c; w <<= 1) cand.push_back(w);
+  cand.push_back(c);
+  for (int n : cand) {
+    const double ms = cost(n);
+    if (ms <= 0) continue;  // width not timed
+    const double rate = n / ms;
+    if (rate > best_rate) {
+      best_rate = rate;
+      best = n;
+    }
+  }
+  return best;
+}
+
 void worker_loop(lm_server *s, int wid) {
   (void)hipSetDevice(s->ctx->device);
   hipStream_t st = nullptr;
@@ -98,8 +124,10 @@ void worker_loop(lm_server *s, int wid) {
       s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
       if (s->pending.empty()) break;  // stop requested and queue drained
       dbi = s->pending.front().db;
-      for (auto it = s->pending.begin(); it != s->pending.end() &&
-                                         (int)batch.size() < s->cfg.max_batch;) {
+      int same = 0;
+      for (const auto &r : s->pending) same += r.db == dbi;
+      const int take = pick_batch(s->dbs[dbi], std::min(same, s->cfg.max_batch));
+      for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < take;) {
         if (it->db == dbi) {
           batch.push_back(*it);
           it = s->pending.erase(it);

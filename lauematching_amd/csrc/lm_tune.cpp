@@ -148,6 +148,13 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->banded_loads = best_loads;
   db->batch_tb = best_btb;
   db->batch_widths = mask;
+  // cost table for the server's batch-size choice
+  for (double &v : db->width_ms) v = 0;
+  db->width_ms[0] = best1;
+  for (int w = 0; w < nw; ++w) {
+    const int k = width_log2(rep->widths[w]);
+    db->width_ms[k] = ((mask >> k) & 1) ? rep->ms_banded[w][best_btb - 9] : rep->ms_lane[w];
+  }
   rc = build_banded(db);  // also bumps the generation: no stale captured launch
   if (rc) return restore(rc);
   return LM_OK;

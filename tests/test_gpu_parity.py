@@ -97,6 +97,45 @@ def test_forward_fp32_guard_bit_identical(ctx, geo):
     assert db.checksum() == c32
 
 
+# Rows of the 100M-orientation DB (seed 0, 2048^2 detector, FCC |hkl| <= 8)
+# with a spot whose energy lies within a float32 rounding of Ehi + eE: an
+# earlier classifier compared E with the rounded sum Ehi + eE and accepted
+# them (E = 30.00045..30.00051 keV > Ehi = 30).
+FP32_EDGE_ROWS = [13621346, 18944713, 21642751, 29386159, 30040475, 38000683, 40986199,
+                  51687398, 56880516, 59475135, 61372750, 62127485, 66307574, 72056249,
+                  75014891, 76294540, 78089371, 78238134, 79530517, 83598450, 84554566,
+                  84604834, 86705729, 87291174, 89099544, 95102346, 97160938, 98347393,
+                  98472412, 99041057, 99827004]
+
+
+def test_forward_fp32_guard_energy_edge_rows(ctx):
+    """Regression: pairs just above the energy window stay rejected."""
+    p = synth.baseline_params(MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.orientation_rows(0, FP32_EDGE_ROWS)
+    _say("forward fp32-guard energy-edge rows")
+    db = _db(ctx, p, O, hk)
+    db.forward(api.LM_FWD_FP32_GUARD)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    np.testing.assert_array_equal(db.export_table(), want)
+
+
+def test_forward_fp32_guard_equals_fp64_4m(ctx):
+    """4M random orientations: FP32+guard and FP64 tables have equal
+    position-tagged checksums (the FP64 path is bit-exact to the oracle)."""
+    p = synth.baseline_params(MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.random_orientations(4_000_000, seed=77)
+    _say("forward fp32-guard vs fp64, 4M orientations")
+    db = _db(ctx, p, O, hk)
+    db.set_banded_tile_bits(0)
+    db.forward(api.LM_FWD_FP64)
+    c64 = db.checksum()
+    db.forward(api.LM_FWD_FP32_GUARD)
+    assert db.checksum() == c64
+    db.close()
+
+
 def test_forward_fp32_guard_config4_hkls(ctx):
     """400-reflection-class list (FCC |hkl| <= 10, 1066 rows), 5-40 keV."""
     p = synth.baseline_params(Ehi=40.0, MaxNrLaueSpots=64)
