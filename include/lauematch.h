@@ -125,7 +125,7 @@ uint64_t lm_db_table_spots(const lm_db *db);
 
 /* Layout options (no reference counterpart; used by the autotuner).
  * LM_OPT_BANDED_TILE_BITS: orientations per super-tile of the pixel-banded
- * table copy used by single-image scoring, as log2 (13 or 14); 0 drops the
+ * table copy used by single-image scoring, as log2 (13, 14 or 15); 0 drops the
  * copy (scoring then reads the orientation-major table only).  Applies at
  * once if a table exists.
  * LM_OPT_BATCH_TILE_BITS: super-tile size (log2, 9..13) of a second banded
@@ -158,7 +158,7 @@ typedef struct lm_tune_report {
   int32_t n_widths;
   int32_t widths[5];          /* batch widths timed (4 .. max_batch)             */
   double ms_single_table;     /* one image: orientation-major table               */
-  double ms_single[2][3];     /* one image: banded 2^13 / 2^14 x loads 0, 1, 2    */
+  double ms_single[3][3];     /* one image: banded 2^13 / 2^14 / 2^15 x loads 0..2 */
   double ms_lane[5];          /* per width: orientation-major table               */
   double ms_banded[5][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
   int32_t banded_tile_bits;   /* chosen LM_OPT_BANDED_TILE_BITS                   */
@@ -239,6 +239,9 @@ typedef struct lm_server_config {
   int32_t workers;      /* concurrent batches / HIP streams (0 = 2)              */
   int32_t max_inflight; /* admitted, not yet completed requests (0 = 2*workers*max_batch) */
   int32_t use_graphs;   /* replay captured launch sequences (hipGraph)           */
+  int32_t batch_after_ms; /* batch only when the pending images of a DB would take
+                             longer than this one at a time (autotuned DBs;
+                             0 = four single-image launches, < 0 = always batch) */
 } lm_server_config;
 typedef struct lm_server_stats {
   uint64_t completed;

@@ -47,7 +47,8 @@ class LaueMatchError(RuntimeError):
 
 class LmServerConfig(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_int32), ("workers", ctypes.c_int32),
-                ("max_inflight", ctypes.c_int32), ("use_graphs", ctypes.c_int32)]
+                ("max_inflight", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
+                ("batch_after_ms", ctypes.c_int32)]
 
 
 class LmServerStats(ctypes.Structure):
@@ -59,7 +60,7 @@ class LmServerStats(ctypes.Structure):
 
 class LmTuneReport(ctypes.Structure):
     _fields_ = [("n_widths", ctypes.c_int32), ("widths", ctypes.c_int32 * 5),
-                ("ms_single_table", ctypes.c_double), ("ms_single", (ctypes.c_double * 3) * 2),
+                ("ms_single_table", ctypes.c_double), ("ms_single", (ctypes.c_double * 3) * 3),
                 ("ms_lane", ctypes.c_double * 5), ("ms_banded", (ctypes.c_double * 5) * 5),
                 ("banded_tile_bits", ctypes.c_int32), ("banded_loads", ctypes.c_int32),
                 ("batch_tile_bits", ctypes.c_int32), ("batch_widths_mask", ctypes.c_uint32)]

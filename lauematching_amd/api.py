@@ -180,7 +180,7 @@ class OrientationDB:
                                    ctypes.byref(r)))
         widths = [r.widths[i] for i in range(r.n_widths)]
         single = {"table": r.ms_single_table}
-        for i, tb in enumerate((13, 14)):
+        for i, tb in enumerate((13, 14, 15)):
             for loads in range(3):
                 single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]
         return {

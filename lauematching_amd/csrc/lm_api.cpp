@@ -363,8 +363,8 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   if (option == LM_OPT_BANDED_TILE_BITS) {
-    if (value != 0 && value != 13 && value != 14) {
-      set_error("LM_OPT_BANDED_TILE_BITS must be 0, 13 or 14");
+    if (value != 0 && (value < 13 || value > 15)) {
+      set_error("LM_OPT_BANDED_TILE_BITS must be 0, 13, 14 or 15");
       return LM_ERR_INVALID;
     }
     db->banded_tb = (int)value;

@@ -500,8 +500,8 @@ void release_banded(lm_db *db) {
 
 int build_banded(lm_db *db) {
   release_banded(db);
-  if (db->banded_tb != 0 && db->banded_tb != 13 && db->banded_tb != 14) {
-    set_error("banded super-tile bits must be 0, 13 or 14");
+  if (db->banded_tb != 0 && (db->banded_tb < 13 || db->banded_tb > 15)) {
+    set_error("banded super-tile bits must be 0, 13, 14 or 15");
     return LM_ERR_INVALID;
   }
   if (db->batch_tb != 0 && (db->batch_tb < 9 || db->batch_tb > 13)) {
@@ -525,6 +525,9 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
                      sc.counters)
   const int key = b.tb * 10 + db->banded_loads;
   switch (key) {
+    case 150: LM_SB(score_band_kernel, 15, 1024, 8); break;
+    case 151: LM_SB(score_band_v4_kernel, 15, 1024, 2); break;
+    case 152: LM_SB(score_band_v4_kernel, 15, 1024, 4); break;
     case 140: LM_SB(score_band_kernel, 14, 1024, 8); break;
     case 141: LM_SB(score_band_v4_kernel, 14, 1024, 2); break;
     case 142: LM_SB(score_band_v4_kernel, 14, 1024, 4); break;

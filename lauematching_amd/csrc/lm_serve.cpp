@@ -82,15 +82,20 @@ namespace {
 
 // How many of the c pending requests of one DB to score in one launch.
 // A batch of nimg images runs at width batch_width(nimg), so its cost is
-// that width's; with the DB's autotuned per-width device times, take the
-// count with the best images per millisecond (fewer than c when padding a
-// partial batch to the next width costs more than it gains; the rest go to
-// the next free worker).  Without timings, take all c.
-int pick_batch(const lm_db *db, int c) {
-  if (c <= 1 || db->width_ms[0] <= 0) return c;
+// that width's autotuned device time.  While the pending images would finish
+// within `after_ms` one at a time, they are taken one at a time (a small
+// batch costs several single launches and would only add latency at light
+// load); past that, the count with the best images per millisecond is taken
+// (fewer than c when padding to the next width costs more than it gains; the
+// rest go to the next free worker).  Without timings, all c are taken.
+int pick_batch(const lm_db *db, int c, int after_ms) {
+  const double one = db->width_ms[0];
+  if (c <= 1 || one <= 0) return c;
+  const double budget = after_ms > 0 ? after_ms : (after_ms == 0 ? 4.0 * one : 0.0);
+  if (c * one <= budget) return 1;
   auto cost = [&](int n) { return db->width_ms[width_log2(batch_width(n))]; };
   int best = 1;
-  double best_rate = 1.0 / db->width_ms[0];
+  double best_rate = 1.0 / one;
   std::vector<int> cand;
   for (int w = 4; w <= c; w <<= 1) cand.push_back(w);
   cand.push_back(c);
@@ -126,7 +131,8 @@ void worker_loop(lm_server *s, int wid) {
       dbi = s->pending.front().db;
       int same = 0;
       for (const auto &r : s->pending) same += r.db == dbi;
-      const int take = pick_batch(s->dbs[dbi], std::min(same, s->cfg.max_batch));
+      const int take =
+          pick_batch(s->dbs[dbi], std::min(same, s->cfg.max_batch), s->cfg.batch_after_ms);
       for (auto it = s->pending.begin(); it != s->pending.end() && (int)batch.size() < take;) {
         if (it->db == dbi) {
           batch.push_back(*it);

@@ -2,7 +2,7 @@
 //
 // The compare step can read three layouts of the same spot table:
 //   - the orientation-major table, one lane per orientation (any batch width),
-//   - the single-image banded copy (super-tiles of 2^13 or 2^14 rows),
+//   - the single-image banded copy (super-tiles of 2^13, 2^14 or 2^15 rows),
 //   - the batch banded copy (super-tiles of 2^9..2^13 rows; a batch of width
 //     B fits when B * 2^tb <= 32768, its LDS counters filling <= 128 KiB).
 // Which one is fastest depends on the image density, the table's spots per
@@ -68,7 +68,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     const int r = build_banded(db);
     return code ? code : r;
   };
-  // single image: the table, then 2^13 and 2^14 super-tiles x entry loads
+  // single image: the table, then 2^13, 2^14 and 2^15 super-tiles x entry loads
   db->batch_tb = 0;
   db->banded_tb = 0;
   rc = build_banded(db);
@@ -77,7 +77,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   if (rc) return restore(rc);
   double best1 = rep->ms_single_table;
   int best_tb = 0, best_loads = 0;
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < 3; ++i) {
     db->banded_tb = 13 + i;
     rc = build_banded(db);
     if (rc) return restore(rc);

@@ -16,8 +16,9 @@ class Server:
     ascending hit lists lm_score gives for each image alone."""
 
     def __init__(self, ctx: Context, max_batch: int = 32, workers: int = 2,
-                 max_inflight: int = 0, use_graphs: bool = True):
-        cfg = LmServerConfig(max_batch, workers, max_inflight, 1 if use_graphs else 0)
+                 max_inflight: int = 0, use_graphs: bool = True, batch_after_ms: int = 0):
+        cfg = LmServerConfig(max_batch, workers, max_inflight, 1 if use_graphs else 0,
+                             batch_after_ms)
         h = ctypes.c_void_p()
         check(lib().lm_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
