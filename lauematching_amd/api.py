@@ -160,7 +160,8 @@ class OrientationDB:
         check(lib().lm_db_set_option(self._h, 3, int(mask)))
 
     def set_banded_loads(self, loads: int) -> None:
-        """Single-image banded walk: 0 = 4-byte loads x8, 1 = 16-byte x2, 2 = 16-byte x4."""
+        """Single-image banded walk: 0 = 4-byte loads x8, 1 = 16-byte x2, 2 = 16-byte x4,
+        3 = image staged through LDS (2^14 / 2^15 super-tiles)."""
         check(lib().lm_db_set_option(self._h, 5, int(loads)))
 
     def set_use_graphs(self, on: bool) -> None:
@@ -181,8 +182,9 @@ class OrientationDB:
         widths = [r.widths[i] for i in range(r.n_widths)]
         single = {"table": r.ms_single_table}
         for i, tb in enumerate((13, 14, 15)):
-            for loads in range(3):
-                single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]
+            for loads in range(4):
+                if r.ms_single[i][loads] > 0:
+                    single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]
         return {
             "ms_single": single, "banded_loads": r.banded_loads,
             "ms_batch": {B: {"table": r.ms_lane[i],

@@ -43,7 +43,8 @@ __global__ void __launch_bounds__(kBuildThreads)
 band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
                   const uint32_t *__restrict__ entries, uint64_t n, int tb, int pb, int bb,
                   int nbuckets, int nbands, uint32_t *__restrict__ bentries,
-                  uint32_t *__restrict__ bandOff) {
+                  uint32_t *__restrict__ bandOff, int cb, int nchunks,
+                  uint32_t *__restrict__ chunkOff) {
   __shared__ uint32_t hist[kMaxBuckets];
   __shared__ uint32_t wsum[kBuildThreads / 64];
   const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
@@ -105,6 +106,13 @@ band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restric
       for (int j = tid; j <= nbands; j += kBuildThreads) {
         const int64_t bk = (int64_t)j << shift;
         bo[j] = (j == nbands || bk >= nbuckets) ? run : hist[bk];
+      }
+      if (chunkOff) {
+        uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
+        for (int j = tid; j <= nchunks; j += kBuildThreads) {
+          const int64_t bk = (int64_t)j << (cb - bb);
+          co[j] = (j == nchunks || bk >= nbuckets) ? run : hist[bk];
+        }
       }
       __syncthreads();
     }
@@ -298,6 +306,113 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
   }
 }
 
+// Single-image compare with the image staged through LDS instead of
+// gathered from L2.  The byte gathers of score_band_kernel keep the texture
+// address and data units busy (one wave instruction touches about a dozen
+// cache lines); here each wave owns a 2^CB-byte slot of LDS and walks the
+// super-tile chunk by chunk (chunk c = pixels [c << CB, (c+1) << CB), the
+// entries of chunk c are chunkOff[c] .. chunkOff[c+1] of the pixel-sorted
+// super-tile): one coalesced 16-byte load per lane brings the chunk's pixels,
+// the chunk's entries come in EMAX coalesced loads per lane, and each entry
+// reads its pixel from LDS and adds into the shared row counters.  Wave w
+// takes chunks w, w + 16, ...; the next chunk's loads are issued before the
+// current chunk is processed.  LDS traffic of a wave is ordered, so its slot
+// needs no barrier between chunks.
+template <int TB, int CB, int EMAX>
+__global__ void __launch_bounds__(1024)
+score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
+                      const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
+                      uint64_t nsuper, const uint8_t *__restrict__ img, uint64_t npx,
+                      const Pred *__restrict__ pred, uint64_t *__restrict__ hits, uint64_t cap,
+                      uint32_t *__restrict__ counters) {
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  constexpr int T = 1 << TB;
+  constexpr int THREADS = 1024;
+  constexpr int NW = THREADS / 64;
+  constexpr int PB = 32 - TB;
+  constexpr int C = 1 << CB;
+  static_assert(C == 64 * 16, "one 16-byte load per lane fills a wave's chunk");
+  __shared__ uint32_t cnt[T];
+  __shared__ v4u slot[NW][C / 16];
+  const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+  const uint8_t *mine = (const uint8_t *)&slot[w][0];
+  for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
+    for (int i = tid; i < T; i += THREADS) cnt[i] = 0;
+    __syncthreads();
+    const uint64_t first = t << TB;
+    const uint32_t *src = bentries + tileOff[first >> kTileBits];
+    const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
+    v4u pim = v4u{0u, 0u, 0u, 0u};
+    uint32_t pe[EMAX];
+    uint32_t pbeg = 0, pend = 0;
+    auto fetch = [&](int c) {
+      pbeg = co[c];
+      pend = co[c + 1];
+      const uint64_t p0 = ((uint64_t)c << CB) + 16ull * lane;
+      pim = p0 + 16 <= npx ? *(const v4u *)(img + p0) : v4u{0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int k = 0; k < EMAX; ++k) {
+        const uint32_t i = pbeg + k * 64 + lane;
+        pe[k] = i < pend ? __builtin_nontemporal_load(src + i) : 0u;
+      }
+    };
+    int c = w;
+    if (c < nchunks) fetch(c);
+    while (c < nchunks) {
+      const v4u cim = pim;
+      uint32_t ce[EMAX];
+#pragma unroll
+      for (int k = 0; k < EMAX; ++k) ce[k] = pe[k];
+      const uint32_t cbeg = pbeg, cend = pend;
+      const int cn = c + NW;
+      if (cn < nchunks) fetch(cn);
+      slot[w][lane] = cim;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < EMAX; ++k) {
+        const uint32_t i = cbeg + k * 64 + lane;
+        if (i < cend) {
+          const uint32_t e = ce[k];
+          const uint32_t raw = mine[e & (C - 1)];
+          if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
+        }
+      }
+      for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {  // dense chunks
+        const uint32_t e = src[i];
+        const uint32_t raw = mine[e & (C - 1)];
+        if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
+      }
+      __builtin_amdgcn_wave_barrier();
+      c = cn;
+    }
+    __syncthreads();
+    for (int rr = tid; rr < T; rr += THREADS) {
+      const uint64_t o = first + rr;
+      const uint32_t acc = cnt[rr];
+      const uint32_t ns = acc >> kSumBits, sum = acc & kSumMask;
+      const bool live = o < n;
+      const int min_spots = pred->min_spots[0];
+      const double min_int = pred->min_int[0];
+      const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= min_int);
+      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int;
+      const uint64_t zm = __ballot(zero_pass);
+      if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
+      const uint64_t m = __ballot(cand);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t b0 = 0;
+        if (lane == leader) b0 = atomicAdd(&counters[0], (uint32_t)__popcll(m));
+        b0 = __shfl(b0, leader);
+        if (cand) {
+          const uint64_t pos = (uint64_t)b0 + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+          if (pos < cap) hits[pos] = (o << 26) | (uint64_t)acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Batch compare over a banded copy: B images interleaved pixel-major
 // ([pixel][B] bytes), one B-byte gather per spot, LDS counters per
 // (image, row) in image-major planes cnt[b*T + row].  Same flat, pipelined
@@ -440,6 +555,7 @@ int reserve_bytes(lm_db *db, uint64_t bytes) {
 void release_one(lm_db *db, Banded &b) {
   if (b.entries) (void)hipFree(b.entries);
   if (b.band_off) (void)hipFree(b.band_off);
+  if (b.chunk_off) (void)hipFree(b.chunk_off);
   if (b.bytes) {
     std::lock_guard<std::mutex> lk(db->ctx->mu);
     db->ctx->in_use -= b.bytes;
@@ -447,7 +563,7 @@ void release_one(lm_db *db, Banded &b) {
   b = Banded{};
 }
 
-int build_one(lm_db *db, Banded &b, int tb) {
+int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
   release_one(db, b);
   if (tb == 0) return LM_OK;
   const Table &t = db->table;
@@ -465,20 +581,25 @@ int build_one(lm_db *db, Banded &b, int tb) {
   b.nsuper = (db->n + (1ull << tb) - 1) >> tb;
   const uint64_t ebytes = (t.total + 64) * 4;
   const uint64_t obytes = b.nsuper * (uint64_t)(b.nbands + 1) * 4;
-  int rc = reserve_bytes(db, ebytes + obytes);
+  // chunk offsets for the LDS-staged walk (chunks must be whole buckets)
+  chunks = chunks && bb <= kWaveChunkBits;
+  b.nchunks = chunks ? (int)((npx + (1ull << kWaveChunkBits) - 1) >> kWaveChunkBits) : 0;
+  const uint64_t cbytes = chunks ? b.nsuper * (uint64_t)(b.nchunks + 1) * 4 : 0;
+  int rc = reserve_bytes(db, ebytes + obytes + cbytes);
   if (rc) {
     b = Banded{};
     return rc;
   }
-  b.bytes = ebytes + obytes;
+  b.bytes = ebytes + obytes + cbytes;
   hipError_t e = hipMalloc(&b.entries, ebytes);
   if (e == hipSuccess) e = hipMalloc(&b.band_off, obytes);
+  if (e == hipSuccess && chunks) e = hipMalloc(&b.chunk_off, cbytes);
   hipStream_t st = db->ctx->stream;
   if (e == hipSuccess) e = hipMemsetAsync(b.entries + t.total, 0, 64 * 4, st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(band_build_kernel, dim3((unsigned)b.nsuper), dim3(kBuildThreads), 0, st,
                        t.counts, t.tile_off, t.entries, db->n, tb, b.pb, bb, nbuckets, b.nbands,
-                       b.entries, b.band_off);
+                       b.entries, b.band_off, kWaveChunkBits, b.nchunks, b.chunk_off);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -508,8 +629,8 @@ int build_banded(lm_db *db) {
     set_error("batch super-tile bits must be 0 or 9..13");
     return LM_ERR_INVALID;
   }
-  int rc = build_one(db, db->banded, db->banded_tb);
-  if (rc == LM_OK) rc = build_one(db, db->banded_b, db->batch_tb);
+  int rc = build_one(db, db->banded, db->banded_tb, db->banded_tb >= 14);
+  if (rc == LM_OK) rc = build_one(db, db->banded_b, db->batch_tb, false);
   if (rc) release_banded(db);
   return rc;
 }
@@ -523,7 +644,27 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   hipLaunchKernelGGL((KERNEL<TB, TH, U>), dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off,   \
                      b.band_off, b.nbands, db->n, b.nsuper, img, sc.pred, sc.hits, cap,         \
                      sc.counters)
-  const int key = b.tb * 10 + db->banded_loads;
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  // the LDS-staged walk loads the image in 16-byte pieces; an image that is
+  // not 16-byte aligned (a caller's device pointer) takes the gather walk
+  if (db->banded_loads == 3 && ((uintptr_t)img & 15) == 0) {
+    if (!b.chunk_off || (b.tb != 14 && b.tb != 15)) {
+      set_error("LDS-staged walk needs 2^14 or 2^15 super-tiles and chunk offsets");
+      return LM_ERR_INVALID;
+    }
+    if (b.tb == 15)
+      hipLaunchKernelGGL((score_band_lds_kernel<15, kWaveChunkBits, 4>), dim3(grid), dim3(1024), 0,
+                         st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n, b.nsuper, img,
+                         npx, sc.pred, sc.hits, cap, sc.counters);
+    else
+      hipLaunchKernelGGL((score_band_lds_kernel<14, kWaveChunkBits, 4>), dim3(grid), dim3(1024), 0,
+                         st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n, b.nsuper, img,
+                         npx, sc.pred, sc.hits, cap, sc.counters);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
+    return LM_OK;
+  }
+  const int key = b.tb * 10 + (db->banded_loads == 3 ? 0 : db->banded_loads);
   switch (key) {
     case 150: LM_SB(score_band_kernel, 15, 1024, 8); break;
     case 151: LM_SB(score_band_v4_kernel, 15, 1024, 2); break;

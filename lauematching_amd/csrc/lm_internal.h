@@ -44,14 +44,17 @@ struct Table {
 struct Banded {
   uint32_t *entries = nullptr;   // same length as Table::entries
   uint32_t *band_off = nullptr;  // nsuper * (nbands + 1)
+  uint32_t *chunk_off = nullptr; // nsuper * (nchunks + 1): offsets of 2^kWaveChunkBits-pixel chunks
   uint64_t nsuper = 0;
   int tb = 0;
   int pb = 0;
   int nbands = 0;
+  int nchunks = 0;
   uint64_t bytes = 0;
   bool ready = false;
 };
 constexpr int kBucketBits = 10;
+constexpr int kWaveChunkBits = 10;  // LDS-staged walk: pixels per wave chunk (log2)
 
 struct Geom {
   double recip[9];
