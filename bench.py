@@ -333,6 +333,19 @@ def config4(ctx, O, n, steps, batch_imgs):
     return res
 
 
+def kernel_name(db):
+    """Name of the single-image kernel the DB's layout options select."""
+    tb, loads = db.banded_layout()
+    if tb == 0:
+        return "score_kernel<1>"
+    if loads == 3 and tb >= 14:
+        return f"score_band_lds_kernel<{tb},10,4>"
+    threads = 1024 if tb >= 14 else 512
+    if loads == 0:
+        return f"score_band_kernel<{tb},{threads},8>"
+    return f"score_band_v4_kernel<{tb},{threads},{2 if loads == 1 else 4}>"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,8 +358,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tile-bits", type=int, default=14,
-                    help="super-tile size (log2) of the pixel-banded table; 0 = none")
+    ap.add_argument("--tile-bits", type=int, default=None,
+                    help="force the single-image super-tile size (log2) of the banded "
+                         "table (0 = none) instead of the autotuned one")
+    ap.add_argument("--loads", type=int, default=0,
+                    help="with --tile-bits: the single-image walk (LM_OPT_BANDED_LOADS)")
     ap.add_argument("--batch", type=str, default="",
                     help="comma-separated batch widths to measure as well, e.g. 8,16,32,64")
     ap.add_argument("--no-autotune", action="store_true",
@@ -401,11 +417,27 @@ def main():
     t0 = time.time()
     db = api.OrientationDB(ctx, O, hk, api.Geometry.from_params(p), p.MaxNrLaueSpots)
     log(f"DB upload {time.time() - t0:.1f} s")
-    db.set_banded_tile_bits(args.tile_bits)
     st = db.forward()
     log(f"forward simulation: {st.n_spots} spots ({st.n_spots / n:.2f}/orientation), "
         f"{st.ms:.1f} ms device time")
     u8, im_scale = api.quantize_u8(img)
+
+    # layouts: the autotuner times every walk on this image (and the batch
+    # images) before anything is measured; --tile-bits forces the single walk
+    tune = None
+    if not args.no_autotune:
+        samples = [u8] + [api.quantize_u8(im)[0] for im, _ in batch_imgs[:3]]
+        t0 = time.time()
+        tune = db.autotune(samples, max_batch=64, reps=2)
+        tune["seconds"] = time.time() - t0
+        log(f"autotune ({tune['seconds']:.1f} s): single {tune['ms_single']} -> "
+            f"{tune['banded_tile_bits']}/{tune['banded_loads']}; batch tile bits "
+            f"{tune['batch_tile_bits']}, widths mask {tune['batch_widths_mask']:#x}; "
+            f"per width {tune['ms_batch']}")
+    if args.tile_bits is not None:
+        db.set_banded_tile_bits(args.tile_bits)
+        db.set_banded_loads(args.loads)
+    single_kernel = kernel_name(db)
     d_img = torch.from_numpy(u8).to("cuda:0")
     torch.cuda.synchronize()
 
@@ -454,21 +486,13 @@ def main():
     if os.path.exists(tf):
         try:
             tj = json.load(open(tf))
-            if tj.get("orients") == n:
+            if tj.get("orients") == n and tj.get("kernel") == single_kernel:
                 traffic = tj.get("bytes_per_launch")
         except Exception:
             traffic = None
 
-    batched, tune = None, None
+    batched = None
     if widths:
-        if not args.no_autotune:
-            samples = [api.quantize_u8(im)[0] for im, _ in batch_imgs[:4]]
-            t0 = time.time()
-            tune = db.autotune(samples, max_batch=max(widths), reps=2)
-            tune["seconds"] = time.time() - t0
-            log(f"autotune ({tune['seconds']:.1f} s): single {tune['ms_single']}, "
-                f"batch tile bits {tune['batch_tile_bits']}, widths mask "
-                f"{tune['batch_widths_mask']:#x}; per width {tune['ms_batch']}")
         db.set_use_graphs(True)
         batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1)
         db.set_use_graphs(False)
@@ -490,6 +514,9 @@ def main():
             d2 = api.OrientationDB(ctx, O if cr == "fcc" else Ohex, hk2,
                                    api.Geometry.from_params(pp), 64)
             d2.forward(api.LM_FWD_FP32_GUARD)
+            if not args.no_autotune:  # per-DB layouts and per-width costs for batching
+                d2.autotune([api.quantize_u8(im)[0] for c, d, im in serve_pool
+                             if (c, d) == (cr, det)][:3], max_batch=64, reps=2)
             dbs[(cr, det)] = d2
         keys = list(dbs)
         log(f"serving DBs ready in {time.time() - t0:.1f} s; HBM in use "
@@ -540,7 +567,7 @@ def main():
                    "grains_recovered": bool(found), "prefix_parity_vs_oracle": parity_ok,
                    "forward_ms": st.ms, "kernel_ms": kernel_ms,
                    "hbm_copy_GBps": copy_gbps,
-                   "banded_tile_bits": args.tile_bits},
+                   "single_kernel": single_kernel},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
         "cpu_baseline": cpu,

@@ -147,6 +147,7 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * (2^14 or 2^15 super-tiles).  Default 0. */
 #define LM_OPT_BANDED_LOADS 5
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
+int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
 /* Autotuner (no reference counterpart): times every scoring layout on the
  * sample images (nimg >= 1 host uint8 images of the DB's detector, repeated

@@ -91,6 +91,8 @@ SIGNATURES = {
     "lm_db_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(LmFwdStats)]),
     "lm_db_table_checksum": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "lm_db_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64]),
+    "lm_db_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int64)]),
     "lm_db_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
                                       ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(LmTuneReport)]),
     "lm_db_import_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),

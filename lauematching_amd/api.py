@@ -164,6 +164,15 @@ class OrientationDB:
         3 = image staged through LDS (2^14 / 2^15 super-tiles)."""
         check(lib().lm_db_set_option(self._h, 5, int(loads)))
 
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        check(lib().lm_db_get_option(self._h, int(option), ctypes.byref(v)))
+        return v.value
+
+    def banded_layout(self):
+        """(super-tile bits, walk) of the single-image layout in use."""
+        return self.get_option(1), self.get_option(5)
+
     def set_use_graphs(self, on: bool) -> None:
         """Replay captured launch sequences (hipGraph) in direct scoring calls."""
         check(lib().lm_db_set_option(self._h, 4, int(bool(on))))

@@ -406,6 +406,22 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
   return LM_ERR_INVALID;
 }
 
+int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
+  if (!db || !value) {
+    set_error("lm_db_get_option: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(db->mu);
+  switch (option) {
+    case LM_OPT_BANDED_TILE_BITS: *value = db->banded_tb; return LM_OK;
+    case LM_OPT_BATCH_TILE_BITS: *value = db->batch_tb; return LM_OK;
+    case LM_OPT_BATCH_WIDTHS: *value = db->batch_widths; return LM_OK;
+    case LM_OPT_USE_GRAPHS: *value = db->scratch.use_graphs ? 1 : 0; return LM_OK;
+    case LM_OPT_BANDED_LOADS: *value = db->banded_loads; return LM_OK;
+    default: set_error("unknown option"); return LM_ERR_INVALID;
+  }
+}
+
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out) {
   if (!db || !images || nimg < 1 || !out || max_batch < 1 || reps < 1) {

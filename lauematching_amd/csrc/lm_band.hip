@@ -312,7 +312,7 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // cache lines); here each wave owns a 2^CB-byte slot of LDS and walks the
 // super-tile chunk by chunk (chunk c = pixels [c << CB, (c+1) << CB), the
 // entries of chunk c are chunkOff[c] .. chunkOff[c+1] of the pixel-sorted
-// super-tile): one coalesced 16-byte load per lane brings the chunk's pixels,
+// super-tile): coalesced 16-byte loads (2^CB / 1024 per lane) bring the chunk's pixels,
 // the chunk's entries come in EMAX coalesced loads per lane, and each entry
 // reads its pixel from LDS and adds into the shared row counters.  Wave w
 // takes chunks w, w + 16, ...; the next chunk's loads are issued before the
@@ -331,7 +331,9 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   constexpr int NW = THREADS / 64;
   constexpr int PB = 32 - TB;
   constexpr int C = 1 << CB;
-  static_assert(C == 64 * 16, "one 16-byte load per lane fills a wave's chunk");
+  constexpr int V = C / (64 * 16);  // 16-byte image loads per lane per chunk
+  static_assert(V >= 1 && V * 64 * 16 == C, "chunk must be a multiple of 1 KiB");
+  static_assert(T * 4 + NW * C <= 163840, "LDS");
   __shared__ uint32_t cnt[T];
   __shared__ v4u slot[NW][C / 16];
   const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
@@ -342,14 +344,17 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     const uint64_t first = t << TB;
     const uint32_t *src = bentries + tileOff[first >> kTileBits];
     const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
-    v4u pim = v4u{0u, 0u, 0u, 0u};
+    v4u pim[V];
     uint32_t pe[EMAX];
     uint32_t pbeg = 0, pend = 0;
     auto fetch = [&](int c) {
       pbeg = co[c];
       pend = co[c + 1];
-      const uint64_t p0 = ((uint64_t)c << CB) + 16ull * lane;
-      pim = p0 + 16 <= npx ? *(const v4u *)(img + p0) : v4u{0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const uint64_t p0 = ((uint64_t)c << CB) + 16ull * (v * 64 + lane);
+        pim[v] = p0 + 16 <= npx ? *(const v4u *)(img + p0) : v4u{0u, 0u, 0u, 0u};
+      }
 #pragma unroll
       for (int k = 0; k < EMAX; ++k) {
         const uint32_t i = pbeg + k * 64 + lane;
@@ -359,14 +364,17 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     int c = w;
     if (c < nchunks) fetch(c);
     while (c < nchunks) {
-      const v4u cim = pim;
+      v4u cim[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) cim[v] = pim[v];
       uint32_t ce[EMAX];
 #pragma unroll
       for (int k = 0; k < EMAX; ++k) ce[k] = pe[k];
       const uint32_t cbeg = pbeg, cend = pend;
       const int cn = c + NW;
       if (cn < nchunks) fetch(cn);
-      slot[w][lane] = cim;
+#pragma unroll
+      for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = cim[v];
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int k = 0; k < EMAX; ++k) {
@@ -582,8 +590,9 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
   const uint64_t ebytes = (t.total + 64) * 4;
   const uint64_t obytes = b.nsuper * (uint64_t)(b.nbands + 1) * 4;
   // chunk offsets for the LDS-staged walk (chunks must be whole buckets)
-  chunks = chunks && bb <= kWaveChunkBits;
-  b.nchunks = chunks ? (int)((npx + (1ull << kWaveChunkBits) - 1) >> kWaveChunkBits) : 0;
+  const int cb = wave_chunk_bits(tb);
+  chunks = chunks && bb <= cb;
+  b.nchunks = chunks ? (int)((npx + (1ull << cb) - 1) >> cb) : 0;
   const uint64_t cbytes = chunks ? b.nsuper * (uint64_t)(b.nchunks + 1) * 4 : 0;
   int rc = reserve_bytes(db, ebytes + obytes + cbytes);
   if (rc) {
@@ -599,7 +608,7 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
   if (e == hipSuccess) {
     hipLaunchKernelGGL(band_build_kernel, dim3((unsigned)b.nsuper), dim3(kBuildThreads), 0, st,
                        t.counts, t.tile_off, t.entries, db->n, tb, b.pb, bb, nbuckets, b.nbands,
-                       b.entries, b.band_off, kWaveChunkBits, b.nchunks, b.chunk_off);
+                       b.entries, b.band_off, cb, b.nchunks, b.chunk_off);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -653,13 +662,13 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
       return LM_ERR_INVALID;
     }
     if (b.tb == 15)
-      hipLaunchKernelGGL((score_band_lds_kernel<15, kWaveChunkBits, 4>), dim3(grid), dim3(1024), 0,
-                         st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n, b.nsuper, img,
-                         npx, sc.pred, sc.hits, cap, sc.counters);
+      hipLaunchKernelGGL((score_band_lds_kernel<15, wave_chunk_bits(15), 4>), dim3(grid),
+                         dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,
+                         b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters);
     else
-      hipLaunchKernelGGL((score_band_lds_kernel<14, kWaveChunkBits, 4>), dim3(grid), dim3(1024), 0,
-                         st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n, b.nsuper, img,
-                         npx, sc.pred, sc.hits, cap, sc.counters);
+      hipLaunchKernelGGL((score_band_lds_kernel<14, wave_chunk_bits(14), 4>), dim3(grid),
+                         dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,
+                         b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
     return LM_OK;

@@ -44,7 +44,7 @@ struct Table {
 struct Banded {
   uint32_t *entries = nullptr;   // same length as Table::entries
   uint32_t *band_off = nullptr;  // nsuper * (nbands + 1)
-  uint32_t *chunk_off = nullptr; // nsuper * (nchunks + 1): offsets of 2^kWaveChunkBits-pixel chunks
+  uint32_t *chunk_off = nullptr; // nsuper * (nchunks + 1): offsets of 2^wave_chunk_bits(tb)-pixel chunks
   uint64_t nsuper = 0;
   int tb = 0;
   int pb = 0;
@@ -54,7 +54,9 @@ struct Banded {
   bool ready = false;
 };
 constexpr int kBucketBits = 10;
-constexpr int kWaveChunkBits = 10;  // LDS-staged walk: pixels per wave chunk (log2)
+// LDS-staged walk: pixels per wave chunk (log2) for a super-tile of 2^tb rows;
+// 16 chunk slots and 2^tb row counters fill at most 160 KiB of LDS.
+constexpr int wave_chunk_bits(int tb) { return tb >= 15 ? 11 : 12; }
 
 struct Geom {
   double recip[9];
