@@ -338,11 +338,14 @@ def kernel_name(db):
     tb, loads = db.banded_layout()
     if tb == 0:
         return "score_kernel<1>"
-    if loads == 3 and tb >= 14:
-        return f"score_band_lds_kernel<{tb},10,4>"
+    if loads in (3, 5) and tb >= 14:
+        return (f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},"
+                f"{'8,2' if loads == 5 else '4,1'}>")
     threads = 1024 if tb >= 14 else 512
     if loads == 0:
         return f"score_band_kernel<{tb},{threads},8>"
+    if loads == 4:
+        return f"score_band_kernel<{tb},{threads},8,true>"
     return f"score_band_v4_kernel<{tb},{threads},{2 if loads == 1 else 4}>"
 
 

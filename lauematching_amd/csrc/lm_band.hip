@@ -129,7 +129,7 @@ band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restric
 // decreases).  The next group's entry loads are issued after the current
 // group's gathers, so they stay in flight across the gather wait and the LDS
 // adds.
-template <int TB, int THREADS, int U>
+template <int TB, int THREADS, int U, bool WIDE = false>
 __global__ void __launch_bounds__(THREADS)
 score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                   const uint32_t *__restrict__ bandOff, int nbands, uint64_t n, uint64_t nsuper,
@@ -169,7 +169,14 @@ score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restr
             ++band;
             bend = bo[band + 1];
           }
-          raw[k] = img[((uint64_t)band << PB) | (e[k] & PMASK)];
+          const uint64_t px = ((uint64_t)band << PB) | (e[k] & PMASK);
+          if constexpr (WIDE) {
+            // the aligned word holding the pixel: a 4-byte gather per lane
+            const uint32_t wd = *(const uint32_t *)(img + (px & ~3ull));
+            raw[k] = (wd >> (8 * (uint32_t)(px & 3))) & 0xffu;
+          } else {
+            raw[k] = img[px];
+          }
         }
       }
 #pragma unroll
@@ -313,12 +320,13 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // super-tile chunk by chunk (chunk c = pixels [c << CB, (c+1) << CB), the
 // entries of chunk c are chunkOff[c] .. chunkOff[c+1] of the pixel-sorted
 // super-tile): coalesced 16-byte loads (2^CB / 1024 per lane) bring the chunk's pixels,
-// the chunk's entries come in EMAX coalesced loads per lane, and each entry
-// reads its pixel from LDS and adds into the shared row counters.  Wave w
-// takes chunks w, w + 16, ...; the next chunk's loads are issued before the
-// current chunk is processed.  LDS traffic of a wave is ordered, so its slot
-// needs no barrier between chunks.
-template <int TB, int CB, int EMAX>
+// the chunk's entries come in EMAX coalesced loads per lane (a chunk with
+// more entries loads the rest as it goes), and each entry reads its pixel
+// from LDS and adds into the shared row counters.  Wave w takes chunks
+// w, w + 16, ...; the loads of its next D chunks are in flight while one is
+// processed.  LDS traffic of a wave is ordered, so its slot needs no barrier
+// between chunks.
+template <int TB, int CB, int EMAX, int D>
 __global__ void __launch_bounds__(1024)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                       const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
@@ -344,35 +352,36 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     const uint64_t first = t << TB;
     const uint32_t *src = bentries + tileOff[first >> kTileBits];
     const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
-    v4u pim[V];
-    uint32_t pe[EMAX];
-    uint32_t pbeg = 0, pend = 0;
-    auto fetch = [&](int c) {
-      pbeg = co[c];
-      pend = co[c + 1];
+    // D sets of prefetch registers: chunk c + d*NW is in flight in set d
+    v4u pim[D][V];
+    uint32_t pe[D][EMAX];
+    uint32_t pbeg[D], pend[D];
+    auto fetch = [&](int d, int c) {
+      pbeg[d] = co[c];
+      pend[d] = co[c + 1];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         const uint64_t p0 = ((uint64_t)c << CB) + 16ull * (v * 64 + lane);
-        pim[v] = p0 + 16 <= npx ? *(const v4u *)(img + p0) : v4u{0u, 0u, 0u, 0u};
+        pim[d][v] = p0 + 16 <= npx ? *(const v4u *)(img + p0) : v4u{0u, 0u, 0u, 0u};
       }
 #pragma unroll
       for (int k = 0; k < EMAX; ++k) {
-        const uint32_t i = pbeg + k * 64 + lane;
-        pe[k] = i < pend ? __builtin_nontemporal_load(src + i) : 0u;
+        const uint32_t i = pbeg[d] + k * 64 + lane;
+        pe[d][k] = i < pend[d] ? __builtin_nontemporal_load(src + i) : 0u;
       }
     };
-    int c = w;
-    if (c < nchunks) fetch(c);
-    while (c < nchunks) {
+    // consume set d (chunk c): image into this wave's slot, entries into the
+    // counters; then refill set d with chunk c + D*NW
+    auto consume = [&](int d, int c) {
       v4u cim[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) cim[v] = pim[v];
       uint32_t ce[EMAX];
 #pragma unroll
-      for (int k = 0; k < EMAX; ++k) ce[k] = pe[k];
-      const uint32_t cbeg = pbeg, cend = pend;
-      const int cn = c + NW;
-      if (cn < nchunks) fetch(cn);
+      for (int v = 0; v < V; ++v) cim[v] = pim[d][v];
+#pragma unroll
+      for (int k = 0; k < EMAX; ++k) ce[k] = pe[d][k];
+      const uint32_t cbeg = pbeg[d], cend = pend[d];
+      const int cn = c + D * NW;
+      if (cn < nchunks) fetch(d, cn);
 #pragma unroll
       for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = cim[v];
       __builtin_amdgcn_wave_barrier();
@@ -391,7 +400,14 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
       }
       __builtin_amdgcn_wave_barrier();
-      c = cn;
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      if (w + d * NW < nchunks) fetch(d, w + d * NW);
+    for (int c = w; c < nchunks; c += D * NW) {
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        if (c + d * NW < nchunks) consume(d, c + d * NW);
     }
     __syncthreads();
     for (int rr = tid; rr < T; rr += THREADS) {
@@ -656,26 +672,40 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
   // the LDS-staged walk loads the image in 16-byte pieces; an image that is
   // not 16-byte aligned (a caller's device pointer) takes the gather walk
-  if (db->banded_loads == 3 && ((uintptr_t)img & 15) == 0) {
+  if ((db->banded_loads == 3 || db->banded_loads == 5) && ((uintptr_t)img & 15) == 0) {
     if (!b.chunk_off || (b.tb != 14 && b.tb != 15)) {
       set_error("LDS-staged walk needs 2^14 or 2^15 super-tiles and chunk offsets");
       return LM_ERR_INVALID;
     }
-    if (b.tb == 15)
-      hipLaunchKernelGGL((score_band_lds_kernel<15, wave_chunk_bits(15), 4>), dim3(grid),
-                         dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,
-                         b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters);
-    else
-      hipLaunchKernelGGL((score_band_lds_kernel<14, wave_chunk_bits(14), 4>), dim3(grid),
-                         dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,
-                         b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters);
+#define LM_LDS(TB, EM, DD)                                                                       \
+  hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD>), dim3(grid),        \
+                     dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,     \
+                     b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
+    const bool deep = db->banded_loads == 5;
+    if (b.tb == 15) {
+      if (deep) LM_LDS(15, 8, 2); else LM_LDS(15, 4, 1);
+    } else {
+      if (deep) LM_LDS(14, 8, 2); else LM_LDS(14, 4, 1);
+    }
+#undef LM_LDS
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
     return LM_OK;
   }
-  const int key = b.tb * 10 + (db->banded_loads == 3 ? 0 : db->banded_loads);
+  // 4-byte gathers read whole words of the image: it must be 4-byte aligned
+  // and its size a multiple of 4 (else the byte-gather walk is used)
+  int loads = (db->banded_loads == 3 || db->banded_loads == 5) ? 0 : db->banded_loads;
+  if (loads == 4 && (((uintptr_t)img & 3) || (npx & 3))) loads = 0;
+  const int key = b.tb * 10 + loads;
+#define LM_SBW(TB, TH)                                                                           \
+  hipLaunchKernelGGL((score_band_kernel<TB, TH, 8, true>), dim3(grid), dim3(TH), 0, st, b.entries, \
+                     t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, sc.pred, sc.hits,   \
+                     cap, sc.counters)
   switch (key) {
     case 150: LM_SB(score_band_kernel, 15, 1024, 8); break;
+    case 154: LM_SBW(15, 1024); break;
+    case 144: LM_SBW(14, 1024); break;
+    case 134: LM_SBW(13, 512); break;
     case 151: LM_SB(score_band_v4_kernel, 15, 1024, 2); break;
     case 152: LM_SB(score_band_v4_kernel, 15, 1024, 4); break;
     case 140: LM_SB(score_band_kernel, 14, 1024, 8); break;
@@ -689,6 +719,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
       return LM_ERR_INVALID;
   }
 #undef LM_SB
+#undef LM_SBW
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
   return LM_OK;
