@@ -254,7 +254,7 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
     from lauematching_amd.serve import Server
 
     res = {}
-    for name, kw in [("batched", dict(max_batch=64, workers=2, use_graphs=True)),
+    for name, kw in [("batched", dict(max_batch=128, workers=2, use_graphs=True)),
                      ("plain", dict(max_batch=1, workers=1, use_graphs=False))]:
         srv = Server(ctx, **kw)
         ids = [srv.add_db(db) for db in dbs]
@@ -431,7 +431,7 @@ def main():
     if not args.no_autotune:
         samples = [u8] + [api.quantize_u8(im)[0] for im, _ in batch_imgs[:3]]
         t0 = time.time()
-        tune = db.autotune(samples, max_batch=64, reps=2)
+        tune = db.autotune(samples, max_batch=128, reps=2)
         tune["seconds"] = time.time() - t0
         log(f"autotune ({tune['seconds']:.1f} s): single {tune['ms_single']} -> "
             f"{tune['banded_tile_bits']}/{tune['banded_loads']}; batch tile bits "
@@ -519,7 +519,7 @@ def main():
             d2.forward(api.LM_FWD_FP32_GUARD)
             if not args.no_autotune:  # per-DB layouts and per-width costs for batching
                 d2.autotune([api.quantize_u8(im)[0] for c, d, im in serve_pool
-                             if (c, d) == (cr, det)][:3], max_batch=64, reps=2)
+                             if (c, d) == (cr, det)][:3], max_batch=128, reps=2)
             dbs[(cr, det)] = d2
         keys = list(dbs)
         log(f"serving DBs ready in {time.time() - t0:.1f} s; HBM in use "

@@ -160,12 +160,12 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
  * table; the timings are device milliseconds per launch sequence. */
 typedef struct lm_tune_report {
   int32_t n_widths;
-  int32_t widths[5];          /* batch widths timed (4 .. max_batch)             */
+  int32_t widths[6];          /* batch widths timed (4 .. max_batch <= 128)      */
   double ms_single_table;     /* one image: orientation-major table               */
   double ms_single[3][6];     /* one image: banded 2^13 / 2^14 / 2^15 x loads 0..5
                                  (0 = not available)                             */
-  double ms_lane[5];          /* per width: orientation-major table               */
-  double ms_banded[5][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
+  double ms_lane[6];          /* per width: orientation-major table               */
+  double ms_banded[6][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
   int32_t banded_tile_bits;   /* chosen LM_OPT_BANDED_TILE_BITS                   */
   int32_t banded_loads;       /* chosen LM_OPT_BANDED_LOADS                       */
   int32_t batch_tile_bits;    /* chosen LM_OPT_BATCH_TILE_BITS                    */
@@ -215,7 +215,7 @@ int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double 
 int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t min_nr_spots,
                  double min_intensity, void *stream, lm_hits *out);
 
-/* Score nimg host images against the DB in one pass over the table.
+/* Score nimg (1..128) host images against the DB in one pass over the table.
  * outs[i] receives image i's hits, identical to lm_score on that image. */
 int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
                    const double *im_scales, int32_t min_nr_spots, double min_intensity,
@@ -240,7 +240,7 @@ void lm_hits_free(lm_hits *h);
  * submit and must not be rebuilt while the server runs. */
 typedef struct lm_server lm_server;
 typedef struct lm_server_config {
-  int32_t max_batch;    /* images per launch, 1..64 (0 = 32)                      */
+  int32_t max_batch;    /* images per launch, 1..128 (0 = 32)                     */
   int32_t workers;      /* concurrent batches / HIP streams (0 = 2)              */
   int32_t max_inflight; /* admitted, not yet completed requests (0 = 2*workers*max_batch) */
   int32_t use_graphs;   /* replay captured launch sequences (hipGraph)           */

@@ -497,8 +497,8 @@ int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
   int rc = check_ctx(db->ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
-  if (nimg > 64) {
-    set_error("lm_score_batch: at most 64 images per call");
+  if (nimg > kMaxBatch) {
+    set_error("lm_score_batch: at most 128 images per call");
     return LM_ERR_INVALID;
   }
   uint8_t *d = nullptr;

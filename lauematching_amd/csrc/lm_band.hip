@@ -557,7 +557,7 @@ score_band_batch_kernel(const uint32_t *__restrict__ bentries,
           base = __shfl(base, leader);
           if (cand) {
             const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (pos < capPer) hits[(uint64_t)b * capPer + pos] = ((uint64_t)b << 58) | (o << 26) | acc;
+            if (pos < capPer) hits[(uint64_t)b * capPer + pos] = (o << 26) | acc;
           }
         }
       }

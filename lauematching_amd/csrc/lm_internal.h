@@ -67,7 +67,7 @@ struct Geom {
   double Elo, Ehi;
 };
 
-constexpr int kMaxBatch = 64;
+constexpr int kMaxBatch = 128;
 
 // Per-image predicate of one scoring launch, read by the kernels from device
 // memory (so a captured launch sequence stays valid for new thresholds).
@@ -75,7 +75,7 @@ struct Pred {
   double min_int[kMaxBatch];
   double cand_thr[kMaxBatch];  // candidate threshold on the integer sum
   int min_spots[kMaxBatch];
-  uint32_t exact_lo, exact_hi; // bit b: image b has imScale == 1
+  uint32_t exact[kMaxBatch / 32]; // bit b: image b has imScale == 1
   int nimg;
 };
 
@@ -139,7 +139,7 @@ struct lm_db {
   int banded_loads = 0;         // single-image walk: 0 scalar x8, 1 16-byte x2, 2 16-byte x4
   int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
-  double width_ms[7] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
+  double width_ms[8] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   std::mutex mu;

@@ -41,7 +41,7 @@ constexpr int kStageWords = 12288;  // 48 KB of staged entries per workgroup
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ bool bit(const Pred &p, int b) {
-  return b < 32 ? (p.exact_lo >> b) & 1u : (p.exact_hi >> (b - 32)) & 1u;
+  return (p.exact[b >> 5] >> (b & 31)) & 1u;
 }
 
 // Words holding one pixel of B interleaved images.
@@ -86,11 +86,22 @@ __device__ __forceinline__ void accum_px(const uint32_t (&w)[PxW<B>::W], uint32_
   }
 }
 
-// Accumulate the c spots at src[0..c) (LDS or global), four gathers in flight.
+// Accumulate the c spots at src[0..c) (LDS or global), four gathers in flight
+// (two for 128 images: 128 accumulators plus 2 x 32 words in flight).
 template <int B>
 __device__ __forceinline__ void accum_spots(const uint8_t *__restrict__ img, const uint32_t *src,
                                             uint32_t c, uint32_t (&acc)[B]) {
   uint32_t s = 0;
+  if constexpr (B >= 128) {
+    for (; s + 2 <= c; s += 2) {
+      uint32_t w0[PxW<B>::W], w1[PxW<B>::W];
+      const uint32_t p0 = src[s], p1 = src[s + 1];
+      load_px<B>(img, p0, w0);
+      load_px<B>(img, p1, w1);
+      accum_px<B>(w0, acc);
+      accum_px<B>(w1, acc);
+    }
+  }
   for (; s + 4 <= c; s += 4) {
     uint32_t w0[PxW<B>::W], w1[PxW<B>::W], w2[PxW<B>::W], w3[PxW<B>::W];
     const uint32_t p0 = src[s], p1 = src[s + 1], p2 = src[s + 2], p3 = src[s + 3];
@@ -194,7 +205,7 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
       }
       const uint64_t zm = __ballot(zero_pass);
       if (zm && lane == 0) atomicAdd(&counters[2 * b + 1], (uint32_t)__popcll(zm));
-      const uint64_t rec = ((uint64_t)b << 58) | (o << 26) | (uint64_t)acc[b];
+      const uint64_t rec = (o << 26) | (uint64_t)acc[b];  // image b's own region
       append(cand, rec, hits + (uint64_t)b * capPer, &counters[2 * b], capPer);
     }
     __syncthreads();
@@ -385,6 +396,7 @@ int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t cap
     case 16: LM_LAUNCH(16); break;
     case 32: LM_LAUNCH(32); break;
     case 64: LM_LAUNCH(64); break;
+    case 128: LM_LAUNCH(128); break;
     default: set_error("unsupported batch width"); return LM_ERR_INVALID;
   }
 #undef LM_LAUNCH
@@ -404,6 +416,7 @@ int launch_interleave(int B, const uint8_t *src, const Pred *pred, uint64_t npx,
     case 16: LM_IL(16); break;
     case 32: LM_IL(32); break;
     case 64: LM_IL(64); break;
+    case 128: LM_IL(128); break;
     default: set_error("unsupported batch width"); return LM_ERR_INVALID;
   }
 #undef LM_IL
@@ -563,7 +576,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     return LM_ERR_STATE;
   }
   if (nimg < 1 || nimg > kMaxBatch || !d_imgs || !im_scales || !outs) {
-    set_error("score: need 1..64 images");
+    set_error("score: need 1..128 images");
     return LM_ERR_INVALID;
   }
   for (int b = 0; b < nimg; ++b)
@@ -586,7 +599,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (s == 1.0) {
       // totInt == sum exactly: sum >= min_int decided on the device
       pred.cand_thr[b] = min_int[b];
-      if (b < 32) pred.exact_lo |= 1u << b; else pred.exact_hi |= 1u << (b - 32);
+      pred.exact[b >> 5] |= 1u << (b & 31);
     } else {
       // superset: the sequential float64 sum is within n*2^-53 relative of sum*s
       pred.cand_thr[b] = min_int[b] <= 0 ? -INFINITY : min_int[b] / s * (1.0 - 1e-12);
