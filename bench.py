@@ -340,7 +340,7 @@ def kernel_name(db):
         return "score_kernel<1>"
     if loads in (3, 5) and tb >= 14:
         return (f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},"
-                f"{'8,2' if loads == 5 else '4,1'}>")
+                f"{'8,3' if loads == 5 else '4,2'}>")
     threads = 1024 if tb >= 14 else 512
     if loads == 0:
         return f"score_band_kernel<{tb},{threads},8>"

@@ -323,9 +323,9 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // the chunk's entries come in EMAX coalesced loads per lane (a chunk with
 // more entries loads the rest as it goes), and each entry reads its pixel
 // from LDS and adds into the shared row counters.  Wave w takes chunks
-// w, w + 16, ...; the loads of its next D chunks are in flight while one is
-// processed.  LDS traffic of a wave is ordered, so its slot needs no barrier
-// between chunks.
+// w, w + 16, ...; the loads of its next D - 1 chunks are in flight while one
+// is processed.  LDS traffic of a wave is ordered, so its slot needs no
+// barrier between chunks.
 template <int TB, int CB, int EMAX, int D>
 __global__ void __launch_bounds__(1024)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
@@ -346,60 +346,60 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   __shared__ v4u slot[NW][C / 16];
   const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
   const uint8_t *mine = (const uint8_t *)&slot[w][0];
+  // buffer loads: wave-uniform base and range in SGPRs, 32-bit lane offsets;
+  // offsets past the range read as zero, so the loads need no guards
+  constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
+  const __amdgpu_buffer_rsrc_t irs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
     for (int i = tid; i < T; i += THREADS) cnt[i] = 0;
     __syncthreads();
     const uint64_t first = t << TB;
-    const uint32_t *src = bentries + tileOff[first >> kTileBits];
     const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
-    // D sets of prefetch registers: chunk c + d*NW is in flight in set d
+    const uint32_t total = co[nchunks];
+    const __amdgpu_buffer_rsrc_t ers = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(bentries + tileOff[first >> kTileBits]), (short)0, (int)(total * 4u),
+        kRawBufferWord3);
+    // D sets of prefetch registers; set d holds chunk c + d*NW while the
+    // other sets are processed
     v4u pim[D][V];
     uint32_t pe[D][EMAX];
     uint32_t pbeg[D], pend[D];
     auto fetch = [&](int d, int c) {
       pbeg[d] = co[c];
       pend[d] = co[c + 1];
+      const uint32_t ib = (uint32_t)c << CB;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const uint64_t p0 = ((uint64_t)c << CB) + 16ull * (v * 64 + lane);
-        pim[d][v] = p0 + 16 <= npx ? *(const v4u *)(img + p0) : v4u{0u, 0u, 0u, 0u};
-      }
+      for (int v = 0; v < V; ++v)
+        pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
 #pragma unroll
-      for (int k = 0; k < EMAX; ++k) {
-        const uint32_t i = pbeg[d] + k * 64 + lane;
-        pe[d][k] = i < pend[d] ? __builtin_nontemporal_load(src + i) : 0u;
-      }
+      for (int k = 0; k < EMAX; ++k)
+        pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (pbeg[d] + k * 64 + lane) * 4u, 0, 0);
     };
-    // consume set d (chunk c): image into this wave's slot, entries into the
-    // counters; then refill set d with chunk c + D*NW
+    // process set d (chunk c): its pixels into this wave's slot, its entries
+    // into the counters; then reload set d with chunk c + D*NW
     auto consume = [&](int d, int c) {
-      v4u cim[V];
-      uint32_t ce[EMAX];
 #pragma unroll
-      for (int v = 0; v < V; ++v) cim[v] = pim[d][v];
-#pragma unroll
-      for (int k = 0; k < EMAX; ++k) ce[k] = pe[d][k];
-      const uint32_t cbeg = pbeg[d], cend = pend[d];
-      const int cn = c + D * NW;
-      if (cn < nchunks) fetch(d, cn);
-#pragma unroll
-      for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = cim[v];
+      for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = pim[d][v];
       __builtin_amdgcn_wave_barrier();
+      const uint32_t cbeg = pbeg[d], cend = pend[d];
 #pragma unroll
       for (int k = 0; k < EMAX; ++k) {
         const uint32_t i = cbeg + k * 64 + lane;
         if (i < cend) {
-          const uint32_t e = ce[k];
+          const uint32_t e = pe[d][k];
           const uint32_t raw = mine[e & (C - 1)];
           if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
         }
       }
       for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {  // dense chunks
-        const uint32_t e = src[i];
+        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
         const uint32_t raw = mine[e & (C - 1)];
         if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
       }
       __builtin_amdgcn_wave_barrier();
+      const int cn = c + D * NW;
+      if (cn < nchunks) fetch(d, cn);
     };
 #pragma unroll
     for (int d = 0; d < D; ++d)
@@ -683,9 +683,9 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
                      b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
     const bool deep = db->banded_loads == 5;
     if (b.tb == 15) {
-      if (deep) LM_LDS(15, 8, 2); else LM_LDS(15, 4, 1);
+      if (deep) LM_LDS(15, 8, 3); else LM_LDS(15, 4, 2);
     } else {
-      if (deep) LM_LDS(14, 8, 2); else LM_LDS(14, 4, 1);
+      if (deep) LM_LDS(14, 8, 3); else LM_LDS(14, 4, 2);
     }
 #undef LM_LDS
     e = hipGetLastError();
