@@ -121,6 +121,59 @@ __device__ __forceinline__ void accum_spots(const uint8_t *__restrict__ img, con
   }
 }
 
+// Packed (SWAR) accumulation of four images per 32-bit word: for the word w
+// holding pixel bytes x0..x3 of images 4j..4j+3, s02 gathers x0 and x2 in two
+// 16-bit sums, s13 gathers x1 and x3, and cn counts the nonzero bytes in four
+// 8-bit fields.  Valid while a row has at most 255 spots (8-bit counts; sums
+// stay below 255 * 255 < 2^16).  A byte x is nonzero exactly when bit 7 of
+// ((x & 0x7f) + 0x7f) | x is set.
+template <int B>
+struct Swar {
+  static constexpr int J = B / 4;
+  uint32_t s02[J], s13[J], cn[J];
+};
+
+template <int B>
+__device__ __forceinline__ void swar_px(const uint32_t (&w)[PxW<B>::W], Swar<B> &a) {
+#pragma unroll
+  for (int j = 0; j < B / 4; ++j) {
+    const uint32_t x = w[j];
+    a.s02[j] += x & 0x00ff00ffu;
+    a.s13[j] += (x >> 8) & 0x00ff00ffu;
+    const uint32_t nz = ((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x;
+    a.cn[j] += (nz >> 7) & 0x01010101u;
+  }
+}
+
+template <int B>
+__device__ __forceinline__ void swar_spots(const uint8_t *__restrict__ img, const uint32_t *src,
+                                           uint32_t c, Swar<B> &a) {
+  constexpr int G = B >= 128 ? 2 : 4;  // gathers in flight
+  uint32_t s = 0;
+  for (; s + G <= c; s += G) {
+    uint32_t w[G][PxW<B>::W];
+#pragma unroll
+    for (int g = 0; g < G; ++g) load_px<B>(img, src[s + g], w[g]);
+#pragma unroll
+    for (int g = 0; g < G; ++g) swar_px<B>(w[g], a);
+  }
+  for (; s < c; ++s) {
+    uint32_t w[PxW<B>::W];
+    load_px<B>(img, src[s], w);
+    swar_px<B>(w, a);
+  }
+}
+
+// Image b's packed accumulator (nspots << 17 | sum) from the SWAR fields.
+template <int B>
+__device__ __forceinline__ uint32_t swar_acc(const Swar<B> &a, int b) {
+  const int j = b >> 2, q = b & 3;
+  const uint32_t s = (q & 1) ? a.s13[j] : a.s02[j];
+  const uint32_t sum = (q & 2) ? (s >> 16) : (s & 0xffffu);
+  const uint32_t ns = (a.cn[j] >> (8 * q)) & 0xffu;
+  return (ns << kSumBits) | sum;
+}
+
 // Wave-aggregated append: one atomic per wave, lanes write at base + rank.
 __device__ __forceinline__ void append(bool flag, uint64_t rec, uint64_t *__restrict__ out,
                                        uint32_t *__restrict__ counter, uint64_t cap) {
@@ -140,7 +193,7 @@ __device__ __forceinline__ void append(bool flag, uint64_t rec, uint64_t *__rest
 
 // counters layout: [2*b] = hits written for image b, [2*b+1] = zero-spot rows
 // passing the predicate for image b.
-template <int B>
+template <int B, bool SW>
 __global__ void __launch_bounds__(kTile)
 score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
              const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
@@ -183,17 +236,10 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
       }
     }
     __syncthreads();
-    uint32_t acc[B];
-#pragma unroll
-    for (int b = 0; b < B; ++b) acc[b] = 0;
-    if (staged)
-      accum_spots<B>(img, stage + excl, c, acc);
-    else
-      accum_spots<B>(img, entries + base + excl, c, acc);
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const uint32_t ns = acc[b] >> kSumBits;
-      const uint32_t sum = acc[b] & kSumMask;
+    // predicate and compaction of image b's accumulator
+    auto finish = [&](int b, uint32_t accb) {
+      const uint32_t ns = accb >> kSumBits;
+      const uint32_t sum = accb & kSumMask;
       const bool live = o < n;
       bool cand = false, zero_pass = false;
       if (live && b < pred->nimg) {
@@ -205,8 +251,29 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
       }
       const uint64_t zm = __ballot(zero_pass);
       if (zm && lane == 0) atomicAdd(&counters[2 * b + 1], (uint32_t)__popcll(zm));
-      const uint64_t rec = (o << 26) | (uint64_t)acc[b];  // image b's own region
+      const uint64_t rec = (o << 26) | (uint64_t)accb;  // image b's own region
       append(cand, rec, hits + (uint64_t)b * capPer, &counters[2 * b], capPer);
+    };
+    if constexpr (SW) {
+      Swar<B> a;
+#pragma unroll
+      for (int j = 0; j < B / 4; ++j) a.s02[j] = a.s13[j] = a.cn[j] = 0;
+      if (staged)
+        swar_spots<B>(img, stage + excl, c, a);
+      else
+        swar_spots<B>(img, entries + base + excl, c, a);
+#pragma unroll
+      for (int b = 0; b < B; ++b) finish(b, swar_acc<B>(a, b));
+    } else {
+      uint32_t acc[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) acc[b] = 0;
+      if (staged)
+        accum_spots<B>(img, stage + excl, c, acc);
+      else
+        accum_spots<B>(img, entries + base + excl, c, acc);
+#pragma unroll
+      for (int b = 0; b < B; ++b) finish(b, acc[b]);
     }
     __syncthreads();
   }
@@ -386,19 +453,25 @@ int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t cap
                  hipStream_t st) {
   const Table &t = db->table;
   const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
-#define LM_LAUNCH(BB)                                                                           \
-  hipLaunchKernelGGL(score_kernel<BB>, dim3(grid), dim3(kTile), 0, st, t.counts, t.tile_off,   \
-                     t.entries, db->n, t.ntiles, img, sc.pred, sc.hits, capPer, sc.counters)
+  // packed four-image accumulators need at most 255 spots per row
+  const bool sw = db->max_spots <= 255;
+#define LM_LAUNCH(BB, SWW)                                                                      \
+  hipLaunchKernelGGL((score_kernel<BB, SWW>), dim3(grid), dim3(kTile), 0, st, t.counts,        \
+                     t.tile_off, t.entries, db->n, t.ntiles, img, sc.pred, sc.hits, capPer,      \
+                     sc.counters)
+#define LM_LAUNCH2(BB) \
+  if (sw) LM_LAUNCH(BB, true); else LM_LAUNCH(BB, false)
   switch (B) {
-    case 1: LM_LAUNCH(1); break;
-    case 4: LM_LAUNCH(4); break;
-    case 8: LM_LAUNCH(8); break;
-    case 16: LM_LAUNCH(16); break;
-    case 32: LM_LAUNCH(32); break;
-    case 64: LM_LAUNCH(64); break;
-    case 128: LM_LAUNCH(128); break;
+    case 1: LM_LAUNCH(1, false); break;
+    case 4: LM_LAUNCH2(4); break;
+    case 8: LM_LAUNCH2(8); break;
+    case 16: LM_LAUNCH2(16); break;
+    case 32: LM_LAUNCH2(32); break;
+    case 64: LM_LAUNCH2(64); break;
+    case 128: LM_LAUNCH2(128); break;
     default: set_error("unsupported batch width"); return LM_ERR_INVALID;
   }
+#undef LM_LAUNCH2
 #undef LM_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "score kernel launch");
