@@ -338,9 +338,9 @@ def kernel_name(db):
     tb, loads = db.banded_layout()
     if tb == 0:
         return "score_kernel<1>"
-    if loads in (3, 5) and tb >= 14:
-        return (f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},"
-                f"{'8,3' if loads == 5 else '4,2'}>")
+    if (loads == 3 or loads >= 5) and tb >= 14:
+        shape = {3: "4,2,0", 5: "8,3,0", 6: "8,3,2", 7: "16,2,2"}[loads]
+        return f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},{shape}>"
     threads = 1024 if tb >= 14 else 512
     if loads == 0:
         return f"score_band_kernel<{tb},{threads},8>"

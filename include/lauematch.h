@@ -145,8 +145,9 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * loads per lane in flight, 1 = two 16-byte loads (8 entries), 2 = four
  * 16-byte loads (16 entries), 3 = the image staged through LDS in per-wave
  * chunks (2^14 or 2^15 super-tiles), 4 = as 0 with each pixel read through a
- * 4-byte gather of its aligned word, 5 = as 3 with two chunks in flight per
- * wave.  Default 0. */
+ * 4-byte gather of its aligned word, 5 = as 3 with 8 entry loads per lane and
+ * two chunks in flight per wave, 6 = as 5 with streaming entry loads, 7 = 16
+ * streaming entry loads per lane and one chunk in flight.  Default 0. */
 #define LM_OPT_BANDED_LOADS 5
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
@@ -162,7 +163,7 @@ typedef struct lm_tune_report {
   int32_t n_widths;
   int32_t widths[6];          /* batch widths timed (4 .. max_batch <= 128)      */
   double ms_single_table;     /* one image: orientation-major table               */
-  double ms_single[3][6];     /* one image: banded 2^13 / 2^14 / 2^15 x loads 0..5
+  double ms_single[3][8];     /* one image: banded 2^13 / 2^14 / 2^15 x loads 0..7
                                  (0 = not available)                             */
   double ms_lane[6];          /* per width: orientation-major table               */
   double ms_banded[6][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */

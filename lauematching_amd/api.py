@@ -162,7 +162,8 @@ class OrientationDB:
     def set_banded_loads(self, loads: int) -> None:
         """Single-image banded walk: 0 = 4-byte loads x8, 1 = 16-byte x2, 2 = 16-byte x4,
         3 = image staged through LDS (2^14 / 2^15 super-tiles), 4 = 4-byte gathers,
-        5 = LDS-staged with two chunks in flight."""
+        5 = LDS-staged, 8 entries per lane, two chunks in flight, 6 = as 5 with
+        streaming entry loads, 7 = LDS-staged, 16 streaming entries per lane."""
         check(lib().lm_db_set_option(self._h, 5, int(loads)))
 
     def get_option(self, option: int) -> int:
@@ -192,7 +193,7 @@ class OrientationDB:
         widths = [r.widths[i] for i in range(r.n_widths)]
         single = {"table": r.ms_single_table}
         for i, tb in enumerate((13, 14, 15)):
-            for loads in range(6):
+            for loads in range(8):
                 if r.ms_single[i][loads] > 0:
                     single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]
         return {

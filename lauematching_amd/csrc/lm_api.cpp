@@ -390,8 +390,8 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     return LM_OK;
   }
   if (option == LM_OPT_BANDED_LOADS) {
-    if (value < 0 || value > 5) {
-      set_error("LM_OPT_BANDED_LOADS must be 0..5");
+    if (value < 0 || value > 7) {
+      set_error("LM_OPT_BANDED_LOADS must be 0..7");
       return LM_ERR_INVALID;
     }
     db->banded_loads = (int)value;

@@ -326,7 +326,7 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // w, w + 16, ...; the loads of its next D - 1 chunks are in flight while one
 // is processed.  LDS traffic of a wave is ordered, so its slot needs no
 // barrier between chunks.
-template <int TB, int CB, int EMAX, int D>
+template <int TB, int CB, int EMAX, int D, int AUX>
 __global__ void __launch_bounds__(1024)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                       const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
@@ -374,7 +374,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
 #pragma unroll
       for (int k = 0; k < EMAX; ++k)
-        pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (pbeg[d] + k * 64 + lane) * 4u, 0, 0);
+        pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (pbeg[d] + k * 64 + lane) * 4u, 0, AUX);
     };
     // process set d (chunk c): its pixels into this wave's slot, its entries
     // into the counters; then reload set d with chunk c + D*NW
@@ -393,7 +393,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         }
       }
       for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {  // dense chunks
-        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
+        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, AUX);
         const uint32_t raw = mine[e & (C - 1)];
         if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
       }
@@ -672,21 +672,32 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
   // the LDS-staged walk loads the image in 16-byte pieces; an image that is
   // not 16-byte aligned (a caller's device pointer) takes the gather walk
-  if ((db->banded_loads == 3 || db->banded_loads == 5) && ((uintptr_t)img & 15) == 0) {
+  const int L = db->banded_loads;
+  const bool lds_walk = L == 3 || L >= 5;
+  if (lds_walk && ((uintptr_t)img & 15) == 0) {
     if (!b.chunk_off || (b.tb != 14 && b.tb != 15)) {
       set_error("LDS-staged walk needs 2^14 or 2^15 super-tiles and chunk offsets");
       return LM_ERR_INVALID;
     }
-#define LM_LDS(TB, EM, DD)                                                                       \
-  hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD>), dim3(grid),        \
+    // shapes: entries per lane per chunk, chunks in flight, cache policy of
+    // the entry loads (2 = streaming)
+#define LM_LDS(TB, EM, DD, AX)                                                                   \
+  hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, AX>), dim3(grid),    \
                      dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,     \
                      b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
-    const bool deep = db->banded_loads == 5;
+#define LM_LDS_ALL(TB)                    \
+  switch (L) {                            \
+    case 3: LM_LDS(TB, 4, 2, 0); break;   \
+    case 5: LM_LDS(TB, 8, 3, 0); break;   \
+    case 6: LM_LDS(TB, 8, 3, 2); break;   \
+    default: LM_LDS(TB, 16, 2, 2); break; \
+  }
     if (b.tb == 15) {
-      if (deep) LM_LDS(15, 8, 3); else LM_LDS(15, 4, 2);
+      LM_LDS_ALL(15);
     } else {
-      if (deep) LM_LDS(14, 8, 3); else LM_LDS(14, 4, 2);
+      LM_LDS_ALL(14);
     }
+#undef LM_LDS_ALL
 #undef LM_LDS
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
@@ -694,7 +705,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   }
   // 4-byte gathers read whole words of the image: it must be 4-byte aligned
   // and its size a multiple of 4 (else the byte-gather walk is used)
-  int loads = (db->banded_loads == 3 || db->banded_loads == 5) ? 0 : db->banded_loads;
+  int loads = lds_walk ? 0 : L;
   if (loads == 4 && (((uintptr_t)img & 3) || (npx & 3))) loads = 0;
   const int key = b.tb * 10 + loads;
 #define LM_SBW(TB, TH)                                                                           \

@@ -81,8 +81,9 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     db->banded_tb = 13 + i;
     rc = build_banded(db);
     if (rc) return restore(rc);
-    for (int loads = 0; loads < 6; ++loads) {
-      if ((loads == 3 || loads == 5) && (!db->banded.chunk_off || db->banded_tb == 13)) continue;
+    for (int loads = 0; loads < 8; ++loads) {
+      const bool lds_walk = loads == 3 || loads >= 5;
+      if (lds_walk && (!db->banded.chunk_off || db->banded_tb == 13)) continue;
       db->banded_loads = loads;
       ++db->generation;
       rc = time_batch(db, d, 1, reps, &rep->ms_single[i][loads]);

@@ -217,7 +217,8 @@ def test_score_matches_oracle_unit_scale(scored_case, min_sp, min_int):
 
 @pytest.mark.parametrize("bits,loads", [(0, 0), (13, 0), (14, 0), (15, 0), (13, 1), (14, 1),
                                         (15, 1), (13, 2), (14, 2), (15, 2), (14, 3), (15, 3),
-                                        (13, 4), (14, 4), (15, 4), (14, 5), (15, 5)])
+                                        (13, 4), (14, 4), (15, 4), (14, 5), (15, 5), (14, 6),
+                                        (15, 6), (14, 7), (15, 7)])
 def test_score_layouts_agree(scored_case, bits, loads):
     """Banded copy with 8K, 16K or 32K super-tiles walked with 4-byte or 16-byte
     entry loads, and no banded copy (the orientation-major kernel): identical
@@ -401,7 +402,7 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
     try:
         assert set(rep["ms_batch"]) == {4, 8, 16, 32}  # max_batch=32
         singles = rep["ms_single"]
-        assert len(singles) == 17 and all(v > 0 for v in singles.values())
+        assert len(singles) == 21 and all(v > 0 for v in singles.values())
         tb = rep["banded_tile_bits"]
         chosen = "table" if tb == 0 else f"banded{tb}_loads{rep['banded_loads']}"
         assert singles[chosen] == min(singles.values())
