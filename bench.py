@@ -259,6 +259,10 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
         srv = Server(ctx, **kw)
         ids = [srv.add_db(db) for db in dbs]
         rq = [(ids[i], u8, s) for i, u8, s in reqs]
+        # warm-up (buffers, captured launch sequences of each width), not measured
+        closed_loop(srv, rq, 4 * kw["max_batch"] + 16, window=2 * kw["max_batch"])
+        for t in [srv.submit(*rq[k % len(rq)], 5, 1000.0) for k in range(3)]:
+            srv.wait(t)
         runs = [poisson_run(srv, rq, r, duration, 100 + j) for j, r in enumerate(rates)]
         cl = closed_loop(srv, rq, nmax)
         st = srv.stats()
@@ -339,7 +343,7 @@ def kernel_name(db):
     if tb == 0:
         return "score_kernel<1>"
     if (loads == 3 or loads >= 5) and tb >= 14:
-        shape = {3: "4,2,0", 5: "8,3,0", 6: "8,3,2", 7: "16,2,2"}[loads]
+        shape = {3: "4,2,1024", 5: "8,3,1024", 6: "6,4,1024", 7: "8,6,512"}[loads]
         return f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},{shape}>"
     threads = 1024 if tb >= 14 else 512
     if loads == 0:

@@ -326,8 +326,8 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // w, w + 16, ...; the loads of its next D - 1 chunks are in flight while one
 // is processed.  LDS traffic of a wave is ordered, so its slot needs no
 // barrier between chunks.
-template <int TB, int CB, int EMAX, int D, int AUX>
-__global__ void __launch_bounds__(1024)
+template <int TB, int CB, int EMAX, int D, int THREADS>
+__global__ void __launch_bounds__(THREADS)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                       const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
                       uint64_t nsuper, const uint8_t *__restrict__ img, uint64_t npx,
@@ -335,7 +335,6 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
                       uint32_t *__restrict__ counters) {
   typedef uint32_t v4u __attribute__((ext_vector_type(4)));
   constexpr int T = 1 << TB;
-  constexpr int THREADS = 1024;
   constexpr int NW = THREADS / 64;
   constexpr int PB = 32 - TB;
   constexpr int C = 1 << CB;
@@ -374,7 +373,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
 #pragma unroll
       for (int k = 0; k < EMAX; ++k)
-        pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (pbeg[d] + k * 64 + lane) * 4u, 0, AUX);
+        pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (pbeg[d] + k * 64 + lane) * 4u, 0, 0);
     };
     // process set d (chunk c): its pixels into this wave's slot, its entries
     // into the counters; then reload set d with chunk c + D*NW
@@ -393,7 +392,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         }
       }
       for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {  // dense chunks
-        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, AUX);
+        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
         const uint32_t raw = mine[e & (C - 1)];
         if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
       }
@@ -679,18 +678,18 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
       set_error("LDS-staged walk needs 2^14 or 2^15 super-tiles and chunk offsets");
       return LM_ERR_INVALID;
     }
-    // shapes: entries per lane per chunk, chunks in flight, cache policy of
-    // the entry loads (2 = streaming)
-#define LM_LDS(TB, EM, DD, AX)                                                                   \
-  hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, AX>), dim3(grid),    \
-                     dim3(1024), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,     \
+    // shapes: entry loads per lane per chunk, chunk sets in flight per wave,
+    // threads per workgroup (1024: 128 VGPRs per lane; 512: up to 256)
+#define LM_LDS(TB, EM, DD, TH)                                                                   \
+  hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH>), dim3(grid),    \
+                     dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,       \
                      b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
-#define LM_LDS_ALL(TB)                    \
-  switch (L) {                            \
-    case 3: LM_LDS(TB, 4, 2, 0); break;   \
-    case 5: LM_LDS(TB, 8, 3, 0); break;   \
-    case 6: LM_LDS(TB, 8, 3, 2); break;   \
-    default: LM_LDS(TB, 16, 2, 2); break; \
+#define LM_LDS_ALL(TB)                      \
+  switch (L) {                              \
+    case 3: LM_LDS(TB, 4, 2, 1024); break;  \
+    case 5: LM_LDS(TB, 8, 3, 1024); break;  \
+    case 6: LM_LDS(TB, 6, 4, 1024); break;  \
+    default: LM_LDS(TB, 8, 6, 512); break;  \
   }
     if (b.tb == 15) {
       LM_LDS_ALL(15);
