@@ -370,8 +370,9 @@ def main():
                          "table (0 = none) instead of the autotuned one")
     ap.add_argument("--loads", type=int, default=0,
                     help="with --tile-bits: the single-image walk (LM_OPT_BANDED_LOADS)")
-    ap.add_argument("--batch", type=str, default="",
-                    help="comma-separated batch widths to measure as well, e.g. 8,16,32,64")
+    ap.add_argument("--batch", type=str, default="64",
+                    help="comma-separated batch widths to measure as well (one pass over the "
+                         "table per batch), e.g. 8,16,32,64,128; empty = none")
     ap.add_argument("--no-autotune", action="store_true",
                     help="score batches with the default layouts instead of the autotuned ones")
     ap.add_argument("--no-precision", action="store_true",

@@ -319,14 +319,19 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // cache lines); here each wave owns a 2^CB-byte slot of LDS and walks the
 // super-tile chunk by chunk (chunk c = pixels [c << CB, (c+1) << CB), the
 // entries of chunk c are chunkOff[c] .. chunkOff[c+1] of the pixel-sorted
-// super-tile): coalesced 16-byte loads (2^CB / 1024 per lane) bring the chunk's pixels,
-// the chunk's entries come in EMAX coalesced loads per lane (a chunk with
-// more entries loads the rest as it goes), and each entry reads its pixel
-// from LDS and adds into the shared row counters.  Wave w takes chunks
-// w, w + 16, ...; the loads of its next D - 1 chunks are in flight while one
-// is processed.  LDS traffic of a wave is ordered, so its slot needs no
-// barrier between chunks.
-template <int TB, int CB, int EMAX, int D, int THREADS>
+// super-tile): coalesced 16-byte loads (2^CB / 1024 per lane) bring the
+// chunk's pixels, EMAX coalesced loads per lane its entries, and each entry
+// reads its pixel from LDS and adds into the shared row counters.  Wave w
+// takes chunks w, w + NW, ...; D sets of loads rotate, so while one chunk is
+// processed the next D - 1 are in flight.  All loads are buffer loads with a
+// wave-uniform base and range: offsets past the range read zeros, so chunk
+// indices past the last chunk are empty chunks, every wave runs the same
+// straight-line loop and each wait covers exactly one set.  A set also loads
+// the entry range of the chunk it will hold next, so refilling a set never
+// waits on a scalar load.  DENSE adds a slow path for chunks with more than
+// EMAX * 64 entries (the builder records the largest chunk).  LDS traffic of
+// a wave is ordered, so its slot needs no barrier between chunks.
+template <int TB, int CB, int EMAX, int D, int THREADS, bool DENSE>
 __global__ void __launch_bounds__(THREADS)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                       const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
@@ -341,13 +346,12 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   constexpr int V = C / (64 * 16);  // 16-byte image loads per lane per chunk
   static_assert(V >= 1 && V * 64 * 16 == C, "chunk must be a multiple of 1 KiB");
   static_assert(T * 4 + NW * C <= 163840, "LDS");
+  constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
   __shared__ uint32_t cnt[T];
   __shared__ v4u slot[NW][C / 16];
-  const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+  const int tid = threadIdx.x, lane = __lane_id();
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint8_t *mine = (const uint8_t *)&slot[w][0];
-  // buffer loads: wave-uniform base and range in SGPRs, 32-bit lane offsets;
-  // offsets past the range read as zero, so the loads need no guards
-  constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
   const __amdgpu_buffer_rsrc_t irs =
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
@@ -359,54 +363,73 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     const __amdgpu_buffer_rsrc_t ers = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(bentries + tileOff[first >> kTileBits]), (short)0, (int)(total * 4u),
         kRawBufferWord3);
-    // D sets of prefetch registers; set d holds chunk c + d*NW while the
-    // other sets are processed
+    // chunk offsets as a buffer too: index nchunks holds `total`; indices are
+    // clamped to it, so chunks past the end are empty
+    const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)co, (short)0, (int)((uint32_t)(nchunks + 1) * 4u), kRawBufferWord3);
+    auto coff = [&](int c) {
+      const uint32_t cc = (uint32_t)(c < nchunks ? c : nchunks);
+      return __builtin_amdgcn_raw_buffer_load_b32(crs, cc * 4u, 0, 0);
+    };
     v4u pim[D][V];
     uint32_t pe[D][EMAX];
-    uint32_t pbeg[D], pend[D];
-    auto fetch = [&](int d, int c) {
-      pbeg[d] = co[c];
-      pend[d] = co[c + 1];
-      const uint32_t ib = (uint32_t)c << CB;
+    uint32_t beg[D], end[D];
+    // The wave's chunks go in rounds of 64: lane l holds the entry range of
+    // the round's chunk l (one load per round), so a set's entry range comes
+    // from a lane read, not from a load the set's entry loads would wait on.
+    const int per_wave = (nchunks + NW - 1) / NW;
+    for (int j0 = 0; j0 < per_wave; j0 += 64) {
+      const int cl = w + NW * (j0 + lane);
+      const uint32_t vbeg = coff(cl), vend = coff(cl + 1);
+      auto fetch = [&](int d, int j) {  // round chunk j (>= 64: empty)
+        const bool in = j < 64;
+        const int jj = in ? j : 63;
+        beg[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)vbeg, jj) : total;
+        end[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)vend, jj) : total;
+        const uint32_t ib = (uint32_t)(w + NW * (j0 + j)) << CB;
 #pragma unroll
-      for (int v = 0; v < V; ++v)
-        pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
+        for (int v = 0; v < V; ++v)
+          pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
 #pragma unroll
-      for (int k = 0; k < EMAX; ++k)
-        pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (pbeg[d] + k * 64 + lane) * 4u, 0, 0);
-    };
-    // process set d (chunk c): its pixels into this wave's slot, its entries
-    // into the counters; then reload set d with chunk c + D*NW
-    auto consume = [&](int d, int c) {
+        for (int k = 0; k < EMAX; ++k)
+          pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (beg[d] + k * 64 + lane) * 4u, 0, 0);
+        // keep the sets' loads in issue order, so the wait before a set
+        // counts only the later sets
+        __builtin_amdgcn_sched_barrier(0);
+      };
+      auto consume = [&](int d, int j) {
 #pragma unroll
-      for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = pim[d][v];
-      __builtin_amdgcn_wave_barrier();
-      const uint32_t cbeg = pbeg[d], cend = pend[d];
+        for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = pim[d][v];
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t cbeg = beg[d], cend = end[d];
+        // all pixel reads first (lanes past the chunk read a masked, in-slot
+        // byte and drop it), then the counter adds
+        uint32_t raw[EMAX];
 #pragma unroll
-      for (int k = 0; k < EMAX; ++k) {
-        const uint32_t i = cbeg + k * 64 + lane;
-        if (i < cend) {
-          const uint32_t e = pe[d][k];
-          const uint32_t raw = mine[e & (C - 1)];
-          if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
+        for (int k = 0; k < EMAX; ++k) {
+          const uint32_t r = mine[pe[d][k] & (C - 1)];
+          raw[k] = cbeg + k * 64 + lane < cend ? r : 0u;
         }
-      }
-      for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {  // dense chunks
-        const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
-        const uint32_t raw = mine[e & (C - 1)];
-        if (raw) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + raw);
-      }
-      __builtin_amdgcn_wave_barrier();
-      const int cn = c + D * NW;
-      if (cn < nchunks) fetch(d, cn);
-    };
 #pragma unroll
-    for (int d = 0; d < D; ++d)
-      if (w + d * NW < nchunks) fetch(d, w + d * NW);
-    for (int c = w; c < nchunks; c += D * NW) {
+        for (int k = 0; k < EMAX; ++k)
+          if (raw[k]) atomicAdd(&cnt[pe[d][k] >> PB], (1u << kSumBits) + raw[k]);
+        if constexpr (DENSE) {
+          for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {
+            const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
+            const uint32_t r = mine[e & (C - 1)];
+            if (r) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + r);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        fetch(d, j + D);
+      };
 #pragma unroll
-      for (int d = 0; d < D; ++d)
-        if (c + d * NW < nchunks) consume(d, c + d * NW);
+      for (int d = 0; d < D; ++d) fetch(d, d);
+      const int n_in = per_wave - j0 < 64 ? per_wave - j0 : 64;
+      for (int j = 0; j < n_in; j += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) consume(d, j + d);
+      }
     }
     __syncthreads();
     for (int rr = tid; rr < T; rr += THREADS) {
@@ -434,6 +457,23 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     }
     __syncthreads();
   }
+}
+
+// Largest number of entries in one chunk of a banded copy.
+__global__ void chunk_max_kernel(const uint32_t *__restrict__ chunkOff, uint64_t nsuper,
+                                 int nchunks, uint32_t *__restrict__ out) {
+  const uint64_t total = nsuper * (uint64_t)nchunks;
+  uint32_t m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t t = i / (uint64_t)nchunks;
+    const uint64_t j = i - t * (uint64_t)nchunks;
+    const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
+    m = max(m, co[j + 1] - co[j]);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
+  if (__lane_id() == 0) atomicMax(out, m);
 }
 
 // Batch compare over a banded copy: B images interleaved pixel-major
@@ -626,6 +666,19 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
                        b.entries, b.band_off, cb, b.nchunks, b.chunk_off);
     e = hipGetLastError();
   }
+  if (e == hipSuccess && chunks) {
+    uint32_t *d_max = nullptr;
+    e = hipMalloc(&d_max, 4);
+    if (e == hipSuccess) e = hipMemsetAsync(d_max, 0, 4, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(chunk_max_kernel, dim3(1024), dim3(256), 0, st, b.chunk_off, b.nsuper,
+                         b.nchunks, d_max);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&b.max_chunk, d_max, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_max) (void)hipFree(d_max);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     release_one(db, b);
@@ -680,10 +733,16 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     }
     // shapes: entry loads per lane per chunk, chunk sets in flight per wave,
     // threads per workgroup (1024: 128 VGPRs per lane; 512: up to 256)
+    // chunks larger than EMAX * 64 entries need the kernel's slow path
 #define LM_LDS(TB, EM, DD, TH)                                                                   \
-  hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH>), dim3(grid),    \
-                     dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off, b.nchunks, db->n,       \
-                     b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
+  if (b.max_chunk <= (uint32_t)(EM)*64u)                                                         \
+    hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, false>),       \
+                       dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
+                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters); \
+  else                                                                                            \
+    hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, true>),        \
+                       dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
+                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
 #define LM_LDS_ALL(TB)                      \
   switch (L) {                              \
     case 3: LM_LDS(TB, 4, 2, 1024); break;  \

@@ -50,6 +50,7 @@ struct Banded {
   int pb = 0;
   int nbands = 0;
   int nchunks = 0;
+  uint32_t max_chunk = 0;        // largest number of entries in one chunk
   uint64_t bytes = 0;
   bool ready = false;
 };
