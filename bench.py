@@ -343,7 +343,7 @@ def kernel_name(db):
     if tb == 0:
         return "score_kernel<1>"
     if (loads == 3 or loads >= 5) and tb >= 14:
-        shape = {3: "4,2,1024", 5: "8,3,1024", 6: "6,4,1024", 7: "8,6,512"}[loads]
+        shape = {3: "4,2,1024", 5: "5,2,1024", 6: "4,3,1024", 7: "5,3,1024"}[loads]
         return f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},{shape}>"
     threads = 1024 if tb >= 14 else 512
     if loads == 0:

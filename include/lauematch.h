@@ -145,10 +145,9 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * loads per lane in flight, 1 = two 16-byte loads (8 entries), 2 = four
  * 16-byte loads (16 entries), 3 = the image staged through LDS in per-wave
  * chunks (2^14 or 2^15 super-tiles), 4 = as 0 with each pixel read through a
- * 4-byte gather of its aligned word, 5 = as 3 with 8 entry loads per lane and
- * two further chunks in flight per wave, 6 = 6 entry loads and three chunks in
- * flight, 7 = 512-thread workgroups, 8 entry loads and five chunks in flight.
- * Default 0. */
+ * 4-byte gather of its aligned word; 3, 5, 6, 7 = the LDS walk with 4, 5, 4, 5
+ * entry loads per lane and 2, 2, 3, 3 chunk sets in flight per wave.  Default
+ * 0. */
 #define LM_OPT_BANDED_LOADS 5
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);

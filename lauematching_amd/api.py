@@ -162,8 +162,8 @@ class OrientationDB:
     def set_banded_loads(self, loads: int) -> None:
         """Single-image banded walk: 0 = 4-byte loads x8, 1 = 16-byte x2, 2 = 16-byte x4,
         3 = image staged through LDS (2^14 / 2^15 super-tiles), 4 = 4-byte gathers,
-        5 = LDS-staged, 8 entry loads per lane, 6 = 6 loads and more chunks in
-        flight, 7 = 512-thread workgroups with five chunks in flight."""
+        5, 6, 7 = LDS-staged with (entry loads per lane, chunk sets in flight)
+        (5, 2), (4, 3), (5, 3); 3 is (4, 2)."""
         check(lib().lm_db_set_option(self._h, 5, int(loads)))
 
     def get_option(self, option: int) -> int:

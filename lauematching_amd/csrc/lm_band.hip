@@ -746,9 +746,9 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
 #define LM_LDS_ALL(TB)                      \
   switch (L) {                              \
     case 3: LM_LDS(TB, 4, 2, 1024); break;  \
-    case 5: LM_LDS(TB, 8, 3, 1024); break;  \
-    case 6: LM_LDS(TB, 6, 4, 1024); break;  \
-    default: LM_LDS(TB, 8, 6, 512); break;  \
+    case 5: LM_LDS(TB, 5, 2, 1024); break;  \
+    case 6: LM_LDS(TB, 4, 3, 1024); break;  \
+    default: LM_LDS(TB, 5, 3, 1024); break; \
   }
     if (b.tb == 15) {
       LM_LDS_ALL(15);
