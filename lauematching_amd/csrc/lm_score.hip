@@ -279,6 +279,109 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
   }
 }
 
+// 128 images: two lanes per orientation, lane h of the pair accumulating
+// images 64h .. 64h+63 with packed accumulators.  A spot's 128 interleaved
+// bytes are one 128-byte line: the pair's two 64-byte loads in one wave
+// instruction fetch it once, so the line traffic per spot is that of a
+// 64-image batch for twice the images.  512 threads per workgroup, one tile
+// of 256 orientations; the tile's entries are staged in LDS as in
+// score_kernel.
+__global__ void __launch_bounds__(2 * kTile)
+score_pair_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+                  const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
+                  const uint8_t *__restrict__ img, const Pred *__restrict__ pred,
+                  uint64_t *__restrict__ hits, uint64_t capPer, uint32_t *__restrict__ counters) {
+  constexpr int B = 128, HB = 64;
+  __shared__ uint32_t stage[kStageWords];
+  __shared__ uint32_t wsum[kTile / 64];
+  __shared__ uint32_t rowExcl[kTile], rowCnt[kTile];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  const int r = tid >> 1, h = tid & 1;  // row in tile, half of the batch
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = tileOff[t];
+    const uint64_t tot = tileOff[t + 1] - base;
+    if (tid < kTile) {  // counts and their exclusive scan over the tile
+      const uint64_t o = t * kTile + tid;
+      const uint32_t c = o < n ? counts[o] : 0u;
+      uint32_t incl = c;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      rowCnt[tid] = c;
+      rowExcl[tid] = incl - c;
+    }
+    __syncthreads();
+    if (tid < kTile) {
+      uint32_t add = 0;
+      for (int w = 0; w < wave; ++w) add += wsum[w];
+      rowExcl[tid] += add;
+    }
+    const bool staged = tot <= (uint64_t)kStageWords;
+    if (staged) {
+      const uint64_t a0 = base & ~3ull;
+      const uint64_t nvec = (base + tot - a0 + 3) >> 2;
+      const u32x4 *src = (const u32x4 *)(entries + a0);
+      for (uint64_t i = tid; i < nvec; i += 2 * kTile) {
+        const u32x4 v = __builtin_nontemporal_load(src + i);
+        const int64_t j = (int64_t)(a0 + 4 * i) - (int64_t)base;
+        if (j >= 0 && j < (int64_t)tot) stage[j] = v.x;
+        if (j + 1 >= 0 && j + 1 < (int64_t)tot) stage[j + 1] = v.y;
+        if (j + 2 >= 0 && j + 2 < (int64_t)tot) stage[j + 2] = v.z;
+        if (j + 3 >= 0 && j + 3 < (int64_t)tot) stage[j + 3] = v.w;
+      }
+    }
+    __syncthreads();
+    const uint64_t o = t * kTile + r;
+    const uint32_t c = rowCnt[r], excl = rowExcl[r];
+    const uint32_t *sp = staged ? stage + excl : entries + base + excl;
+    const uint8_t *himg = img + HB * h;  // this lane's half of each 128-byte pixel
+    Swar<HB> a;
+#pragma unroll
+    for (int j = 0; j < HB / 4; ++j) a.s02[j] = a.s13[j] = a.cn[j] = 0;
+    uint32_t s = 0;
+    for (; s + 2 <= c; s += 2) {
+      uint32_t w0[HB / 4], w1[HB / 4];
+      load_px<HB>(himg, sp[s] * 2u, w0);  // pixel p's half h sits at p*128 + 64h
+      load_px<HB>(himg, sp[s + 1] * 2u, w1);
+      swar_px<HB>(w0, a);
+      swar_px<HB>(w1, a);
+    }
+    for (; s < c; ++s) {
+      uint32_t w0[HB / 4];
+      load_px<HB>(himg, sp[s] * 2u, w0);
+      swar_px<HB>(w0, a);
+    }
+    const bool live = o < n;
+#pragma unroll
+    for (int b = 0; b < HB; ++b) {
+      const uint32_t accb = swar_acc<HB>(a, b);
+      const uint32_t ns = accb >> kSumBits, sum = accb & kSumMask;
+      const int img_b = HB * h + b;
+      bool cand = false, zero_pass = false;
+      if (live && img_b < pred->nimg) {
+        if (ns == 0)
+          zero_pass = (0 >= pred->min_spots[img_b]) && (0.0 >= pred->min_int[img_b]);
+        else
+          cand = ((int)ns >= pred->min_spots[img_b]) && ((double)sum >= pred->cand_thr[img_b]);
+      }
+      const uint64_t rec = (o << 26) | (uint64_t)accb;
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {  // the two halves append to different images
+        const int ib = HB * hh + b;
+        const uint64_t zm = __ballot(zero_pass && h == hh);
+        if (zm && lane == 0) atomicAdd(&counters[2 * ib + 1], (uint32_t)__popcll(zm));
+        append(cand && h == hh, rec, hits + (uint64_t)ib * capPer, &counters[2 * ib], capPer);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // In-order float64 scoring, one lane per orientation: totInt accumulates the
 // pixel values > 0 in table order exactly like the reference loops
 // (src/LaueMatchingCPU.c:647-653 for the float64 image of the DoFwd=1
@@ -468,7 +571,15 @@ int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t cap
     case 16: LM_LAUNCH2(16); break;
     case 32: LM_LAUNCH2(32); break;
     case 64: LM_LAUNCH2(64); break;
-    case 128: LM_LAUNCH2(128); break;
+    case 128:
+      // the pair kernel addresses pixel halves as 2 * pixel in 32 bits
+      if (sw && (uint64_t)db->geom.nrPxX * db->geom.nrPxY <= (1ull << 31))
+        hipLaunchKernelGGL(score_pair_kernel, dim3(grid), dim3(2 * kTile), 0, st, t.counts,
+                           t.tile_off, t.entries, db->n, t.ntiles, img, sc.pred, sc.hits, capPer,
+                           sc.counters);
+      else
+        LM_LAUNCH(128, false);
+      break;
     default: set_error("unsupported batch width"); return LM_ERR_INVALID;
   }
 #undef LM_LAUNCH2
