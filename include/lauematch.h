@@ -149,6 +149,10 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * entry loads per lane and 2, 2, 3, 3 chunk sets in flight per wave.  Default
  * 0. */
 #define LM_OPT_BANDED_LOADS 5
+/* LM_OPT_BATCH_GROUPED: bit 6 / bit 7 set = table batches of 64 / 128 images
+ * give each orientation B/16 lanes of 16 images each (one 16-byte load per
+ * lane and spot) instead of one lane holding all B.  Default 0xc0. */
+#define LM_OPT_BATCH_GROUPED 6
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
@@ -166,11 +170,13 @@ typedef struct lm_tune_report {
   double ms_single[3][8];     /* one image: banded 2^13 / 2^14 / 2^15 x loads 0..7
                                  (0 = not available)                             */
   double ms_lane[6];          /* per width: orientation-major table               */
+  double ms_grouped[6];       /* per width: table, grouped lanes (64, 128 only)   */
   double ms_banded[6][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
   int32_t banded_tile_bits;   /* chosen LM_OPT_BANDED_TILE_BITS                   */
   int32_t banded_loads;       /* chosen LM_OPT_BANDED_LOADS                       */
   int32_t batch_tile_bits;    /* chosen LM_OPT_BATCH_TILE_BITS                    */
   uint32_t batch_widths_mask; /* chosen LM_OPT_BATCH_WIDTHS                       */
+  uint32_t grouped_mask;      /* chosen LM_OPT_BATCH_GROUPED                      */
 } lm_tune_report;
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out);

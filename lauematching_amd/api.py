@@ -199,11 +199,12 @@ class OrientationDB:
         return {
             "ms_single": single, "banded_loads": r.banded_loads,
             "ms_batch": {B: {"table": r.ms_lane[i],
+                             **({"grouped": r.ms_grouped[i]} if r.ms_grouped[i] > 0 else {}),
                              **{f"banded{tb}": r.ms_banded[i][tb - 9]
                                 for tb in range(9, 14) if r.ms_banded[i][tb - 9] > 0}}
                          for i, B in enumerate(widths)},
             "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
-            "batch_widths_mask": r.batch_widths_mask,
+            "batch_widths_mask": r.batch_widths_mask, "grouped_mask": r.grouped_mask,
         }
 
     def forward(self, mode: int = LM_FWD_FP64) -> LmFwdStats:

@@ -402,6 +402,15 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     db->scratch.use_graphs = value != 0;
     return LM_OK;
   }
+  if (option == LM_OPT_BATCH_GROUPED) {
+    if (value < 0 || value > 0xc0 || (value & 0x3f)) {
+      set_error("LM_OPT_BATCH_GROUPED is a mask of bits 6 and 7 (64 and 128 images)");
+      return LM_ERR_INVALID;
+    }
+    db->group_widths = (uint32_t)value;
+    ++db->generation;
+    return LM_OK;
+  }
   set_error("unknown option");
   return LM_ERR_INVALID;
 }
@@ -418,6 +427,7 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
     case LM_OPT_BATCH_WIDTHS: *value = db->batch_widths; return LM_OK;
     case LM_OPT_USE_GRAPHS: *value = db->scratch.use_graphs ? 1 : 0; return LM_OK;
     case LM_OPT_BANDED_LOADS: *value = db->banded_loads; return LM_OK;
+    case LM_OPT_BATCH_GROUPED: *value = db->group_widths; return LM_OK;
     default: set_error("unknown option"); return LM_ERR_INVALID;
   }
 }

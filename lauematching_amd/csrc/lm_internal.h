@@ -141,6 +141,7 @@ struct lm_db {
   int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
   double width_ms[8] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
+  uint32_t group_widths = 0xc0; // bit log2(B): table batches of width B use grouped lanes
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   std::mutex mu;

@@ -279,30 +279,36 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
   }
 }
 
-// 128 images: two lanes per orientation, lane h of the pair accumulating
-// images 64h .. 64h+63 with packed accumulators.  A spot's 128 interleaved
-// bytes are one 128-byte line: the pair's two 64-byte loads in one wave
-// instruction fetch it once, so the line traffic per spot is that of a
-// 64-image batch for twice the images.  512 threads per workgroup, one tile
-// of 256 orientations; the tile's entries are staged in LDS as in
-// score_kernel.
-__global__ void __launch_bounds__(2 * kTile)
-score_pair_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
-                  const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
-                  const uint8_t *__restrict__ img, const Pred *__restrict__ pred,
-                  uint64_t *__restrict__ hits, uint64_t capPer, uint32_t *__restrict__ counters) {
-  constexpr int B = 128, HB = 64;
+// Batches of B = 64 or 128 images with L = B / 16 lanes per orientation:
+// lane `sub` of a group gathers the 16 bytes of images 16*sub .. 16*sub+15
+// of each spot (one 16-byte load) and keeps 12 packed accumulators.  A wave
+// instruction then touches 64 / L distinct lines, each fully used, instead
+// of 64 lines at B bytes each.  1024 threads per workgroup; a 256-orientation
+// tile is done in L / 4 passes over its LDS-staged entries.  The epilogue
+// first builds per-lane masks of passing images, so waves without a hit
+// (nearly all) skip the per-image compaction.
+template <int B>
+__global__ void __launch_bounds__(1024)
+score_group_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+                   const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
+                   const uint8_t *__restrict__ img, const Pred *__restrict__ pred,
+                   uint64_t *__restrict__ hits, uint64_t capPer, uint32_t *__restrict__ counters) {
+  constexpr int L = B / 16;
+  constexpr int OPP = 1024 / L;       // orientations per pass
+  constexpr int PASSES = kTile / OPP;
+  static_assert(PASSES * OPP == kTile, "tile must split into passes");
   __shared__ uint32_t stage[kStageWords];
   __shared__ uint32_t wsum[kTile / 64];
   __shared__ uint32_t rowExcl[kTile], rowCnt[kTile];
   const int tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
-  const int r = tid >> 1, h = tid & 1;  // row in tile, half of the batch
+  const int sub = tid % L, rl = tid / L;
+  const int nimg = pred->nimg;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t base = tileOff[t];
     const uint64_t tot = tileOff[t + 1] - base;
-    if (tid < kTile) {  // counts and their exclusive scan over the tile
+    if (tid < kTile) {
       const uint64_t o = t * kTile + tid;
       const uint32_t c = o < n ? counts[o] : 0u;
       uint32_t incl = c;
@@ -326,7 +332,7 @@ score_pair_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restric
       const uint64_t a0 = base & ~3ull;
       const uint64_t nvec = (base + tot - a0 + 3) >> 2;
       const u32x4 *src = (const u32x4 *)(entries + a0);
-      for (uint64_t i = tid; i < nvec; i += 2 * kTile) {
+      for (uint64_t i = tid; i < nvec; i += 1024) {
         const u32x4 v = __builtin_nontemporal_load(src + i);
         const int64_t j = (int64_t)(a0 + 4 * i) - (int64_t)base;
         if (j >= 0 && j < (int64_t)tot) stage[j] = v.x;
@@ -336,46 +342,63 @@ score_pair_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restric
       }
     }
     __syncthreads();
-    const uint64_t o = t * kTile + r;
-    const uint32_t c = rowCnt[r], excl = rowExcl[r];
-    const uint32_t *sp = staged ? stage + excl : entries + base + excl;
-    const uint8_t *himg = img + HB * h;  // this lane's half of each 128-byte pixel
-    Swar<HB> a;
+    for (int pass = 0; pass < PASSES; ++pass) {
+      const int r = pass * OPP + rl;
+      const uint64_t o = t * kTile + r;
+      const uint32_t c = rowCnt[r], excl = rowExcl[r];
+      const uint32_t *sp = staged ? stage + excl : entries + base + excl;
+      const uint8_t *simg = img + 16 * sub;
+      Swar<16> a;
 #pragma unroll
-    for (int j = 0; j < HB / 4; ++j) a.s02[j] = a.s13[j] = a.cn[j] = 0;
-    uint32_t s = 0;
-    for (; s + 2 <= c; s += 2) {
-      uint32_t w0[HB / 4], w1[HB / 4];
-      load_px<HB>(himg, sp[s] * 2u, w0);  // pixel p's half h sits at p*128 + 64h
-      load_px<HB>(himg, sp[s + 1] * 2u, w1);
-      swar_px<HB>(w0, a);
-      swar_px<HB>(w1, a);
-    }
-    for (; s < c; ++s) {
-      uint32_t w0[HB / 4];
-      load_px<HB>(himg, sp[s] * 2u, w0);
-      swar_px<HB>(w0, a);
-    }
-    const bool live = o < n;
+      for (int j = 0; j < 4; ++j) a.s02[j] = a.s13[j] = a.cn[j] = 0;
+      uint32_t s = 0;
+      for (; s + 4 <= c; s += 4) {
+        uint32_t w[4][4];
 #pragma unroll
-    for (int b = 0; b < HB; ++b) {
-      const uint32_t accb = swar_acc<HB>(a, b);
-      const uint32_t ns = accb >> kSumBits, sum = accb & kSumMask;
-      const int img_b = HB * h + b;
-      bool cand = false, zero_pass = false;
-      if (live && img_b < pred->nimg) {
-        if (ns == 0)
-          zero_pass = (0 >= pred->min_spots[img_b]) && (0.0 >= pred->min_int[img_b]);
-        else
-          cand = ((int)ns >= pred->min_spots[img_b]) && ((double)sum >= pred->cand_thr[img_b]);
+        for (int g = 0; g < 4; ++g) {
+          const u32x4 v = *(const u32x4 *)(simg + (uint64_t)sp[s + g] * B);
+          w[g][0] = v.x;
+          w[g][1] = v.y;
+          w[g][2] = v.z;
+          w[g][3] = v.w;
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) swar_px<16>(w[g], a);
       }
-      const uint64_t rec = (o << 26) | (uint64_t)accb;
+      for (; s < c; ++s) {
+        const u32x4 v = *(const u32x4 *)(simg + (uint64_t)sp[s] * B);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        swar_px<16>(w, a);
+      }
+      // masks of this lane's 16 images that pass (cand) or pass with no spot
+      const bool live = o < n;
+      uint32_t cmask = 0, zmask = 0;
 #pragma unroll
-      for (int hh = 0; hh < 2; ++hh) {  // the two halves append to different images
-        const int ib = HB * hh + b;
-        const uint64_t zm = __ballot(zero_pass && h == hh);
-        if (zm && lane == 0) atomicAdd(&counters[2 * ib + 1], (uint32_t)__popcll(zm));
-        append(cand && h == hh, rec, hits + (uint64_t)ib * capPer, &counters[2 * ib], capPer);
+      for (int b = 0; b < 16; ++b) {
+        const int ib = 16 * sub + b;
+        const uint32_t accb = swar_acc<16>(a, b);
+        const uint32_t ns = accb >> kSumBits, sum = accb & kSumMask;
+        if (live && ib < nimg) {
+          if (ns == 0) {
+            if ((0 >= pred->min_spots[ib]) && (0.0 >= pred->min_int[ib])) zmask |= 1u << b;
+          } else if (((int)ns >= pred->min_spots[ib]) && ((double)sum >= pred->cand_thr[ib])) {
+            cmask |= 1u << b;
+          }
+        }
+      }
+      if (__ballot((cmask | zmask) != 0)) {
+        for (int ss = 0; ss < L; ++ss) {
+#pragma unroll
+          for (int b = 0; b < 16; ++b) {
+            const int ib = 16 * ss + b;
+            const bool mine_sub = sub == ss;
+            const uint64_t zm = __ballot(mine_sub && ((zmask >> b) & 1u));
+            if (zm && lane == 0) atomicAdd(&counters[2 * ib + 1], (uint32_t)__popcll(zm));
+            const bool cand = mine_sub && ((cmask >> b) & 1u);
+            const uint64_t rec = (o << 26) | (uint64_t)swar_acc<16>(a, b);
+            append(cand, rec, hits + (uint64_t)ib * capPer, &counters[2 * ib], capPer);
+          }
+        }
       }
     }
     __syncthreads();
@@ -570,11 +593,17 @@ int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t cap
     case 8: LM_LAUNCH2(8); break;
     case 16: LM_LAUNCH2(16); break;
     case 32: LM_LAUNCH2(32); break;
-    case 64: LM_LAUNCH2(64); break;
+    case 64:
+      if (sw && ((db->group_widths >> 6) & 1))
+        hipLaunchKernelGGL((score_group_kernel<64>), dim3(grid), dim3(1024), 0, st, t.counts,
+                           t.tile_off, t.entries, db->n, t.ntiles, img, sc.pred, sc.hits, capPer,
+                           sc.counters);
+      else
+        LM_LAUNCH2(64);
+      break;
     case 128:
-      // the pair kernel addresses pixel halves as 2 * pixel in 32 bits
-      if (sw && (uint64_t)db->geom.nrPxX * db->geom.nrPxY <= (1ull << 31))
-        hipLaunchKernelGGL(score_pair_kernel, dim3(grid), dim3(2 * kTile), 0, st, t.counts,
+      if (sw && ((db->group_widths >> 7) & 1))
+        hipLaunchKernelGGL((score_group_kernel<128>), dim3(grid), dim3(1024), 0, st, t.counts,
                            t.tile_off, t.entries, db->n, t.ntiles, img, sc.pred, sc.hits, capPer,
                            sc.counters);
       else

@@ -59,12 +59,13 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   }
   if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
   const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
-  const uint32_t saved_mask = db->batch_widths;
+  const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths;
   auto restore = [&](int code) {
     db->banded_tb = saved_tb;
     db->batch_tb = saved_btb;
     db->batch_widths = saved_mask;
     db->banded_loads = saved_loads;
+    db->group_widths = saved_group;
     const int r = build_banded(db);
     return code ? code : r;
   };
@@ -103,10 +104,28 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->batch_tb = 0;
   rc = build_banded(db);
   if (rc) return restore(rc);
+  // the table walk: one lane per orientation, and for 64 / 128 images also
+  // grouped lanes; ms_lane keeps the faster, grouped_mask records which
+  uint32_t gmask = 0;
   for (int w = 0; w < nw; ++w) {
-    rc = time_batch(db, d, rep->widths[w], reps, &rep->ms_lane[w]);
+    const int B = rep->widths[w];
+    db->group_widths = 0;
+    ++db->generation;
+    rc = time_batch(db, d, B, reps, &rep->ms_lane[w]);
     if (rc) return restore(rc);
+    if (B == 64 || B == 128) {
+      db->group_widths = 1u << width_log2(B);
+      ++db->generation;
+      rc = time_batch(db, d, B, reps, &rep->ms_grouped[w]);
+      if (rc) return restore(rc);
+      if (rep->ms_grouped[w] < rep->ms_lane[w]) {
+        gmask |= 1u << width_log2(B);
+        std::swap(rep->ms_grouped[w], rep->ms_lane[w]);  // ms_lane: the table walk kept
+      }
+    }
   }
+  db->group_widths = gmask;
+  ++db->generation;
   db->batch_widths = 0x7f;
   for (int tb = 9; tb <= 13; ++tb) {
     db->batch_tb = tb;
@@ -142,6 +161,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
       const double ms = rep->ms_banded[w][best_btb - 9];
       if (ms > 0 && ms < rep->ms_lane[w]) mask |= 1u << width_log2(rep->widths[w]);
     }
+  rep->grouped_mask = gmask;
   rep->banded_tile_bits = best_tb;
   rep->banded_loads = best_loads;
   rep->batch_tile_bits = best_btb;
