@@ -338,19 +338,30 @@ def config4(ctx, O, n, steps, batch_imgs):
 
 
 def kernel_name(db):
-    """Name of the single-image kernel the DB's layout options select."""
+    """Name of the single-image kernel the DB's layout options select, with
+    every template argument as rocprofv3 prints it (the LDS walk's last
+    argument, the slow path for oversized chunks, depends on the table and
+    is left as ``*``)."""
     tb, loads = db.banded_layout()
     if tb == 0:
-        return "score_kernel<1>"
+        return "score_kernel<1,false>"
     if (loads == 3 or loads >= 5) and tb >= 14:
         shape = {3: "4,2,1024", 5: "5,2,1024", 6: "4,3,1024", 7: "5,3,1024"}[loads]
-        return f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},{shape}>"
+        return f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},{shape},*>"
     threads = 1024 if tb >= 14 else 512
     if loads == 0:
-        return f"score_band_kernel<{tb},{threads},8>"
+        return f"score_band_kernel<{tb},{threads},8,false>"
     if loads == 4:
         return f"score_band_kernel<{tb},{threads},8,true>"
     return f"score_band_v4_kernel<{tb},{threads},{2 if loads == 1 else 4}>"
+
+
+def same_kernel(profiled, name):
+    """True when a profiled kernel name is the one ``kernel_name`` gave
+    (spaces ignored, ``*`` matching one template argument)."""
+    import fnmatch
+    profiled = profiled.replace(" ", "")
+    return fnmatch.fnmatchcase(profiled, name.replace("[", "[[]")) and profiled.count(",") == name.count(",")
 
 
 def main():
@@ -494,7 +505,7 @@ def main():
     if os.path.exists(tf):
         try:
             tj = json.load(open(tf))
-            if tj.get("orients") == n and tj.get("kernel") == single_kernel:
+            if tj.get("orients") == n and same_kernel(tj.get("kernel", ""), single_kernel):
                 traffic = tj.get("bytes_per_launch")
         except Exception:
             traffic = None

@@ -228,6 +228,14 @@ int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
                    const double *im_scales, int32_t min_nr_spots, double min_intensity,
                    lm_hits *outs);
 
+/* Same with thresholds per image (min_nr_spots[i], min_intensity[i]), as
+ * requests of different clients share one batch in the server.  No
+ * counterpart in the reference, whose programs take one threshold pair per
+ * run (src/LaueMatchingCPU.c:683-689). */
+int lm_score_batch_each(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
+                        const double *im_scales, const int32_t *min_nr_spots,
+                        const double *min_intensity, lm_hits *outs);
+
 /* Device-resident batch: d_imgs points to nimg device images laid out back to
  * back (nimg * nrPxX * nrPxY bytes). */
 int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,

@@ -110,6 +110,10 @@ SIGNATURES = {
     "lm_score_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_void_p), c_double_p, ctypes.c_int32,
                                       ctypes.c_double, ctypes.POINTER(LmHits)]),
+    "lm_score_batch_each": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                           ctypes.POINTER(ctypes.c_void_p), c_double_p,
+                                           ctypes.POINTER(ctypes.c_int32), c_double_p,
+                                           ctypes.POINTER(LmHits)]),
     "lm_score_batch_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                           c_double_p, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.POINTER(LmHits)]),

@@ -166,6 +166,12 @@ class OrientationDB:
         (5, 2), (4, 3), (5, 3); 3 is (4, 2)."""
         check(lib().lm_db_set_option(self._h, 5, int(loads)))
 
+    def set_batch_grouped(self, mask: int) -> None:
+        """Bit 6 / bit 7 set: table batches of 64 / 128 images use 16 images
+        per lane (several lanes per orientation) instead of one lane holding
+        every image."""
+        check(lib().lm_db_set_option(self._h, 6, int(mask)))
+
     def get_option(self, option: int) -> int:
         v = ctypes.c_int64()
         check(lib().lm_db_get_option(self._h, int(option), ctypes.byref(v)))
@@ -267,6 +273,23 @@ class OrientationDB:
         hs = (LmHits * n)()
         check(lib().lm_score_batch(self._h, n, ptrs, sc, int(min_nr_spots), float(min_intensity),
                                    hs))
+        return [Hits._take(hs[i]) for i in range(n)]
+
+    def score_batch_each(self, imgs_u8, im_scales, min_nr_spots, min_intensity):
+        """Batch with thresholds per image (sequences as long as the batch)."""
+        ims = [np.ascontiguousarray(i, dtype=np.uint8).ravel() for i in imgs_u8]
+        npx = self.geom.nrPxX * self.geom.nrPxY
+        n = len(ims)
+        if any(i.size != npx for i in ims):
+            raise ValueError("image size does not match the detector")
+        if not (len(im_scales) == len(min_nr_spots) == len(min_intensity) == n):
+            raise ValueError("one scale and one threshold pair per image")
+        ptrs = (ctypes.c_void_p * n)(*[i.ctypes.data for i in ims])
+        sc = (ctypes.c_double * n)(*[float(s) for s in im_scales])
+        sp = (ctypes.c_int32 * n)(*[int(v) for v in min_nr_spots])
+        mi = (ctypes.c_double * n)(*[float(v) for v in min_intensity])
+        hs = (LmHits * n)()
+        check(lib().lm_score_batch_each(self._h, n, ptrs, sc, sp, mi, hs))
         return [Hits._take(hs[i]) for i in range(n)]
 
     def score_batch_dev(self, dev_ptr: int, nimg: int, im_scales, min_nr_spots: int,

@@ -518,6 +518,35 @@ int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
                        outs);
 }
 
+int lm_score_batch_each(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
+                        const double *im_scales, const int32_t *min_nr_spots,
+                        const double *min_intensity, lm_hits *outs) {
+  if (!db || !imgs_u8 || !im_scales || !min_nr_spots || !min_intensity || !outs || nimg < 1) {
+    set_error("lm_score_batch_each: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  if (nimg > kMaxBatch) {
+    set_error("lm_score_batch_each: at most 128 images per call");
+    return LM_ERR_INVALID;
+  }
+  uint8_t *d = nullptr;
+  rc = stage_host_images(db, nimg, imgs_u8, &d);
+  if (rc) return rc;
+  if (nimg == 1 && im_scales[0] != 1.0)
+    return score_seq(db, db->scratch, d, false, im_scales[0], min_nr_spots[0], min_intensity[0],
+                     db->ctx->stream, outs);
+  int sp[kMaxBatch];
+  double mi[kMaxBatch];
+  for (int b = 0; b < kMaxBatch; ++b) {
+    sp[b] = b < nimg ? min_nr_spots[b] : 0;
+    mi[b] = b < nimg ? min_intensity[b] : 0.0;
+  }
+  return score_images(db, db->scratch, nimg, d, im_scales, sp, mi, db->ctx->stream, outs);
+}
+
 int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double min_intensity,
                  lm_hits *out) {
   if (!db || !img_f64 || !out) {

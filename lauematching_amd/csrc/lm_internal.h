@@ -77,6 +77,14 @@ struct Pred {
   double cand_thr[kMaxBatch];  // candidate threshold on the integer sum
   int min_spots[kMaxBatch];
   uint32_t exact[kMaxBatch / 32]; // bit b: image b has imScale == 1
+  // The same candidate test in integers, for kernels whose lanes hold
+  // different images (per-lane loads of the doubles above would cost a
+  // vector load per image and row): for nspots > 0 a row passes exactly when
+  // nspots >= pk_thr >> 23 and sum >= pk_thr & 0x7fffff (both clamped so
+  // that "never" stays never for nspots <= 255 and sums < 2^23); zero_ok
+  // bit b says a row with no spot passes (0 >= min_spots and 0 >= min_int).
+  uint32_t pk_thr[kMaxBatch];
+  uint32_t zero_ok[kMaxBatch / 32];
   int nimg;
 };
 

@@ -40,6 +40,24 @@ constexpr int kStageWords = 12288;  // 48 KB of staged entries per workgroup
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Packed integer form of "nspots >= min_spots && (double)sum >= thr" for
+// nspots in 1..255 and integer sums below 2^23: the smallest passing nspots
+// (0..256) in bits 23..31, the smallest passing sum in bits 0..22 (a sum
+// never reaches 2^23 - 1, so that value means "never", as for a NaN or an
+// unreachable threshold).
+static uint32_t pack_threshold(int min_spots, double thr) {
+  const uint32_t ns = min_spots <= 0 ? 0u : (min_spots > 256 ? 256u : (uint32_t)min_spots);
+  constexpr uint32_t kNever = (1u << 23) - 1u;
+  uint32_t sum;
+  if (thr <= 0.0)
+    sum = 0u;
+  else if (!(thr < (double)kNever))  // NaN, +inf, or beyond any sum
+    sum = kNever;
+  else
+    sum = (uint32_t)std::ceil(thr);
+  return (ns << 23) | sum;
+}
+
 __host__ __device__ __forceinline__ bool bit(const Pred &p, int b) {
   return (p.exact[b >> 5] >> (b & 31)) & 1u;
 }
@@ -305,6 +323,14 @@ score_group_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restri
   const int wave = tid >> 6;
   const int sub = tid % L, rl = tid / L;
   const int nimg = pred->nimg;
+  // this lane's 16 images: packed thresholds, images in the batch, images
+  // for which a row with no spot passes
+  uint32_t thr[16];
+#pragma unroll
+  for (int b = 0; b < 16; ++b) thr[b] = pred->pk_thr[16 * sub + b];
+  const int nmine = nimg - 16 * sub;
+  const uint32_t inmask = nmine >= 16 ? 0xffffu : (nmine <= 0 ? 0u : (1u << nmine) - 1u);
+  const uint32_t zok = (pred->zero_ok[sub >> 1] >> (16 * (sub & 1))) & inmask;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t base = tileOff[t];
     const uint64_t tot = tileOff[t + 1] - base;
@@ -371,20 +397,19 @@ score_group_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restri
         swar_px<16>(w, a);
       }
       // masks of this lane's 16 images that pass (cand) or pass with no spot
-      const bool live = o < n;
       uint32_t cmask = 0, zmask = 0;
+      if (o < n) {
 #pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        const int ib = 16 * sub + b;
-        const uint32_t accb = swar_acc<16>(a, b);
-        const uint32_t ns = accb >> kSumBits, sum = accb & kSumMask;
-        if (live && ib < nimg) {
-          if (ns == 0) {
-            if ((0 >= pred->min_spots[ib]) && (0.0 >= pred->min_int[ib])) zmask |= 1u << b;
-          } else if (((int)ns >= pred->min_spots[ib]) && ((double)sum >= pred->cand_thr[ib])) {
+        for (int b = 0; b < 16; ++b) {
+          const uint32_t accb = swar_acc<16>(a, b);
+          const uint32_t ns = accb >> kSumBits, sum = accb & kSumMask;
+          if (ns == 0)
+            zmask |= 1u << b;
+          else if (ns >= (thr[b] >> 23) && sum >= (thr[b] & 0x7fffffu))
             cmask |= 1u << b;
-          }
         }
+        zmask &= zok;
+        cmask &= inmask;
       }
       if (__ballot((cmask | zmask) != 0)) {
         for (int ss = 0; ss < L; ++ss) {
@@ -817,6 +842,8 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       // superset: the sequential float64 sum is within n*2^-53 relative of sum*s
       pred.cand_thr[b] = min_int[b] <= 0 ? -INFINITY : min_int[b] / s * (1.0 - 1e-12);
     }
+    pred.pk_thr[b] = pack_threshold(min_spots[b], pred.cand_thr[b]);
+    if (0 >= min_spots[b] && 0.0 >= min_int[b]) pred.zero_ok[b >> 5] |= 1u << (b & 31);
   }
   hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "predicate upload");

@@ -356,6 +356,48 @@ def test_batch_equals_single(scored_case, nimg):
         np.testing.assert_array_equal(got[i].nspots, single.nspots)
 
 
+@pytest.mark.parametrize("nimg,grouped", [(128, 0xC0), (100, 0xC0), (64, 0xC0), (64, 0),
+                                          (128, 0), (16, 0xC0)])
+def test_batch_thresholds_per_image(scored_case, nimg, grouped):
+    """Each image of a batch with its own thresholds (as the server mixes
+    requests): rows with no spot passing, thresholds no row reaches, negative
+    intensities, images with imScale != 1 and blank images, through the
+    grouped (16 images per lane) and the one-lane table walks; every image
+    bit-exact against the oracle with its own thresholds."""
+    p, hk, O, grains, img, table, db = scored_case
+    rng = np.random.default_rng(500 + nimg)
+    base, _ = api.quantize_u8(img)
+    pairs = [(5, 600.0), (0, 0.0), (0, 300.0), (1, 0.0), (300, 10.0), (5, 1e12), (3, -5.0),
+             (8, 1500.0), (2, 2500.5), (256, 0.0), (-3, 0.0)]
+    imgs, scales, msp, mint = [], [], [], []
+    for i in range(nimg):
+        if i % 6 == 5:
+            u8, s = api.quantize_u8(img * 0.43 + 0.02 * (img > 0))
+        elif i % 11 == 7:
+            u8, s = np.zeros_like(base), 1.0
+        else:
+            u8, s = (base if i == 0 else base[rng.permutation(base.size)]), 1.0
+        ms, mi = pairs[i % len(pairs)]
+        imgs.append(u8)
+        scales.append(s)
+        msp.append(ms)
+        mint.append(mi * s if mi > 0 else mi)
+    _say(f"batch per-image thresholds nimg={nimg} grouped={grouped:#x}")
+    db.set_batch_grouped(grouped)
+    try:
+        got = db.score_batch_each(imgs, scales, msp, mint)
+    finally:
+        db.set_batch_grouped(0xC0)
+    for i in range(nimg):
+        matched, ns, tot, nres = ob.compare(table, 64, imgs[i], p.NrPxX, scales[i], msp[i],
+                                            mint[i])
+        rows, mA = ob.collect(matched)
+        assert got[i].nr_results == nres, (i, msp[i], mint[i])
+        np.testing.assert_array_equal(got[i].ids, rows)
+        np.testing.assert_array_equal(got[i].score, mA)
+        np.testing.assert_array_equal(got[i].nspots, ns[rows.astype(np.int64)])
+
+
 @pytest.mark.parametrize("tb,nimg", [(9, 64), (9, 20), (10, 32), (10, 12), (11, 16), (11, 5),
                                      (12, 8), (13, 4), (13, 3), (0, 16)])
 def test_batch_banded_copy_matches_oracle(scored_case, tb, nimg):
