@@ -345,9 +345,10 @@ def kernel_name(db):
     tb, loads = db.banded_layout()
     if tb == 0:
         return "score_kernel<1,false>"
-    if (loads == 3 or loads >= 5) and tb >= 14:
-        shape = {3: "4,2,1024", 5: "5,2,1024", 6: "4,3,1024", 7: "5,3,1024"}[loads]
-        return f"score_band_lds_kernel<{tb},{11 if tb == 15 else 12},{shape},*>"
+    if ((loads == 3 or loads >= 5) and tb >= 14) or tb == 16:
+        e1, e2 = (8, 12) if tb == 16 else (4, 8)
+        shape = {3: f"{e1},2", 5: f"{e2},2", 6: f"{e1},3", 7: f"{e2},3"}.get(loads, f"{e1},2")
+        return f"score_band_lds_kernel<{tb},{12 if tb == 14 else 11},{shape},1024,*,true>"
     threads = 1024 if tb >= 14 else 512
     if loads == 0:
         return f"score_band_kernel<{tb},{threads},8,false>"
@@ -381,7 +382,7 @@ def main():
                          "table (0 = none) instead of the autotuned one")
     ap.add_argument("--loads", type=int, default=0,
                     help="with --tile-bits: the single-image walk (LM_OPT_BANDED_LOADS)")
-    ap.add_argument("--batch", type=str, default="64",
+    ap.add_argument("--batch", type=str, default="64,128",
                     help="comma-separated batch widths to measure as well (one pass over the "
                          "table per batch), e.g. 8,16,32,64,128; empty = none")
     ap.add_argument("--no-autotune", action="store_true",

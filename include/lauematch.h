@@ -125,9 +125,12 @@ uint64_t lm_db_table_spots(const lm_db *db);
 
 /* Layout options (no reference counterpart; used by the autotuner).
  * LM_OPT_BANDED_TILE_BITS: orientations per super-tile of the pixel-banded
- * table copy used by single-image scoring, as log2 (13, 14 or 15); 0 drops the
+ * table copy used by single-image scoring, as log2 (13..16); 0 drops the
  * copy (scoring then reads the orientation-major table only).  Applies at
- * once if a table exists.
+ * once if a table exists.  16 (MaxNrLaueSpots <= 257 only) keeps 16-bit row
+ * sums and recounts the rows whose sum reaches MinIntensity; calls with
+ * MinIntensity <= 0, an image that is not 16-byte aligned, or more such rows
+ * than n/16 (at least 4096, at most 2^22) read the orientation-major table.
  * LM_OPT_BATCH_TILE_BITS: super-tile size (log2, 9..13) of a second banded
  * copy used by batches of width B (4..64) with B * 2^bits <= 32768 (the
  * per-(image, orientation) counters of a super-tile fill at most 128 KiB of
@@ -144,10 +147,11 @@ uint64_t lm_db_table_spots(const lm_db *db);
 /* LM_OPT_BANDED_LOADS: the single-image banded walk: 0 = eight 4-byte entry
  * loads per lane in flight, 1 = two 16-byte loads (8 entries), 2 = four
  * 16-byte loads (16 entries), 3 = the image staged through LDS in per-wave
- * chunks (2^14 or 2^15 super-tiles), 4 = as 0 with each pixel read through a
- * 4-byte gather of its aligned word; 3, 5, 6, 7 = the LDS walk with 4, 5, 4, 5
- * entry loads per lane and 2, 2, 3, 3 chunk sets in flight per wave.  Default
- * 0. */
+ * chunks (2^14..2^16 super-tiles), 4 = as 0 with each pixel read through a
+ * 4-byte gather of its aligned word; 3, 5, 6, 7 = the LDS walk with 4, 8, 4, 8
+ * entries per lane (8, 12, 8, 12 for 2^16 super-tiles) and 2, 2, 3, 3 chunk
+ * sets in flight per wave.  2^16 super-tiles always take the LDS walk (3 for
+ * 0, 1, 2, 4).  Default 0. */
 #define LM_OPT_BANDED_LOADS 5
 /* LM_OPT_BATCH_GROUPED: bit 6 / bit 7 set = table batches of 64 / 128 images
  * give each orientation B/16 lanes of 16 images each (one 16-byte load per
@@ -167,8 +171,8 @@ typedef struct lm_tune_report {
   int32_t n_widths;
   int32_t widths[6];          /* batch widths timed (4 .. max_batch <= 128)      */
   double ms_single_table;     /* one image: orientation-major table               */
-  double ms_single[3][8];     /* one image: banded 2^13 / 2^14 / 2^15 x loads 0..7
-                                 (0 = not available)                             */
+  double ms_single[4][8];     /* one image: banded 2^13 / 2^14 / 2^15 / 2^16 x
+                                 loads 0..7 (0 = not available)                  */
   double ms_lane[6];          /* per width: orientation-major table               */
   double ms_grouped[6];       /* per width: table, grouped lanes (64, 128 only)   */
   double ms_banded[6][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */

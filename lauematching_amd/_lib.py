@@ -60,7 +60,7 @@ class LmServerStats(ctypes.Structure):
 
 class LmTuneReport(ctypes.Structure):
     _fields_ = [("n_widths", ctypes.c_int32), ("widths", ctypes.c_int32 * 6),
-                ("ms_single_table", ctypes.c_double), ("ms_single", (ctypes.c_double * 8) * 3),
+                ("ms_single_table", ctypes.c_double), ("ms_single", (ctypes.c_double * 8) * 4),
                 ("ms_lane", ctypes.c_double * 6), ("ms_grouped", ctypes.c_double * 6),
                 ("ms_banded", (ctypes.c_double * 5) * 6),
                 ("banded_tile_bits", ctypes.c_int32), ("banded_loads", ctypes.c_int32),

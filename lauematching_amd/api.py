@@ -148,7 +148,9 @@ class OrientationDB:
         return int(lib().lm_db_table_spots(self._h))
 
     def set_banded_tile_bits(self, bits: int) -> None:
-        """Super-tile size (log2) of the pixel-banded copy; 0 drops it."""
+        """Super-tile size (log2, 13..16) of the pixel-banded copy; 0 drops it.
+        16 keeps 16-bit row sums (MaxNrLaueSpots <= 257) and recounts rows
+        whose sum reaches MinIntensity."""
         check(lib().lm_db_set_option(self._h, 1, int(bits)))
 
     def set_batch_tile_bits(self, bits: int) -> None:
@@ -161,9 +163,9 @@ class OrientationDB:
 
     def set_banded_loads(self, loads: int) -> None:
         """Single-image banded walk: 0 = 4-byte loads x8, 1 = 16-byte x2, 2 = 16-byte x4,
-        3 = image staged through LDS (2^14 / 2^15 super-tiles), 4 = 4-byte gathers,
-        5, 6, 7 = LDS-staged with (entry loads per lane, chunk sets in flight)
-        (5, 2), (4, 3), (5, 3); 3 is (4, 2)."""
+        3 = image staged through LDS (2^14..2^16 super-tiles), 4 = 4-byte gathers,
+        5, 6, 7 = LDS-staged with (entries per lane, chunk sets in flight)
+        (8, 2), (4, 3), (8, 3); 3 is (4, 2); twice the entries for 2^16."""
         check(lib().lm_db_set_option(self._h, 5, int(loads)))
 
     def set_batch_grouped(self, mask: int) -> None:
@@ -198,7 +200,7 @@ class OrientationDB:
                                    ctypes.byref(r)))
         widths = [r.widths[i] for i in range(r.n_widths)]
         single = {"table": r.ms_single_table}
-        for i, tb in enumerate((13, 14, 15)):
+        for i, tb in enumerate((13, 14, 15, 16)):
             for loads in range(8):
                 if r.ms_single[i][loads] > 0:
                     single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]

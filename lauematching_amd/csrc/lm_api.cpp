@@ -363,8 +363,12 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   if (option == LM_OPT_BANDED_TILE_BITS) {
-    if (value != 0 && (value < 13 || value > 15)) {
-      set_error("LM_OPT_BANDED_TILE_BITS must be 0, 13, 14 or 15");
+    if (value != 0 && (value < 13 || value > kSum16TileBits)) {
+      set_error("LM_OPT_BANDED_TILE_BITS must be 0 or 13..16");
+      return LM_ERR_INVALID;
+    }
+    if (value == kSum16TileBits && db->max_spots > kSum16MaxSpots) {
+      set_error("LM_OPT_BANDED_TILE_BITS 16 keeps 16-bit sums: MaxNrLaueSpots must be <= 257");
       return LM_ERR_INVALID;
     }
     db->banded_tb = (int)value;

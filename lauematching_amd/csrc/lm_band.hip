@@ -331,13 +331,14 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 // waits on a scalar load.  DENSE adds a slow path for chunks with more than
 // EMAX * 64 entries (the builder records the largest chunk).  LDS traffic of
 // a wave is ordered, so its slot needs no barrier between chunks.
-template <int TB, int CB, int EMAX, int D, int THREADS, bool DENSE>
+template <int TB, int CB, int EMAX, int D, int THREADS, bool DENSE, bool E16>
 __global__ void __launch_bounds__(THREADS)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                       const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
                       uint64_t nsuper, const uint8_t *__restrict__ img, uint64_t npx,
                       const Pred *__restrict__ pred, uint64_t *__restrict__ hits, uint64_t cap,
-                      uint32_t *__restrict__ counters) {
+                      uint32_t *__restrict__ counters, uint64_t *__restrict__ cands,
+                      uint64_t candCap) {
   typedef uint32_t v4u __attribute__((ext_vector_type(4)));
   constexpr int T = 1 << TB;
   constexpr int NW = THREADS / 64;
@@ -345,9 +346,13 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   constexpr int C = 1 << CB;
   constexpr int V = C / (64 * 16);  // 16-byte image loads per lane per chunk
   static_assert(V >= 1 && V * 64 * 16 == C, "chunk must be a multiple of 1 KiB");
-  static_assert(T * 4 + NW * C <= 163840, "LDS");
+  static_assert(!E16 || EMAX % 4 == 0, "16-byte entry loads hold 4 entries per lane");
+  // 2^16-row super-tiles: 16-bit sums, two rows per counter word
+  constexpr bool S16 = TB >= kSum16TileBits;
+  constexpr int NCNT = S16 ? T / 2 : T;
+  static_assert(NCNT * 4 + NW * C <= 163840, "LDS");
   constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
-  __shared__ uint32_t cnt[T];
+  __shared__ uint32_t cnt[NCNT];
   __shared__ v4u slot[NW][C / 16];
   const int tid = threadIdx.x, lane = __lane_id();
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -355,14 +360,18 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   const __amdgpu_buffer_rsrc_t irs =
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
-    for (int i = tid; i < T; i += THREADS) cnt[i] = 0;
+    for (int i = tid; i < NCNT; i += THREADS) cnt[i] = 0;
     __syncthreads();
     const uint64_t first = t << TB;
     const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
     const uint32_t total = co[nchunks];
+    // E16: the entry buffer starts at the 16-byte boundary at or below the
+    // super-tile's first entry (mis entries earlier), so 16-byte entry loads
+    // are aligned
+    const uint64_t e0 = tileOff[first >> kTileBits];
+    const uint32_t mis = E16 ? (uint32_t)(e0 & 3u) : 0u;
     const __amdgpu_buffer_rsrc_t ers = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(bentries + tileOff[first >> kTileBits]), (short)0, (int)(total * 4u),
-        kRawBufferWord3);
+        (void *)(bentries + (e0 - mis)), (short)0, (int)((total + mis) * 4u), kRawBufferWord3);
     // chunk offsets as a buffer too: index nchunks holds `total`; indices are
     // clamped to it, so chunks past the end are empty
     const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
@@ -390,9 +399,25 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
 #pragma unroll
         for (int v = 0; v < V; ++v)
           pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
+        if constexpr (E16) {
+          // lane l holds entries a + 4l .. a + 4l + 3 (+ 256 per load), a the
+          // chunk's first entry rounded down to a 16-byte boundary
+          const uint32_t a = (beg[d] + mis) & ~3u;
 #pragma unroll
-        for (int k = 0; k < EMAX; ++k)
-          pe[d][k] = __builtin_amdgcn_raw_buffer_load_b32(ers, (beg[d] + k * 64 + lane) * 4u, 0, 0);
+          for (int q = 0; q < EMAX / 4; ++q) {
+            const v4u v = __builtin_amdgcn_raw_buffer_load_b128(ers, (a + q * 256 + 4 * lane) * 4u,
+                                                                0, 0);
+            pe[d][4 * q + 0] = v.x;
+            pe[d][4 * q + 1] = v.y;
+            pe[d][4 * q + 2] = v.z;
+            pe[d][4 * q + 3] = v.w;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < EMAX; ++k)
+            pe[d][k] =
+                __builtin_amdgcn_raw_buffer_load_b32(ers, (beg[d] + k * 64 + lane) * 4u, 0, 0);
+        }
         // keep the sets' loads in issue order, so the wait before a set
         // counts only the later sets
         __builtin_amdgcn_sched_barrier(0);
@@ -402,22 +427,36 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         for (int v = 0; v < V; ++v) slot[w][v * 64 + lane] = pim[d][v];
         __builtin_amdgcn_wave_barrier();
         const uint32_t cbeg = beg[d], cend = end[d];
-        // all pixel reads first (lanes past the chunk read a masked, in-slot
-        // byte and drop it), then the counter adds
+        // buffer index of this lane's k-th entry, and of the first entry the
+        // loads above did not cover
+        const uint32_t a = E16 ? ((cbeg + mis) & ~3u) : cbeg + mis;
+        auto idx = [&](int k) {
+          return E16 ? a + (k >> 2) * 256 + 4 * lane + (k & 3) : a + k * 64 + lane;
+        };
+        // all pixel reads first (lanes outside the chunk read a masked,
+        // in-slot byte and drop it), then the counter adds
         uint32_t raw[EMAX];
 #pragma unroll
         for (int k = 0; k < EMAX; ++k) {
           const uint32_t r = mine[pe[d][k] & (C - 1)];
-          raw[k] = cbeg + k * 64 + lane < cend ? r : 0u;
+          const uint32_t i = idx(k);
+          raw[k] = i >= cbeg + mis && i < cend + mis ? r : 0u;
         }
+        auto add = [&](uint32_t e, uint32_t r) {
+          const uint32_t row = e >> PB;
+          if constexpr (S16)
+            atomicAdd(&cnt[row >> 1], r << ((row & 1u) << 4));
+          else
+            atomicAdd(&cnt[row], (1u << kSumBits) + r);
+        };
 #pragma unroll
         for (int k = 0; k < EMAX; ++k)
-          if (raw[k]) atomicAdd(&cnt[pe[d][k] >> PB], (1u << kSumBits) + raw[k]);
+          if (raw[k]) add(pe[d][k], raw[k]);
         if constexpr (DENSE) {
-          for (uint32_t i = cbeg + EMAX * 64 + lane; i < cend; i += 64) {
+          for (uint32_t i = a + EMAX * 64 + lane; i < cend + mis; i += 64) {
             const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
             const uint32_t r = mine[e & (C - 1)];
-            if (r) atomicAdd(&cnt[e >> PB], (1u << kSumBits) + r);
+            if (r) add(e, r);
           }
         }
         __builtin_amdgcn_wave_barrier();
@@ -432,6 +471,35 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
       }
     }
     __syncthreads();
+    if constexpr (S16) {
+      // a row can pass only if its sum reaches MinIntensity and the spot
+      // count it bounds (each nonzero pixel adds at least 1) can reach
+      // MinNrSpots; such rows go to the candidate list for an exact recount
+      const uint32_t pk = pred->pk_thr[0];
+      const uint32_t sthr = pk & 0x7fffffu, nthr = (pk >> 23) > 1u ? (pk >> 23) : 1u;
+      const bool zok = pred->zero_ok[0] & 1u;
+      for (int rr = tid; rr < T; rr += THREADS) {
+        const uint64_t o = first + rr;
+        const uint32_t sum = (cnt[rr >> 1] >> ((rr & 1) << 4)) & 0xffffu;
+        const bool live = o < n;
+        const uint64_t zm = __ballot(live && sum == 0 && zok);
+        if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
+        const bool cand = live && sum >= nthr && sum >= sthr;
+        const uint64_t m = __ballot(cand);
+        if (m) {
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          uint32_t b0 = 0;
+          if (lane == leader) b0 = atomicAdd(&counters[kCandSlot], (uint32_t)__popcll(m));
+          b0 = __shfl(b0, leader);
+          if (cand) {
+            const uint64_t pos = (uint64_t)b0 + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (pos < candCap) cands[pos] = o;
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     for (int rr = tid; rr < T; rr += THREADS) {
       const uint64_t o = first + rr;
       const uint32_t acc = cnt[rr];
@@ -456,6 +524,52 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
       }
     }
     __syncthreads();
+  }
+}
+
+// Exact recount of the candidate rows of the 16-bit-sum walk, one wave per
+// row (grid-stride over the list): the row's spots are read from the
+// orientation-major table (its offset is the tile offset plus the counts of
+// the rows before it in the tile), and nspots (nonzero pixels) and the sum
+// are counted again; rows passing nspots >= MinNrSpots and sum >= MinIntensity
+// (src/LaueMatchingCPU.c:683-689) are appended as (row << 26 | nspots << 17 |
+// sum), the record of the other walks.
+__global__ void __launch_bounds__(256)
+recount_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+               const uint32_t *__restrict__ entries, uint64_t n, const uint8_t *__restrict__ img,
+               const Pred *__restrict__ pred, const uint64_t *__restrict__ cands,
+               uint64_t candCap, uint64_t *__restrict__ hits, uint64_t cap,
+               uint32_t *__restrict__ counters) {
+  const int lane = __lane_id();
+  const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t ncand = std::min<uint64_t>(counters[kCandSlot], candCap);
+  const int min_spots = pred->min_spots[0];
+  const double min_int = pred->min_int[0];
+  auto wave_sum = [](uint32_t v) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d);
+    return v;
+  };
+  for (uint64_t c = wid; c < ncand; c += nw) {
+    const uint64_t o = cands[c];
+    if (o >= n) continue;  // never written by the walk (wave-uniform)
+    const uint64_t t0 = (o >> kTileBits) << kTileBits;
+    const uint32_t r = (uint32_t)(o - t0);
+    uint32_t part = 0;
+    for (uint32_t i = lane; i < r; i += 64) part += counts[t0 + i];
+    const uint64_t off = tileOff[o >> kTileBits] + wave_sum(part);
+    const uint32_t no = counts[o];
+    uint32_t ns = 0, sum = 0;
+    for (uint32_t s0 = 0; s0 < no; s0 += 64) {
+      const uint32_t raw = s0 + lane < no ? (uint32_t)img[entries[off + s0 + lane]] : 0u;
+      ns += (uint32_t)__popcll(__ballot(raw != 0));
+      sum += wave_sum(raw);
+    }
+    if (lane == 0 && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int) {
+      const uint32_t pos = atomicAdd(&counters[0], 1u);
+      if (pos < cap) hits[pos] = (o << 26) | ((uint64_t)ns << kSumBits) | sum;
+    }
   }
 }
 
@@ -698,8 +812,12 @@ void release_banded(lm_db *db) {
 
 int build_banded(lm_db *db) {
   release_banded(db);
-  if (db->banded_tb != 0 && (db->banded_tb < 13 || db->banded_tb > 15)) {
-    set_error("banded super-tile bits must be 0, 13, 14 or 15");
+  if (db->banded_tb != 0 && (db->banded_tb < 13 || db->banded_tb > kSum16TileBits)) {
+    set_error("banded super-tile bits must be 0 or 13..16");
+    return LM_ERR_INVALID;
+  }
+  if (db->banded_tb == kSum16TileBits && db->max_spots > kSum16MaxSpots) {
+    set_error("2^16-row super-tiles keep 16-bit sums: MaxNrLaueSpots must be at most 257");
     return LM_ERR_INVALID;
   }
   if (db->batch_tb != 0 && (db->batch_tb < 9 || db->batch_tb > 13)) {
@@ -710,6 +828,10 @@ int build_banded(lm_db *db) {
   if (rc == LM_OK) rc = build_one(db, db->banded_b, db->batch_tb, false);
   if (rc) release_banded(db);
   return rc;
+}
+
+uint64_t cand_capacity(const lm_db *db) {
+  return std::min<uint64_t>(std::max<uint64_t>(db->n / 16, 4096), 1ull << 22);
 }
 
 int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st) {
@@ -724,39 +846,60 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
   // the LDS-staged walk loads the image in 16-byte pieces; an image that is
   // not 16-byte aligned (a caller's device pointer) takes the gather walk
-  const int L = db->banded_loads;
+  // 2^16-row super-tiles have only the LDS-staged walk (with a 16-byte
+  // aligned image; the caller takes the table walk otherwise)
+  const bool s16 = b.tb >= kSum16TileBits;
+  const int L = s16 && !(db->banded_loads == 3 || db->banded_loads >= 5) ? 3 : db->banded_loads;
   const bool lds_walk = L == 3 || L >= 5;
+  if (s16 && (((uintptr_t)img & 15) || !sc.cands)) {
+    set_error("2^16-row walk needs a 16-byte aligned image and a candidate list");
+    return LM_ERR_INVALID;
+  }
   if (lds_walk && ((uintptr_t)img & 15) == 0) {
-    if (!b.chunk_off || (b.tb != 14 && b.tb != 15)) {
-      set_error("LDS-staged walk needs 2^14 or 2^15 super-tiles and chunk offsets");
+    if (!b.chunk_off || b.tb < 14 || b.tb > kSum16TileBits) {
+      set_error("LDS-staged walk needs 2^14..2^16 super-tiles and chunk offsets");
       return LM_ERR_INVALID;
     }
     // shapes: entry loads per lane per chunk, chunk sets in flight per wave,
     // threads per workgroup (1024: 128 VGPRs per lane; 512: up to 256)
     // chunks larger than EMAX * 64 entries need the kernel's slow path
+    // 16-byte entry loads start up to 3 entries before a chunk, so a chunk
+    // fits the fast loads when it has at most EM * 64 - 3 entries
+    constexpr bool kE16 = true;
+    const uint32_t slack = kE16 ? 3u : 0u;
 #define LM_LDS(TB, EM, DD, TH)                                                                   \
-  if (b.max_chunk <= (uint32_t)(EM)*64u)                                                         \
-    hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, false>),       \
+  if (b.max_chunk + slack <= (uint32_t)(EM)*64u)                                                 \
+    hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, false, kE16>), \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
-                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters); \
+                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters,  \
+                       sc.cands, sc.cand_cap);                                                    \
   else                                                                                            \
-    hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, true>),        \
+    hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, true, kE16>),  \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
-                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters)
-#define LM_LDS_ALL(TB)                      \
-  switch (L) {                              \
-    case 3: LM_LDS(TB, 4, 2, 1024); break;  \
-    case 5: LM_LDS(TB, 5, 2, 1024); break;  \
-    case 6: LM_LDS(TB, 4, 3, 1024); break;  \
-    default: LM_LDS(TB, 5, 3, 1024); break; \
+                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters,  \
+                       sc.cands, sc.cand_cap)
+#define LM_LDS_ALL(TB, E1, E2)               \
+  switch (L) {                               \
+    case 3: LM_LDS(TB, E1, 2, 1024); break;  \
+    case 5: LM_LDS(TB, E2, 2, 1024); break;  \
+    case 6: LM_LDS(TB, E1, 3, 1024); break;  \
+    default: LM_LDS(TB, E2, 3, 1024); break; \
   }
-    if (b.tb == 15) {
-      LM_LDS_ALL(15);
+    // twice the rows per super-tile, twice the entries per chunk
+    if (b.tb == 16) {
+      LM_LDS_ALL(16, 8, 12);
+    } else if (b.tb == 15) {
+      LM_LDS_ALL(15, 4, 8);
     } else {
-      LM_LDS_ALL(14);
+      LM_LDS_ALL(14, 4, 8);
     }
 #undef LM_LDS_ALL
 #undef LM_LDS
+    if (s16) {
+      hipLaunchKernelGGL(recount_kernel, dim3(1024), dim3(256), 0, st, t.counts, t.tile_off,
+                         t.entries, db->n, img, sc.pred, sc.cands, sc.cand_cap, sc.hits, cap,
+                         sc.counters);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
     return LM_OK;

@@ -94,7 +94,9 @@ struct Pred {
 struct Scratch {
   uint64_t *hits = nullptr;  // packed hit records, cap_per * nimg
   uint64_t hit_cap = 0;
-  uint32_t *counters = nullptr;  // 2 * kMaxBatch
+  uint32_t *counters = nullptr;  // kCounterWords
+  uint64_t *cands = nullptr;     // candidate rows of the 16-bit-sum walk
+  uint64_t cand_cap = 0;
   double *vals = nullptr;        // float64 sums of in-order hits
   uint64_t val_cap = 0;
   uint8_t *batch = nullptr;      // interleaved images [npx][B]
@@ -166,6 +168,18 @@ void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 
 int pix_bits_for(uint64_t npx);
+// Device counters of a scoring call: per image b, hits at [2b] and rows
+// passing with no spot at [2b + 1]; candidate rows of the 16-bit-sum walk at
+// kCandSlot.
+constexpr int kCandSlot = 2 * kMaxBatch;
+constexpr int kCounterWords = 2 * kMaxBatch + 2;
+
+// Single-image banded copies of 2^16-row super-tiles keep 16-bit sums (two
+// rows per LDS word): exact while 255 * max_spots < 2^16; the spot counts of
+// rows whose sum reaches the threshold are then recounted from the table.
+constexpr int kSum16TileBits = 16;
+constexpr int kSum16MaxSpots = 257;
+
 // Orientations per table tile = 1 << kTileBits (one scoring workgroup).
 constexpr int kTileBits = 8;
 
@@ -182,6 +196,9 @@ int table_checksum(lm_db *db, uint64_t *sum);
 int build_banded(lm_db *db);
 void release_banded(lm_db *db);
 int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st);
+// Candidate list of the 16-bit-sum walk: rows / 16, at least 4096, at most
+// 2^22 (32 MiB); a call with more candidates reruns over the table.
+uint64_t cand_capacity(const lm_db *db);
 bool banded_batch_fits(const lm_db *db, int B);
 int launch_banded_batch(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
                         hipStream_t st);

@@ -588,6 +588,18 @@ int ensure_vals(Scratch &sc, uint64_t need) {
   return LM_OK;
 }
 
+int ensure_cands(Scratch &sc, uint64_t need) {
+  if (sc.cand_cap >= need) return LM_OK;
+  drop_graphs(sc);
+  if (sc.cands) (void)hipFree(sc.cands);
+  sc.cands = nullptr;
+  sc.cand_cap = 0;
+  hipError_t e = hipMalloc(&sc.cands, need * 8);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(candidate rows)");
+  sc.cand_cap = need;
+  return LM_OK;
+}
+
 int ensure_batch(Scratch &sc, uint64_t need) {
   if (sc.batch_cap >= need) return LM_OK;
   drop_graphs(sc);
@@ -695,7 +707,7 @@ int width_log2(int B) {
 
 int scratch_init(Scratch &s) {
   if (s.pred) return LM_OK;
-  hipError_t e = hipMalloc(&s.counters, 2 * kMaxBatch * 4);
+  hipError_t e = hipMalloc(&s.counters, kCounterWords * 4);
   if (e == hipSuccess) e = hipMalloc(&s.pred, sizeof(Pred));
   if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&s.ev0);
@@ -711,6 +723,7 @@ void scratch_release(Scratch &s) {
   for (auto &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
   if (s.hits) (void)hipFree(s.hits);
   if (s.counters) (void)hipFree(s.counters);
+  if (s.cands) (void)hipFree(s.cands);
   if (s.vals) (void)hipFree(s.vals);
   if (s.batch) (void)hipFree(s.batch);
   if (s.img) (void)hipFree(s.img);
@@ -754,8 +767,8 @@ int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
     rc = ensure_hits(sc, cap);
     if (rc == LM_OK) rc = ensure_vals(sc, cap);
     if (rc) break;
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(2 * kMaxBatch), 0, st, sc.counters,
-                       2 * kMaxBatch);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kCounterWords), 0, st, sc.counters,
+                       kCounterWords);
     (void)hipEventRecord(sc.ev0, st);
     if (f64)
       hipLaunchKernelGGL(score_seq_kernel<double>, dim3(grid), dim3(kTile), 0, st, t.counts,
@@ -851,7 +864,17 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   // kernel: 0 lane-per-orientation over the table, 1 single-image banded
   // copy, 2 batch banded copy
   int kind = 0;
-  if (B == 1 && db->banded.ready && bit(pred, 0)) kind = 1;
+  // the 16-bit-sum walk keeps rows whose sum reaches MinIntensity as
+  // candidates: with MinIntensity <= 0 that is every row, so the table walk
+  // is taken instead
+  const bool sum16 = db->banded.ready && db->banded.tb >= kSum16TileBits;
+  if (B == 1 && db->banded.ready && bit(pred, 0) &&
+      (!sum16 || (min_int[0] > 0 && ((uintptr_t)d_imgs & 15) == 0)))
+    kind = 1;
+  if (kind == 1 && sum16) {
+    rc = ensure_cands(sc, cand_capacity(db));
+    if (rc) return rc;
+  }
   if (B > 1 && banded_batch_fits(db, B) && ((db->batch_widths >> width_log2(B)) & 1)) kind = 2;
   if (B > 1) {
     rc = ensure_batch(sc, npx * B);
@@ -865,8 +888,8 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       const int r = launch_interleave(B, d_imgs, sc.pred, npx, sc.batch, st);
       if (r) return r;
     }
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(2 * kMaxBatch), 0, st, sc.counters,
-                       2 * kMaxBatch);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kCounterWords), 0, st, sc.counters,
+                       kCounterWords);
     if (kind == 1) return launch_banded(db, sc, img, cp, st);
     if (kind == 2) return launch_banded_batch(db, sc, B, img, cp, st);
     return launch_score(db, sc, B, img, cp, st);
@@ -898,25 +921,33 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     return le == hipSuccess ? LM_OK : hip_fail(le, "hipGraphLaunch");
   };
   float ms = 0;
-  std::vector<uint32_t> cnt(2 * kMaxBatch);
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  std::vector<uint32_t> cnt(kCounterWords);
+  // at most three runs: a candidate overflow switches to the table walk
+  // once, a hit overflow regrows the hit buffer once
+  bool grown = false;
+  for (;;) {
     rc = ensure_hits(sc, capPer * (uint64_t)B);  // a width-B launch serves any nimg <= B
     if (rc) return rc;
     (void)hipEventRecord(sc.ev0, st);
     rc = run(capPer);
     if (rc) return rc;
     (void)hipEventRecord(sc.ev1, st);
-    e = hipMemcpyAsync(cnt.data(), sc.counters, 2 * kMaxBatch * 4, hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(cnt.data(), sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "score kernel");
     (void)hipEventElapsedTime(&ms, sc.ev0, sc.ev1);
+    if (kind == 1 && sum16 && cnt[kCandSlot] > sc.cand_cap) {
+      kind = 0;  // more candidates than the list holds: rerun over the table
+      continue;
+    }
     uint64_t need = 0;
     for (int b = 0; b < nimg; ++b) need = std::max<uint64_t>(need, cnt[2 * b]);
     if (need <= capPer) break;
-    if (attempt == 1) {
+    if (grown) {
       set_error("score: hit list overflow");
       return LM_ERR_OVERFLOW;
     }
+    grown = true;
     capPer = need + need / 4 + 1024;  // rerun once with a buffer that fits
   }
   std::vector<uint64_t> rec;

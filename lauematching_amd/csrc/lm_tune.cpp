@@ -2,7 +2,7 @@
 //
 // The compare step can read three layouts of the same spot table:
 //   - the orientation-major table, one lane per orientation (any batch width),
-//   - the single-image banded copy (super-tiles of 2^13, 2^14 or 2^15 rows),
+//   - the single-image banded copy (super-tiles of 2^13 .. 2^16 rows),
 //   - the batch banded copy (super-tiles of 2^9..2^13 rows; a batch of width
 //     B fits when B * 2^tb <= 32768, its LDS counters filling <= 128 KiB).
 // Which one is fastest depends on the image density, the table's spots per
@@ -69,7 +69,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     const int r = build_banded(db);
     return code ? code : r;
   };
-  // single image: the table, then 2^13, 2^14 and 2^15 super-tiles x entry loads
+  // single image: the table, then 2^13 .. 2^16 super-tiles x entry loads
   db->batch_tb = 0;
   db->banded_tb = 0;
   rc = build_banded(db);
@@ -78,13 +78,17 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   if (rc) return restore(rc);
   double best1 = rep->ms_single_table;
   int best_tb = 0, best_loads = 0;
-  for (int i = 0; i < 3; ++i) {
+  // (2^16 rows keep 16-bit sums: only the LDS-staged walks, only when a
+  // row's sum fits)
+  const int max_tb = db->max_spots <= kSum16MaxSpots ? kSum16TileBits : 15;
+  for (int i = 0; 13 + i <= max_tb; ++i) {
     db->banded_tb = 13 + i;
     rc = build_banded(db);
     if (rc) return restore(rc);
     for (int loads = 0; loads < 8; ++loads) {
       const bool lds_walk = loads == 3 || loads >= 5;
       if (lds_walk && (!db->banded.chunk_off || db->banded_tb == 13)) continue;
+      if (!lds_walk && db->banded_tb == kSum16TileBits) continue;
       db->banded_loads = loads;
       ++db->generation;
       rc = time_batch(db, d, 1, reps, &rep->ms_single[i][loads]);

@@ -242,6 +242,65 @@ def test_score_layouts_agree(scored_case, bits, loads):
         db.set_banded_loads(0)
 
 
+@pytest.mark.parametrize("loads", [3, 5, 6, 7, 0])
+def test_score_sum16_walk(scored_case, loads):
+    """2^16-row super-tiles with 16-bit row sums: rows whose sum reaches
+    MinIntensity are recounted from the table; thresholds with more such rows
+    than the candidate list holds (4096 here), MinIntensity <= 0 and an
+    unaligned device image take the table walk.  All bit-exact."""
+    import torch
+
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img)
+    _say(f"score sum16 loads={loads}")
+    db.set_banded_tile_bits(16)
+    db.set_banded_loads(loads)
+    assert db.banded_layout()[0] == 16
+    recounted = overflowed = 0
+    try:
+        for min_sp, min_int in [(5, 1000.0), (3, 2500.0), (0, 600.0), (5, 250.0), (8, 0.0),
+                                (0, 0.0), (1, 1.0), (300, 900.0), (2, 1e9)]:
+            hits = db.score(u8, s, min_sp, min_int)
+            matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, min_sp, min_int)
+            rows, mA = ob.collect(matched)
+            assert hits.nr_results == nres, (min_sp, min_int)
+            np.testing.assert_array_equal(hits.ids, rows)
+            np.testing.assert_array_equal(hits.score, mA)
+            np.testing.assert_array_equal(hits.nspots, ns[rows.astype(np.int64)])
+            np.testing.assert_array_equal(hits.totint, tot[rows.astype(np.int64)])
+            if min_int > 0:
+                ncand = int(np.count_nonzero((tot >= min_int) & (tot >= max(min_sp, 1))))
+                recounted += 0 < ncand <= 4096
+                overflowed += ncand > 4096
+        assert recounted >= 2 and overflowed >= 1
+        # a device image at an odd address takes the table walk
+        buf = torch.zeros(u8.size + 16, dtype=torch.uint8, device="cuda:0")
+        buf[1:1 + u8.size] = torch.from_numpy(u8).to("cuda:0")
+        torch.cuda.synchronize()
+        hits = db.score_dev(buf.data_ptr() + 1, 1.0, 5, 1000.0)
+        matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, 1.0, 5, 1000.0)
+        rows, mA = ob.collect(matched)
+        np.testing.assert_array_equal(hits.ids, rows)
+        np.testing.assert_array_equal(hits.score, mA)
+    finally:
+        db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
+
+
+def test_sum16_needs_small_rows(ctx):
+    """16-bit sums hold 255 * 257 at most: a DB with MaxNrLaueSpots > 257
+    refuses 2^16 super-tiles."""
+    p = synth.baseline_params(MaxNrLaueSpots=300)
+    hk = synth.fcc_hkls(4, "box")
+    O = synth.random_orientations(2000, seed=3)
+    db = api.OrientationDB(ctx, O, hk, api.Geometry.from_params(p), 300)
+    db.forward()
+    with pytest.raises(_lib.LaueMatchError):
+        db.set_banded_tile_bits(16)
+    db.set_banded_tile_bits(15)
+    db.close()
+
+
 @pytest.mark.parametrize("geo", ["small_rect", "det1024"])
 def test_score_other_detectors(ctx, geo):
     over = dict(GEOMS.get(geo, {}))
@@ -444,7 +503,7 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
     try:
         assert set(rep["ms_batch"]) == {4, 8, 16, 32}  # max_batch=32
         singles = rep["ms_single"]
-        assert len(singles) == 21 and all(v > 0 for v in singles.values())
+        assert len(singles) == 25 and all(v > 0 for v in singles.values())
         tb = rep["banded_tile_bits"]
         chosen = "table" if tb == 0 else f"banded{tb}_loads{rep['banded_loads']}"
         assert singles[chosen] == min(singles.values())
