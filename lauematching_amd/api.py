@@ -98,10 +98,15 @@ class Hits:
     def _take(cls, h: LmHits) -> "Hits":
         k = int(h.count)
         if k:
-            ids = np.ctypeslib.as_array(h.ids, (k,)).copy()
-            ns = np.ctypeslib.as_array(h.nspots, (k,)).copy()
-            tot = np.ctypeslib.as_array(h.totint, (k,)).copy()
-            sc = np.ctypeslib.as_array(h.score, (k,)).copy()
+            def grab(ptr, dtype):  # one byte copy out of the C arrays (writable result)
+                a = np.empty(k, dtype)
+                ctypes.memmove(a.ctypes.data, ptr, k * a.itemsize)
+                return a
+
+            ids = grab(h.ids, np.uint64)
+            ns = grab(h.nspots, np.int32)
+            tot = grab(h.totint, np.float64)
+            sc = grab(h.score, np.float64)
         else:
             ids = np.zeros(0, np.uint64)
             ns = np.zeros(0, np.int32)

@@ -401,11 +401,16 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
           pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
         if constexpr (E16) {
           // lane l holds entries a + 4l .. a + 4l + 3 (+ 256 per load), a the
-          // chunk's first entry rounded down to a 16-byte boundary
+          // chunk's first entry rounded down to a 16-byte boundary; the
+          // descriptor ends at the chunk's last entry, so loads past it read
+          // zeros instead of the next chunk's entries
           const uint32_t a = (beg[d] + mis) & ~3u;
+          const __amdgpu_buffer_rsrc_t crs16 = __builtin_amdgcn_make_buffer_rsrc(
+              (void *)(bentries + (e0 - mis) + a), (short)0, (int)((end[d] + mis - a) * 4u),
+              kRawBufferWord3);
 #pragma unroll
           for (int q = 0; q < EMAX / 4; ++q) {
-            const v4u v = __builtin_amdgcn_raw_buffer_load_b128(ers, (a + q * 256 + 4 * lane) * 4u,
+            const v4u v = __builtin_amdgcn_raw_buffer_load_b128(crs16, (q * 256 + 4 * lane) * 4u,
                                                                 0, 0);
             pe[d][4 * q + 0] = v.x;
             pe[d][4 * q + 1] = v.y;

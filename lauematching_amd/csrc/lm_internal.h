@@ -75,6 +75,7 @@ constexpr int kMaxBatch = 128;
 struct Pred {
   double min_int[kMaxBatch];
   double cand_thr[kMaxBatch];  // candidate threshold on the integer sum
+  double scale[kMaxBatch];     // imScale of image b
   int min_spots[kMaxBatch];
   uint32_t exact[kMaxBatch / 32]; // bit b: image b has imScale == 1
   // The same candidate test in integers, for kernels whose lanes hold

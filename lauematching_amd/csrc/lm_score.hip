@@ -529,23 +529,75 @@ __global__ void interleave_kernel(const uint8_t *__restrict__ src, const Pred *_
   }
 }
 
+// The same interleave for B >= 16 with 4-byte aligned images and a pixel
+// count divisible by 4: a thread takes 4 pixels of 16 images (one 4-byte
+// load per image), transposes the 4 x 16 bytes in registers and stores each
+// pixel's 16 bytes with one 16-byte store; the B / 16 threads of a pixel
+// quad are neighbouring lanes, so each store instruction writes whole
+// B-byte pixel rows.
+template <int B>
+__global__ void interleave4_kernel(const uint8_t *__restrict__ src, const Pred *__restrict__ pred,
+                                   uint64_t nquads, uint64_t npx, uint8_t *__restrict__ dst) {
+  constexpr int G = B / 16;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = tid / G;
+  const int g = (int)(tid % G);
+  if (q >= nquads) return;
+  const int nimg = pred->nimg;
+  uint32_t v[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int b = 16 * g + j;
+    v[j] = b < nimg ? *(const uint32_t *)(src + (uint64_t)b * npx + 4 * q) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      w[k] = ((v[4 * k + 0] >> (8 * i)) & 0xffu) | (((v[4 * k + 1] >> (8 * i)) & 0xffu) << 8) |
+             (((v[4 * k + 2] >> (8 * i)) & 0xffu) << 16) |
+             (((v[4 * k + 3] >> (8 * i)) & 0xffu) << 24);
+    *(u32x4 *)(dst + (4 * q + i) * B + 16 * g) = u32x4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+// Packs the hit records of every image of a batch back to back (image b's
+// region starts after the records of images 0..b-1), so the host copies them
+// with one transfer.  One workgroup per image; counts <= capPer by the time
+// this runs.
+__global__ void pack_hits_kernel(const uint64_t *__restrict__ hits, uint64_t capPer,
+                                 const uint32_t *__restrict__ counters, int nimg,
+                                 uint64_t *__restrict__ out) {
+  const int b = blockIdx.x;
+  if (b >= nimg) return;
+  uint64_t off = 0;
+  for (int j = 0; j < b; ++j) off += std::min<uint64_t>(counters[2 * j], capPer);
+  const uint64_t n = std::min<uint64_t>(counters[2 * b], capPer);
+  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) out[off + i] = hits[b * capPer + i];
+}
+
 // Exact float64 sums of candidates (src/LaueMatchingCPU.c:674-681 order).
 // img points at image b's bytes with pixel stride `stride`.
 __global__ void exact_kernel(const uint16_t *__restrict__ counts,
                              const uint64_t *__restrict__ tileOff,
-                             const uint32_t *__restrict__ entries, const uint32_t *__restrict__ ids,
+                             const uint32_t *__restrict__ entries, const uint64_t *__restrict__ xids,
                              uint64_t ncand, const uint8_t *__restrict__ img, uint32_t stride,
-                             double scale, double *__restrict__ out) {
+                             const Pred *__restrict__ pred, double *__restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncand) return;
-  const uint64_t o = ids[i];
+  const uint64_t x = xids[i];
+  const uint32_t b = (uint32_t)(x >> 32);
+  const uint64_t o = x & 0xffffffffull;
   const uint64_t t = o / kTile;
   uint64_t off = tileOff[t];
   for (uint64_t r = t * kTile; r < o; ++r) off += counts[r];
   const int c = counts[o];
+  const double scale = pred->scale[b];
+  const uint8_t *im = img + (stride > 1 ? b : 0u);
   double tot = 0;
   for (int s = 0; s < c; ++s) {
-    const uint8_t raw = img[(uint64_t)entries[off + s] * stride];
+    const uint8_t raw = im[(uint64_t)entries[off + s] * stride];
     if (raw > 0) tot += (double)raw * scale;
   }
   out[i] = tot;
@@ -657,6 +709,23 @@ int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t cap
 
 int launch_interleave(int B, const uint8_t *src, const Pred *pred, uint64_t npx, uint8_t *dst,
                       hipStream_t st) {
+  if (B >= 16 && (npx & 3) == 0 && ((uintptr_t)src & 3) == 0) {
+    const uint64_t nquads = npx / 4;
+    const unsigned g4 = (unsigned)((nquads * (B / 16) + 255) / 256);
+#define LM_IL4(BB)                                                                             \
+  hipLaunchKernelGGL(interleave4_kernel<BB>, dim3(g4), dim3(256), 0, st, src, pred, nquads, npx, \
+                     dst)
+    switch (B) {
+      case 16: LM_IL4(16); break;
+      case 32: LM_IL4(32); break;
+      case 64: LM_IL4(64); break;
+      default: LM_IL4(128); break;
+    }
+#undef LM_IL4
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "interleave kernel launch");
+    return LM_OK;
+  }
   const unsigned grid = (unsigned)((npx + 255) / 256);
 #define LM_IL(BB) \
   hipLaunchKernelGGL(interleave_kernel<BB>, dim3(grid), dim3(256), 0, st, src, pred, npx, dst)
@@ -847,6 +916,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     const double s = im_scales[b];
     pred.min_spots[b] = min_spots[b];
     pred.min_int[b] = min_int[b];
+    pred.scale[b] = s;
     if (s == 1.0) {
       // totInt == sum exactly: sum >= min_int decided on the device
       pred.cand_thr[b] = min_int[b];
@@ -950,45 +1020,61 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     grown = true;
     capPer = need + need / 4 + 1024;  // rerun once with a buffer that fits
   }
-  std::vector<uint64_t> rec;
-  uint32_t *d_ids = nullptr;
-  double *d_exact = nullptr;
+  // every image's hit records in one host buffer, copied with one wait
+  std::vector<uint64_t> first(nimg + 1, 0);
+  for (int b = 0; b < nimg; ++b) first[b + 1] = first[b] + cnt[2 * b];
+  std::vector<uint64_t> all(first[nimg]);
+  if (nimg == 1 && first[1]) {
+    e = hipMemcpyAsync(all.data(), sc.hits, first[1] * 8, hipMemcpyDeviceToHost, st);
+  } else if (first[nimg]) {
+    rc = ensure_vals(sc, first[nimg]);
+    if (rc) return rc;
+    uint64_t *packed = (uint64_t *)sc.vals;
+    hipLaunchKernelGGL(pack_hits_kernel, dim3(nimg), dim3(256), 0, st, sc.hits, capPer,
+                       sc.counters, nimg, packed);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(all.data(), packed, first[nimg] * 8, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "hit copy");
+  // images with imScale != 1: the float64 in-order sums of all their
+  // candidates in one launch (candidate = image << 32 | row; the hit buffer
+  // is free again once the records are on the host)
+  for (int b = 0; b < nimg; ++b) std::sort(all.begin() + first[b], all.begin() + first[b + 1]);
+  std::vector<uint64_t> xids;
+  std::vector<uint64_t> xfirst(nimg + 1, 0);
+  for (int b = 0; b < nimg; ++b) {
+    if (!bit(pred, b))
+      for (uint64_t i = first[b]; i < first[b + 1]; ++i)
+        xids.push_back(((uint64_t)b << 32) | ((all[i] >> 26) & 0xffffffffull));
+    xfirst[b + 1] = xids.size();
+  }
+  std::vector<double> xtot(xids.size());
+  if (!xids.empty()) {
+    const uint64_t k = xids.size();
+    rc = ensure_vals(sc, k);
+    if (rc) return rc;
+    e = hipMemcpyAsync(sc.hits, xids.data(), k * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(exact_kernel, dim3((unsigned)((k + 127) / 128)), dim3(128), 0, st,
+                         t.counts, t.tile_off, t.entries, sc.hits, k, img, (uint32_t)B, sc.pred,
+                         sc.vals);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(xtot.data(), sc.vals, k * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "exact pass");
+  }
   for (int b = 0; b < nimg; ++b) memset(&outs[b], 0, sizeof(lm_hits));
   for (int b = 0; b < nimg && rc == LM_OK; ++b) {
     lm_hits &h = outs[b];
     h.kernel_ms = ms;
     const uint64_t k = cnt[2 * b];
-    rec.resize(k);
-    if (k) {
-      e = hipMemcpyAsync(rec.data(), sc.hits + (uint64_t)b * capPer, k * 8,
-                         hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) { rc = hip_fail(e, "hit copy"); break; }
-      std::sort(rec.begin(), rec.end());
-    }
+    const uint64_t *rec = all.data() + first[b];
     std::vector<double> tot(k);
-    if (bit(pred, b)) {
-      for (uint64_t i = 0; i < k; ++i) tot[i] = (double)(rec[i] & kSumMask);
-    } else if (k) {
-      std::vector<uint32_t> ids(k);
-      for (uint64_t i = 0; i < k; ++i) ids[i] = (uint32_t)((rec[i] >> 26) & 0xffffffffull);
-      if (d_ids) (void)hipFree(d_ids);
-      if (d_exact) (void)hipFree(d_exact);
-      d_ids = nullptr;
-      d_exact = nullptr;
-      e = hipMalloc(&d_ids, k * 4);
-      if (e == hipSuccess) e = hipMalloc(&d_exact, k * 8);
-      if (e == hipSuccess) e = hipMemcpyAsync(d_ids, ids.data(), k * 4, hipMemcpyHostToDevice, st);
-      if (e != hipSuccess) { rc = hip_fail(e, "exact pass setup"); break; }
-      const uint8_t *ib = (B > 1) ? img + b : img;
-      hipLaunchKernelGGL(exact_kernel, dim3((unsigned)((k + 127) / 128)), dim3(128), 0, st,
-                         t.counts, t.tile_off, t.entries, d_ids, k, ib, (uint32_t)B,
-                         im_scales[b], d_exact);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipMemcpyAsync(tot.data(), d_exact, k * 8, hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) { rc = hip_fail(e, "exact pass"); break; }
-    }
+    for (uint64_t i = 0; i < k; ++i)
+      tot[i] = bit(pred, b) ? (double)(rec[i] & kSumMask) : xtot[xfirst[b] + i];
     // final predicate and arrays (ascending id)
     uint64_t m = 0;
     for (uint64_t i = 0; i < k; ++i)
@@ -1011,8 +1097,6 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   }
   if (rc)
     for (int b = 0; b < nimg; ++b) lm_hits_free(&outs[b]);
-  if (d_ids) (void)hipFree(d_ids);
-  if (d_exact) (void)hipFree(d_exact);
   return rc;
 }
 
