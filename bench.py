@@ -274,6 +274,48 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
     return res
 
 
+def merge_check(ctx, O):
+    """Merge of duplicate candidates (the hit list's consumer): the device pair
+    tests at the reference's 50000-candidate cap, and the host merge (16
+    threads) beside it on a 5000-candidate sample (its O(n^2) cost makes the
+    full cap about 100x the sample); both produce the same clusters."""
+    from scipy.spatial.transform import Rotation
+
+    from lauematching_amd import api
+
+    rng = np.random.default_rng(11)
+    # candidates as a hit list would give them: members of a few thousand
+    # grains (within 1.6 degrees of their grain) in shuffled list order
+    n_grains, per = 2500, 20
+    centres = Rotation.from_matrix(O[rng.choice(len(O), n_grains, replace=False)].reshape(-1, 3, 3))
+    axes = rng.normal(size=(n_grains * (per - 1), 3))
+    axes /= np.linalg.norm(axes, axis=1, keepdims=True)
+    ang = np.radians(rng.uniform(0, 1.6, len(axes)))
+    c = Rotation.from_quat(np.repeat(centres.as_quat(), per - 1, axis=0))
+    mats = np.concatenate([centres.as_matrix().reshape(-1, 9),
+                           (c * Rotation.from_rotvec(axes * ang[:, None])).as_matrix().reshape(-1, 9)])
+    rows = rng.permutation(len(mats)).astype(np.uint64)
+    scores = rng.uniform(100, 1000, len(rows))
+    out = {"candidates": int(len(rows)), "max_angle": 2.0, "space_group": 225}
+    api.merge_candidates(mats, rows[:64], scores[:64], 2.0, 225, ctx=ctx)
+    t0 = time.perf_counter()
+    g = api.merge_candidates(mats, rows, scores, 2.0, 225, ctx=ctx)
+    out["gpu_ms"] = (time.perf_counter() - t0) * 1e3
+    out["clusters"] = int(len(g[0]))
+    k = 5000
+    t0 = time.perf_counter()
+    h = api.merge_candidates(mats, rows[:k], scores[:k], 2.0, 225, threads=16)
+    out["cpu_sample"] = {"candidates": k, "ms": (time.perf_counter() - t0) * 1e3, "threads": 16}
+    t0 = time.perf_counter()
+    gk = api.merge_candidates(mats, rows[:k], scores[:k], 2.0, 225, ctx=ctx)
+    out["gpu_sample_ms"] = (time.perf_counter() - t0) * 1e3
+    out["sample_equal"] = all(np.array_equal(a, b) for a, b in zip(gk, h))
+    log(f"merge: {out['candidates']} candidates on the device in {out['gpu_ms']:.0f} ms "
+        f"({out['clusters']} clusters); {k}: device {out['gpu_sample_ms']:.1f} ms, host "
+        f"{out['cpu_sample']['ms']:.0f} ms (16 threads), equal={out['sample_equal']}")
+    return out
+
+
 def precision_check(db, fp64_ms):
     """Rebuild the table with the FP32 projection + FP64 guard band and check
     it is the FP64 table (64-bit position-tagged checksum over counts and
@@ -387,6 +429,8 @@ def main():
                          "table per batch), e.g. 8,16,32,64,128; empty = none")
     ap.add_argument("--no-autotune", action="store_true",
                     help="score batches with the default layouts instead of the autotuned ones")
+    ap.add_argument("--no-merge", action="store_true",
+                    help="skip the duplicate-merge timing (device vs host)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32+guard rebuild of the table")
     ap.add_argument("--config4", action="store_true",
@@ -553,6 +597,7 @@ def main():
         serving["hbm_in_use_gb"] = ctx.bytes_in_use() / 1e9
         for k in keys[1:]:
             dbs[k].close()
+    merge = None if args.no_merge else merge_check(ctx, O)
     precision = None if args.no_precision else precision_check(db, st.ms)
     c4 = None
     if args.config4:
@@ -613,6 +658,8 @@ def main():
         line["serving"] = serving
     if precision:
         line["precision"] = precision
+    if merge:
+        line["merge"] = merge
     if c4:
         line["config4"] = c4
     print(json.dumps(line), flush=True)

@@ -295,7 +295,7 @@ void lm_server_reset_stats(lm_server *s);
 /* Finishes every admitted request, then releases the server. */
 void lm_server_destroy(lm_server *s);
 
-/* ---- merge of duplicate candidates (CPU) -------------------------------------------- */
+/* ---- merge of duplicate candidates ------------------------------------------------- */
 
 /* Cluster the nr candidates (orientation rows + coarse scores, in hit-list
  * order) by misorientation <= max_angle degrees under the point group of
@@ -308,6 +308,16 @@ int lm_merge_candidates(const double *orients, uint64_t n_orients, const uint64_
                         const double *scores, int32_t nr, double max_angle, int32_t space_group,
                         int32_t threads, uint64_t *best_row, double *best_score,
                         int32_t *cluster_size, int32_t *n_clusters);
+
+/* The same merge with the pairwise misorientation tests on the device of
+ * ctx (replaces the O(nr^2) loop of src/LaueMatchingHeaders.h:969-1065); the
+ * result is identical to lm_merge_candidates.  The greedy pass stays on the
+ * host over the device's pair bits (nr x ceil(nr/64) words, 313 MB at the
+ * 50000 cap). */
+int lm_merge_candidates_dev(lm_ctx *ctx, const double *orients, uint64_t n_orients,
+                            const uint64_t *rows, const double *scores, int32_t nr,
+                            double max_angle, int32_t space_group, uint64_t *best_row,
+                            double *best_score, int32_t *cluster_size, int32_t *n_clusters);
 
 /* Unit quaternion (w >= 0) of a row-major rotation matrix
  * (OrientMat2Quat, src/LaueMatchingHeaders.h:262-296). */

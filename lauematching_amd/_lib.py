@@ -118,6 +118,11 @@ SIGNATURES = {
                                           c_double_p, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.POINTER(LmHits)]),
     "lm_hits_free": (None, [ctypes.POINTER(LmHits)]),
+    "lm_merge_candidates_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                               ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.POINTER(ctypes.c_int32)]),
     "lm_merge_candidates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,

@@ -309,6 +309,34 @@ class OrientationDB:
         return [Hits._take(hs[i]) for i in range(nimg)]
 
 
+def merge_candidates(orients, rows, scores, max_angle: float, space_group: int,
+                     ctx: "Context | None" = None, threads: int = 8):
+    """Merge of duplicate candidates (mergeDuplicateOrientations,
+    src/LaueMatchingHeaders.h:969-1065): greedy clustering in list order by
+    misorientation <= max_angle.  With a context the pair tests run on its
+    device (same result).  Returns (best rows, best scores, cluster sizes)."""
+    o = np.ascontiguousarray(orients, dtype=np.float64).reshape(-1, 9)
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    s = np.ascontiguousarray(scores, dtype=np.float64)
+    nr = int(r.size)
+    if s.size != nr:
+        raise ValueError("one score per candidate row")
+    cap = max(1, min(nr, 50000))
+    best = np.zeros(cap, np.uint64)
+    bs = np.zeros(cap)
+    cs = np.zeros(cap, np.int32)
+    k = ctypes.c_int32()
+    if ctx is None:
+        check(lib().lm_merge_candidates(_ptr(o), o.shape[0], _ptr(r), _ptr(s), nr,
+                                        float(max_angle), int(space_group), int(threads),
+                                        _ptr(best), _ptr(bs), _ptr(cs), ctypes.byref(k)))
+    else:
+        check(lib().lm_merge_candidates_dev(ctx.handle, _ptr(o), o.shape[0], _ptr(r), _ptr(s),
+                                            nr, float(max_angle), int(space_group), _ptr(best),
+                                            _ptr(bs), _ptr(cs), ctypes.byref(k)))
+    return best[:k.value], bs[:k.value], cs[:k.value]
+
+
 def quantize_u8(img: np.ndarray):
     """uint8 quantization + imScale (reference src/LaueMatchingCPU.c:355-377)."""
     im = np.ascontiguousarray(img, dtype=np.float64).ravel()

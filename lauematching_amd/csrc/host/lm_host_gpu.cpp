@@ -190,6 +190,7 @@ int main(int argc, char **argv) {
   in.nhkls = nhkls;
   in.image = image.data();
   in.num_procs = numProcs;
+  in.ctx = ctx;
   in.image_nr = 0;
   const std::string base = argv[4];
   int rc = write_solutions(in, (base + ".solutions.txt").c_str(), (base + ".spots.txt").c_str());

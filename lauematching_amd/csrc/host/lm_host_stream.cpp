@@ -324,6 +324,7 @@ int main(int argc, char **argv) {
       in.nhkls = nhkls;
       in.image_f = job.image.data();
       in.num_procs = numProcs;
+      in.ctx = ctx;
       in.image_nr = job.image_num;
       const int ns = append_solutions(in, sol, spots);
       printf("[Image %u] %d unique orientations found\n", job.image_num, ns);

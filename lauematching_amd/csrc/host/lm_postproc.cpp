@@ -197,10 +197,17 @@ int emit(const PostInput &in, FILE *sol, FILE *spots) {
   std::vector<double> bestScore(cap + 1);
   std::vector<int32_t> csize(cap + 1);
   int32_t nsol = 0;
+  // the pair tests are O(nr^2): from a few thousand candidates on, the device
+  // does them (same clusters)
+  const bool on_device = in.ctx && nr >= 2048;
   if (nr > 0 &&
-      lm_merge_candidates(in.orients, in.n_orients, rows.data(), scores.data(), nr, p.max_angle,
-                          p.space_group, in.num_procs, best.data(), bestScore.data(),
-                          csize.data(), &nsol) != LM_OK) {
+      (on_device
+           ? lm_merge_candidates_dev(in.ctx, in.orients, in.n_orients, rows.data(), scores.data(),
+                                     nr, p.max_angle, p.space_group, best.data(),
+                                     bestScore.data(), csize.data(), &nsol)
+           : lm_merge_candidates(in.orients, in.n_orients, rows.data(), scores.data(), nr,
+                                 p.max_angle, p.space_group, in.num_procs, best.data(),
+                                 bestScore.data(), csize.data(), &nsol)) != LM_OK) {
     fprintf(stderr, "merge failed: %s\n", lm_last_error());
     return -1;
   }

@@ -30,6 +30,7 @@ struct PostInput {
   const double *image = nullptr;    // float64 image (converted to float for fitting)
   const float *image_f = nullptr;   // or directly a float image
   int num_procs = 1;
+  lm_ctx *ctx = nullptr;            // set: large candidate lists merge on its device
   int image_nr = 0;                 // > 0: streaming output with an ImageNr column
 };
 

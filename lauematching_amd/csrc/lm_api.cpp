@@ -612,6 +612,27 @@ int lm_merge_candidates(const double *orients, uint64_t n_orients, const uint64_
   return LM_OK;
 }
 
+int lm_merge_candidates_dev(lm_ctx *ctx, const double *orients, uint64_t n_orients,
+                            const uint64_t *rows, const double *scores, int32_t nr,
+                            double max_angle, int32_t space_group, uint64_t *best_row,
+                            double *best_score, int32_t *cluster_size, int32_t *n_clusters) {
+  if (!ctx || nr < 0 ||
+      (nr && (!orients || !rows || !scores || !best_row || !best_score || !cluster_size)) ||
+      !n_clusters) {
+    set_error("lm_merge_candidates_dev: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  std::vector<int> sizes((size_t)std::min(nr, 50000));
+  const int k = merge_candidates_dev(ctx->stream, orients, n_orients, rows, scores, nr,
+                                     max_angle, space_group, best_row, best_score, sizes.data());
+  if (k < 0) return k;  // an LM_ERR_* code
+  for (int i = 0; i < k; ++i) cluster_size[i] = sizes[i];
+  *n_clusters = k;
+  return LM_OK;
+}
+
 void lm_quat_from_matrix(const double m[9], double q[4]) { quat_from_matrix(m, q); }
 
 double lm_misorientation_deg(const double q1[4], const double q2[4], int32_t space_group) {

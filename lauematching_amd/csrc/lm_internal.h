@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "lauematch.h"
+#include "lm_quat.h"
 
 namespace lm {
 
@@ -228,5 +229,16 @@ double misorientation_deg(const double a[4], const double b[4], int space_group)
 int merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows,
                      const double *scores, int nr, double max_angle, int space_group,
                      int threads, uint64_t *best_row, double *best_score, int *cluster_size);
+// Same clustering with the pair tests on the device (stream st).
+int merge_candidates_dev(hipStream_t st, const double *orients, uint64_t n_orients,
+                         const uint64_t *rows, const double *scores, int nr, double max_angle,
+                         int space_group, uint64_t *best_row, double *best_score,
+                         int *cluster_size);
+
+// lm_merge_dev.hip: bit l of row i (l > i, rows of W = ceil(n / 64) words)
+// set when the clamped misorientation cosine of candidates l and i (their
+// fundamental-region quaternions fq, n x 4) is at least cos_min.
+int merge_bits_dev(hipStream_t st, const double *fq, int n, const SymSet &sym, double cos_min,
+                   uint64_t *bits_host);
 
 }  // namespace lm

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "lm_internal.h"
+#include "lm_quat.h"
 
 namespace lm {
 namespace {
@@ -55,11 +56,6 @@ const double kCub[24][4] = {
     {0, 0.70711, 0.70711, 0},  {0, -0.70711, 0.70711, 0},  {0, 0.70711, 0, 0.70711},
     {0, 0.70711, 0, -0.70711}, {0, 0, 0.70711, 0.70711},   {0, 0, 0.70711, -0.70711}};
 
-struct SymSet {
-  int n = 1;
-  double q[24][4];
-};
-
 SymSet symmetry_for(int sg) {
   SymSet s;
   const double(*src)[4];
@@ -75,49 +71,11 @@ SymSet symmetry_for(int sg) {
   return s;
 }
 
-void normalize4(double q[4]) {
-  const double nrm = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-  for (int i = 0; i < 4; ++i) q[i] /= nrm;
-}
-
-// Q = q (x) r, scalar part made non-negative, normalised.
-void qmul(const double q[4], const double r[4], double Q[4]) {
-  Q[0] = r[0] * q[0] - r[1] * q[1] - r[2] * q[2] - r[3] * q[3];
-  Q[1] = r[1] * q[0] + r[0] * q[1] + r[3] * q[2] - r[2] * q[3];
-  Q[2] = r[2] * q[0] + r[0] * q[2] + r[1] * q[3] - r[3] * q[1];
-  Q[3] = r[3] * q[0] + r[0] * q[3] + r[2] * q[1] - r[1] * q[2];
-  if (Q[0] < 0)
-    for (int i = 0; i < 4; ++i) Q[i] = -Q[i];
-  normalize4(Q);
-}
-
-// Symmetry-equivalent with the largest scalar part (first one on ties).
-void fundamental(const SymSet &s, const double in[4], double out[4]) {
-  double best[4] = {1, 0, 0, 0}, qt[4];
-  double mc = -10000;
-  for (int i = 0; i < s.n; ++i) {
-    qmul(in, s.q[i], qt);
-    if (mc < qt[0]) {
-      mc = qt[0];
-      std::memcpy(best, qt, sizeof best);
-    }
-  }
-  std::memcpy(out, best, sizeof best);
-  normalize4(out);
-}
-
 double misorientation(const SymSet &s, const double a[4], const double b[4]) {
-  double fa[4], fb[4], inv[4], p[4], m[4];
-  fundamental(s, a, fa);
-  fundamental(s, b, fb);
-  inv[0] = -fa[0];
-  inv[1] = fa[1];
-  inv[2] = fa[2];
-  inv[3] = fa[3];
-  qmul(inv, fb, p);
-  fundamental(s, p, m);
-  if (m[0] > 1) m[0] = 1;
-  return 2 * (std::acos(m[0])) * kRad2Deg;
+  double fa[4], fb[4];
+  quat_fundamental(s, a, fa);
+  quat_fundamental(s, b, fb);
+  return 2 * (std::acos(quat_miso_cos(s, fa, fb))) * kRad2Deg;
 }
 
 template <typename F>
@@ -166,62 +124,164 @@ void quat_from_matrix(const double m[9], double q[4]) {
   }
   if (q[0] < 0)
     for (int i = 0; i < 4; ++i) q[i] = -q[i];
-  normalize4(q);
+  quat_normalize(q);
 }
 
 double misorientation_deg(const double a[4], const double b[4], int space_group) {
   return misorientation(symmetry_for(space_group), a, b);
 }
 
-int merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows_in,
-                     const double *scores_in, int nr, double max_angle, int space_group,
-                     int threads, uint64_t *best_row, double *best_score, int *cluster_size) {
-  std::vector<uint64_t> rows(rows_in, rows_in + nr);
-  std::vector<double> scores(scores_in, scores_in + nr);
+namespace {
+
+// The candidates the merge sees: at most the 50000 best-scored (ties in list
+// order, as glibc's merge-sort qsort keeps them), their unit quaternions and
+// the fundamental-region reduction of each.
+struct Prepared {
+  std::vector<uint64_t> rows;
+  std::vector<double> scores;
+  std::vector<double> fq;  // nr x 4
+  int nr = 0;
+};
+
+int prepare(const double *orients, uint64_t n_orients, const uint64_t *rows_in,
+            const double *scores_in, int nr, const SymSet &sym, Prepared &P) {
+  P.rows.assign(rows_in, rows_in + nr);
+  P.scores.assign(scores_in, scores_in + nr);
   for (int i = 0; i < nr; ++i)
-    if (rows[i] >= n_orients) {
+    if (P.rows[i] >= n_orients) {
       set_error("merge: candidate row outside the orientation DB");
       return -1;
     }
   if (nr > kMergeCap) {
-    // best-scored first, ties in list order (glibc qsort is a merge sort)
     std::vector<int> ord(nr);
     for (int i = 0; i < nr; ++i) ord[i] = i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return scores[a] > scores[b]; });
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int b) { return P.scores[a] > P.scores[b]; });
     std::vector<uint64_t> r2(kMergeCap);
     std::vector<double> s2(kMergeCap);
     for (int i = 0; i < kMergeCap; ++i) {
-      r2[i] = rows[ord[i]];
-      s2[i] = scores[ord[i]];
+      r2[i] = P.rows[ord[i]];
+      s2[i] = P.scores[ord[i]];
     }
-    rows.swap(r2);
-    scores.swap(s2);
+    P.rows.swap(r2);
+    P.scores.swap(s2);
     nr = kMergeCap;
   }
+  P.nr = nr;
+  P.fq.resize((size_t)nr * 4);
+  for (int i = 0; i < nr; ++i) {
+    double q[4];
+    quat_from_matrix(orients + P.rows[i] * 9, q);
+    quat_fundamental(sym, q, &P.fq[(size_t)i * 4]);
+  }
+  return 0;
+}
+
+// The clustering test of the reference, (float)(2 acos(c) 180/pi) <= max_angle
+// on the clamped cosine c in [0, 1], is monotone in c; returns the smallest
+// double c that passes (so "c >= t" is the same test for every c), -1 when
+// every c passes, 2 when none does, or NaN if the test is not monotone near
+// the boundary (then the caller keeps the float matrix).
+double cos_threshold(double max_angle) {
+  auto pass = [&](double c) { return (float)(2 * (std::acos(c)) * kRad2Deg) <= max_angle; };
+  if (!pass(1.0)) return 2.0;
+  if (pass(0.0)) return -1.0;
+  uint64_t lo, hi;  // bit patterns: pass(lo) false, pass(hi) true
+  const double zero = 0.0, one = 1.0;
+  std::memcpy(&lo, &zero, 8);
+  std::memcpy(&hi, &one, 8);
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    double c;
+    std::memcpy(&c, &mid, 8);
+    (pass(c) ? hi : lo) = mid;
+  }
+  // every double within 2^16 representable steps on either side must agree
+  for (uint64_t d = 0; d < (1u << 16); ++d) {
+    double c;
+    if (hi + d <= ((uint64_t)0x3ff0000000000000ull)) {
+      const uint64_t b = hi + d;
+      std::memcpy(&c, &b, 8);
+      if (!pass(c)) return NAN;
+    }
+    if (lo >= d) {
+      const uint64_t b = lo - d;
+      std::memcpy(&c, &b, 8);
+      if (pass(c)) return NAN;
+    }
+  }
+  double t;
+  std::memcpy(&t, &hi, 8);
+  return t;
+}
+
+// Greedy clustering in list order over a pass test given as bit rows:
+// bit l of row i (l > i) set when candidate l is within max_angle of i.
+int greedy_bits(const Prepared &P, const std::vector<uint64_t> &bits, int W, uint64_t *best_row,
+                double *best_score, int *cluster_size) {
+  std::vector<uint64_t> done((size_t)W, 0);
+  int k = 0;
+  for (int i = 0; i < P.nr; ++i) {
+    if ((done[i >> 6] >> (i & 63)) & 1) continue;
+    done[i >> 6] |= 1ull << (i & 63);
+    int size = 1;
+    uint64_t br = P.rows[i];
+    double bs = P.scores[i];
+    const uint64_t *row = bits.data() + (size_t)i * W;
+    for (int wd = (i + 1) >> 6; wd < W; ++wd) {
+      uint64_t m = row[wd] & ~done[wd];
+      done[wd] |= m;
+      while (m) {
+        const int l = wd * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        ++size;
+        if (P.scores[l] > bs) {
+          bs = P.scores[l];
+          br = P.rows[l];
+        }
+      }
+    }
+    best_row[k] = br;
+    best_score[k] = bs;
+    cluster_size[k] = size;
+    ++k;
+  }
+  return k;
+}
+
+}  // namespace
+
+int merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *rows_in,
+                     const double *scores_in, int nr, double max_angle, int space_group,
+                     int threads, uint64_t *best_row, double *best_score, int *cluster_size) {
   const SymSet sym = symmetry_for(space_group);
-  std::vector<double> q((size_t)nr * 4);
-  for (int i = 0; i < nr; ++i) quat_from_matrix(orients + rows[i] * 9, &q[(size_t)i * 4]);
+  Prepared P;
+  if (prepare(orients, n_orients, rows_in, scores_in, nr, sym, P)) return -1;
+  nr = P.nr;
+  // pairwise misorientation (float, as the reference stores it), each pair
+  // from the two candidates' fundamental-region reductions
   std::vector<float> miso((size_t)nr * (nr > 0 ? nr - 1 : 0) / 2);
   parallel_for(nr, threads, [&](int i) {
     for (int j = 0; j < i; ++j)
-      miso[(size_t)i * (i - 1) / 2 + j] =
-          (float)misorientation(sym, &q[(size_t)i * 4], &q[(size_t)j * 4]);
+      miso[(size_t)i * (i - 1) / 2 + j] = (float)(
+          2 * (std::acos(quat_miso_cos(sym, &P.fq[(size_t)i * 4], &P.fq[(size_t)j * 4]))) *
+          kRad2Deg);
   });
   std::vector<int> done(nr, 0);
   int k = 0;
   for (int i = 0; i < nr; ++i) {
     if (done[i]) continue;
     done[i] = 1;
-    uint64_t br = rows[i];
-    double bs = scores[i];
+    uint64_t br = P.rows[i];
+    double bs = P.scores[i];
     for (int l = i + 1; l < nr; ++l) {
       if (done[l] > 0) continue;
       if (miso[(size_t)l * (l - 1) / 2 + i] <= max_angle) {
         done[l] = 1;
         done[i]++;
-        if (scores[l] > bs) {
-          bs = scores[l];
-          br = rows[l];
+        if (P.scores[l] > bs) {
+          bs = P.scores[l];
+          br = P.rows[l];
         }
       }
     }
@@ -231,6 +291,26 @@ int merge_candidates(const double *orients, uint64_t n_orients, const uint64_t *
     ++k;
   }
   return k;
+}
+
+int merge_candidates_dev(hipStream_t st, const double *orients, uint64_t n_orients,
+                         const uint64_t *rows_in, const double *scores_in, int nr,
+                         double max_angle, int space_group, uint64_t *best_row,
+                         double *best_score, int *cluster_size) {
+  const SymSet sym = symmetry_for(space_group);
+  const double t = cos_threshold(max_angle);
+  if (std::isnan(t))  // keep the reference's float test on the host
+    return merge_candidates(orients, n_orients, rows_in, scores_in, nr, max_angle, space_group,
+                            (int)std::thread::hardware_concurrency(), best_row, best_score,
+                            cluster_size);
+  Prepared P;
+  if (prepare(orients, n_orients, rows_in, scores_in, nr, sym, P)) return LM_ERR_INVALID;
+  if (P.nr == 0) return 0;
+  const int W = (P.nr + 63) / 64;
+  std::vector<uint64_t> bits((size_t)P.nr * W);
+  const int rc = merge_bits_dev(st, P.fq.data(), P.nr, sym, t, bits.data());
+  if (rc != LM_OK) return rc;
+  return greedy_bits(P, bits, W, best_row, best_score, cluster_size);
 }
 
 }  // namespace lm
