@@ -327,6 +327,51 @@ void lm_quat_from_matrix(const double m[9], double q[4]);
  * point group of space_group (GetMisOrientation, src/LaueMatchingHeaders.h:247-260). */
 double lm_misorientation_deg(const double q1[4], const double q2[4], int32_t space_group);
 
+/* ---- image preprocessing ----------------------------------------------------------- */
+
+/* The preprocessing before the compare (preprocess_image,
+ * laue_index/preprocess.py:221-337, thresholds laue_index/thresholds.py:29-110)
+ * on the device: background subtraction max(raw - background, 0); a
+ * threshold (pixels <= t become 0) that is fixed, a percentile of all pixels
+ * (numpy's linear interpolation), or adaptive (median of the nonzero pixels
+ * + 4 * 1.4826 * their MAD, at least 1; also used for "otsu", as the
+ * reference does without scikit-image); the cast to uint16; 8-connected
+ * components of the nonzero pixels, those with fewer than min_area pixels
+ * removed; the Gaussian sigma max(0.25 * min(nearest centre-of-mass
+ * distance, distance * tan(orientation_spacing) / px), 1) (3 with fewer than
+ * two components); a float64 Gaussian blur (radius int(4 sigma + 0.5),
+ * reflecting edges) and the uint8 quantization of lm_quantize_u8.  The
+ * background median filter of the reference is not computed here: pass the
+ * background (or NULL for none).  No counterpart runs on the GPU in the
+ * reference. */
+#define LM_THRESH_ADAPTIVE 0
+#define LM_THRESH_PERCENTILE 1
+#define LM_THRESH_FIXED 2
+#define LM_THRESH_OTSU 3
+typedef struct lm_prep_config {
+  int32_t threshold_method;    /* LM_THRESH_*                                       */
+  double threshold_value;      /* LM_THRESH_FIXED                                   */
+  double threshold_percentile; /* LM_THRESH_PERCENTILE (0..100)                     */
+  int32_t min_area;            /* components with fewer pixels are removed          */
+  double px_x;                 /* pixel size (mm), detector distance (mm) and the   */
+  double distance;             /* DB's orientation spacing (degrees) bound sigma    */
+  double orientation_spacing;
+} lm_prep_config;
+typedef struct lm_prep_result {
+  double threshold;      /* threshold applied                                  */
+  double sigma;          /* Gaussian sigma (pixels; 0 when nothing was kept)   */
+  int32_t n_components;  /* 8-connected components before the area filter      */
+  int32_t n_kept;        /* components kept                                    */
+  double im_scale;       /* imScale of the uint8 image                         */
+  double ms;             /* device time of the whole pipeline                  */
+} lm_prep_result;
+/* raw and background: nrPxY x nrPxX float64 host images (background may be
+ * NULL); the uint8 image goes to u8_host and/or d_u8 (a device buffer of the
+ * context's device), blurred (optional, host) receives the float64 image. */
+int lm_preprocess(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *raw,
+                  const double *background, const lm_prep_config *cfg, uint8_t *u8_host,
+                  void *d_u8, double *blurred, lm_prep_result *res);
+
 /* ---- diagnostics ------------------------------------------------------------------ */
 
 /* Evaluate on the device, with the forward simulation's arithmetic, q[i] =

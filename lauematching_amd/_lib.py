@@ -45,6 +45,19 @@ class LaueMatchError(RuntimeError):
         self.code = code
 
 
+class LmPrepConfig(ctypes.Structure):
+    _fields_ = [("threshold_method", ctypes.c_int32), ("threshold_value", ctypes.c_double),
+                ("threshold_percentile", ctypes.c_double), ("min_area", ctypes.c_int32),
+                ("px_x", ctypes.c_double), ("distance", ctypes.c_double),
+                ("orientation_spacing", ctypes.c_double)]
+
+
+class LmPrepResult(ctypes.Structure):
+    _fields_ = [("threshold", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("n_components", ctypes.c_int32), ("n_kept", ctypes.c_int32),
+                ("im_scale", ctypes.c_double), ("ms", ctypes.c_double)]
+
+
 class LmServerConfig(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_int32), ("workers", ctypes.c_int32),
                 ("max_inflight", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
@@ -118,6 +131,11 @@ SIGNATURES = {
                                           c_double_p, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.POINTER(LmHits)]),
     "lm_hits_free": (None, [ctypes.POINTER(LmHits)]),
+    "lm_preprocess": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.POINTER(LmPrepConfig), ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.POINTER(LmPrepResult)]),
     "lm_merge_candidates_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,

@@ -337,6 +337,44 @@ def merge_candidates(orients, rows, scores, max_angle: float, space_group: int,
     return best[:k.value], bs[:k.value], cs[:k.value]
 
 
+PREP_METHODS = {"adaptive": 0, "percentile": 1, "fixed": 2, "otsu": 3}
+
+
+def preprocess(ctx: "Context", raw: np.ndarray, background: np.ndarray | None = None,
+               method: str = "adaptive", fixed_value: float = 0.0, percentile: float = 90.0,
+               min_area: int = 10, pixel_size: float = 0.2, distance: float = 0.513,
+               orient_spacing: float = 0.4, d_u8: int = 0, want_blurred: bool = False):
+    """Image preprocessing on the device (preprocess_image,
+    laue_index/preprocess.py:221-337, enhancement off): background
+    subtraction, threshold, uint16 cast, 8-connected components with the
+    small ones removed, Gaussian blur, uint8 quantization.  Returns (u8 image,
+    imScale, info dict[, blurred float64 image]); with d_u8 (a device pointer
+    of npx bytes) the uint8 image is also left there for scoring."""
+    img = np.ascontiguousarray(raw, dtype=np.float64)
+    if img.ndim != 2:
+        raise ValueError("raw must be a 2-D (NrPxY, NrPxX) image")
+    H, W = img.shape
+    bg = None
+    if background is not None:
+        bg = np.ascontiguousarray(background, dtype=np.float64)
+        if bg.shape != img.shape:
+            raise ValueError("background must have the image's shape")
+    cfg = _lib.LmPrepConfig(PREP_METHODS[method], float(fixed_value), float(percentile),
+                            int(min_area), float(pixel_size), float(distance),
+                            float(orient_spacing))
+    u8 = np.zeros((H, W), np.uint8)
+    blurred = np.zeros((H, W)) if want_blurred else None
+    res = _lib.LmPrepResult()
+    check(lib().lm_preprocess(ctx.handle, W, H, _ptr(img), None if bg is None else _ptr(bg),
+                              ctypes.byref(cfg), _ptr(u8), ctypes.c_void_p(d_u8 or None),
+                              None if blurred is None else _ptr(blurred), ctypes.byref(res)))
+    info = {"threshold": res.threshold, "sigma": res.sigma, "n_components": res.n_components,
+            "n_kept": res.n_kept, "ms": res.ms}
+    if want_blurred:
+        return u8, res.im_scale, info, blurred
+    return u8, res.im_scale, info
+
+
 def quantize_u8(img: np.ndarray):
     """uint8 quantization + imScale (reference src/LaueMatchingCPU.c:355-377)."""
     im = np.ascontiguousarray(img, dtype=np.float64).ravel()

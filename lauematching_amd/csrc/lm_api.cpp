@@ -633,6 +633,27 @@ int lm_merge_candidates_dev(lm_ctx *ctx, const double *orients, uint64_t n_orien
   return LM_OK;
 }
 
+int lm_preprocess(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *raw,
+                  const double *background, const lm_prep_config *cfg, uint8_t *u8_host,
+                  void *d_u8, double *blurred, lm_prep_result *res) {
+  if (!ctx || !raw || !cfg || !res || nr_px_x < 1 || nr_px_y < 1 ||
+      (uint64_t)nr_px_x * (uint64_t)nr_px_y >= (1ull << 31)) {
+    set_error("lm_preprocess: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  if (cfg->threshold_method < LM_THRESH_ADAPTIVE || cfg->threshold_method > LM_THRESH_OTSU ||
+      (cfg->threshold_method == LM_THRESH_PERCENTILE &&
+       !(cfg->threshold_percentile >= 0 && cfg->threshold_percentile <= 100))) {
+    set_error("lm_preprocess: threshold method or percentile out of range");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return preprocess(ctx, nr_px_x, nr_px_y, raw, background, *cfg, u8_host, (uint8_t *)d_u8,
+                    blurred, *res);
+}
+
 void lm_quat_from_matrix(const double m[9], double q[4]) { quat_from_matrix(m, q); }
 
 double lm_misorientation_deg(const double q1[4], const double q2[4], int32_t space_group) {

@@ -223,6 +223,12 @@ int batch_width(int nimg);   // kernel width for nimg images: 1, 4, 8, 16, 32, 6
 int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int reps,
              lm_tune_report *rep);
 
+// lm_prep.hip: the preprocessing pipeline of lm_preprocess (u8 to the host
+// and/or the device buffer d_u8; blurred optional).
+int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *background,
+               const lm_prep_config &cfg, uint8_t *u8_host, uint8_t *d_u8, double *blurred,
+               lm_prep_result &res);
+
 // lm_merge.cpp
 void quat_from_matrix(const double m[9], double q[4]);
 double misorientation_deg(const double a[4], const double b[4], int space_group);

@@ -1,0 +1,109 @@
+"""Pins of the preprocessing oracle (oracle/lm_prep_oracle.py) against
+brute force and closed forms, so the device pipeline is checked against a
+restatement that is itself checked: 8-connectivity against a flood fill,
+centres of mass against explicit sums, the nearest-centre distance against
+all pairs, the blur's edge mode against an explicit padded correlation, the
+noise floor against a hand-computed case."""
+import os
+import sys
+
+import numpy as np
+import scipy.ndimage as ndi
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "oracle"))
+import lm_prep_oracle as po  # noqa: E402
+
+
+def _flood(mask):
+    """Brute-force 8-connected components: sets of pixels."""
+    H, W = mask.shape
+    seen = np.zeros_like(mask, bool)
+    comps = []
+    for y in range(H):
+        for x in range(W):
+            if mask[y, x] and not seen[y, x]:
+                stack, comp = [(y, x)], set()
+                seen[y, x] = True
+                while stack:
+                    cy, cx = stack.pop()
+                    comp.add((cy, cx))
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            ny, nx = cy + dy, cx + dx
+                            if 0 <= ny < H and 0 <= nx < W and mask[ny, nx] and not seen[ny, nx]:
+                                seen[ny, nx] = True
+                                stack.append((ny, nx))
+                comps.append(frozenset(comp))
+    return set(comps)
+
+
+def test_components_are_8_connected():
+    rng = np.random.default_rng(4)
+    for trial in range(20):
+        u16 = (rng.random((12, 15)) < 0.3).astype(np.uint16) * rng.integers(1, 9, (12, 15))
+        labels, n = po.components(u16.astype(np.uint16))
+        got = {frozenset(zip(*np.nonzero(labels == k))) for k in range(1, n + 1)}
+        assert got == _flood(u16 > 0)
+    # diagonal neighbours join, as with 8-connectivity
+    u = np.zeros((4, 4), np.uint16)
+    u[0, 0] = u[1, 1] = u[2, 2] = 5
+    assert po.components(u)[1] == 1
+
+
+def test_centres_and_area_filter():
+    u = np.zeros((10, 12), np.uint16)
+    u[2, 3], u[2, 4], u[3, 4] = 2, 6, 4          # area 3
+    u[7, 8:11] = [1, 1, 1]                      # area 3
+    u[0, 11] = 9                                # area 1, removed
+    labels, n = po.components(u)
+    filt, centres = po.filter_small(u, labels, n, 2)
+    assert filt[0, 11] == 0 and filt[2, 4] == 6
+    want = sorted([((3 * 2 + 4 * 6 + 4 * 4) / 12, (2 * 2 + 2 * 6 + 3 * 4) / 12), (9.0, 7.0)])
+    assert sorted(centres) == want
+
+
+def test_sigma_from_nearest_centres():
+    rng = np.random.default_rng(5)
+    pts = [tuple(p) for p in rng.uniform(0, 500, (40, 2))]
+    brute = min(np.hypot(a[0] - b[0], a[1] - b[1]) for i, a in enumerate(pts) for b in pts[i + 1:])
+    # delta large: sigma = brute / 4 (>= 1)
+    assert po.gaussian_sigma(pts, 0.2, 1e6, 0.4) == max(0.25 * brute, 1.0)
+    # delta small: sigma = 0.25 * delta, delta = distance tan(spacing) / px
+    d = 513.0 * np.tan(np.radians(0.01)) / 0.2
+    assert abs(po.gaussian_sigma(pts, 0.2, 513.0, 0.01) - max(0.25 * min(brute, d), 1.0)) < 1e-12
+    assert po.gaussian_sigma(pts[:1], 0.2, 513.0, 0.4) == 3.0
+
+
+def test_blur_reflects_at_the_edges():
+    rng = np.random.default_rng(6)
+    img = rng.random((9, 13))
+    sigma = 1.3
+    r = int(4.0 * sigma + 0.5)
+    x = np.arange(-r, r + 1)
+    w = np.exp(-0.5 / (sigma * sigma) * x ** 2)
+    w /= w.sum()
+    pad = np.pad(img, r, mode="symmetric")   # d c b a | a b c d | d c b a
+    ref = np.zeros_like(img)
+    for dy in range(2 * r + 1):
+        for dx in range(2 * r + 1):
+            ref += w[dy] * w[dx] * pad[dy:dy + img.shape[0], dx:dx + img.shape[1]]
+    np.testing.assert_allclose(ndi.gaussian_filter(img, sigma), ref, rtol=1e-12, atol=1e-14)
+
+
+def test_noise_floor_known_answer():
+    img = np.zeros((3, 4))
+    img.ravel()[:5] = [1.0, 2.0, 3.0, 10.0, 4.0]
+    # nonzero median 3, |v - 3| = 2 1 0 7 1 -> median 1; 3 + 4 * 1.4826 * 1
+    assert po.noise_floor(img) == 3.0 + 4.0 * (1.4826 * 1.0)
+    assert po.noise_floor(np.zeros((2, 2))) == 1.0
+    assert po.threshold_value(img, "otsu", 0, 0) == po.noise_floor(img)
+
+
+def test_empty_and_all_small():
+    raw = np.zeros((20, 30))
+    out, info = po.preprocess(raw, method="fixed", fixed_value=5.0)
+    assert not out.any() and info["n_components"] == 0
+    raw[3, 3] = 100.0
+    out, info = po.preprocess(raw, method="fixed", fixed_value=5.0, min_area=2)
+    assert not out.any() and info["n_components"] == 1 and info["n_kept"] == 0
