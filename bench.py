@@ -316,6 +316,39 @@ def merge_check(ctx, O):
     return out
 
 
+def prep_check(ctx, reps=5):
+    """Device preprocessing of one raw 2048x2048 frame (background
+    subtraction, adaptive threshold, components, blur, uint8 quantization):
+    device time of the pipeline and wall time of the call (host copies in
+    and out included)."""
+    from lauematching_amd import api
+
+    rng = np.random.default_rng(21)
+    H = W = 2048
+    raw = rng.poisson(50.0, (H, W)).astype(np.float64)
+    yy, xx = np.mgrid[0:21, 0:21]
+    for _ in range(3000):
+        yc, xc = rng.uniform(10, H - 11), rng.uniform(10, W - 11)
+        a, sg = rng.uniform(300, 4000), rng.uniform(0.7, 2.5)
+        y0, x0 = int(yc) - 10, int(xc) - 10
+        raw[y0:y0 + 21, x0:x0 + 21] += a * np.exp(-((yy + y0 - yc) ** 2 + (xx + x0 - xc) ** 2)
+                                                  / (2 * sg * sg))
+    bg = 45.0 + rng.normal(0, 2.0, (H, W))
+    api.preprocess(ctx, raw, bg)
+    dev, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        u8, scale, info = api.preprocess(ctx, raw, bg)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(info["ms"])
+    out = {"frame": "2048x2048 float64, 3000 spots, adaptive threshold, min_area 10",
+           "device_ms": float(np.median(dev)), "wall_ms": float(np.median(wall)),
+           "components": info["n_components"], "kept": info["n_kept"], "sigma": info["sigma"]}
+    log(f"preprocess: {out['device_ms']:.2f} ms device, {out['wall_ms']:.1f} ms per call "
+        f"({out['kept']}/{out['components']} components kept, sigma {out['sigma']:.2f})")
+    return out
+
+
 def precision_check(db, fp64_ms):
     """Rebuild the table with the FP32 projection + FP64 guard band and check
     it is the FP64 table (64-bit position-tagged checksum over counts and
@@ -430,7 +463,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="score batches with the default layouts instead of the autotuned ones")
     ap.add_argument("--no-merge", action="store_true",
-                    help="skip the duplicate-merge timing (device vs host)")
+                    help="skip the duplicate-merge and preprocessing timings")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32+guard rebuild of the table")
     ap.add_argument("--config4", action="store_true",
@@ -598,6 +631,7 @@ def main():
         for k in keys[1:]:
             dbs[k].close()
     merge = None if args.no_merge else merge_check(ctx, O)
+    prep = None if args.no_merge else prep_check(ctx)
     precision = None if args.no_precision else precision_check(db, st.ms)
     c4 = None
     if args.config4:
@@ -660,6 +694,8 @@ def main():
         line["precision"] = precision
     if merge:
         line["merge"] = merge
+    if prep:
+        line["preprocess"] = prep
     if c4:
         line["config4"] = c4
     print(json.dumps(line), flush=True)
