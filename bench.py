@@ -462,6 +462,8 @@ def main():
                          "table per batch), e.g. 8,16,32,64,128; empty = none")
     ap.add_argument("--no-autotune", action="store_true",
                     help="score batches with the default layouts instead of the autotuned ones")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the single-image launch sequence as a graph")
     ap.add_argument("--no-merge", action="store_true",
                     help="skip the duplicate-merge and preprocessing timings")
     ap.add_argument("--no-precision", action="store_true",
@@ -537,6 +539,8 @@ def main():
     single_kernel = kernel_name(db)
     d_img = torch.from_numpy(u8).to("cuda:0")
     torch.cuda.synchronize()
+    # optionally the launch sequence (counter fill, walk, recount) as one graph
+    db.set_use_graphs(args.graphs)
 
     def step():
         return db.score_dev(d_img.data_ptr(), im_scale, p.MinNrSpots, p.MinIntensity)
@@ -666,7 +670,7 @@ def main():
                    "grains_recovered": bool(found), "prefix_parity_vs_oracle": parity_ok,
                    "forward_ms": st.ms, "kernel_ms": kernel_ms,
                    "hbm_copy_GBps": copy_gbps,
-                   "single_kernel": single_kernel},
+                   "single_kernel": single_kernel, "graph_replay": bool(args.graphs)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
         "cpu_baseline": cpu,

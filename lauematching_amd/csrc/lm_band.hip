@@ -352,7 +352,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   constexpr int NCNT = S16 ? T / 2 : T;
   static_assert(NCNT * 4 + NW * C <= 163840, "LDS");
   constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
-  __shared__ uint32_t cnt[NCNT];
+  __shared__ __attribute__((aligned(16))) uint32_t cnt[NCNT];
   __shared__ v4u slot[NW][C / 16];
   const int tid = threadIdx.x, lane = __lane_id();
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -360,7 +360,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   const __amdgpu_buffer_rsrc_t irs =
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
-    for (int i = tid; i < NCNT; i += THREADS) cnt[i] = 0;
+    for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
     __syncthreads();
     const uint64_t first = t << TB;
     const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
@@ -383,19 +383,32 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     v4u pim[D][V];
     uint32_t pe[D][EMAX];
     uint32_t beg[D], end[D];
-    // The wave's chunks go in rounds of 64: lane l holds the entry range of
+    // The wave's chunks w, w + NW, ... (j-th: chunk w + NW j) go in rounds of
+    // RC = the largest multiple of D up to 64: lane l holds the entry range of
     // the round's chunk l (one load per round), so a set's entry range comes
     // from a lane read, not from a load the set's entry loads would wait on.
+    // Round r + 1's ranges are loaded when round r starts; inside round r the
+    // fetches read round r's ranges only, and the last D consumes of the round
+    // (unrolled) fetch round r + 1's first chunks from the ranges loaded a
+    // round earlier, so the pipeline runs through round boundaries without a
+    // drain and every wait stays an exact count.
+    constexpr int RC = (64 / D) * D;
     const int per_wave = (nchunks + NW - 1) / NW;
-    for (int j0 = 0; j0 < per_wave; j0 += 64) {
-      const int cl = w + NW * (j0 + lane);
-      const uint32_t vbeg = coff(cl), vend = coff(cl + 1);
-      auto fetch = [&](int d, int j) {  // round chunk j (>= 64: empty)
-        const bool in = j < 64;
-        const int jj = in ? j : 63;
-        beg[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)vbeg, jj) : total;
-        end[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)vend, jj) : total;
-        const uint32_t ib = (uint32_t)(w + NW * (j0 + j)) << CB;
+    const int nround = (per_wave + RC - 1) / RC;
+    auto load_round = [&](int r, uint32_t &vb, uint32_t &ve) {
+      const int cl = w + NW * (r * RC + (lane < RC ? lane : RC - 1));
+      vb = coff(cl);
+      ve = coff(cl + 1);
+    };
+    uint32_t cur_b, cur_e, nxt_b, nxt_e;
+    load_round(0, cur_b, cur_e);
+    {
+      // chunk j of the wave, entry range from lane jj of (vb, ve)
+      auto fetch = [&](int d, int j, int jj, uint32_t vb, uint32_t ve) {
+        const bool in = j < per_wave;
+        beg[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)vb, jj) : total;
+        end[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)ve, jj) : total;
+        const uint32_t ib = (uint32_t)(w + NW * j) << CB;
 #pragma unroll
         for (int v = 0; v < V; ++v)
           pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
@@ -465,14 +478,27 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
           }
         }
         __builtin_amdgcn_wave_barrier();
-        fetch(d, j + D);
       };
 #pragma unroll
-      for (int d = 0; d < D; ++d) fetch(d, d);
-      const int n_in = per_wave - j0 < 64 ? per_wave - j0 : 64;
-      for (int j = 0; j < n_in; j += D) {
+      for (int d = 0; d < D; ++d) fetch(d, d, d, cur_b, cur_e);
+      for (int r = 0; r < nround; ++r) {
+        load_round(r + 1, nxt_b, nxt_e);
+        const int j0 = r * RC;
+        for (int j = j0; j < j0 + RC - D && j < per_wave; j += D) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) consume(d, j + d);
+          for (int d = 0; d < D; ++d) {
+            consume(d, j + d);
+            fetch(d, j + d + D, j + d + D - j0, cur_b, cur_e);
+          }
+        }
+        if (j0 + RC - D >= per_wave) break;  // every later chunk is empty
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          consume(d, j0 + RC - D + d);
+          fetch(d, j0 + RC + d, d, nxt_b, nxt_e);
+        }
+        cur_b = nxt_b;
+        cur_e = nxt_e;
       }
     }
     __syncthreads();
@@ -487,8 +513,10 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         const uint64_t o = first + rr;
         const uint32_t sum = (cnt[rr >> 1] >> ((rr & 1) << 4)) & 0xffffu;
         const bool live = o < n;
-        const uint64_t zm = __ballot(live && sum == 0 && zok);
-        if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
+        if (zok) {  // rows with no spot pass only when both thresholds are <= 0
+          const uint64_t zm = __ballot(live && sum == 0);
+          if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
+        }
         const bool cand = live && sum >= nthr && sum >= sthr;
         const uint64_t m = __ballot(cand);
         if (m) {
@@ -901,7 +929,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
 #undef LM_LDS_ALL
 #undef LM_LDS
     if (s16) {
-      hipLaunchKernelGGL(recount_kernel, dim3(1024), dim3(256), 0, st, t.counts, t.tile_off,
+      hipLaunchKernelGGL(recount_kernel, dim3(256), dim3(256), 0, st, t.counts, t.tile_off,
                          t.entries, db->n, img, sc.pred, sc.cands, sc.cand_cap, sc.hits, cap,
                          sc.counters);
     }
