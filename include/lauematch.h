@@ -83,7 +83,10 @@ typedef struct lm_hits {
   int32_t *nspots;     /* spots with intensity > 0                           */
   double *totint;      /* float64 intensity sum, record order                */
   double *score;       /* totint * sqrt(nspots)                              */
-  double kernel_ms;    /* device time of the scoring kernels                 */
+  double kernel_ms;    /* device time of the scoring launch sequence         */
+  double walk_ms;      /* device time of the compare kernel alone (single
+                          image over the banded copy, eager launches; else
+                          equal to kernel_ms)                                */
 } lm_hits;
 
 /* ---- context -------------------------------------------------------------- */

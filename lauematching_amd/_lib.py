@@ -36,7 +36,8 @@ class LmFwdStats(ctypes.Structure):
 class LmHits(ctypes.Structure):
     _fields_ = [("nr_results", ctypes.c_uint64), ("count", ctypes.c_uint64),
                 ("ids", ctypes.POINTER(ctypes.c_uint64)), ("nspots", ctypes.POINTER(ctypes.c_int32)),
-                ("totint", c_double_p), ("score", c_double_p), ("kernel_ms", ctypes.c_double)]
+                ("totint", c_double_p), ("score", c_double_p), ("kernel_ms", ctypes.c_double),
+                ("walk_ms", ctypes.c_double)]
 
 
 class LaueMatchError(RuntimeError):

@@ -93,6 +93,7 @@ class Hits:
     totint: np.ndarray
     score: np.ndarray
     kernel_ms: float
+    walk_ms: float = 0.0
 
     @classmethod
     def _take(cls, h: LmHits) -> "Hits":
@@ -112,7 +113,7 @@ class Hits:
             ns = np.zeros(0, np.int32)
             tot = np.zeros(0)
             sc = np.zeros(0)
-        out = cls(int(h.nr_results), ids, ns, tot, sc, float(h.kernel_ms))
+        out = cls(int(h.nr_results), ids, ns, tot, sc, float(h.kernel_ms), float(h.walk_ms))
         lib().lm_hits_free(ctypes.byref(h))
         return out
 

@@ -888,6 +888,15 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     set_error("2^16-row walk needs a 16-byte aligned image and a candidate list");
     return LM_ERR_INVALID;
   }
+  // events around the compare kernel alone (not inside a captured graph)
+  const bool timed = !sc.use_graphs && sc.ew0 && sc.ew1;
+  if (timed) (void)hipEventRecord(sc.ew0, st);
+  auto walk_done = [&] {
+    if (timed) {
+      (void)hipEventRecord(sc.ew1, st);
+      sc.walk_timed = true;
+    }
+  };
   if (lds_walk && ((uintptr_t)img & 15) == 0) {
     if (!b.chunk_off || b.tb < 14 || b.tb > kSum16TileBits) {
       set_error("LDS-staged walk needs 2^14..2^16 super-tiles and chunk offsets");
@@ -928,6 +937,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     }
 #undef LM_LDS_ALL
 #undef LM_LDS
+    walk_done();
     if (s16) {
       hipLaunchKernelGGL(recount_kernel, dim3(256), dim3(256), 0, st, t.counts, t.tile_off,
                          t.entries, db->n, img, sc.pred, sc.cands, sc.cand_cap, sc.hits, cap,
@@ -965,6 +975,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   }
 #undef LM_SB
 #undef LM_SBW
+  walk_done();
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "banded score kernel launch");
   return LM_OK;

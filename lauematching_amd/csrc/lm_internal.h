@@ -108,6 +108,9 @@ struct Scratch {
   Pred *pred = nullptr;          // device copy of the predicate
   Pred *pred_host = nullptr;     // pinned host copy
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // around the compare kernel alone (eager single-image banded launches)
+  hipEvent_t ew0 = nullptr, ew1 = nullptr;
+  bool walk_timed = false;
   // Captured launch sequences (interleave + counter fill + score kernel),
   // keyed by every pointer and size their kernel arguments depend on.
   bool use_graphs = false;

@@ -781,6 +781,8 @@ int scratch_init(Scratch &s) {
   if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&s.ev0);
   if (e == hipSuccess) e = hipEventCreate(&s.ev1);
+  if (e == hipSuccess) e = hipEventCreate(&s.ew0);
+  if (e == hipSuccess) e = hipEventCreate(&s.ew1);
   if (e != hipSuccess) {
     scratch_release(s);
     return hip_fail(e, "scratch buffers");
@@ -800,6 +802,8 @@ void scratch_release(Scratch &s) {
   if (s.pred_host) (void)hipHostFree(s.pred_host);
   if (s.ev0) (void)hipEventDestroy(s.ev0);
   if (s.ev1) (void)hipEventDestroy(s.ev1);
+  if (s.ew0) (void)hipEventDestroy(s.ew0);
+  if (s.ew1) (void)hipEventDestroy(s.ew1);
   s = Scratch{};
 }
 
@@ -874,6 +878,7 @@ int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
   out->nr_results = (uint64_t)cnt[1] + k;
   out->count = k;
   out->kernel_ms = ms;
+  out->walk_ms = ms;
   rc = fill_hits(*out, k);
   if (rc) return rc;
   for (uint64_t j = 0; j < k; ++j) {
@@ -990,7 +995,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     const hipError_t le = hipGraphLaunch(exec, st);
     return le == hipSuccess ? LM_OK : hip_fail(le, "hipGraphLaunch");
   };
-  float ms = 0;
+  float ms = 0, walk_ms = 0;
   std::vector<uint32_t> cnt(kCounterWords);
   // at most three runs: a candidate overflow switches to the table walk
   // once, a hit overflow regrows the hit buffer once
@@ -999,6 +1004,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     rc = ensure_hits(sc, capPer * (uint64_t)B);  // a width-B launch serves any nimg <= B
     if (rc) return rc;
     (void)hipEventRecord(sc.ev0, st);
+    sc.walk_timed = false;
     rc = run(capPer);
     if (rc) return rc;
     (void)hipEventRecord(sc.ev1, st);
@@ -1006,6 +1012,8 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "score kernel");
     (void)hipEventElapsedTime(&ms, sc.ev0, sc.ev1);
+    walk_ms = ms;
+    if (sc.walk_timed) (void)hipEventElapsedTime(&walk_ms, sc.ew0, sc.ew1);
     if (kind == 1 && sum16 && cnt[kCandSlot] > sc.cand_cap) {
       kind = 0;  // more candidates than the list holds: rerun over the table
       continue;
@@ -1070,6 +1078,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   for (int b = 0; b < nimg && rc == LM_OK; ++b) {
     lm_hits &h = outs[b];
     h.kernel_ms = ms;
+    h.walk_ms = walk_ms;
     const uint64_t k = cnt[2 * b];
     const uint64_t *rec = all.data() + first[b];
     std::vector<double> tot(k);
