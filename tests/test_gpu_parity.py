@@ -556,6 +556,43 @@ def test_graph_replay_of_direct_calls(scored_case):
         db.set_use_graphs(False)
 
 
+@pytest.mark.parametrize("bits,loads", [(16, 3), (15, 3)])
+def test_graph_replay_single_image(scored_case, bits, loads):
+    """A captured single-image sequence (counter fill, LDS walk and, for 2^16
+    rows, the candidate recount) replays bit-exactly, with eager calls of
+    another layout and thresholds between replays; eager calls time the
+    compare kernel alone (walk_ms <= kernel_ms)."""
+    p, hk, O, grains, img, table, db = scored_case
+    base, _ = api.quantize_u8(img)
+    rng = np.random.default_rng(12)
+    imgs = [base] + [base[rng.permutation(base.size)] for _ in range(3)]
+    db.set_banded_tile_bits(bits)
+    db.set_banded_loads(loads)
+    try:
+        eager = [db.score(u, 1.0, 5, 400.0) for u in imgs]
+        for h in eager:
+            assert 0 < h.walk_ms <= h.kernel_ms
+        db.set_use_graphs(True)
+        for rep in range(6):
+            _say(f"single-image graph replay tb={bits} rep={rep}")
+            i = rep % len(imgs)
+            got = db.score(imgs[i], 1.0, 5, 400.0)
+            assert got.nr_results == eager[i].nr_results
+            np.testing.assert_array_equal(got.ids, eager[i].ids)
+            np.testing.assert_array_equal(got.score, eager[i].score)
+            np.testing.assert_array_equal(got.nspots, eager[i].nspots)
+            db.set_use_graphs(False)
+            other = db.score(imgs[i], 1.0, 3, 900.0)  # eager work between replays
+            assert other.nr_results >= 0
+            db.set_use_graphs(True)
+        matched, ns, tot, nres = ob.compare(table, 64, imgs[0], p.NrPxX, 1.0, 5, 400.0)
+        assert eager[0].nr_results == nres
+    finally:
+        db.set_use_graphs(False)
+        db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
+
+
 def test_budget_admission(ctx):
     small = api.Context(0, hbm_budget=1 << 20)
     p = synth.baseline_params()
