@@ -128,6 +128,118 @@ def test_stream_daemon_matches_single_image_program(tmp_path):
         assert got.get(num, []) == want, num
 
 
+def _v2_frame(num, db, img, dtype):
+    import struct
+    codes = {np.float32: 0, np.float64: 1, np.uint16: 2, np.uint8: 3}
+    H, W = img.shape
+    return struct.pack("<IIHHIII", 0x32464D4C, num, db, codes[dtype], W, H, 0) + \
+        np.ascontiguousarray(img, dtype=dtype).tobytes()
+
+
+@pytest.mark.gpu
+def test_stream_daemon_two_dbs_and_wire_v2(tmp_path):
+    """Two DBs (different detectors and orientation sets) in one daemon: v1
+    frames to DB 0, v2 frames (float64, uint16, uint8, float32) to both; a
+    malformed v2 header closes only its own connection.  Every result row
+    equals the single-image program's for the same image and DB, in
+    solutions.txt (DB 0) and solutions_db1.txt (DB 1)."""
+    import signal
+    import socket
+    import time
+
+    p0, hk, O0, img0 = _case(tmp_path)
+    p0.extra = {"MinGoodSpots": p0.MinNrSpots, "ResultDir": str(tmp_path / "res")}
+    p0.DoFwd = 1
+    p0.write(str(tmp_path / "params.txt"))
+    d1 = tmp_path / "db1"
+    d1.mkdir()
+    p1 = synth.baseline_params(NrPxX=512, NrPxY=384, PxX=0.0008, PxY=0.0008, MaxNrLaueSpots=40,
+                               MinNrSpots=4, MinIntensity=300.0,
+                               ForwardFile=str(d1 / "fwd.bin"), DoFwd=1)
+    p1.extra = {"MinGoodSpots": 4, "ResultDir": str(tmp_path / "res")}
+    O1 = synth.random_orientations(12000, seed=8)
+    O1.tofile(d1 / "orients.bin")
+    synth.write_hkl_file(str(d1 / "hkls.csv"), hk)
+    p1.write(str(d1 / "params.txt"))
+    rng = np.random.default_rng(5)
+
+    def frame_for(p, O, seed, ints):
+        im = synth.synth_image(p, hk, O[rng.choice(len(O), 3, replace=False)], seed=seed)
+        return np.floor(im * (60000.0 / im.max())) if ints == 16 else (
+            np.floor(im * (250.0 / im.max())) if ints == 8 else im.astype(np.float32))
+
+    sends = [  # (wire, number, db, image, dtype)
+        ("v1", 1, 0, img0.astype(np.float32), np.float32),
+        ("v2", 2, 0, frame_for(p0, O0, 2, 0).astype(np.float64), np.float64),
+        ("v2", 3, 1, frame_for(p1, O1, 3, 16), np.uint16),
+        ("v2", 4, 1, frame_for(p1, O1, 4, 0), np.float32),
+        ("v2", 5, 0, frame_for(p0, O0, 5, 8), np.uint8),
+    ]
+    port = _free_port()
+    env = dict(os.environ, LM_STREAM_PORT=str(port))
+    exe = os.path.join(ROOT, "bin", "LaueMatchingGPUStream")
+    proc = subprocess.Popen([exe, str(tmp_path / "params.txt"), str(tmp_path / "orients.bin"),
+                             str(tmp_path / "hkls.csv"), "4", str(d1 / "params.txt"),
+                             str(d1 / "orients.bin"), str(d1 / "hkls.csv")],
+                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
+    try:
+        deadline = time.time() + 120
+        socks = None
+        while time.time() < deadline:
+            try:
+                socks = (socket.create_connection(("127.0.0.1", port), timeout=2),
+                         socket.create_connection(("127.0.0.1", port + 1), timeout=2))
+                break
+            except OSError:
+                assert proc.poll() is None, proc.stdout.read()
+                time.sleep(0.3)
+        assert socks is not None
+        bad = socket.create_connection(("127.0.0.1", port + 1), timeout=2)
+        bad.sendall(b"\x00" * 24)  # wrong magic: this connection is closed
+        bad.close()
+        for wire, num, db, im, dt in sends:
+            if wire == "v1":
+                socks[0].sendall(np.array([num], "<u2").tobytes() + im.astype(np.float32).tobytes())
+            else:
+                socks[1].sendall(_v2_frame(num, db, im, dt))
+        for sk in socks:
+            sk.close()
+        time.sleep(2.0)
+        proc.send_signal(signal.SIGTERM)
+        out, _ = proc.communicate(timeout=120)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+    assert proc.returncode == 0, out
+    assert "bad frame header" in out and "exited cleanly" in out
+
+    def rows_of(path):
+        got = {}
+        for r in [l.rstrip("\n").split("\t") for l in open(path)][1:]:
+            got.setdefault(int(r[0]), []).append(r[1:])
+        return got
+
+    got0 = rows_of(tmp_path / "res" / "solutions.txt")
+    got1 = rows_of(tmp_path / "res" / "solutions_db1.txt")
+    p0.DoFwd = 0
+    p1.DoFwd = 0
+    found = 0
+    for wire, num, db, im, dt in sends:
+        base = tmp_path if db == 0 else d1
+        p = p0 if db == 0 else p1
+        synth.write_image(str(base / "image.bin"), im.astype(np.float32).astype(np.float64))
+        p.write(str(base / "params.txt"))
+        r = subprocess.run([EXE, str(base / "params.txt"), str(base / "orients.bin"),
+                            str(base / "hkls.csv"), str(base / "image.bin"), "4"],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        want = [l.rstrip("\n").split("\t") for l in open(base / "image.bin.solutions.txt")][1:]
+        got = (got0 if db == 0 else got1).get(num, [])
+        assert got == want, (wire, num, db)
+        found += bool(want)
+    assert found >= 4  # the frames hold grains of their DB, so solutions appear
+
+
 @pytest.mark.gpu
 def test_end_to_end_dofwd1_then_dofwd0(tmp_path):
     p, hk, O, img = _case(tmp_path)
