@@ -132,8 +132,6 @@ int stage_host_images(lm_db *db, int nimg, const uint8_t *const *imgs, uint8_t *
 // Direct scoring call on the DB's own buffers with one threshold pair.
 int score_uniform(lm_db *db, int nimg, const uint8_t *d, const double *scales, int min_sp,
                   double min_int, hipStream_t st, lm_hits *outs) {
-  if (nimg == 1 && scales && scales[0] != 1.0)
-    return score_seq(db, db->scratch, d, false, scales[0], min_sp, min_int, st, outs);
   int sp[kMaxBatch];
   double mi[kMaxBatch];
   for (int b = 0; b < kMaxBatch; ++b) {
@@ -539,9 +537,6 @@ int lm_score_batch_each(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
   uint8_t *d = nullptr;
   rc = stage_host_images(db, nimg, imgs_u8, &d);
   if (rc) return rc;
-  if (nimg == 1 && im_scales[0] != 1.0)
-    return score_seq(db, db->scratch, d, false, im_scales[0], min_nr_spots[0], min_intensity[0],
-                     db->ctx->stream, outs);
   int sp[kMaxBatch];
   double mi[kMaxBatch];
   for (int b = 0; b < kMaxBatch; ++b) {

@@ -196,8 +196,10 @@ score_band_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restr
       const bool live = o < n;
       const int min_spots = pred->min_spots[0];
       const double min_int = pred->min_int[0];
+      // imScale != 1: a superset, decided by the float64 in-order re-sum
+      const double cand_thr = pred->cand_thr[0];
       const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= min_int);
-      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int;
+      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= cand_thr;
       const uint64_t zm = __ballot(zero_pass);
       if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
       const uint64_t m = __ballot(cand);
@@ -293,8 +295,10 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
       const bool live = o < n;
       const int min_spots = pred->min_spots[0];
       const double min_int = pred->min_int[0];
+      // imScale != 1: a superset, decided by the float64 in-order re-sum
+      const double cand_thr = pred->cand_thr[0];
       const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= min_int);
-      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int;
+      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= cand_thr;
       const uint64_t zm = __ballot(zero_pass);
       if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
       const uint64_t m = __ballot(cand);
@@ -540,8 +544,10 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
       const bool live = o < n;
       const int min_spots = pred->min_spots[0];
       const double min_int = pred->min_int[0];
+      // imScale != 1: a superset, decided by the float64 in-order re-sum
+      const double cand_thr = pred->cand_thr[0];
       const bool zero_pass = live && ns == 0 && (0 >= min_spots) && (0.0 >= min_int);
-      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int;
+      const bool cand = live && ns > 0 && (int)ns >= min_spots && (double)sum >= cand_thr;
       const uint64_t zm = __ballot(zero_pass);
       if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
       const uint64_t m = __ballot(cand);
@@ -578,7 +584,9 @@ recount_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t ncand = std::min<uint64_t>(counters[kCandSlot], candCap);
   const int min_spots = pred->min_spots[0];
-  const double min_int = pred->min_int[0];
+  // the integer sum against the candidate threshold (MinIntensity itself
+  // when imScale == 1; a superset otherwise, decided by the in-order re-sum)
+  const double cand_thr = pred->cand_thr[0];
   auto wave_sum = [](uint32_t v) {
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) v += __shfl_xor(v, d);
@@ -599,7 +607,7 @@ recount_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__
       ns += (uint32_t)__popcll(__ballot(raw != 0));
       sum += wave_sum(raw);
     }
-    if (lane == 0 && ns > 0 && (int)ns >= min_spots && (double)sum >= min_int) {
+    if (lane == 0 && ns > 0 && (int)ns >= min_spots && (double)sum >= cand_thr) {
       const uint32_t pos = atomicAdd(&counters[0], 1u);
       if (pos < cap) hits[pos] = (o << 26) | ((uint64_t)ns << kSumBits) | sum;
     }

@@ -943,7 +943,9 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   // candidates: with MinIntensity <= 0 that is every row, so the table walk
   // is taken instead
   const bool sum16 = db->banded.ready && db->banded.tb >= kSum16TileBits;
-  if (B == 1 && db->banded.ready && bit(pred, 0) &&
+  // (imScale != 1: the walk keeps a candidate superset, re-summed in order
+  // in float64 below, as for batches)
+  if (B == 1 && db->banded.ready &&
       (!sum16 || (min_int[0] > 0 && ((uintptr_t)d_imgs & 15) == 0)))
     kind = 1;
   if (kind == 1 && sum16) {

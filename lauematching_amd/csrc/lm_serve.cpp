@@ -169,10 +169,7 @@ void worker_loop(lm_server *s, int wid) {
       mint[i] = batch[i].min_int;
     }
     if (rc == LM_OK) {
-      if (nb == 1 && scales[0] != 1.0)
-        rc = score_seq(db, sc, stage, false, scales[0], msp[0], mint[0], st, outs.data());
-      else
-        rc = score_images(db, sc, nb, stage, scales.data(), msp.data(), mint.data(), st,
+      rc = score_images(db, sc, nb, stage, scales.data(), msp.data(), mint.data(), st,
                           outs.data());
     }
     const auto now = Clock::now();

@@ -342,6 +342,36 @@ def test_score_matches_oracle_nonunit_scale(scored_case, factor):
         np.testing.assert_array_equal(hits.score, mA)
 
 
+@pytest.mark.parametrize("bits,loads", [(0, 0), (14, 0), (15, 3), (16, 3), (16, 5)])
+def test_nonunit_scale_every_layout(scored_case, bits, loads):
+    """Single images with imScale != 1 (every image whose maximum is not 255)
+    through every layout: the integer walk keeps a candidate superset, the
+    float64 in-order re-sum decides.  MinIntensity set exactly to one row's
+    in-order totInt checks the boundary (that row passes, rows a few ulps
+    below do not)."""
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img * 0.731 + (img > 0) * 0.013)
+    assert s != 1.0
+    _, _, tot0, _ = ob.compare(table, 64, u8, p.NrPxX, s, 0, -1.0)
+    order = np.argsort(tot0)
+    edge = float(tot0[order[int(len(order) * 0.9995)]])  # a row near the top
+    db.set_banded_tile_bits(bits)
+    db.set_banded_loads(loads)
+    try:
+        for min_sp, min_int in [(5, 700.0 * s), (3, edge), (0, 0.0), (4, np.nextafter(edge, 0))]:
+            _say(f"nonunit tb={bits} loads={loads} min_sp={min_sp} min_int={min_int!r}")
+            hits = db.score(u8, s, min_sp, min_int)
+            matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, min_sp, min_int)
+            rows, mA = ob.collect(matched)
+            assert hits.nr_results == nres, (min_sp, min_int)
+            np.testing.assert_array_equal(hits.ids, rows)
+            np.testing.assert_array_equal(hits.totint, tot[rows.astype(np.int64)])
+            np.testing.assert_array_equal(hits.score, mA)
+    finally:
+        db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
+
+
 @pytest.mark.parametrize("min_sp,min_int", [(5, 300.0), (0, 0.0), (6, 4000.0)])
 def test_score_f64_matches_dofwd1_oracle(scored_case, min_sp, min_int):
     """Float64 image (non-integer values): the DoFwd=1 scoring of the oracle's
