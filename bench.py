@@ -565,6 +565,25 @@ def main():
         f"(nrResults {h.nr_results})")
     found = all(g in set(h.ids.tolist()) for g in grains)
 
+    # the same frame as a detector would give it (maximum not 255, so
+    # imScale != 1): candidate superset from the integer walk, then the
+    # float64 in-order re-sum of the candidates
+    u8n, scale_n = api.quantize_u8(img * 0.731 + (img > 0) * 0.013)
+    d_imgn = torch.from_numpy(u8n).to("cuda:0")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        db.score_dev(d_imgn.data_ptr(), scale_n, p.MinNrSpots, p.MinIntensity * 0.731)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hn = db.score_dev(d_imgn.data_ptr(), scale_n, p.MinNrSpots, p.MinIntensity * 0.731)
+    torch.cuda.synchronize()
+    nonunit = {"im_scale": scale_n, "ms_per_step": (time.perf_counter() - t0) / args.steps * 1e3,
+               "kernel_ms": hn.kernel_ms, "hits": int(len(hn.ids))}
+    log(f"imScale {scale_n:.4f} frame: step {nonunit['ms_per_step']:.3f} ms, sequence "
+        f"{hn.kernel_ms:.3f} ms, hits {len(hn.ids)}")
+    del d_imgn
+
     # a plain device copy of known size: the HBM rate reachable on this box,
     # and the reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters
     csrc = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0")
@@ -699,6 +718,7 @@ def main():
         line["serving"] = serving
     if precision:
         line["precision"] = precision
+    line["nonunit_scale"] = nonunit
     if merge:
         line["merge"] = merge
     if prep:
