@@ -130,10 +130,12 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * LM_OPT_BANDED_TILE_BITS: orientations per super-tile of the pixel-banded
  * table copy used by single-image scoring, as log2 (13..16); 0 drops the
  * copy (scoring then reads the orientation-major table only).  Applies at
- * once if a table exists.  16 (MaxNrLaueSpots <= 257 only) keeps 16-bit row
- * sums and recounts the rows whose sum reaches MinIntensity; calls with
- * MinIntensity <= 0, an image that is not 16-byte aligned, or more such rows
- * than n/16 (at least 4096, at most 2^22) read the orientation-major table.
+ * once if a table exists.  16 (MaxNrLaueSpots <= 257 only) keeps a 16-bit
+ * counter per row — the sum of its pixels, or for sparse images (under a
+ * quarter of the pixels nonzero) with MinNrSpots >= 2 its count of nonzero
+ * pixels — and recounts the rows that can pass on it; calls with MinIntensity
+ * <= 0, an image that is not 16-byte aligned, or more such rows than n/16
+ * (at least 4096, at most 2^22) read the orientation-major table.
  * LM_OPT_BATCH_TILE_BITS: super-tile size (log2, 9..13) of a second banded
  * copy used by batches of width B (4..64) with B * 2^bits <= 32768 (the
  * per-(image, orientation) counters of a super-tile fill at most 128 KiB of

@@ -363,6 +363,11 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   const uint8_t *mine = (const uint8_t *)&slot[w][0];
   const __amdgpu_buffer_rsrc_t irs =
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
+  // 16-bit counters: the sum of each row's pixels, or, when fewer than a
+  // quarter of the image's pixels are nonzero and MinNrSpots >= 2, its count
+  // of nonzero pixels (the selective test for sparse images)
+  const uint32_t pk0 = pred->pk_thr[0];
+  const bool count_mode = S16 && (pk0 >> 23) >= 2u && (uint64_t)counters[kNzSlot] * 4 < npx;
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
     for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
     __syncthreads();
@@ -467,7 +472,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         auto add = [&](uint32_t e, uint32_t r) {
           const uint32_t row = e >> PB;
           if constexpr (S16)
-            atomicAdd(&cnt[row >> 1], r << ((row & 1u) << 4));
+            atomicAdd(&cnt[row >> 1], (count_mode ? 1u : r) << ((row & 1u) << 4));
           else
             atomicAdd(&cnt[row], (1u << kSumBits) + r);
         };
@@ -507,11 +512,13 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     }
     __syncthreads();
     if constexpr (S16) {
-      // a row can pass only if its sum reaches MinIntensity and the spot
-      // count it bounds (each nonzero pixel adds at least 1) can reach
+      // sum mode: a row can pass only if its sum reaches MinIntensity and the
+      // spot count it bounds (each nonzero pixel adds at least 1) can reach
       // MinNrSpots; such rows go to the candidate list for an exact recount
-      const uint32_t pk = pred->pk_thr[0];
-      const uint32_t sthr = pk & 0x7fffffu, nthr = (pk >> 23) > 1u ? (pk >> 23) : 1u;
+      // (count mode: the counter is the exact spot count, tested against
+      // MinNrSpots; the sum is recounted)
+      const uint32_t sthr = count_mode ? 0u : (pk0 & 0x7fffffu);
+      const uint32_t nthr = (pk0 >> 23) > 1u ? (pk0 >> 23) : 1u;
       const bool zok = pred->zero_ok[0] & 1u;
       for (int rr = tid; rr < T; rr += THREADS) {
         const uint64_t o = first + rr;
@@ -564,6 +571,31 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     }
     __syncthreads();
   }
+}
+
+// Nonzero pixels of the image (16 bytes per lane, one atomic per workgroup;
+// launched with few workgroups, so the atomics do not queue on one address).
+__global__ void __launch_bounds__(256)
+nonzero_kernel(const uint8_t *__restrict__ img, uint64_t npx, uint32_t *__restrict__ counters) {
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, 0x00020000);
+  uint32_t c = 0;
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q * 16 < npx;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const v4u v = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)(q * 16), 0, 0);
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t x = v[k];
+      // nonzero bytes of x (bit 7 of each byte of ((x & 0x7f..) + 0x7f..) | x)
+      c += __popc((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u);
+    }
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) c += __shfl_xor(c, d);
+  __shared__ uint32_t wsum[4];
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&counters[kNzSlot], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
 
 // Exact recount of the candidate rows of the 16-bit-sum walk, one wave per
@@ -896,6 +928,12 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     set_error("2^16-row walk needs a 16-byte aligned image and a candidate list");
     return LM_ERR_INVALID;
   }
+  // the 16-bit-counter walk picks sums or spot counts from the image's
+  // nonzero fraction, counted first on the device
+  if (s16)
+    hipLaunchKernelGGL(nonzero_kernel,
+                       dim3((unsigned)std::min<uint64_t>(((npx + 15) / 16 + 255) / 256, 128)),
+                       dim3(256), 0, st, img, npx, sc.counters);
   // events around the compare kernel alone (not inside a captured graph)
   const bool timed = !sc.use_graphs && sc.ew0 && sc.ew1;
   if (timed) (void)hipEventRecord(sc.ew0, st);

@@ -174,14 +174,16 @@ int hip_fail(hipError_t e, const char *what);
 
 int pix_bits_for(uint64_t npx);
 // Device counters of a scoring call: per image b, hits at [2b] and rows
-// passing with no spot at [2b + 1]; candidate rows of the 16-bit-sum walk at
-// kCandSlot.
+// passing with no spot at [2b + 1]; candidate rows of the 16-bit-counter walk
+// at kCandSlot, nonzero pixels of its image at kNzSlot.
 constexpr int kCandSlot = 2 * kMaxBatch;
+constexpr int kNzSlot = 2 * kMaxBatch + 1;
 constexpr int kCounterWords = 2 * kMaxBatch + 2;
 
-// Single-image banded copies of 2^16-row super-tiles keep 16-bit sums (two
-// rows per LDS word): exact while 255 * max_spots < 2^16; the spot counts of
-// rows whose sum reaches the threshold are then recounted from the table.
+// Single-image banded copies of 2^16-row super-tiles keep a 16-bit counter
+// per row (two rows per LDS word): the sum of its pixels (exact while
+// 255 * max_spots < 2^16), or, for sparse images, its count of nonzero
+// pixels; rows that can pass on that counter are recounted from the table.
 constexpr int kSum16TileBits = 16;
 constexpr int kSum16MaxSpots = 257;
 

@@ -287,6 +287,36 @@ def test_score_sum16_walk(scored_case, loads):
         db.set_banded_loads(0)
 
 
+@pytest.mark.parametrize("loads", [3, 6])
+def test_score_count16_walk_sparse_images(scored_case, loads):
+    """Sparse images (a few percent of pixels nonzero, as preprocessed frames
+    are) with MinNrSpots >= 2: the 2^16-row walk counts nonzero pixels per
+    row instead of summing them, candidates are the rows with enough spots,
+    and the recount gives the exact sums.  Thresholds as the reference's
+    parameter files set them (MinIntensity 50) and others; imScale 1 and not."""
+    p, hk, O, grains, img, table, db = scored_case
+    sparse = np.where(img > np.percentile(img, 97.0), img, 0.0)
+    cases = [api.quantize_u8(sparse), api.quantize_u8(sparse * 3.7)]
+    db.set_banded_tile_bits(16)
+    db.set_banded_loads(loads)
+    try:
+        for u8, s in cases:
+            assert np.count_nonzero(u8) < u8.size // 4
+            for min_sp, min_int in [(5, 50.0 * s), (2, 0.5), (3, 2000.0 * s), (7, 1e-3)]:
+                _say(f"count16 loads={loads} s={s:.3f} min_sp={min_sp} min_int={min_int}")
+                hits = db.score(u8, s, min_sp, min_int)
+                matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, min_sp, min_int)
+                rows, mA = ob.collect(matched)
+                assert hits.nr_results == nres, (min_sp, min_int)
+                np.testing.assert_array_equal(hits.ids, rows)
+                np.testing.assert_array_equal(hits.score, mA)
+                np.testing.assert_array_equal(hits.nspots, ns[rows.astype(np.int64)])
+                np.testing.assert_array_equal(hits.totint, tot[rows.astype(np.int64)])
+    finally:
+        db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
+
+
 def test_sum16_needs_small_rows(ctx):
     """16-bit sums hold 255 * 257 at most: a DB with MaxNrLaueSpots > 257
     refuses 2^16 super-tiles."""
