@@ -66,6 +66,7 @@ struct lm_server {
   uint64_t next_ticket = 1;
   std::unordered_set<uint64_t> open_tickets;  // submitted, result not yet collected
   bool stop = false;
+  int busy = 0;  // workers with a batch on the device
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_slot;
   std::vector<std::thread> workers;
@@ -126,7 +127,14 @@ void worker_loop(lm_server *s, int wid) {
     int dbi = -1;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+      // while another worker's batch is on the device, let the queue fill
+      // (up to a full batch) until that batch is done, instead of taking
+      // the few requests that arrived meanwhile one by one
+      s->cv_work.wait(lk, [&] {
+        if (s->stop) return true;
+        if (s->pending.empty()) return false;
+        return s->busy == 0 || (int)s->pending.size() >= s->cfg.max_batch;
+      });
       if (s->pending.empty()) break;  // stop requested and queue drained
       dbi = s->pending.front().db;
       int same = 0;
@@ -141,6 +149,7 @@ void worker_loop(lm_server *s, int wid) {
           ++it;
         }
       }
+      ++s->busy;
     }
     lm_db *db = s->dbs[dbi];
     const int nb = (int)batch.size();
@@ -190,9 +199,11 @@ void worker_loop(lm_server *s, int wid) {
       s->last_done = now;
       s->graph_launches += sc.graph_launches - seen_graphs;
       seen_graphs = sc.graph_launches;
+      --s->busy;
     }
     s->cv_done.notify_all();
     s->cv_slot.notify_all();
+    s->cv_work.notify_all();  // a waiting worker takes what queued meanwhile
   }
   scratch_release(sc);
   if (stage) (void)hipFree(stage);
