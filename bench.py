@@ -207,44 +207,59 @@ def poisson_run(srv, reqs, rate, duration, seed):
             "images_per_s": nreq / wall, "mean_batch": st["mean_batch"]}
 
 
-def closed_loop(srv, reqs, count, window=128):
-    """Sustained throughput: keep `window` requests outstanding (submit one
-    as soon as the oldest completes), `count` requests in all."""
+def closed_loop(srv, reqs, count, window=128, clients=4):
+    """Sustained throughput: `clients` submitting threads keep `window`
+    requests outstanding (a new one as soon as one completes), `count`
+    requests in all; two threads collect results in submission order."""
     import collections
     import threading
 
     srv.reset_stats()
     tickets = collections.deque()
     lock = threading.Condition()
-    done = [0]
+    slots = threading.Semaphore(window)
+    done, nxt = [0], [0]
+
+    def submit():
+        while True:
+            with lock:
+                k = nxt[0]
+                if k >= count:
+                    return
+                nxt[0] += 1
+            slots.acquire()
+            db_id, u8, s = reqs[k % len(reqs)]
+            t = srv.submit(db_id, u8, s, 5, 1000.0)
+            with lock:
+                tickets.append(t)
+                lock.notify_all()
 
     def collect():
-        while done[0] < count:
+        while True:
             with lock:
-                while not tickets:
-                    lock.wait()
+                while not tickets and done[0] < count:
+                    lock.wait(0.1)
+                if not tickets:
+                    if done[0] >= count:
+                        return
+                    continue
                 t = tickets.popleft()
             srv.wait(t)
+            slots.release()
             with lock:
                 done[0] += 1
                 lock.notify_all()
 
-    th = threading.Thread(target=collect)
-    th.start()
+    ths = [threading.Thread(target=submit) for _ in range(clients)]
+    ths += [threading.Thread(target=collect) for _ in range(2)]
     t0 = time.perf_counter()
-    for k in range(count):
-        with lock:
-            while k - done[0] >= window:
-                lock.wait()
-        db_id, u8, s = reqs[k % len(reqs)]
-        t = srv.submit(db_id, u8, s, 5, 1000.0)
-        with lock:
-            tickets.append(t)
-            lock.notify_all()
-    th.join()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
     dt = time.perf_counter() - t0
     st = srv.stats()
-    return {"images": count, "window": window, "images_per_s": count / dt,
+    return {"images": count, "window": window, "clients": clients, "images_per_s": count / dt,
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]}
 
 
@@ -254,7 +269,7 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
     from lauematching_amd.serve import Server
 
     res = {}
-    for name, kw in [("batched", dict(max_batch=128, workers=2, use_graphs=True)),
+    for name, kw in [("batched", dict(max_batch=256, workers=2, use_graphs=True)),
                      ("plain", dict(max_batch=1, workers=1, use_graphs=False))]:
         srv = Server(ctx, **kw)
         ids = [srv.add_db(db) for db in dbs]
@@ -264,7 +279,7 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
         for t in [srv.submit(*rq[k % len(rq)], 5, 1000.0) for k in range(3)]:
             srv.wait(t)
         runs = [poisson_run(srv, rq, r, duration, 100 + j) for j, r in enumerate(rates)]
-        cl = closed_loop(srv, rq, nmax)
+        cl = closed_loop(srv, rq, nmax, window=2 * kw["max_batch"] if kw["max_batch"] > 1 else 128)
         st = srv.stats()
         srv.close()
         res[name] = {"poisson": runs, "closed_loop": cl, "graph_launches": st["graph_launches"]}
@@ -457,7 +472,7 @@ def main():
                          "table (0 = none) instead of the autotuned one")
     ap.add_argument("--loads", type=int, default=0,
                     help="with --tile-bits: the single-image walk (LM_OPT_BANDED_LOADS)")
-    ap.add_argument("--batch", type=str, default="64,128",
+    ap.add_argument("--batch", type=str, default="64,128,256",
                     help="comma-separated batch widths to measure as well (one pass over the "
                          "table per batch), e.g. 8,16,32,64,128; empty = none")
     ap.add_argument("--no-autotune", action="store_true",
@@ -527,7 +542,7 @@ def main():
     if not args.no_autotune:
         samples = [u8] + [api.quantize_u8(im)[0] for im, _ in batch_imgs[:3]]
         t0 = time.time()
-        tune = db.autotune(samples, max_batch=128, reps=2)
+        tune = db.autotune(samples, max_batch=256, reps=2)
         tune["seconds"] = time.time() - t0
         log(f"autotune ({tune['seconds']:.1f} s): single {tune['ms_single']} -> "
             f"{tune['banded_tile_bits']}/{tune['banded_loads']}; batch tile bits "
@@ -646,7 +661,7 @@ def main():
     latency = None
     if not args.no_latency:
         reqs = [(0, *api.quantize_u8(img)) for c, d, img in serve_pool if (c, d) == ("fcc", 2048)]
-        latency = run_serving(ctx, [db], reqs, [50, 200, 400, 800], 2.0, 512)
+        latency = run_serving(ctx, [db], reqs, [50, 200, 400, 800, 1600], 2.0, 2048)
     serving = None
     if args.serve:
         from lauematching_amd import synth as sy
@@ -662,7 +677,7 @@ def main():
             d2.forward(api.LM_FWD_FP32_GUARD)
             if not args.no_autotune:  # per-DB layouts and per-width costs for batching
                 d2.autotune([api.quantize_u8(im)[0] for c, d, im in serve_pool
-                             if (c, d) == (cr, det)][:3], max_batch=128, reps=2)
+                             if (c, d) == (cr, det)][:3], max_batch=256, reps=2)
             dbs[(cr, det)] = d2
         keys = list(dbs)
         log(f"serving DBs ready in {time.time() - t0:.1f} s; HBM in use "

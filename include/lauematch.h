@@ -162,6 +162,15 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * give each orientation B/16 lanes of 16 images each (one 16-byte load per
  * lane and spot) instead of one lane holding all B.  Default 0xc0. */
 #define LM_OPT_BATCH_GROUPED 6
+/* LM_OPT_BATCH_NIBBLE: bit 7 / bit 8 set = batches of 128 / 129..256 images
+ * read 4-bit upper bounds of the pixels (ceil(v / 17): zero exactly when v
+ * is, so spot counts stay exact), interleaved as B/2 bytes per pixel, and
+ * recount the (row, image) pairs that can pass from the uint8 images.  Needs
+ * MinIntensity > 0 for every image, 4-byte aligned images and a pixel count
+ * divisible by 4; otherwise (and when more pairs can pass than the candidate
+ * list holds) 128 images take the uint8 walks and wider batches are scored
+ * as two calls of at most 128.  Default 0x180. */
+#define LM_OPT_BATCH_NIBBLE 7
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
@@ -169,23 +178,25 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
  * sample images (nimg >= 1 host uint8 images of the DB's detector, repeated
  * to fill a batch) for one image and for batch widths 4, 8, ..., max_batch,
  * best of `reps` runs each, then sets LM_OPT_BANDED_TILE_BITS,
- * LM_OPT_BANDED_LOADS, LM_OPT_BATCH_TILE_BITS and LM_OPT_BATCH_WIDTHS to the
- * fastest.  Needs a
+ * LM_OPT_BANDED_LOADS, LM_OPT_BATCH_TILE_BITS, LM_OPT_BATCH_WIDTHS,
+ * LM_OPT_BATCH_GROUPED and LM_OPT_BATCH_NIBBLE to the fastest.  Needs a
  * table; the timings are device milliseconds per launch sequence. */
 typedef struct lm_tune_report {
   int32_t n_widths;
-  int32_t widths[6];          /* batch widths timed (4 .. max_batch <= 128)      */
+  int32_t widths[7];          /* batch widths timed (4 .. max_batch <= 256)      */
   double ms_single_table;     /* one image: orientation-major table               */
   double ms_single[4][8];     /* one image: banded 2^13 / 2^14 / 2^15 / 2^16 x
                                  loads 0..7 (0 = not available)                  */
-  double ms_lane[6];          /* per width: orientation-major table               */
-  double ms_grouped[6];       /* per width: table, grouped lanes (64, 128 only)   */
-  double ms_banded[6][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
+  double ms_lane[7];          /* per width: orientation-major table               */
+  double ms_grouped[7];       /* per width: table, grouped lanes (64, 128 only)   */
+  double ms_banded[7][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
+  double ms_nibble[7];        /* per width: 4-bit bounds (128, 256 only)          */
   int32_t banded_tile_bits;   /* chosen LM_OPT_BANDED_TILE_BITS                   */
   int32_t banded_loads;       /* chosen LM_OPT_BANDED_LOADS                       */
   int32_t batch_tile_bits;    /* chosen LM_OPT_BATCH_TILE_BITS                    */
   uint32_t batch_widths_mask; /* chosen LM_OPT_BATCH_WIDTHS                       */
   uint32_t grouped_mask;      /* chosen LM_OPT_BATCH_GROUPED                      */
+  uint32_t nibble_mask;       /* chosen LM_OPT_BATCH_NIBBLE                       */
 } lm_tune_report;
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out);
@@ -231,7 +242,7 @@ int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double 
 int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t min_nr_spots,
                  double min_intensity, void *stream, lm_hits *out);
 
-/* Score nimg (1..128) host images against the DB in one pass over the table.
+/* Score nimg (1..256) host images against the DB in one pass over the table.
  * outs[i] receives image i's hits, identical to lm_score on that image. */
 int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
                    const double *im_scales, int32_t min_nr_spots, double min_intensity,
@@ -264,7 +275,7 @@ void lm_hits_free(lm_hits *h);
  * submit and must not be rebuilt while the server runs. */
 typedef struct lm_server lm_server;
 typedef struct lm_server_config {
-  int32_t max_batch;    /* images per launch, 1..128 (0 = 32)                     */
+  int32_t max_batch;    /* images per launch, 1..256 (0 = 32)                     */
   int32_t workers;      /* concurrent batches / HIP streams (0 = 2)              */
   int32_t max_inflight; /* admitted, not yet completed requests (0 = 2*workers*max_batch) */
   int32_t use_graphs;   /* replay captured launch sequences (hipGraph)           */

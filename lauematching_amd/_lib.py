@@ -73,13 +73,13 @@ class LmServerStats(ctypes.Structure):
 
 
 class LmTuneReport(ctypes.Structure):
-    _fields_ = [("n_widths", ctypes.c_int32), ("widths", ctypes.c_int32 * 6),
+    _fields_ = [("n_widths", ctypes.c_int32), ("widths", ctypes.c_int32 * 7),
                 ("ms_single_table", ctypes.c_double), ("ms_single", (ctypes.c_double * 8) * 4),
-                ("ms_lane", ctypes.c_double * 6), ("ms_grouped", ctypes.c_double * 6),
-                ("ms_banded", (ctypes.c_double * 5) * 6),
+                ("ms_lane", ctypes.c_double * 7), ("ms_grouped", ctypes.c_double * 7),
+                ("ms_banded", (ctypes.c_double * 5) * 7), ("ms_nibble", ctypes.c_double * 7),
                 ("banded_tile_bits", ctypes.c_int32), ("banded_loads", ctypes.c_int32),
                 ("batch_tile_bits", ctypes.c_int32), ("batch_widths_mask", ctypes.c_uint32),
-                ("grouped_mask", ctypes.c_uint32)]
+                ("grouped_mask", ctypes.c_uint32), ("nibble_mask", ctypes.c_uint32)]
 
 
 _lib = None

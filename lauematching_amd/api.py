@@ -180,6 +180,12 @@ class OrientationDB:
         every image."""
         check(lib().lm_db_set_option(self._h, 6, int(mask)))
 
+    def set_batch_nibble(self, mask: int) -> None:
+        """Bit 7 / bit 8 set: batches of 128 / 129..256 images read 4-bit
+        upper bounds of the pixels and recount their candidates exactly;
+        clear: 128 images take the uint8 walks, wider batches two calls."""
+        check(lib().lm_db_set_option(self._h, 7, int(mask)))
+
     def get_option(self, option: int) -> int:
         v = ctypes.c_int64()
         check(lib().lm_db_get_option(self._h, int(option), ctypes.byref(v)))
@@ -212,13 +218,15 @@ class OrientationDB:
                     single[f"banded{tb}_loads{loads}"] = r.ms_single[i][loads]
         return {
             "ms_single": single, "banded_loads": r.banded_loads,
-            "ms_batch": {B: {"table": r.ms_lane[i],
+            "ms_batch": {B: {**({"table": r.ms_lane[i]} if r.ms_lane[i] > 0 else {}),
                              **({"grouped": r.ms_grouped[i]} if r.ms_grouped[i] > 0 else {}),
+                             **({"nibble": r.ms_nibble[i]} if r.ms_nibble[i] > 0 else {}),
                              **{f"banded{tb}": r.ms_banded[i][tb - 9]
                                 for tb in range(9, 14) if r.ms_banded[i][tb - 9] > 0}}
                          for i, B in enumerate(widths)},
             "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
             "batch_widths_mask": r.batch_widths_mask, "grouped_mask": r.grouped_mask,
+            "nibble_mask": r.nibble_mask,
         }
 
     def forward(self, mode: int = LM_FWD_FP64) -> LmFwdStats:

@@ -413,6 +413,15 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     ++db->generation;
     return LM_OK;
   }
+  if (option == LM_OPT_BATCH_NIBBLE) {
+    if (value < 0 || value > 0x180 || (value & 0x7f)) {
+      set_error("LM_OPT_BATCH_NIBBLE is a mask of bits 7 and 8 (128 and 256 images)");
+      return LM_ERR_INVALID;
+    }
+    db->nib_widths = (uint32_t)value;
+    ++db->generation;
+    return LM_OK;
+  }
   set_error("unknown option");
   return LM_ERR_INVALID;
 }
@@ -430,6 +439,7 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
     case LM_OPT_USE_GRAPHS: *value = db->scratch.use_graphs ? 1 : 0; return LM_OK;
     case LM_OPT_BANDED_LOADS: *value = db->banded_loads; return LM_OK;
     case LM_OPT_BATCH_GROUPED: *value = db->group_widths; return LM_OK;
+    case LM_OPT_BATCH_NIBBLE: *value = db->nib_widths; return LM_OK;
     default: set_error("unknown option"); return LM_ERR_INVALID;
   }
 }
@@ -510,7 +520,7 @@ int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   if (nimg > kMaxBatch) {
-    set_error("lm_score_batch: at most 128 images per call");
+    set_error("lm_score_batch: at most 256 images per call");
     return LM_ERR_INVALID;
   }
   uint8_t *d = nullptr;
@@ -531,7 +541,7 @@ int lm_score_batch_each(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   if (nimg > kMaxBatch) {
-    set_error("lm_score_batch_each: at most 128 images per call");
+    set_error("lm_score_batch_each: at most 256 images per call");
     return LM_ERR_INVALID;
   }
   uint8_t *d = nullptr;

@@ -461,13 +461,16 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
           return E16 ? a + (k >> 2) * 256 + 4 * lane + (k & 3) : a + k * 64 + lane;
         };
         // all pixel reads first (lanes outside the chunk read a masked,
-        // in-slot byte and drop it), then the counter adds
+        // in-slot byte and drop it), then the counter adds.  In range:
+        // cbeg + mis <= idx(k) < cend + mis, i.e. (idx(k) - cbeg - mis),
+        // wrapping below the chunk, < cend - cbeg; idx(k) - idx(0) is a
+        // compile-time constant, so each test is one add and one compare.
+        const uint32_t rel0 = idx(0) - (cbeg + mis), len = cend - cbeg;
         uint32_t raw[EMAX];
 #pragma unroll
         for (int k = 0; k < EMAX; ++k) {
           const uint32_t r = mine[pe[d][k] & (C - 1)];
-          const uint32_t i = idx(k);
-          raw[k] = i >= cbeg + mis && i < cend + mis ? r : 0u;
+          raw[k] = rel0 + (idx(k) - idx(0)) < len ? r : 0u;
         }
         auto add = [&](uint32_t e, uint32_t r) {
           const uint32_t row = e >> PB;

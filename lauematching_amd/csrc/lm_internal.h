@@ -69,7 +69,9 @@ struct Geom {
   double Elo, Ehi;
 };
 
-constexpr int kMaxBatch = 128;
+constexpr int kMaxBatch = 256;
+// Widest batch the uint8 interleaved walks take; wider ones read 4-bit bounds.
+constexpr int kMaxByteBatch = 128;
 
 // Per-image predicate of one scoring launch, read by the kernels from device
 // memory (so a captured launch sequence stays valid for new thresholds).
@@ -87,6 +89,11 @@ struct Pred {
   // bit b says a row with no spot passes (0 >= min_spots and 0 >= min_int).
   uint32_t pk_thr[kMaxBatch];
   uint32_t zero_ok[kMaxBatch / 32];
+  // The 4-bit walk's candidate test: sum of ceil(v / 17) >= nib_sum and
+  // nonzero count >= nib_cnt (both <= 0x7fff; 0x7fff = never for images
+  // outside the batch).
+  uint16_t nib_sum[kMaxBatch];
+  uint16_t nib_cnt[kMaxBatch];
   int nimg;
 };
 
@@ -155,8 +162,9 @@ struct lm_db {
   int banded_loads = 0;         // single-image walk: 0 scalar x8, 1 16-byte x2, 2 16-byte x4
   int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
-  double width_ms[8] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
+  double width_ms[9] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
   uint32_t group_widths = 0xc0; // bit log2(B): table batches of width B use grouped lanes
+  uint32_t nib_widths = 0x180;  // bit log2(B): batches of width B (128, 256) use 4-bit bounds
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   std::mutex mu;

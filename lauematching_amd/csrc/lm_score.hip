@@ -297,6 +297,31 @@ score_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ t
   }
 }
 
+// One row of the grouped walk: 16 images of each of the row's c spots.
+template <int B>
+__device__ __forceinline__ void group_row(const uint32_t *sp, uint32_t c, const uint8_t *simg,
+                                          Swar<16> &a) {
+  uint32_t s = 0;
+  for (; s + 4 <= c; s += 4) {
+    uint32_t w[4][4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const u32x4 v = *(const u32x4 *)(simg + (uint64_t)sp[s + g] * B);
+      w[g][0] = v.x;
+      w[g][1] = v.y;
+      w[g][2] = v.z;
+      w[g][3] = v.w;
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) swar_px<16>(w[g], a);
+  }
+  for (; s < c; ++s) {
+    const u32x4 v = *(const u32x4 *)(simg + (uint64_t)sp[s] * B);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    swar_px<16>(w, a);
+  }
+}
+
 // Batches of B = 64 or 128 images with L = B / 16 lanes per orientation:
 // lane `sub` of a group gathers the 16 bytes of images 16*sub .. 16*sub+15
 // of each spot (one 16-byte load) and keeps 12 packed accumulators.  A wave
@@ -372,30 +397,17 @@ score_group_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restri
       const int r = pass * OPP + rl;
       const uint64_t o = t * kTile + r;
       const uint32_t c = rowCnt[r], excl = rowExcl[r];
-      const uint32_t *sp = staged ? stage + excl : entries + base + excl;
       const uint8_t *simg = img + 16 * sub;
       Swar<16> a;
 #pragma unroll
       for (int j = 0; j < 4; ++j) a.s02[j] = a.s13[j] = a.cn[j] = 0;
-      uint32_t s = 0;
-      for (; s + 4 <= c; s += 4) {
-        uint32_t w[4][4];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const u32x4 v = *(const u32x4 *)(simg + (uint64_t)sp[s + g] * B);
-          w[g][0] = v.x;
-          w[g][1] = v.y;
-          w[g][2] = v.z;
-          w[g][3] = v.w;
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) swar_px<16>(w[g], a);
-      }
-      for (; s < c; ++s) {
-        const u32x4 v = *(const u32x4 *)(simg + (uint64_t)sp[s] * B);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-        swar_px<16>(w, a);
-      }
+      // (two calls, so the staged one reads its indices from LDS: through
+      // one generic pointer they would be flat loads, and every gather would
+      // wait for all earlier ones)
+      if (staged)
+        group_row<B>(stage + excl, c, simg, a);
+      else
+        group_row<B>(entries + base + excl, c, simg, a);
       // masks of this lane's 16 images that pass (cand) or pass with no spot
       uint32_t cmask = 0, zmask = 0;
       if (o < n) {
@@ -427,6 +439,252 @@ score_group_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restri
       }
     }
     __syncthreads();
+  }
+}
+
+// Batches of B = 128 or 256 images read as 4-bit upper bounds.  The batch
+// walk is limited by its random line fetches from the interleaved images, so
+// halving the bytes per pixel serves twice the images per fetch.  Nibble
+// q = ceil(v / 17) of pixel value v (0..15) is zero exactly when v is, and
+// 17 q >= v, so per (row, image) the nibble walk gives the exact count of
+// nonzero spots and an upper bound 17 * sum(q) of the raw sum: a row can
+// pass only if count >= MinNrSpots and sum(q) >= ceil(ceil(cand_thr) / 17).
+// Those (row, image) pairs are candidates; nib_recount_kernel recounts them
+// exactly from the uint8 images.  Layout [pixel][B / 2] bytes; byte k of a
+// pixel's 16-byte group g holds image 32 g + 2 k in its low nibble and
+// 32 g + 2 k + 1 in its high nibble.
+//
+// Threads: a wave per (32-image group g, 64 pixel quads); a lane loads one
+// 4-pixel word of each of its 32 images (256 contiguous bytes per load
+// instruction) and writes each pixel's 16 nibble bytes with one store.
+template <int B>
+__global__ void nibble_kernel(const uint8_t *__restrict__ src, const Pred *__restrict__ pred,
+                              uint64_t nquads, uint64_t npx, uint8_t *__restrict__ dst) {
+  constexpr int PB = B / 2;
+  const int g = threadIdx.x >> 6;
+  const uint64_t q = (uint64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  if (q >= nquads) return;
+  const int nimg = pred->nimg;
+  uint32_t v[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int b = 32 * g + j;
+    v[j] = b < nimg ? *(const uint32_t *)(src + (uint64_t)b * npx + 4 * q) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t w[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t x = ((v[8 * m + k] >> (8 * i)) & 0xffu) + 16u;
+        word |= ((x * 241u) >> 12) << (4 * k);  // floor(x / 17) for x < 4096
+      }
+      w[m] = word;
+    }
+    *(u32x4 *)(dst + (4 * q + i) * PB + 16 * g) = u32x4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+// One pixel word (8 nibbles, images 0..7 of the word) into the byte sums of
+// the even (bl) and odd (bh) images and the nibble counts of nonzero values
+// (cn): at most 15 words between flushes.
+__device__ __forceinline__ void nib_word(uint32_t x, uint32_t &bl, uint32_t &bh, uint32_t &cn) {
+  bl += x & 0x0f0f0f0fu;
+  bh += (x >> 4) & 0x0f0f0f0fu;
+  // bit 3 of each nibble of ((x & 7) + 7) | x is set exactly when it is nonzero
+  cn += ((((x & 0x77777777u) + 0x77777777u) | x) >> 3) & 0x11111111u;
+}
+
+// The 16-bit accumulator fields of a word's 8 images come in pairs (low
+// field, high field) = (0, 4), (2, 6), (1, 5), (3, 7) for p = 0..3.
+__host__ __device__ constexpr int nib_pair_lo(int p) { return p == 0 ? 0 : (p == 1 ? 2 : (p == 2 ? 1 : 3)); }
+
+// One row of the nibble walk: the row's c spots in groups of four loads in
+// flight (past the row's end a group reloads the row's last spot and adds
+// zero, so the loads need no branch and each wait counts exactly), byte
+// sums and nibble counts over at most 12 spots, then widened into the
+// paired 16-bit sums S and counts C.
+template <int PB>
+__device__ __forceinline__ void nib_row(const uint32_t *sp, uint32_t c, const uint8_t *simg,
+                                        uint32_t (&S)[16], uint32_t (&C)[16]) {
+  uint32_t s = 0;
+  while (s < c) {
+    uint32_t bl[4] = {0, 0, 0, 0}, bh[4] = {0, 0, 0, 0}, cn[4] = {0, 0, 0, 0};
+    for (int grp = 0; grp < 3 && s < c; ++grp, s += 4) {
+      uint32_t idx[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) idx[g] = sp[min(s + g, c - 1u)];
+      u32x4 v[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) v[g] = *(const u32x4 *)(simg + (uint64_t)idx[g] * PB);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (s + g >= c) v[g] = u32x4{0u, 0u, 0u, 0u};
+        nib_word(v[g].x, bl[0], bh[0], cn[0]);
+        nib_word(v[g].y, bl[1], bh[1], cn[1]);
+        nib_word(v[g].z, bl[2], bh[2], cn[2]);
+        nib_word(v[g].w, bl[3], bh[3], cn[3]);
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      S[4 * w + 0] += bl[w] & 0x00ff00ffu;
+      S[4 * w + 1] += (bl[w] >> 8) & 0x00ff00ffu;
+      S[4 * w + 2] += bh[w] & 0x00ff00ffu;
+      S[4 * w + 3] += (bh[w] >> 8) & 0x00ff00ffu;
+      C[4 * w + 0] += cn[w] & 0x000f000fu;
+      C[4 * w + 1] += (cn[w] >> 8) & 0x000f000fu;
+      C[4 * w + 2] += (cn[w] >> 4) & 0x000f000fu;
+      C[4 * w + 3] += (cn[w] >> 12) & 0x000f000fu;
+    }
+  }
+}
+
+// L = B / 32 lanes per orientation, each owning 32 images (one 16-byte load
+// per spot); the rest as score_group_kernel.  Per lane 16 words of paired
+// 16-bit nibble sums and 16 of paired counts (128 VGPRs at 1024 threads:
+// the paired thresholds live in LDS, the count thresholds are read only for
+// rows whose sums pass).  Candidates go to cands[] as image << 32 | row,
+// counted at counters[kCandSlot].
+template <int B>
+__global__ void __launch_bounds__(1024)
+score_nib_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
+                 const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
+                 const uint8_t *__restrict__ nib, const Pred *__restrict__ pred,
+                 uint64_t *__restrict__ cands, uint64_t cap, uint32_t *__restrict__ counters) {
+  constexpr int L = B / 32;
+  constexpr int PB = B / 2;
+  constexpr int OPP = 1024 / L;
+  constexpr int PASSES = kTile / OPP;
+  static_assert(PASSES * OPP == kTile, "tile must split into passes");
+  __shared__ uint32_t stage[kStageWords];
+  __shared__ uint32_t wsum[kTile / 64];
+  __shared__ uint32_t rowExcl[kTile], rowCnt[kTile];
+  __shared__ uint32_t sthr[L * 16], cthr[L * 16];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  const int sub = tid % L, rl = tid / L;
+  if (tid < L * 16) {
+    const int k = tid & 15, w = k >> 2, p = k & 3;
+    const int b = 32 * (tid >> 4) + 8 * w + nib_pair_lo(p);
+    sthr[tid] = (uint32_t)pred->nib_sum[b] | ((uint32_t)pred->nib_sum[b + 4] << 16);
+    cthr[tid] = (uint32_t)pred->nib_cnt[b] | ((uint32_t)pred->nib_cnt[b + 4] << 16);
+  }
+  const uint32_t *mys = sthr + 16 * sub, *myc = cthr + 16 * sub;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = tileOff[t];
+    const uint64_t tot = tileOff[t + 1] - base;
+    if (tid < kTile) {
+      const uint64_t o = t * kTile + tid;
+      const uint32_t c = o < n ? counts[o] : 0u;
+      uint32_t incl = c;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      rowCnt[tid] = c;
+      rowExcl[tid] = incl - c;
+    }
+    __syncthreads();
+    if (tid < kTile) {
+      uint32_t add = 0;
+      for (int w = 0; w < wave; ++w) add += wsum[w];
+      rowExcl[tid] += add;
+    }
+    const bool staged = tot <= (uint64_t)kStageWords;
+    if (staged) {
+      const uint64_t a0 = base & ~3ull;
+      const uint64_t nvec = (base + tot - a0 + 3) >> 2;
+      const u32x4 *src = (const u32x4 *)(entries + a0);
+      for (uint64_t i = tid; i < nvec; i += 1024) {
+        const u32x4 v = __builtin_nontemporal_load(src + i);
+        const int64_t j = (int64_t)(a0 + 4 * i) - (int64_t)base;
+        if (j >= 0 && j < (int64_t)tot) stage[j] = v.x;
+        if (j + 1 >= 0 && j + 1 < (int64_t)tot) stage[j + 1] = v.y;
+        if (j + 2 >= 0 && j + 2 < (int64_t)tot) stage[j + 2] = v.z;
+        if (j + 3 >= 0 && j + 3 < (int64_t)tot) stage[j + 3] = v.w;
+      }
+    }
+    __syncthreads();
+    for (int pass = 0; pass < PASSES; ++pass) {
+      const int r = pass * OPP + rl;
+      const uint64_t o = t * kTile + r;
+      const uint32_t c = rowCnt[r], excl = rowExcl[r];
+      const uint8_t *simg = nib + 16 * sub;
+      uint32_t S[16], C[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) S[k] = C[k] = 0;
+      // (LDS-staged indices through their own call: see score_group_kernel)
+      if (staged)
+        nib_row<PB>(stage + excl, c, simg, S, C);
+      else
+        nib_row<PB>(entries + base + excl, c, simg, S, C);
+      // fields < 2^15 and thresholds <= 0x7fff: bit 15 of each field of
+      // (a | 0x8000) - t is set exactly when a >= t
+      uint32_t any = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) any |= ((S[k] | 0x80008000u) - mys[k]) & 0x80008000u;
+      if (o < n && any) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            const int k = 4 * w + p;
+            const int b = 32 * sub + 8 * w + nib_pair_lo(p);
+            const uint32_t g = ((S[k] | 0x80008000u) - mys[k]) &
+                               ((C[k] | 0x80008000u) - myc[k]) & 0x80008000u;
+            if (g & 0x8000u) {
+              const uint32_t pos = atomicAdd(&counters[kCandSlot], 1u);
+              if (pos < cap) cands[pos] = ((uint64_t)b << 32) | o;
+            }
+            if (g & 0x80000000u) {
+              const uint32_t pos = atomicAdd(&counters[kCandSlot], 1u);
+              if (pos < cap) cands[pos] = ((uint64_t)(b + 4) << 32) | o;
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Exact count and sum of the nibble walk's candidates from the uint8 images
+// ([nimg][npx]), one lane per candidate; rows passing nspots >= MinNrSpots
+// and sum >= cand_thr get the usual hit record in their image's region.
+__global__ void nib_recount_kernel(const uint16_t *__restrict__ counts,
+                                   const uint64_t *__restrict__ tileOff,
+                                   const uint32_t *__restrict__ entries,
+                                   const uint64_t *__restrict__ cands, uint64_t cap,
+                                   const uint8_t *__restrict__ imgs, uint64_t npx,
+                                   const Pred *__restrict__ pred, uint64_t *__restrict__ hits,
+                                   uint64_t capPer, uint32_t *__restrict__ counters) {
+  const uint64_t ncand = min((uint64_t)counters[kCandSlot], cap);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncand;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t x = cands[i];
+    const uint32_t b = (uint32_t)(x >> 32);
+    const uint64_t o = x & 0xffffffffull;
+    const uint64_t t = o / kTile;
+    uint64_t off = tileOff[t];
+    for (uint64_t r = t * kTile; r < o; ++r) off += counts[r];
+    const uint32_t c = counts[o];
+    const uint8_t *im = imgs + (uint64_t)b * npx;
+    uint32_t ns = 0, sum = 0;
+    for (uint32_t s = 0; s < c; ++s) {
+      const uint32_t v = im[entries[off + s]];
+      ns += v != 0;
+      sum += v;
+    }
+    if (ns > 0 && (int)ns >= pred->min_spots[b] && (double)sum >= pred->cand_thr[b]) {
+      const uint32_t pos = atomicAdd(&counters[2 * b], 1u);
+      if (pos < capPer) hits[(uint64_t)b * capPer + pos] = (o << 26) | ((uint64_t)ns << kSumBits) | sum;
+    }
   }
 }
 
@@ -578,12 +836,14 @@ __global__ void pack_hits_kernel(const uint64_t *__restrict__ hits, uint64_t cap
 }
 
 // Exact float64 sums of candidates (src/LaueMatchingCPU.c:674-681 order).
-// img points at image b's bytes with pixel stride `stride`.
+// Image b's pixel p is img[b * bstep + p * stride] (interleaved batches:
+// bstep 1, stride B; separate images: bstep npx, stride 1).
 __global__ void exact_kernel(const uint16_t *__restrict__ counts,
                              const uint64_t *__restrict__ tileOff,
                              const uint32_t *__restrict__ entries, const uint64_t *__restrict__ xids,
                              uint64_t ncand, const uint8_t *__restrict__ img, uint32_t stride,
-                             const Pred *__restrict__ pred, double *__restrict__ out) {
+                             uint64_t bstep, const Pred *__restrict__ pred,
+                             double *__restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncand) return;
   const uint64_t x = xids[i];
@@ -594,7 +854,7 @@ __global__ void exact_kernel(const uint16_t *__restrict__ counts,
   for (uint64_t r = t * kTile; r < o; ++r) off += counts[r];
   const int c = counts[o];
   const double scale = pred->scale[b];
-  const uint8_t *im = img + (stride > 1 ? b : 0u);
+  const uint8_t *im = img + (uint64_t)b * bstep;
   double tot = 0;
   for (int s = 0; s < c; ++s) {
     const uint8_t raw = im[(uint64_t)entries[off + s] * stride];
@@ -741,6 +1001,46 @@ int launch_interleave(int B, const uint8_t *src, const Pred *pred, uint64_t npx,
 #undef LM_IL
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "interleave kernel launch");
+  return LM_OK;
+}
+
+// Nibble layout of a batch of B = 128 or 256 images: needs 4-byte aligned
+// images and a pixel count divisible by 4 (checked by the caller).
+int launch_nibble_pack(int B, const uint8_t *src, const Pred *pred, uint64_t npx, uint8_t *dst,
+                       hipStream_t st) {
+  const uint64_t nquads = npx / 4;
+  const unsigned grid = (unsigned)((nquads + 63) / 64);
+  if (B == 256)
+    hipLaunchKernelGGL(nibble_kernel<256>, dim3(grid), dim3(64 * 8), 0, st, src, pred, nquads,
+                       npx, dst);
+  else
+    hipLaunchKernelGGL(nibble_kernel<128>, dim3(grid), dim3(64 * 4), 0, st, src, pred, nquads,
+                       npx, dst);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "nibble kernel launch");
+  return LM_OK;
+}
+
+// Nibble walk over the table, then the exact recount of its candidates.
+int launch_nibble_score(lm_db *db, Scratch &sc, int B, const uint8_t *imgs, uint64_t npx,
+                        uint64_t capPer, hipStream_t st) {
+  const Table &t = db->table;
+  const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
+  if (B == 256)
+    hipLaunchKernelGGL((score_nib_kernel<256>), dim3(grid), dim3(1024), 0, st, t.counts,
+                       t.tile_off, t.entries, db->n, t.ntiles, sc.batch, sc.pred, sc.cands,
+                       sc.cand_cap, sc.counters);
+  else
+    hipLaunchKernelGGL((score_nib_kernel<128>), dim3(grid), dim3(1024), 0, st, t.counts,
+                       t.tile_off, t.entries, db->n, t.ntiles, sc.batch, sc.pred, sc.cands,
+                       sc.cand_cap, sc.counters);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "nibble score kernel launch");
+  hipLaunchKernelGGL(nib_recount_kernel, dim3(512), dim3(256), 0, st, t.counts, t.tile_off,
+                     t.entries, sc.cands, sc.cand_cap, imgs, npx, sc.pred, sc.hits, capPer,
+                     sc.counters);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "nibble recount launch");
   return LM_OK;
 }
 
@@ -901,7 +1201,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     return LM_ERR_STATE;
   }
   if (nimg < 1 || nimg > kMaxBatch || !d_imgs || !im_scales || !outs) {
-    set_error("score: need 1..128 images");
+    set_error("score: need 1..256 images");
     return LM_ERR_INVALID;
   }
   for (int b = 0; b < nimg; ++b)
@@ -913,10 +1213,32 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   if (rc) return rc;
   const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
   const int B = batch_width(nimg);
+  // 4-bit bounds for 128 / 256 images (when enabled for the width): every
+  // image needs MinIntensity > 0 (else every row is a candidate), and the
+  // packing kernel 4-byte aligned images of a pixel count divisible by 4
+  bool nib = B >= 128 && ((db->nib_widths >> width_log2(B)) & 1) && (npx & 3) == 0 &&
+             ((uintptr_t)d_imgs & 3) == 0;
+  for (int b = 0; b < nimg && nib; ++b)
+    if (!(min_int[b] > 0)) nib = false;
+  // more images than the uint8 walks take, without the 4-bit walk: two calls
+  auto halves = [&]() -> int {
+    const int h = kMaxByteBatch;
+    int r = score_images(db, sc, h, d_imgs, im_scales, min_spots, min_int, st, outs);
+    if (r) return r;
+    r = score_images(db, sc, nimg - h, d_imgs + (uint64_t)h * npx, im_scales + h, min_spots + h,
+                     min_int + h, st, outs + h);
+    if (r)
+      for (int b = 0; b < h; ++b) lm_hits_free(&outs[b]);
+    return r;
+  };
+  if (B > kMaxByteBatch && !nib) return halves();
   Pred &pred = *sc.pred_host;
   memset(&pred, 0, sizeof pred);
   pred.nimg = nimg;
-  for (int b = 0; b < kMaxBatch; ++b) pred.cand_thr[b] = INFINITY;
+  for (int b = 0; b < kMaxBatch; ++b) {
+    pred.cand_thr[b] = INFINITY;
+    pred.nib_sum[b] = pred.nib_cnt[b] = 0x7fff;
+  }
   for (int b = 0; b < nimg; ++b) {
     const double s = im_scales[b];
     pred.min_spots[b] = min_spots[b];
@@ -931,6 +1253,15 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       pred.cand_thr[b] = min_int[b] <= 0 ? -INFINITY : min_int[b] / s * (1.0 - 1e-12);
     }
     pred.pk_thr[b] = pack_threshold(min_spots[b], pred.cand_thr[b]);
+    // 4-bit test: a sum >= cand_thr is an integer >= smin = ceil(cand_thr),
+    // and 17 * sum(q) >= sum, so sum(q) >= ceil(smin / 17); sums never
+    // exceed 255 * 511 < 2^17
+    const double ct = pred.cand_thr[b];
+    if (ct <= 0)
+      pred.nib_sum[b] = 0;
+    else if (ct < 131072.0)
+      pred.nib_sum[b] = (uint16_t)std::min<uint32_t>(((uint32_t)std::ceil(ct) + 16u) / 17u, 0x7fffu);
+    pred.nib_cnt[b] = (uint16_t)std::min(std::max(min_spots[b], 0), 0x7fff);
     if (0 >= min_spots[b] && 0.0 >= min_int[b]) pred.zero_ok[b >> 5] |= 1u << (b & 31);
   }
   hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
@@ -953,20 +1284,32 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (rc) return rc;
   }
   if (B > 1 && banded_batch_fits(db, B) && ((db->batch_widths >> width_log2(B)) & 1)) kind = 2;
+  // 3: 4-bit bounds of 128 / 256 images over the table plus the exact recount
+  if (nib) {
+    kind = 3;
+    rc = ensure_cands(sc, std::min<uint64_t>(
+                              1u << 22, std::max<uint64_t>(cand_capacity(db), db->n * B / 64)));
+    if (rc) return rc;
+  }
   if (B > 1) {
-    rc = ensure_batch(sc, npx * B);
+    // (uint8 interleave up to 128 images; 256 images only as nibbles)
+    rc = ensure_batch(sc, npx * std::min(B, kMaxByteBatch));
     if (rc) return rc;
   }
   const uint8_t *img = B > 1 ? sc.batch : d_imgs;
   // interleave (B > 1), counter fill, score kernel: enqueued directly, or
   // replayed from a captured graph keyed by everything the arguments use
   auto sequence = [&](uint64_t cp) -> int {
-    if (B > 1) {
+    if (kind == 3) {
+      const int r = launch_nibble_pack(B, d_imgs, sc.pred, npx, sc.batch, st);
+      if (r) return r;
+    } else if (B > 1) {
       const int r = launch_interleave(B, d_imgs, sc.pred, npx, sc.batch, st);
       if (r) return r;
     }
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kCounterWords), 0, st, sc.counters,
                        kCounterWords);
+    if (kind == 3) return launch_nibble_score(db, sc, B, d_imgs, npx, cp, st);
     if (kind == 1) return launch_banded(db, sc, img, cp, st);
     if (kind == 2) return launch_banded_batch(db, sc, B, img, cp, st);
     return launch_score(db, sc, B, img, cp, st);
@@ -1020,6 +1363,12 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       kind = 0;  // more candidates than the list holds: rerun over the table
       continue;
     }
+    if (kind == 3 && cnt[kCandSlot] > sc.cand_cap) {
+      // the same for the 4-bit walk: uint8 walks (two calls above 128 images)
+      if (B > kMaxByteBatch) return halves();
+      kind = 0;
+      continue;
+    }
     uint64_t need = 0;
     for (int b = 0; b < nimg; ++b) need = std::max<uint64_t>(need, cnt[2 * b]);
     if (need <= capPer) break;
@@ -1067,8 +1416,13 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (rc) return rc;
     e = hipMemcpyAsync(sc.hits, xids.data(), k * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
+      // image b's bytes: separate images for the 4-bit walk and single
+      // images, interleaved for the uint8 batch walks
+      const uint8_t *xi = kind == 3 ? d_imgs : img;
+      const uint32_t stride = kind == 3 ? 1u : (uint32_t)B;
+      const uint64_t bstep = kind == 3 ? npx : (B > 1 ? 1u : 0u);
       hipLaunchKernelGGL(exact_kernel, dim3((unsigned)((k + 127) / 128)), dim3(128), 0, st,
-                         t.counts, t.tile_off, t.entries, sc.hits, k, img, (uint32_t)B, sc.pred,
+                         t.counts, t.tile_off, t.entries, sc.hits, k, xi, stride, bstep, sc.pred,
                          sc.vals);
       e = hipGetLastError();
     }

@@ -1,7 +1,9 @@
 // lm_tune.cpp - autotuner for the scoring layouts of one DB.
 //
 // The compare step can read three layouts of the same spot table:
-//   - the orientation-major table, one lane per orientation (any batch width),
+//   - the orientation-major table: one lane per orientation (up to 128
+//     images), B/16 lanes of 16 images (64, 128), or 4-bit bounds of the
+//     pixels with an exact recount of the candidates (128, 256),
 //   - the single-image banded copy (super-tiles of 2^13 .. 2^16 rows),
 //   - the batch banded copy (super-tiles of 2^9..2^13 rows; a batch of width
 //     B fits when B * 2^tb <= 32768, its LDS counters filling <= 128 KiB).
@@ -11,7 +13,7 @@
 // `reps`) and keeps the winners: the single-image super-tile size and entry
 // loads, the batch copy's super-tile size (least per-image time summed over
 // the batch widths) and, per batch width, whether it reads the batch copy or
-// the orientation-major table.
+// the orientation-major table, and with which walk.
 
 #include <hip/hip_runtime.h>
 
@@ -59,13 +61,15 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   }
   if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
   const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
-  const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths;
+  const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths,
+                 saved_nib = db->nib_widths;
   auto restore = [&](int code) {
     db->banded_tb = saved_tb;
     db->batch_tb = saved_btb;
     db->batch_widths = saved_mask;
     db->banded_loads = saved_loads;
     db->group_widths = saved_group;
+    db->nib_widths = saved_nib;
     const int r = build_banded(db);
     return code ? code : r;
   };
@@ -108,27 +112,51 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->batch_tb = 0;
   rc = build_banded(db);
   if (rc) return restore(rc);
-  // the table walk: one lane per orientation, and for 64 / 128 images also
-  // grouped lanes; ms_lane keeps the faster, grouped_mask records which
-  uint32_t gmask = 0;
+  // the table walks: one lane per orientation (up to 128 images), grouped
+  // lanes (64, 128), 4-bit bounds (128, 256); tbl[w] keeps the fastest,
+  // grouped_mask / nibble_mask record which
+  uint32_t gmask = 0, nmask = 0;
+  std::vector<double> tbl(nw, INFINITY);
   for (int w = 0; w < nw; ++w) {
     const int B = rep->widths[w];
+    const uint32_t bitB = 1u << width_log2(B);
+    int pick = -1;  // 0 lane, 1 grouped, 2 nibble
     db->group_widths = 0;
+    db->nib_widths = 0;
     ++db->generation;
-    rc = time_batch(db, d, B, reps, &rep->ms_lane[w]);
-    if (rc) return restore(rc);
+    if (B <= kMaxByteBatch) {
+      rc = time_batch(db, d, B, reps, &rep->ms_lane[w]);
+      if (rc) return restore(rc);
+      tbl[w] = rep->ms_lane[w];
+      pick = 0;
+    }
     if (B == 64 || B == 128) {
-      db->group_widths = 1u << width_log2(B);
+      db->group_widths = bitB;
       ++db->generation;
       rc = time_batch(db, d, B, reps, &rep->ms_grouped[w]);
       if (rc) return restore(rc);
-      if (rep->ms_grouped[w] < rep->ms_lane[w]) {
-        gmask |= 1u << width_log2(B);
-        std::swap(rep->ms_grouped[w], rep->ms_lane[w]);  // ms_lane: the table walk kept
+      db->group_widths = 0;
+      if (rep->ms_grouped[w] < tbl[w]) {
+        tbl[w] = rep->ms_grouped[w];
+        pick = 1;
       }
     }
+    if (B >= 128) {
+      db->nib_widths = bitB;
+      ++db->generation;
+      rc = time_batch(db, d, B, reps, &rep->ms_nibble[w]);
+      if (rc) return restore(rc);
+      db->nib_widths = 0;
+      if (rep->ms_nibble[w] < tbl[w]) {
+        tbl[w] = rep->ms_nibble[w];
+        pick = 2;
+      }
+    }
+    if (pick == 1) gmask |= bitB;
+    if (pick == 2) nmask |= bitB;
   }
   db->group_widths = gmask;
+  db->nib_widths = nmask;
   ++db->generation;
   db->batch_widths = 0x7f;
   for (int tb = 9; tb <= 13; ++tb) {
@@ -146,13 +174,13 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   // widths (each width taking the faster of the copy and the table); none if
   // the table alone is as fast
   double best_cost = 0;
-  for (int w = 0; w < nw; ++w) best_cost += rep->ms_lane[w] / rep->widths[w];
+  for (int w = 0; w < nw; ++w) best_cost += tbl[w] / rep->widths[w];
   int best_btb = 0;
   for (int tb = 9; tb <= 13; ++tb) {
     double cost = 0;
     for (int w = 0; w < nw; ++w) {
       const double ms = rep->ms_banded[w][tb - 9];
-      cost += (ms > 0 ? std::min(ms, rep->ms_lane[w]) : rep->ms_lane[w]) / rep->widths[w];
+      cost += (ms > 0 ? std::min(ms, tbl[w]) : tbl[w]) / rep->widths[w];
     }
     if (cost < best_cost) {
       best_cost = cost;
@@ -163,9 +191,10 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   if (best_btb)
     for (int w = 0; w < nw; ++w) {
       const double ms = rep->ms_banded[w][best_btb - 9];
-      if (ms > 0 && ms < rep->ms_lane[w]) mask |= 1u << width_log2(rep->widths[w]);
+      if (ms > 0 && ms < tbl[w]) mask |= 1u << width_log2(rep->widths[w]);
     }
   rep->grouped_mask = gmask;
+  rep->nibble_mask = nmask;
   rep->banded_tile_bits = best_tb;
   rep->banded_loads = best_loads;
   rep->batch_tile_bits = best_btb;
@@ -179,7 +208,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->width_ms[0] = best1;
   for (int w = 0; w < nw; ++w) {
     const int k = width_log2(rep->widths[w]);
-    db->width_ms[k] = ((mask >> k) & 1) ? rep->ms_banded[w][best_btb - 9] : rep->ms_lane[w];
+    db->width_ms[k] = ((mask >> k) & 1) ? rep->ms_banded[w][best_btb - 9] : tbl[w];
   }
   rc = build_banded(db);  // also bumps the generation: no stale captured launch
   if (rc) return restore(rc);
