@@ -97,7 +97,8 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
       ++db->generation;
       rc = time_batch(db, d, 1, reps, &rep->ms_single[i][loads]);
       if (rc) return restore(rc);
-      if (rep->ms_single[i][loads] < best1) {
+      // a later layout must be faster by more than the timing noise (1%)
+      if (rep->ms_single[i][loads] < 0.99 * best1) {
         best1 = rep->ms_single[i][loads];
         best_tb = 13 + i;
         best_loads = loads;

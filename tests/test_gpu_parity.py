@@ -566,7 +566,8 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
         assert len(singles) == 25 and all(v > 0 for v in singles.values())
         tb = rep["banded_tile_bits"]
         chosen = "table" if tb == 0 else f"banded{tb}_loads{rep['banded_loads']}"
-        assert singles[chosen] == min(singles.values())
+        # the kept layout is within 1% of the fastest (a later one must win by more)
+        assert singles[chosen] <= min(singles.values()) / 0.99
         u8 = samples[0]
         one = db.score(u8, 1.0, 5, 500.0)
         matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, 1.0, 5, 500.0)
