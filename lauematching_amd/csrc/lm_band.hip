@@ -342,7 +342,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
                       uint64_t nsuper, const uint8_t *__restrict__ img, uint64_t npx,
                       const Pred *__restrict__ pred, uint64_t *__restrict__ hits, uint64_t cap,
                       uint32_t *__restrict__ counters, uint64_t *__restrict__ cands,
-                      uint64_t candCap) {
+                      uint64_t candCap, const uint32_t *__restrict__ chunkNz) {
   typedef uint32_t v4u __attribute__((ext_vector_type(4)));
   constexpr int T = 1 << TB;
   constexpr int NW = THREADS / 64;
@@ -363,6 +363,11 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   const uint8_t *mine = (const uint8_t *)&slot[w][0];
   const __amdgpu_buffer_rsrc_t irs =
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
+  // image chunks with no nonzero pixel (chunkNz[c] == 0, from
+  // chunk_nz_kernel on this image): their spots add nothing, so their entry
+  // ranges are taken as empty and never loaded; flags past nchunks read 0
+  const __amdgpu_buffer_rsrc_t frs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)chunkNz, (short)0, (int)((uint32_t)nchunks * 4u), kRawBufferWord3);
   // 16-bit counters: the sum of each row's pixels, or, when fewer than a
   // quarter of the image's pixels are nonzero and MinNrSpots >= 2, its count
   // of nonzero pixels (the selective test for sparse images)
@@ -407,7 +412,10 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     auto load_round = [&](int r, uint32_t &vb, uint32_t &ve) {
       const int cl = w + NW * (r * RC + (lane < RC ? lane : RC - 1));
       vb = coff(cl);
-      ve = coff(cl + 1);
+      const uint32_t e1 = coff(cl + 1);
+      const uint32_t nz = __builtin_amdgcn_raw_buffer_load_b32(
+          frs, (uint32_t)(cl < nchunks ? cl : nchunks) * 4u, 0, 0);
+      ve = nz ? e1 : vb;
     };
     uint32_t cur_b, cur_e, nxt_b, nxt_e;
     load_round(0, cur_b, cur_e);
@@ -576,27 +584,43 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   }
 }
 
-// Nonzero pixels of the image (16 bytes per lane, one atomic per workgroup;
-// launched with few workgroups, so the atomics do not queue on one address).
+// Per image chunk of 2^cb pixels (the LDS walk's chunks), whether any pixel
+// is nonzero (flags[c] = 0 or 1; the walk skips the entries of all-zero
+// chunks, whose spots add nothing), and the image's count of nonzero pixels
+// (counters[kNzSlot], which picks sums or counts for the 16-bit counters).
+// One wave per chunk in turn, 16 bytes per lane and load; one atomic per
+// workgroup, launched with few workgroups so the atomics do not queue on one
+// address.  Bytes past npx read as zero.
 __global__ void __launch_bounds__(256)
-nonzero_kernel(const uint8_t *__restrict__ img, uint64_t npx, uint32_t *__restrict__ counters) {
+chunk_nz_kernel(const uint8_t *__restrict__ img, uint64_t npx, int cb, uint32_t nchunks,
+                uint32_t *__restrict__ flags, uint32_t *__restrict__ counters) {
   typedef uint32_t v4u __attribute__((ext_vector_type(4)));
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, 0x00020000);
+  const int lane = __lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t per = (1u << cb) / (64u * 16u);  // 16-byte loads per lane and chunk
   uint32_t c = 0;
-  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q * 16 < npx;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const v4u v = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)(q * 16), 0, 0);
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t x = v[k];
-      // nonzero bytes of x (bit 7 of each byte of ((x & 0x7f..) + 0x7f..) | x)
-      c += __popc((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u);
+  for (uint32_t ch = wid; ch < nchunks; ch += nw) {
+    uint32_t any = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+      const uint32_t off = (ch << cb) + 16u * (k * 64u + (uint32_t)lane);
+      const v4u v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t x = v[q];
+        // nonzero bytes of x (bit 7 of each byte of ((x & 0x7f..) + 0x7f..) | x)
+        c += __popc((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u);
+        any |= x;
+      }
     }
+    const bool nz = __ballot(any != 0) != 0;
+    if (lane == 0) flags[ch] = nz ? 1u : 0u;
   }
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) c += __shfl_xor(c, d);
   __shared__ uint32_t wsum[4];
-  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = c;
+  if (lane == 0) wsum[threadIdx.x >> 6] = c;
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(&counters[kNzSlot], wsum[0] + wsum[1] + wsum[2] + wsum[3]);
 }
@@ -931,12 +955,19 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     set_error("2^16-row walk needs a 16-byte aligned image and a candidate list");
     return LM_ERR_INVALID;
   }
-  // the 16-bit-counter walk picks sums or spot counts from the image's
-  // nonzero fraction, counted first on the device
-  if (s16)
-    hipLaunchKernelGGL(nonzero_kernel,
-                       dim3((unsigned)std::min<uint64_t>(((npx + 15) / 16 + 255) / 256, 128)),
-                       dim3(256), 0, st, img, npx, sc.counters);
+  const bool lds_ok = lds_walk && ((uintptr_t)img & 15) == 0;
+  if (lds_ok && (!sc.chunk_nz || sc.chunk_nz_cap < (uint64_t)b.nchunks)) {
+    set_error("LDS-staged walk needs the chunk flag buffer");
+    return LM_ERR_INVALID;
+  }
+  // the LDS walk skips the image's all-zero chunks, and the 16-bit-counter
+  // walk picks sums or spot counts from the image's nonzero fraction: both
+  // counted first on the device
+  if (lds_ok)
+    hipLaunchKernelGGL(chunk_nz_kernel,
+                       dim3((unsigned)std::min<uint64_t>(((uint64_t)b.nchunks + 3) / 4, 128)),
+                       dim3(256), 0, st, img, npx, wave_chunk_bits(b.tb), (uint32_t)b.nchunks,
+                       sc.chunk_nz, sc.counters);
   // events around the compare kernel alone (not inside a captured graph)
   const bool timed = !sc.use_graphs && sc.ew0 && sc.ew1;
   if (timed) (void)hipEventRecord(sc.ew0, st);
@@ -946,7 +977,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
       sc.walk_timed = true;
     }
   };
-  if (lds_walk && ((uintptr_t)img & 15) == 0) {
+  if (lds_ok) {
     if (!b.chunk_off || b.tb < 14 || b.tb > kSum16TileBits) {
       set_error("LDS-staged walk needs 2^14..2^16 super-tiles and chunk offsets");
       return LM_ERR_INVALID;
@@ -963,12 +994,12 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, false, kE16>), \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
                        b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters,  \
-                       sc.cands, sc.cand_cap);                                                    \
+                       sc.cands, sc.cand_cap, sc.chunk_nz);                                       \
   else                                                                                            \
     hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, true, kE16>),  \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
                        b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters,  \
-                       sc.cands, sc.cand_cap)
+                       sc.cands, sc.cand_cap, sc.chunk_nz)
 #define LM_LDS_ALL(TB, E1, E2)               \
   switch (L) {                               \
     case 3: LM_LDS(TB, E1, 2, 1024); break;  \

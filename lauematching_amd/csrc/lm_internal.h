@@ -106,6 +106,8 @@ struct Scratch {
   uint32_t *counters = nullptr;  // kCounterWords
   uint64_t *cands = nullptr;     // candidate rows of the 16-bit-sum walk
   uint64_t cand_cap = 0;
+  uint32_t *chunk_nz = nullptr;  // per image chunk of the LDS walk: any nonzero pixel
+  uint64_t chunk_nz_cap = 0;
   double *vals = nullptr;        // float64 sums of in-order hits
   uint64_t val_cap = 0;
   uint8_t *batch = nullptr;      // interleaved images [npx][B]
@@ -136,6 +138,8 @@ struct Scratch {
 int scratch_init(Scratch &s);
 void scratch_release(Scratch &s);
 int scratch_images(Scratch &s, uint64_t bytes);
+// chunk flag buffer of the single-image LDS walk (at least n words)
+int ensure_chunk_flags(Scratch &s, uint64_t n);
 
 }  // namespace lm
 

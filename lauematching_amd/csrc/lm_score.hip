@@ -1095,6 +1095,7 @@ void scratch_release(Scratch &s) {
   if (s.hits) (void)hipFree(s.hits);
   if (s.counters) (void)hipFree(s.counters);
   if (s.cands) (void)hipFree(s.cands);
+  if (s.chunk_nz) (void)hipFree(s.chunk_nz);
   if (s.vals) (void)hipFree(s.vals);
   if (s.batch) (void)hipFree(s.batch);
   if (s.img) (void)hipFree(s.img);
@@ -1105,6 +1106,18 @@ void scratch_release(Scratch &s) {
   if (s.ew0) (void)hipEventDestroy(s.ew0);
   if (s.ew1) (void)hipEventDestroy(s.ew1);
   s = Scratch{};
+}
+
+int ensure_chunk_flags(Scratch &s, uint64_t n) {
+  if (s.chunk_nz_cap >= n) return LM_OK;
+  drop_graphs(s);
+  if (s.chunk_nz) (void)hipFree(s.chunk_nz);
+  s.chunk_nz = nullptr;
+  s.chunk_nz_cap = 0;
+  hipError_t e = hipMalloc(&s.chunk_nz, std::max<uint64_t>(n, 1) * 4);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(chunk flags)");
+  s.chunk_nz_cap = n;
+  return LM_OK;
 }
 
 int scratch_images(Scratch &s, uint64_t bytes) {
@@ -1281,6 +1294,10 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     kind = 1;
   if (kind == 1 && sum16) {
     rc = ensure_cands(sc, cand_capacity(db));
+    if (rc) return rc;
+  }
+  if (kind == 1 && db->banded.chunk_off) {
+    rc = ensure_chunk_flags(sc, (uint64_t)db->banded.nchunks);
     if (rc) return rc;
   }
   if (B > 1 && banded_batch_fits(db, B) && ((db->batch_widths >> width_log2(B)) & 1)) kind = 2;

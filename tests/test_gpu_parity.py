@@ -317,6 +317,46 @@ def test_score_count16_walk_sparse_images(scored_case, loads):
         db.set_banded_loads(0)
 
 
+@pytest.mark.parametrize("bits", [14, 15, 16])
+def test_lds_walk_skips_zero_chunks(scored_case, bits):
+    """The LDS-staged walks skip the entries of image chunks (2^11 or 2^12
+    pixels) with no nonzero pixel.  Images whose nonzero pixels sit in a few
+    bands of detector rows, lone nonzero pixels at the first and last pixel
+    of a chunk, and a blank image: every row still equals the oracle, with
+    sums and with spot counts as the 16-bit counters."""
+    p, hk, O, grains, img, table, db = scored_case
+    base, _ = api.quantize_u8(img)
+    H, W = p.NrPxY, p.NrPxX
+    banded = base.reshape(H, W).copy()
+    for y0 in range(0, H, 300):
+        banded[y0 + 40:y0 + 300] = 0  # keep 40 rows of every 300
+    edge = np.zeros(H * W, np.uint8)
+    cb = 12 if bits == 14 else 11
+    nch = (H * W) >> cb
+    for c in (3, nch // 2, nch - 1):
+        edge[(c << cb)] = 200
+        edge[((c + 1) << cb) - 1] = 255
+    hot = base.ravel()[np.flatnonzero(base.ravel() > 200)[:5000]]
+    edge[np.flatnonzero(base.ravel() > 200)[:5000]] = hot  # the brightest spots of the grains
+    images = [banded.ravel(), edge, np.zeros(H * W, np.uint8)]
+    db.set_banded_tile_bits(bits)
+    db.set_banded_loads(3)
+    try:
+        for k, u8 in enumerate(images):
+            for min_sp, min_int in [(5, 300.0), (1, 1.0), (3, 50.0)]:
+                _say(f"zero chunks bits={bits} image={k} min_sp={min_sp} min_int={min_int}")
+                hits = db.score(u8, 1.0, min_sp, min_int)
+                matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, 1.0, min_sp, min_int)
+                rows, mA = ob.collect(matched)
+                assert hits.nr_results == nres, (k, min_sp, min_int)
+                np.testing.assert_array_equal(hits.ids, rows)
+                np.testing.assert_array_equal(hits.score, mA)
+                np.testing.assert_array_equal(hits.nspots, ns[rows.astype(np.int64)])
+    finally:
+        db.set_banded_tile_bits(14)
+        db.set_banded_loads(0)
+
+
 def test_sum16_needs_small_rows(ctx):
     """16-bit sums hold 255 * 257 at most: a DB with MaxNrLaueSpots > 257
     refuses 2^16 super-tiles."""
