@@ -152,3 +152,31 @@ def test_nibble_option_is_checked(case):
         with pytest.raises(_lib.LaueMatchError):
             db.set_batch_nibble(bad)
     assert db.get_option(7) == 0x180
+
+
+def test_nibble_unaligned_device_images(case):
+    """Device images at an address that is not 4-byte aligned cannot be packed
+    as nibbles: 256 such images are scored as two uint8 calls, 128 through the
+    uint8 walks, with the same results as aligned images."""
+    import torch
+
+    p, img, table, db = case
+    base, _ = api.quantize_u8(img)
+    rng = np.random.default_rng(6)
+    imgs = [base] + [base[rng.permutation(base.size)] for _ in range(255)]
+    npx = base.size
+    flat = torch.zeros(256 * npx + 16, dtype=torch.uint8, device="cuda:0")
+    host = torch.from_numpy(np.concatenate(imgs))
+    flat[1:1 + 256 * npx].copy_(host)
+    aligned = torch.from_numpy(np.stack(imgs)).to("cuda:0")
+    torch.cuda.synchronize()
+    for nimg in (256, 128):
+        _say(f"nibble unaligned device images nimg={nimg}")
+        got = db.score_batch_dev(flat.data_ptr() + 1, nimg, [1.0] * nimg, 5, 500.0)
+        want = db.score_batch_dev(aligned.data_ptr(), nimg, [1.0] * nimg, 5, 500.0)
+        torch.cuda.synchronize()
+        for a, b in zip(got, want):
+            assert a.nr_results == b.nr_results
+            np.testing.assert_array_equal(a.ids, b.ids)
+            np.testing.assert_array_equal(a.score, b.score)
+        _check(p, table, imgs[:2], [1.0] * 2, [5] * 2, [500.0] * 2, got[:2])
