@@ -425,7 +425,9 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         const bool in = j < per_wave;
         beg[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)vb, jj) : total;
         end[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)ve, jj) : total;
-        const uint32_t ib = (uint32_t)(w + NW * j) << CB;
+        // a chunk with no entries to walk (none, or an all-zero image chunk)
+        // needs no pixels: its loads read chunk 0 again, which stays cached
+        const uint32_t ib = beg[d] != end[d] ? (uint32_t)(w + NW * j) << CB : 0u;
 #pragma unroll
         for (int v = 0; v < V; ++v)
           pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
