@@ -116,6 +116,9 @@ struct Scratch {
   uint64_t img_cap = 0;
   Pred *pred = nullptr;          // device copy of the predicate
   Pred *pred_host = nullptr;     // pinned host copy
+  // pinned host copies of the counters and of the first hit records of a
+  // single-image call, fetched together after the launch sequence
+  uint32_t *host_cnt = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // around the compare kernel alone (eager single-image banded launches)
   hipEvent_t ew0 = nullptr, ew1 = nullptr;
@@ -191,6 +194,9 @@ int pix_bits_for(uint64_t npx);
 constexpr int kCandSlot = 2 * kMaxBatch;
 constexpr int kNzSlot = 2 * kMaxBatch + 1;
 constexpr int kCounterWords = 2 * kMaxBatch + 2;
+// Hit records of a single-image call copied back with its counters (one
+// round trip when there are at most this many).
+constexpr uint64_t kHitPrefix = 4096;
 
 // Single-image banded copies of 2^16-row super-tiles keep a 16-bit counter
 // per row (two rows per LDS word): the sum of its pixels (exact while

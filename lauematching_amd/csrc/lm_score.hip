@@ -1079,6 +1079,8 @@ int scratch_init(Scratch &s) {
   hipError_t e = hipMalloc(&s.counters, kCounterWords * 4);
   if (e == hipSuccess) e = hipMalloc(&s.pred, sizeof(Pred));
   if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc(&s.host_cnt, kCounterWords * 4 + kHitPrefix * 8, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&s.ev0);
   if (e == hipSuccess) e = hipEventCreate(&s.ev1);
   if (e == hipSuccess) e = hipEventCreate(&s.ew0);
@@ -1101,6 +1103,7 @@ void scratch_release(Scratch &s) {
   if (s.img) (void)hipFree(s.img);
   if (s.pred) (void)hipFree(s.pred);
   if (s.pred_host) (void)hipHostFree(s.pred_host);
+  if (s.host_cnt) (void)hipHostFree(s.host_cnt);
   if (s.ev0) (void)hipEventDestroy(s.ev0);
   if (s.ev1) (void)hipEventDestroy(s.ev1);
   if (s.ew0) (void)hipEventDestroy(s.ew0);
@@ -1359,6 +1362,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   };
   float ms = 0, walk_ms = 0;
   std::vector<uint32_t> cnt(kCounterWords);
+  uint64_t *const host_pre = (uint64_t *)(sc.host_cnt + kCounterWords);
   // at most three runs: a candidate overflow switches to the table walk
   // once, a hit overflow regrows the hit buffer once
   bool grown = false;
@@ -1370,9 +1374,14 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     rc = run(capPer);
     if (rc) return rc;
     (void)hipEventRecord(sc.ev1, st);
-    e = hipMemcpyAsync(cnt.data(), sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost, st);
+    // the counters and, for one image, the first hit records in one wait
+    e = hipMemcpyAsync(sc.host_cnt, sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nimg == 1)
+      e = hipMemcpyAsync(host_pre, sc.hits, std::min(capPer, kHitPrefix) * 8,
+                         hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "score kernel");
+    memcpy(cnt.data(), sc.host_cnt, kCounterWords * 4);
     (void)hipEventElapsedTime(&ms, sc.ev0, sc.ev1);
     walk_ms = ms;
     if (sc.walk_timed) (void)hipEventElapsedTime(&walk_ms, sc.ew0, sc.ew1);
@@ -1400,7 +1409,9 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   std::vector<uint64_t> first(nimg + 1, 0);
   for (int b = 0; b < nimg; ++b) first[b + 1] = first[b] + cnt[2 * b];
   std::vector<uint64_t> all(first[nimg]);
-  if (nimg == 1 && first[1]) {
+  if (nimg == 1 && first[1] && first[1] <= std::min(capPer, kHitPrefix)) {
+    memcpy(all.data(), host_pre, first[1] * 8);  // already here
+  } else if (nimg == 1 && first[1]) {
     e = hipMemcpyAsync(all.data(), sc.hits, first[1] * 8, hipMemcpyDeviceToHost, st);
   } else if (first[nimg]) {
     rc = ensure_vals(sc, first[nimg]);
