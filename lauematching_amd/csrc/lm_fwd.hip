@@ -195,20 +195,23 @@ __device__ __forceinline__ int project32(const float *M, int hi, int ki, int li,
 // WRITE=true: spots of orientation o go to entries[tile_off[tile] + in-tile
 // prefix of counts], in spot order.  GUARD: float32 projection, float64
 // recompute of undecided pairs (counted in *recomputed by the count pass).
-template <bool WRITE, bool LDS_LIST, bool GUARD>
+// KEEP (count pass): the spot list of orientation o is also stored in
+// keep[s * n + o], so the write pass (FROMKEEP) copies it instead of
+// projecting again.
+template <bool WRITE, bool LDS_LIST, bool GUARD, bool FROMKEEP = false>
 __global__ void __launch_bounds__(kFwdBlock)
 fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__restrict__ hkls,
            int nhkls, double r00, double r01, double r02, double r10, double r11, double r12,
            double r20, double r21, double r22, DevGeom g, DevGeom32 g32, int maxSpots,
            int tileBits, uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
            uint32_t *__restrict__ entries, uint32_t *__restrict__ spill,
-           unsigned long long *__restrict__ recomputed) {
+           unsigned long long *__restrict__ recomputed, uint32_t *__restrict__ keep) {
   __shared__ uint32_t lst[LDS_LIST ? kLdsSlots * kFwdBlock : 1];
   __shared__ uint32_t scan[kFwdBlock];
   const uint64_t o = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
   const bool live = o < n;
   double M[9];
-  if (live) {
+  if (live && !FROMKEEP) {
     double t[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) t[i] = orients[o * 9 + i];
@@ -229,7 +232,8 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   uint32_t *col = LDS_LIST ? lst + threadIdx.x : spill + o;
   const uint64_t colStride = LDS_LIST ? (uint64_t)kFwdBlock : (uint64_t)gridDim.x * kFwdBlock;
   int ns = live ? 0 : maxSpots;
-  for (int hi = 0; hi < nhkls; ++hi) {
+  if (FROMKEEP) ns = live ? (int)counts[o] : 0;
+  for (int hi = 0; hi < (FROMKEEP ? 0 : nhkls); ++hi) {
     if (__all(ns >= maxSpots)) break;
     const int hI = hkls[3 * hi + 0], kI = hkls[3 * hi + 1], lI = hkls[3 * hi + 2];
     if (ns < maxSpots) {
@@ -259,6 +263,8 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   }
   if (!WRITE) {
     if (live) counts[o] = (uint16_t)ns;
+    if (keep && live)
+      for (int s = 0; s < ns; ++s) keep[(uint64_t)s * n + o] = col[(uint64_t)s * colStride];
     if (GUARD) {
       uint32_t w = nrec;
 #pragma unroll
@@ -294,7 +300,8 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   const uint32_t excl = scan[threadIdx.x] - (uint32_t)c;
   if (live) {
     uint32_t *dst = entries + tileOff[tile] + blockBase + excl;
-    for (int s = 0; s < c; ++s) dst[s] = col[(uint64_t)s * colStride];
+    for (int s = 0; s < c; ++s)
+      dst[s] = FROMKEEP ? keep[(uint64_t)s * n + o] : col[(uint64_t)s * colStride];
   }
 }
 
@@ -544,6 +551,23 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
   const uint64_t chunkBlocks = 1u << 18;  // 32M orientations per launch
   uint32_t *spill = nullptr;
   unsigned long long *d_rec = nullptr;
+  // One projection pass when the HBM budget holds every orientation's spot
+  // list (n * M words, slot-major per launch chunk) beside the table: the
+  // count pass keeps the lists and the write pass copies them; otherwise
+  // (no room for the lists, or then none for the table beside them) the
+  // write pass projects again.
+  uint32_t *keep = nullptr;
+  const uint64_t keepBytes = ldsList ? db->n * (uint64_t)db->max_spots * 4 : 0;
+  if (keepBytes && reserve(db, keepBytes) == LM_OK) {
+    if (hipMalloc(&keep, keepBytes) != hipSuccess) {
+      keep = nullptr;
+      (void)hipGetLastError();
+    }
+    if (!keep) {
+      std::lock_guard<std::mutex> lk(db->ctx->mu);
+      db->ctx->in_use -= keepBytes;
+    }
+  }
   hipError_t e = hipMalloc(&d_rec, 8);
   if (e == hipSuccess) e = hipMemsetAsync(d_rec, 0, 8, st);
   if (e == hipSuccess && !ldsList)
@@ -560,6 +584,18 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
   for (int pass = 0; pass < 2 && rc == LM_OK; ++pass) {
     if (pass == 1) {
       rc = finish_offsets(db, st);
+      if (rc == LM_ERR_BUDGET && keep) {
+        // no room for the table beside the kept lists: drop them and
+        // project again in the write pass
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(keep);
+        keep = nullptr;
+        {
+          std::lock_guard<std::mutex> lk(db->ctx->mu);
+          db->ctx->in_use -= keepBytes;
+        }
+        rc = finish_offsets(db, st);
+      }
       if (rc) break;
     }
     for (uint64_t b0 = 0; b0 * kFwdBlock < db->n; b0 += chunkBlocks) {
@@ -571,12 +607,20 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
       const double *op = db->d_orients + first * 9;
       uint16_t *cp = t.counts + first;
       const uint64_t *tp = t.tile_off + (first >> t.tile_bits);
+      uint32_t *kp = keep ? keep + first * (uint64_t)db->max_spots : nullptr;
 #define LM_FWD(W, L, G)                                                                         \
   hipLaunchKernelGGL((fwd_kernel<W, L, G>), dim3(blocks), dim3(kFwdBlock), 0, st, op, rows,      \
                      db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], \
-                     g, g32, db->max_spots, t.tile_bits, cp, tp, t.entries, spill, d_rec)
+                     g, g32, db->max_spots, t.tile_bits, cp, tp, t.entries, spill, d_rec, kp)
       const int sel = (pass << 2) | (ldsList << 1) | (guard ? 1 : 0);
-      switch (sel) {
+      if (pass == 1 && keep) {
+        // copy the kept lists (no projection and no list: LDS_LIST and GUARD
+        // do not matter, false keeps the copy kernel free of LDS)
+        hipLaunchKernelGGL((fwd_kernel<true, false, false, true>), dim3(blocks), dim3(kFwdBlock), 0,
+                           st, op, rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4],
+                           R[5], R[6], R[7], R[8], g, g32, db->max_spots, t.tile_bits, cp, tp,
+                           t.entries, spill, d_rec, kp);
+      } else switch (sel) {
         case 0: LM_FWD(false, false, false); break;
         case 1: LM_FWD(false, false, true); break;
         case 2: LM_FWD(false, true, false); break;
@@ -599,6 +643,11 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
   e = hipStreamSynchronize(st);
   if (e == hipSuccess) e = hipMemcpy(&nrec, d_rec, 8, hipMemcpyDeviceToHost);
   if (spill) (void)hipFree(spill);
+  if (keep) {
+    (void)hipFree(keep);
+    std::lock_guard<std::mutex> lk(db->ctx->mu);
+    db->ctx->in_use -= keepBytes;
+  }
   (void)hipFree(d_rec);
   if (rc == LM_OK && e != hipSuccess) rc = hip_fail(e, "forward simulation");
   float ms = 0;

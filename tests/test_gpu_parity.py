@@ -701,3 +701,27 @@ def test_budget_admission(ctx):
     with pytest.raises(_lib.LaueMatchError) as e:
         api.OrientationDB(small, O, synth.fcc_hkls(8), api.Geometry.from_params(p), 64)
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("guard", [False, True])
+def test_forward_without_room_for_kept_lists(ctx, guard):
+    """With an HBM budget that holds the DB and its table but not every
+    orientation's spot list (n x M words), the forward simulation projects
+    twice instead of keeping the lists: the same table, bit for bit."""
+    p = synth.baseline_params(MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(8, "box")
+    n = 20000
+    O = synth.random_orientations(n, seed=19)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    spots = int(want[:, 0].astype(np.int64).sum())
+    # orientations + hkls + counts/tile offsets + entries + banded copy, with
+    # slack, and well under the n * 64 * 4 bytes of kept lists on top
+    need = n * 72 + hk.size * 4 + n * 2 + 4096 + 2 * (spots + 64) * 4 + (1 << 20)
+    assert need + n * 64 * 4 // 2 < need + n * 64 * 4
+    small = api.Context(0, hbm_budget=need + n * 64 * 4 // 2)
+    _say(f"forward with a small budget guard={guard}")
+    db = api.OrientationDB(small, O, hk, api.Geometry.from_params(p), 64)
+    db.set_banded_tile_bits(0)
+    db.forward(api.LM_FWD_FP32_GUARD if guard else api.LM_FWD_FP64)
+    np.testing.assert_array_equal(db.export_table(), want)
+    db.close()
