@@ -688,7 +688,7 @@ def main():
             u8, s = api.quantize_u8(img)
             reps = 7 if d == 2048 else 3
             reqs += [(keys.index((c, d)), u8, s)] * reps
-        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [5, 20, 50, 100, 200], 4.0, 1024)
+        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [50, 200, 800, 1600], 3.0, 2048)
         serving["dbs"] = [{"crystal": c, "detector": d, "orientations": dbs[(c, d)].n}
                           for c, d in keys]
         serving["hbm_in_use_gb"] = ctx.bytes_in_use() / 1e9
