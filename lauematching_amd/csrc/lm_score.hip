@@ -505,32 +505,33 @@ __host__ __device__ constexpr int nib_pair_lo(int p) { return p == 0 ? 0 : (p ==
 // One row of the nibble walk: the row's c spots in groups of four loads in
 // flight (past the row's end a group reloads the row's last spot and adds
 // zero, so the loads need no branch and each wait counts exactly), byte
-// sums and nibble counts over at most 12 spots, then widened into the
-// paired 16-bit sums S and counts C.
-template <int PB>
+// sums and nibble counts of the lane's WPL words (8 images each) over at
+// most 12 spots, then widened into the paired 16-bit sums S and counts C.
+template <int PB, int WPL>
 __device__ __forceinline__ void nib_row(const uint32_t *sp, uint32_t c, const uint8_t *simg,
-                                        uint32_t (&S)[16], uint32_t (&C)[16]) {
+                                        uint32_t (&S)[4 * WPL], uint32_t (&C)[4 * WPL]) {
+  typedef uint32_t vw __attribute__((ext_vector_type(WPL)));
   uint32_t s = 0;
   while (s < c) {
-    uint32_t bl[4] = {0, 0, 0, 0}, bh[4] = {0, 0, 0, 0}, cn[4] = {0, 0, 0, 0};
+    uint32_t bl[WPL], bh[WPL], cn[WPL];
+#pragma unroll
+    for (int w = 0; w < WPL; ++w) bl[w] = bh[w] = cn[w] = 0;
     for (int grp = 0; grp < 3 && s < c; ++grp, s += 4) {
       uint32_t idx[4];
 #pragma unroll
       for (int g = 0; g < 4; ++g) idx[g] = sp[min(s + g, c - 1u)];
-      u32x4 v[4];
+      vw v[4];
 #pragma unroll
-      for (int g = 0; g < 4; ++g) v[g] = *(const u32x4 *)(simg + (uint64_t)idx[g] * PB);
+      for (int g = 0; g < 4; ++g) v[g] = *(const vw *)(simg + (uint64_t)idx[g] * PB);
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        if (s + g >= c) v[g] = u32x4{0u, 0u, 0u, 0u};
-        nib_word(v[g].x, bl[0], bh[0], cn[0]);
-        nib_word(v[g].y, bl[1], bh[1], cn[1]);
-        nib_word(v[g].z, bl[2], bh[2], cn[2]);
-        nib_word(v[g].w, bl[3], bh[3], cn[3]);
+        const bool in = s + g < c;
+#pragma unroll
+        for (int w = 0; w < WPL; ++w) nib_word(in ? v[g][w] : 0u, bl[w], bh[w], cn[w]);
       }
     }
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < WPL; ++w) {
       S[4 * w + 0] += bl[w] & 0x00ff00ffu;
       S[4 * w + 1] += (bl[w] >> 8) & 0x00ff00ffu;
       S[4 * w + 2] += bh[w] & 0x00ff00ffu;
@@ -543,38 +544,42 @@ __device__ __forceinline__ void nib_row(const uint32_t *sp, uint32_t c, const ui
   }
 }
 
-// L = B / 32 lanes per orientation, each owning 32 images (one 16-byte load
-// per spot); the rest as score_group_kernel.  Per lane 16 words of paired
-// 16-bit nibble sums and 16 of paired counts (128 VGPRs at 1024 threads:
-// the paired thresholds live in LDS, the count thresholds are read only for
-// rows whose sums pass).  Candidates go to cands[] as image << 32 | row,
-// counted at counters[kCandSlot].
-template <int B>
-__global__ void __launch_bounds__(1024)
+// L = B / IPL lanes per orientation, each owning IPL images (one IPL/2-byte
+// load per spot); the rest as score_group_kernel.  Per lane IPL/2 words of
+// paired 16-bit nibble sums and as many of paired counts; the paired
+// thresholds live in LDS, the count thresholds are read only for rows whose
+// sums pass.  IPL = 16 keeps the lane's state small enough for two 1024-thread
+// workgroups per CU (8 waves per SIMD), IPL = 32 needs 128 VGPRs (4 waves).
+// Candidates go to cands[] as image << 32 | row, counted at
+// counters[kCandSlot].
+template <int B, int IPL>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(IPL == 16 ? 8 : 4)))
 score_nib_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
                  const uint32_t *__restrict__ entries, uint64_t n, uint64_t ntiles,
                  const uint8_t *__restrict__ nib, const Pred *__restrict__ pred,
                  uint64_t *__restrict__ cands, uint64_t cap, uint32_t *__restrict__ counters) {
-  constexpr int L = B / 32;
+  constexpr int L = B / IPL;
   constexpr int PB = B / 2;
+  constexpr int WPL = IPL / 8;  // 32-bit words (8 images) per lane and spot
+  constexpr int NF = 4 * WPL;   // paired accumulator words per lane
   constexpr int OPP = 1024 / L;
   constexpr int PASSES = kTile / OPP;
   static_assert(PASSES * OPP == kTile, "tile must split into passes");
   __shared__ uint32_t stage[kStageWords];
   __shared__ uint32_t wsum[kTile / 64];
   __shared__ uint32_t rowExcl[kTile], rowCnt[kTile];
-  __shared__ uint32_t sthr[L * 16], cthr[L * 16];
+  __shared__ uint32_t sthr[L * NF], cthr[L * NF];
   const int tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
   const int sub = tid % L, rl = tid / L;
-  if (tid < L * 16) {
-    const int k = tid & 15, w = k >> 2, p = k & 3;
-    const int b = 32 * (tid >> 4) + 8 * w + nib_pair_lo(p);
+  if (tid < L * NF) {
+    const int k = tid % NF, w = k >> 2, p = k & 3;
+    const int b = IPL * (tid / NF) + 8 * w + nib_pair_lo(p);
     sthr[tid] = (uint32_t)pred->nib_sum[b] | ((uint32_t)pred->nib_sum[b + 4] << 16);
     cthr[tid] = (uint32_t)pred->nib_cnt[b] | ((uint32_t)pred->nib_cnt[b + 4] << 16);
   }
-  const uint32_t *mys = sthr + 16 * sub, *myc = cthr + 16 * sub;
+  const uint32_t *mys = sthr + NF * sub, *myc = cthr + NF * sub;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t base = tileOff[t];
     const uint64_t tot = tileOff[t + 1] - base;
@@ -616,27 +621,27 @@ score_nib_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict
       const int r = pass * OPP + rl;
       const uint64_t o = t * kTile + r;
       const uint32_t c = rowCnt[r], excl = rowExcl[r];
-      const uint8_t *simg = nib + 16 * sub;
-      uint32_t S[16], C[16];
+      const uint8_t *simg = nib + (IPL / 2) * sub;
+      uint32_t S[NF], C[NF];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) S[k] = C[k] = 0;
+      for (int k = 0; k < NF; ++k) S[k] = C[k] = 0;
       // (LDS-staged indices through their own call: see score_group_kernel)
       if (staged)
-        nib_row<PB>(stage + excl, c, simg, S, C);
+        nib_row<PB, WPL>(stage + excl, c, simg, S, C);
       else
-        nib_row<PB>(entries + base + excl, c, simg, S, C);
+        nib_row<PB, WPL>(entries + base + excl, c, simg, S, C);
       // fields < 2^15 and thresholds <= 0x7fff: bit 15 of each field of
       // (a | 0x8000) - t is set exactly when a >= t
       uint32_t any = 0;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) any |= ((S[k] | 0x80008000u) - mys[k]) & 0x80008000u;
+      for (int k = 0; k < NF; ++k) any |= ((S[k] | 0x80008000u) - mys[k]) & 0x80008000u;
       if (o < n && any) {
 #pragma unroll
-        for (int w = 0; w < 4; ++w)
+        for (int w = 0; w < WPL; ++w)
 #pragma unroll
           for (int p = 0; p < 4; ++p) {
             const int k = 4 * w + p;
-            const int b = 32 * sub + 8 * w + nib_pair_lo(p);
+            const int b = IPL * sub + 8 * w + nib_pair_lo(p);
             const uint32_t g = ((S[k] | 0x80008000u) - mys[k]) &
                                ((C[k] | 0x80008000u) - myc[k]) & 0x80008000u;
             if (g & 0x8000u) {
@@ -1021,17 +1026,23 @@ int launch_nibble_pack(int B, const uint8_t *src, const Pred *pred, uint64_t npx
   return LM_OK;
 }
 
+// Images per lane of the nibble walk (measured at 100M orientations: 256
+// images 34.8 ms with 16, 37.2 ms with 32; 128 images 28.1 ms with 32,
+// 30.6 ms with 16).
+template <int B>
+constexpr int nib_ipl() { return B == 256 ? 16 : 32; }
+
 // Nibble walk over the table, then the exact recount of its candidates.
 int launch_nibble_score(lm_db *db, Scratch &sc, int B, const uint8_t *imgs, uint64_t npx,
                         uint64_t capPer, hipStream_t st) {
   const Table &t = db->table;
   const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
   if (B == 256)
-    hipLaunchKernelGGL((score_nib_kernel<256>), dim3(grid), dim3(1024), 0, st, t.counts,
+    hipLaunchKernelGGL((score_nib_kernel<256, nib_ipl<256>()>), dim3(grid), dim3(1024), 0, st, t.counts,
                        t.tile_off, t.entries, db->n, t.ntiles, sc.batch, sc.pred, sc.cands,
                        sc.cand_cap, sc.counters);
   else
-    hipLaunchKernelGGL((score_nib_kernel<128>), dim3(grid), dim3(1024), 0, st, t.counts,
+    hipLaunchKernelGGL((score_nib_kernel<128, nib_ipl<128>()>), dim3(grid), dim3(1024), 0, st, t.counts,
                        t.tile_off, t.entries, db->n, t.ntiles, sc.batch, sc.pred, sc.cands,
                        sc.cand_cap, sc.counters);
   hipError_t e = hipGetLastError();
