@@ -1036,7 +1036,9 @@ constexpr int nib_ipl() { return B == 256 ? 16 : 32; }
 int launch_nibble_score(lm_db *db, Scratch &sc, int B, const uint8_t *imgs, uint64_t npx,
                         uint64_t capPer, hipStream_t st) {
   const Table &t = db->table;
-  const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 1u << 20);
+  // a few workgroups per CU looping over the tiles: the kernel's start-up
+  // (its register spills among it) is paid per workgroup, not per tile
+  const unsigned grid = (unsigned)std::min<uint64_t>(t.ntiles, 4096);
   if (B == 256)
     hipLaunchKernelGGL((score_nib_kernel<256, nib_ipl<256>()>), dim3(grid), dim3(1024), 0, st, t.counts,
                        t.tile_off, t.entries, db->n, t.ntiles, sc.batch, sc.pred, sc.cands,
