@@ -14,7 +14,12 @@
 // (the image stays hot in L2 / Infinity Cache) and keeps the packed integer
 // accumulator (nspots << 17 | sum of raw) in a register.  B images can be
 // scored in the same pass: they are interleaved pixel-major so one gather of
-// B bytes serves all of them, and each lane keeps B accumulators.
+// B bytes serves all of them, and each lane keeps B accumulators (or B / 16
+// lanes keep 16 each).  For 128 and 256 images the interleave holds 4-bit
+// upper bounds of the pixels instead (half the bytes per image, so 256
+// images share one cache line per spot); their walk keeps exact spot counts
+// and bounded sums, and the (row, image) pairs that can pass are recounted
+// from the uint8 images.
 //
 // Exactness.  The integer sum is exact.  When imScale == 1 (the quantizer's
 // scale for any image whose maximum is 255) totInt equals that sum and the
