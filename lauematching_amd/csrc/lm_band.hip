@@ -373,9 +373,11 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   // of nonzero pixels (the selective test for sparse images)
   const uint32_t pk0 = pred->pk_thr[0];
   const bool count_mode = S16 && (pk0 >> 23) >= 2u && (uint64_t)counters[kNzSlot] * 4 < npx;
+  // counters start at zero; each epilogue clears the words it reads, so the
+  // next super-tile starts without a clearing pass
+  for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
+  __syncthreads();
   for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
-    for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
-    __syncthreads();
     const uint64_t first = t << TB;
     const uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
     const uint32_t total = co[nchunks];
@@ -533,6 +535,33 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
       const uint32_t sthr = count_mode ? 0u : (pk0 & 0x7fffffu);
       const uint32_t nthr = (pk0 >> 23) > 1u ? (pk0 >> 23) : 1u;
       const bool zok = pred->zero_ok[0] & 1u;
+      if (!zok) {
+        // 8 rows per 16-byte LDS read, one max test for the 8 (rows that
+        // pass are rare), and the words cleared for the next super-tile
+        const uint32_t thr = sthr > nthr ? sthr : nthr;
+        for (int i = tid; i < NCNT / 4; i += THREADS) {
+          const v4u q = ((v4u *)cnt)[i];
+          ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
+          uint32_t m = 0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) m = max(m, max(q[k] & 0xffffu, q[k] >> 16));
+          if (m >= thr) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const uint32_t sum = (q[k] >> (16 * h)) & 0xffffu;
+                const uint64_t o = first + 8u * (uint32_t)i + 2u * k + h;
+                if (o < n && sum >= thr) {
+                  const uint32_t pos = atomicAdd(&counters[kCandSlot], 1u);
+                  if (pos < candCap) cands[pos] = o;
+                }
+              }
+          }
+        }
+        __syncthreads();
+        continue;
+      }
       for (int rr = tid; rr < T; rr += THREADS) {
         const uint64_t o = first + rr;
         const uint32_t sum = (cnt[rr >> 1] >> ((rr & 1) << 4)) & 0xffffu;
@@ -555,11 +584,14 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         }
       }
       __syncthreads();
+      for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
+      __syncthreads();
       continue;
     }
     for (int rr = tid; rr < T; rr += THREADS) {
       const uint64_t o = first + rr;
       const uint32_t acc = cnt[rr];
+      cnt[rr] = 0;
       const uint32_t ns = acc >> kSumBits, sum = acc & kSumMask;
       const bool live = o < n;
       const int min_spots = pred->min_spots[0];
