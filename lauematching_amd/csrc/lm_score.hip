@@ -873,6 +873,35 @@ __global__ void exact_kernel(const uint16_t *__restrict__ counts,
   out[i] = tot;
 }
 
+// The same float64 in-order sums for the hit records of a single image
+// (imScale != 1), in the launch sequence right after its walk: vals[i] for
+// hits[i], i < min(counters[0], cap), so the host fetches them with the
+// records instead of sending the rows back.
+__global__ void exact_hits_kernel(const uint16_t *__restrict__ counts,
+                                  const uint64_t *__restrict__ tileOff,
+                                  const uint32_t *__restrict__ entries,
+                                  const uint64_t *__restrict__ hits, uint64_t cap,
+                                  const uint32_t *__restrict__ counters,
+                                  const uint8_t *__restrict__ img, const Pred *__restrict__ pred,
+                                  double *__restrict__ vals) {
+  const uint64_t k = min((uint64_t)counters[0], cap);
+  const double scale = pred->scale[0];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t o = (hits[i] >> 26) & 0xffffffffull;
+    const uint64_t t = o / kTile;
+    uint64_t off = tileOff[t];
+    for (uint64_t r = t * kTile; r < o; ++r) off += counts[r];
+    const int c = counts[o];
+    double tot = 0;
+    for (int s = 0; s < c; ++s) {
+      const uint8_t raw = img[entries[off + s]];
+      if (raw > 0) tot += (double)raw * scale;
+    }
+    vals[i] = tot;
+  }
+}
+
 // Fill kernel for the launch counters (kept as a kernel so that a captured
 // launch sequence holds no memset node).
 __global__ void zero_kernel(uint32_t *__restrict__ p, int n) {
@@ -901,6 +930,7 @@ int ensure_hits(Scratch &sc, uint64_t need) {
 
 int ensure_vals(Scratch &sc, uint64_t need) {
   if (sc.val_cap >= need) return LM_OK;
+  drop_graphs(sc);  // a captured single-image sequence may write vals
   if (sc.vals) (void)hipFree(sc.vals);
   sc.vals = nullptr;
   sc.val_cap = 0;
@@ -1098,7 +1128,7 @@ int scratch_init(Scratch &s) {
   if (e == hipSuccess) e = hipMalloc(&s.pred, sizeof(Pred));
   if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
   if (e == hipSuccess)
-    e = hipHostMalloc(&s.host_cnt, kCounterWords * 4 + kHitPrefix * 8, hipHostMallocDefault);
+    e = hipHostMalloc(&s.host_cnt, kCounterWords * 4 + kHitPrefix * 16, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&s.ev0);
   if (e == hipSuccess) e = hipEventCreate(&s.ev1);
   if (e == hipSuccess) e = hipEventCreate(&s.ew0);
@@ -1335,6 +1365,9 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (rc) return rc;
   }
   const uint8_t *img = B > 1 ? sc.batch : d_imgs;
+  // a single image with imScale != 1: the float64 re-sum of its hits runs in
+  // the launch sequence (its values come back with the records)
+  const bool dev_exact = nimg == 1 && !bit(pred, 0);
   // interleave (B > 1), counter fill, score kernel: enqueued directly, or
   // replayed from a captured graph keyed by everything the arguments use
   auto sequence = [&](uint64_t cp) -> int {
@@ -1348,16 +1381,21 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kCounterWords), 0, st, sc.counters,
                        kCounterWords);
     if (kind == 3) return launch_nibble_score(db, sc, B, d_imgs, npx, cp, st);
-    if (kind == 1) return launch_banded(db, sc, img, cp, st);
     if (kind == 2) return launch_banded_batch(db, sc, B, img, cp, st);
-    return launch_score(db, sc, B, img, cp, st);
+    const int r = kind == 1 ? launch_banded(db, sc, img, cp, st) : launch_score(db, sc, B, img, cp, st);
+    if (r || !dev_exact) return r;
+    hipLaunchKernelGGL(exact_hits_kernel, dim3(64), dim3(256), 0, st, t.counts, t.tile_off,
+                       t.entries, sc.hits, cp, sc.counters, img, sc.pred, sc.vals);
+    const hipError_t le = hipGetLastError();
+    return le == hipSuccess ? LM_OK : hip_fail(le, "exact pass launch");
   };
   auto run = [&](uint64_t cp) -> int {
     if (!sc.use_graphs) return sequence(cp);
+    const int gkind = kind + (dev_exact ? 16 : 0);
     hipGraphExec_t exec = nullptr;
     for (auto &g : sc.graphs)
       if (g.db == db && g.generation == db->generation && g.src == d_imgs &&
-          g.hits == sc.hits && g.width == B && g.cap_per == cp && g.kind == kind)
+          g.hits == sc.hits && g.width == B && g.cap_per == cp && g.kind == gkind)
         exec = g.exec;
     if (!exec) {
       hipGraph_t graph = nullptr;
@@ -1372,7 +1410,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       if (ge == hipSuccess) ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (graph) (void)hipGraphDestroy(graph);
       if (ge != hipSuccess) return hip_fail(ge, "graph capture");
-      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, B, cp, kind, exec});
+      sc.graphs.push_back({db, d_imgs, sc.hits, db->generation, B, cp, gkind, exec});
     }
     ++sc.graph_launches;
     const hipError_t le = hipGraphLaunch(exec, st);
@@ -1381,11 +1419,13 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   float ms = 0, walk_ms = 0;
   std::vector<uint32_t> cnt(kCounterWords);
   uint64_t *const host_pre = (uint64_t *)(sc.host_cnt + kCounterWords);
+  double *const host_vals = (double *)(host_pre + kHitPrefix);
   // at most three runs: a candidate overflow switches to the table walk
   // once, a hit overflow regrows the hit buffer once
   bool grown = false;
   for (;;) {
     rc = ensure_hits(sc, capPer * (uint64_t)B);  // a width-B launch serves any nimg <= B
+    if (rc == LM_OK && dev_exact) rc = ensure_vals(sc, capPer);
     if (rc) return rc;
     (void)hipEventRecord(sc.ev0, st);
     sc.walk_timed = false;
@@ -1396,6 +1436,9 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     e = hipMemcpyAsync(sc.host_cnt, sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && nimg == 1)
       e = hipMemcpyAsync(host_pre, sc.hits, std::min(capPer, kHitPrefix) * 8,
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dev_exact)
+      e = hipMemcpyAsync(host_vals, sc.vals, std::min(capPer, kHitPrefix) * 8,
                          hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "score kernel");
@@ -1427,10 +1470,14 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   std::vector<uint64_t> first(nimg + 1, 0);
   for (int b = 0; b < nimg; ++b) first[b + 1] = first[b] + cnt[2 * b];
   std::vector<uint64_t> all(first[nimg]);
+  std::vector<double> dvals(dev_exact ? first[1] : 0);  // device order, as `all`
   if (nimg == 1 && first[1] && first[1] <= std::min(capPer, kHitPrefix)) {
     memcpy(all.data(), host_pre, first[1] * 8);  // already here
+    if (dev_exact) memcpy(dvals.data(), host_vals, first[1] * 8);
   } else if (nimg == 1 && first[1]) {
     e = hipMemcpyAsync(all.data(), sc.hits, first[1] * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dev_exact)
+      e = hipMemcpyAsync(dvals.data(), sc.vals, first[1] * 8, hipMemcpyDeviceToHost, st);
   } else if (first[nimg]) {
     rc = ensure_vals(sc, first[nimg]);
     if (rc) return rc;
@@ -1446,16 +1493,33 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   // images with imScale != 1: the float64 in-order sums of all their
   // candidates in one launch (candidate = image << 32 | row; the hit buffer
   // is free again once the records are on the host)
-  for (int b = 0; b < nimg; ++b) std::sort(all.begin() + first[b], all.begin() + first[b + 1]);
   std::vector<uint64_t> xids;
   std::vector<uint64_t> xfirst(nimg + 1, 0);
-  for (int b = 0; b < nimg; ++b) {
+  std::vector<double> xtot;
+  if (dev_exact) {
+    // sort the records with their device-computed sums
+    const uint64_t k = first[1];
+    std::vector<uint32_t> ord(k);
+    for (uint64_t i = 0; i < k; ++i) ord[i] = (uint32_t)i;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b2) { return all[a] < all[b2]; });
+    std::vector<uint64_t> sorted(k);
+    xtot.resize(k);
+    for (uint64_t i = 0; i < k; ++i) {
+      sorted[i] = all[ord[i]];
+      xtot[i] = dvals[ord[i]];
+    }
+    all.swap(sorted);
+    xfirst[1] = k;
+  } else {
+    for (int b = 0; b < nimg; ++b) std::sort(all.begin() + first[b], all.begin() + first[b + 1]);
+  }
+  for (int b = 0; b < nimg && !dev_exact; ++b) {
     if (!bit(pred, b))
       for (uint64_t i = first[b]; i < first[b + 1]; ++i)
         xids.push_back(((uint64_t)b << 32) | ((all[i] >> 26) & 0xffffffffull));
     xfirst[b + 1] = xids.size();
   }
-  std::vector<double> xtot(xids.size());
+  if (!dev_exact) xtot.resize(xids.size());
   if (!xids.empty()) {
     const uint64_t k = xids.size();
     rc = ensure_vals(sc, k);

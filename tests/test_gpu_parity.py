@@ -688,6 +688,21 @@ def test_graph_replay_single_image(scored_case, bits, loads):
             db.set_use_graphs(True)
         matched, ns, tot, nres = ob.compare(table, 64, imgs[0], p.NrPxX, 1.0, 5, 400.0)
         assert eager[0].nr_results == nres
+        # imScale != 1: the float64 re-sum of the hits is part of the
+        # captured sequence; alternate with imScale == 1 calls (no re-sum)
+        u8n, sn = api.quantize_u8(img * 0.57 + 0.02 * (img > 0))
+        assert sn != 1.0
+        matched, ns, tot, nres = ob.compare(table, 64, u8n, p.NrPxX, sn, 5, 400.0 * sn)
+        rows, mA = ob.collect(matched)
+        for rep in range(4):
+            _say(f"single-image graph replay, imScale {sn:.4f}, rep={rep}")
+            got = db.score(u8n, sn, 5, 400.0 * sn)
+            assert got.nr_results == nres
+            np.testing.assert_array_equal(got.ids, rows)
+            np.testing.assert_array_equal(got.score, mA)
+            np.testing.assert_array_equal(got.totint, tot[rows.astype(np.int64)])
+            one = db.score(imgs[rep], 1.0, 5, 400.0)
+            np.testing.assert_array_equal(one.ids, eager[rep].ids)
     finally:
         db.set_use_graphs(False)
         db.set_banded_tile_bits(14)
