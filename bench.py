@@ -35,10 +35,9 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(db, u8, im_scale, min_sp, min_int, sample, gpu_hits, budget_s=12.0):
+def cpu_baseline(db, u8, im_scale, min_sp, min_int, sample, budget_s=12.0):
     """The CPU oracle (C + OpenMP restatement of the reference DoFwd=0 loop)
-    timed on all host cores over a prefix of the same table and image; also
-    checks the GPU hits on that prefix bit for bit."""
+    timed on all host cores over a prefix of the same table and image."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
 
@@ -47,19 +46,54 @@ def cpu_baseline(db, u8, im_scale, min_sp, min_int, sample, gpu_hits, budget_s=1
     passes, t_cpu = 0, 0.0
     while t_cpu < budget_s and passes < 2000:
         t0 = time.perf_counter()
-        matched, ns, tot, nres = ob.compare(table, db.max_spots, u8, db.geom.nrPxX, im_scale,
-                                            min_sp, min_int, threads=cores)
+        ob.compare(table, db.max_spots, u8, db.geom.nrPxX, im_scale, min_sp, min_int,
+                   threads=cores)
         t_cpu += time.perf_counter() - t0
         passes += 1
-    rows, mA = ob.collect(matched)
-    sel = gpu_hits.ids < sample
-    ok = (np.array_equal(gpu_hits.ids[sel], rows) and np.array_equal(gpu_hits.score[sel], mA)
-          and np.array_equal(gpu_hits.nspots[sel], ns[rows.astype(np.int64)]))
     rate = sample * passes / t_cpu
     return {"value": rate, "unit": "orient×image pairs/s", "cores": cores, "kind": "port",
             "sample": f"first {sample} orientations of the same table and image, "
                       f"{passes} passes of the oracle compare (OpenMP, {cores} threads), "
-                      f"{t_cpu:.1f} s"}, ok
+                      f"{t_cpu:.1f} s"}
+
+
+def oracle_parity(db, cases, chunk=10_000_000):
+    """Every GPU result of the run against the CPU oracle over the WHOLE
+    table: the table is exported in chunks of `chunk` rows (ForwardFile
+    records), each chunk scored by the oracle for every case, and each case's
+    GPU hits in that row range compared bit for bit (ids, nspots, float64
+    totInt and score, nrResults).  cases: (name, u8, imScale, MinNrSpots,
+    MinIntensity, gpu Hits)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    cores = len(os.sched_getaffinity(0))
+    n = db.n
+    ok = {c[0]: True for c in cases}
+    nres = {c[0]: 0 for c in cases}
+    t0 = time.perf_counter()
+    for first in range(0, n, chunk):
+        cnt = min(chunk, n - first)
+        table = db.export_table(first, cnt)
+        for name, u8, s, ms, mi, h in cases:
+            matched, ns, tot, nr = ob.compare(table, db.max_spots, u8, db.geom.nrPxX, s, ms, mi,
+                                              threads=cores)
+            rows, mA = ob.collect(matched)
+            ri = rows.astype(np.int64)
+            sel = (h.ids >= first) & (h.ids < first + cnt)
+            ok[name] &= bool(np.array_equal(h.ids[sel], rows + np.uint64(first))
+                             and np.array_equal(h.score[sel], mA)
+                             and np.array_equal(h.nspots[sel], ns[ri])
+                             and np.array_equal(h.totint[sel], tot[ri]))
+            nres[name] += int(nr)
+        del table
+    for name, u8, s, ms, mi, h in cases:
+        ok[name] &= nres[name] == int(h.nr_results)
+    out = {"rows": int(n), "cases": len(cases), "all_equal": all(ok.values()),
+           "per_case": ok, "seconds": time.perf_counter() - t0}
+    log(f"oracle parity over all {n} rows: {sum(ok.values())}/{len(ok)} cases bit-exact "
+        f"({out['seconds']:.1f} s)")
+    return out
 
 
 def _batch_image(args):
@@ -86,9 +120,10 @@ def make_batch_images(count, n, db_seed, seed=1234, workers=16):
         return list(ex.map(_batch_image, jobs))
 
 
-def run_batches(db, batch_imgs, widths, p, steps, warmup):
+def run_batches(db, batch_imgs, widths, p, steps, warmup, parity_cases):
     """Scores batches of B images in one pass over the table; returns per-B
-    rates and checks each batched image against scoring it alone."""
+    rates, checks each batched image against scoring it alone and adds three
+    images per width to the oracle parity cases."""
     import torch
     from lauematching_amd import api
 
@@ -118,6 +153,9 @@ def run_batches(db, batch_imgs, widths, p, steps, warmup):
                    and one.nr_results == hs[i].nr_results)
         found = sum(set(rows.tolist()) <= set(hs[i].ids.tolist())
                     for i, (_, rows) in enumerate(batch_imgs[:B]))
+        for i in sorted({0, B // 2, B - 1}):
+            parity_cases.append((f"batch{B}_img{i}", u8s[i], scales[i], p.MinNrSpots,
+                                 p.MinIntensity, hs[i]))
         out.append({"images": B, "ms_per_batch": ms, "kernel_ms": float(np.mean(kms)),
                     "pairs_per_s": db.n * B / (ms / 1e3), "bit_exact_vs_single": bool(ok),
                     "images_with_all_grains_recovered": int(found),
@@ -433,6 +471,8 @@ def kernel_name(db):
     argument, the slow path for oversized chunks, depends on the table and
     is left as ``*``)."""
     tb, loads = db.banded_layout()
+    if tb == 13 and (loads == 3 or loads >= 5):
+        loads = 0  # no LDS walk at 2^13 rows: the byte-gather walk runs
     if tb == 0:
         return "score_kernel<1,false>"
     if ((loads == 3 or loads >= 5) and tb >= 14) or tb == 16:
@@ -579,6 +619,8 @@ def main():
         f"hits {len(h.ids)} "
         f"(nrResults {h.nr_results})")
     found = all(g in set(h.ids.tolist()) for g in grains)
+    # every result below is checked against the oracle over the whole table
+    parity_cases = [("single", u8, im_scale, p.MinNrSpots, p.MinIntensity, h)]
 
     # the same frame as a detector would give it (maximum not 255, so
     # imScale != 1): candidate superset from the integer walk, then the
@@ -595,6 +637,7 @@ def main():
     torch.cuda.synchronize()
     nonunit = {"im_scale": scale_n, "ms_per_step": (time.perf_counter() - t0) / args.steps * 1e3,
                "kernel_ms": hn.kernel_ms, "hits": int(len(hn.ids))}
+    parity_cases.append(("nonunit_scale", u8n, scale_n, p.MinNrSpots, p.MinIntensity * 0.731, hn))
     log(f"imScale {scale_n:.4f} frame: step {nonunit['ms_per_step']:.3f} ms, sequence "
         f"{hn.kernel_ms:.3f} ms, hits {len(hn.ids)}")
     del d_imgn
@@ -617,6 +660,7 @@ def main():
                    "ms_per_step": (time.perf_counter() - t0) / args.steps * 1e3,
                    "kernel_ms": hs_.kernel_ms, "walk_ms": hs_.walk_ms,
                    "nr_results": int(hs_.nr_results)}
+    parity_cases.append(("sparse", u8s, scale_s, 5, 50.0, hs_))
     log(f"sparse frame ({100 * sparse_line['nonzero_fraction']:.1f}% nonzero, MinNrSpots 5, "
         f"MinIntensity 50): step {sparse_line['ms_per_step']:.3f} ms, sequence "
         f"{hs_.kernel_ms:.3f} ms, {hs_.nr_results} results")
@@ -655,7 +699,8 @@ def main():
     batched = None
     if widths:
         db.set_use_graphs(True)
-        batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1)
+        batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1,
+                              parity_cases)
         db.set_use_graphs(False)
 
     latency = None
@@ -684,10 +729,10 @@ def main():
             f"{ctx.bytes_in_use() / 1e9:.1f} GB")
         # 70% 2048x2048, 30% 1024x1024; crystals alternate
         reqs = []
-        for c, d, img in serve_pool:
-            u8, s = api.quantize_u8(img)
+        for c, d, img_ in serve_pool:
+            u8_, s_ = api.quantize_u8(img_)
             reps = 7 if d == 2048 else 3
-            reqs += [(keys.index((c, d)), u8, s)] * reps
+            reqs += [(keys.index((c, d)), u8_, s_)] * reps
         serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [50, 200, 800, 1600], 3.0, 2048)
         serving["dbs"] = [{"crystal": c, "detector": d, "orientations": dbs[(c, d)].n}
                           for c, d in keys]
@@ -701,12 +746,12 @@ def main():
     if args.config4:
         c4 = config4(ctx, O, n, max(3, args.steps // 2), batch_imgs)
 
-    cpu, parity_ok = None, None
+    cpu, parity = None, None
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
-        cpu, parity_ok = cpu_baseline(db, u8, im_scale, p.MinNrSpots, p.MinIntensity, sample, h)
-        log(f"cpu baseline {cpu['value']:.3e} pairs/s ({cpu['cores']} cores); "
-            f"prefix parity {'ok' if parity_ok else 'MISMATCH'}")
+        cpu = cpu_baseline(db, u8, im_scale, p.MinNrSpots, p.MinIntensity, sample)
+        log(f"cpu baseline {cpu['value']:.3e} pairs/s ({cpu['cores']} cores)")
+        parity = oracle_parity(db, parity_cases)
 
     line = {
         "metric": "orient×image pairs/s at 100M orients + p50/p99 image latency vs plain GPU",
@@ -727,7 +772,9 @@ def main():
                    "max_spots": 64, "min_nr_spots": p.MinNrSpots,
                    "min_intensity": p.MinIntensity, "spots_per_orientation": st.n_spots / n,
                    "hits": int(len(h.ids)), "nr_results": int(h.nr_results),
-                   "grains_recovered": bool(found), "prefix_parity_vs_oracle": parity_ok,
+                   "grains_recovered": bool(found),
+                   "oracle_parity_rows": parity["rows"] if parity else 0,
+                   "oracle_parity_all_equal": parity["all_equal"] if parity else None,
                    "forward_ms": st.ms, "kernel_ms": kernel_ms, "walk_ms": walk_ms,
                    "hbm_copy_GBps": copy_gbps,
                    "single_kernel": single_kernel, "graph_replay": bool(args.graphs)},
@@ -739,6 +786,8 @@ def main():
         for b in batched:
             b["speedup_vs_plain"] = b["pairs_per_s"] / value
         line["batched"] = batched
+    if parity:
+        line["oracle_parity"] = parity
     if tune:
         line["autotune"] = tune
     if latency:

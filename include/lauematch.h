@@ -71,6 +71,8 @@ typedef struct lm_fwd_stats {
   uint64_t n_pairs;      /* (orientation, hkl) pairs evaluated                 */
   uint64_t n_recomputed; /* pairs recomputed in float64 (FP32_GUARD only)      */
   double ms;             /* device time of the simulation                      */
+  uint64_t n_projected;  /* pairs past the division-free pre-test, projected
+                            in full (FP64, or FP32 with the guard band)       */
 } lm_fwd_stats;
 
 /* Compact result of scoring one image against one DB. */
@@ -129,7 +131,8 @@ uint64_t lm_db_table_spots(const lm_db *db);
 /* Layout options (no reference counterpart; used by the autotuner).
  * LM_OPT_BANDED_TILE_BITS: orientations per super-tile of the pixel-banded
  * table copy used by single-image scoring, as log2 (13..16); 0 drops the
- * copy (scoring then reads the orientation-major table only).  Applies at
+ * copy (scoring then reads the orientation-major table only).  Default 16
+ * (15 when MaxNrLaueSpots > 257).  Applies at
  * once if a table exists.  16 (MaxNrLaueSpots <= 257 only) keeps a 16-bit
  * counter per row — the sum of its pixels, or for sparse images (under a
  * quarter of the pixels nonzero) with MinNrSpots >= 2 its count of nonzero
@@ -156,7 +159,7 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * 4-byte gather of its aligned word; 3, 5, 6, 7 = the LDS walk with 4, 8, 4, 8
  * entries per lane (8, 12, 8, 12 for 2^16 super-tiles) and 2, 2, 3, 3 chunk
  * sets in flight per wave.  2^16 super-tiles always take the LDS walk (3 for
- * 0, 1, 2, 4).  Default 0. */
+ * 0, 1, 2, 4).  Default 3. */
 #define LM_OPT_BANDED_LOADS 5
 /* LM_OPT_BATCH_GROUPED: bit 6 / bit 7 set = table batches of 64 / 128 images
  * give each orientation B/16 lanes of 16 images each (one 16-byte load per

@@ -30,7 +30,7 @@ class LmGeom(ctypes.Structure):
 class LmFwdStats(ctypes.Structure):
     _fields_ = [("n_orients", ctypes.c_uint64), ("n_spots", ctypes.c_uint64),
                 ("n_pairs", ctypes.c_uint64), ("n_recomputed", ctypes.c_uint64),
-                ("ms", ctypes.c_double)]
+                ("ms", ctypes.c_double), ("n_projected", ctypes.c_uint64)]
 
 
 class LmHits(ctypes.Structure):

@@ -58,6 +58,7 @@
 namespace {
 
 constexpr int kPort = 60517;  // src/LaueMatchingGPUStream.cu:42
+constexpr int kMaxBatch = 256;  // images per launch of the server
 constexpr uint32_t kV2Magic = 0x32464D4Cu;  // "LMF2"
 
 std::atomic<bool> g_running{true};
@@ -246,6 +247,62 @@ void accept_loop(Daemon *d, int lfd, bool v2) {
   }
 }
 
+// Sample frames for the autotuner: the spots of a few rows of the DB's own
+// table painted as 5 x 5 blobs on an empty detector (sparse, like
+// preprocessed frames), so the layouts are timed on this DB's geometry.
+void sample_frames(DbSlot &s, std::vector<std::vector<uint8_t>> &frames) {
+  const int M = s.p.max_spots, W = s.p.nrPxX, H = s.p.nrPxY;
+  std::vector<uint16_t> rec((size_t)(1 + 2 * M));
+  uint64_t x = 0x9e3779b97f4a7c15ull;
+  auto next = [&] {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  for (int f = 0; f < 3; ++f) {
+    std::vector<uint8_t> img((size_t)W * H, 0);
+    for (int g = 0; g < 5 && s.n; ++g) {
+      if (lm_db_export_table(s.db, next() % s.n, 1, rec.data()) != LM_OK) break;
+      for (int k = 0; k < rec[0]; ++k) {
+        const int cx = rec[1 + 2 * k], cy = rec[2 + 2 * k];
+        const int amp = 60 + (int)(next() % 196);
+        for (int dy = -2; dy <= 2; ++dy)
+          for (int dx = -2; dx <= 2; ++dx) {
+            const int px = cx + dx, py = cy + dy;
+            if (px < 0 || py < 0 || px >= W || py >= H) continue;
+            const int v = amp >> (std::abs(dx) + std::abs(dy));
+            uint8_t &t = img[(size_t)py * W + px];
+            if (v > t) t = (uint8_t)v;
+          }
+      }
+    }
+    frames.push_back(std::move(img));
+  }
+}
+
+// Times the scoring layouts on this DB (single image and batches up to the
+// server's width) and keeps the fastest; LM_STREAM_AUTOTUNE=0 keeps the
+// default layouts.
+int autotune_slot(DbSlot &s) {
+  const char *env = getenv("LM_STREAM_AUTOTUNE");
+  if (env && atoi(env) == 0) return 0;
+  std::vector<std::vector<uint8_t>> frames;
+  sample_frames(s, frames);
+  std::vector<const uint8_t *> ptrs;
+  for (auto &f : frames) ptrs.push_back(f.data());
+  lm_tune_report rep{};
+  const double t0 = omp_get_wtime();
+  if (lm_db_autotune(s.db, ptrs.data(), (int)ptrs.size(), kMaxBatch, 2, &rep) != LM_OK) {
+    fprintf(stderr, "FATAL: autotune: %s\n", lm_last_error());
+    return 1;
+  }
+  printf("Autotuned scoring layouts in %.1f s (single image: 2^%d-row super-tiles, walk %d)\n",
+         omp_get_wtime() - t0, rep.banded_tile_bits, rep.banded_loads);
+  fflush(stdout);
+  return 0;
+}
+
 // Loads DB k of the command line: parameters, orientations, hkls, the spot
 // table (ForwardFile or the GPU forward simulation) and its output files.
 int load_slot(lm_ctx *ctx, Daemon &d, int k, const char *par, const char *ori, const char *hkl) {
@@ -302,6 +359,7 @@ int load_slot(lm_ctx *ctx, Daemon &d, int k, const char *par, const char *ori, c
     if (!s->p.forward_file.empty())
       write_forward_file(s->p.forward_file.c_str(), s->db, s->n, s->p.max_spots);
   }
+  if (autotune_slot(*s) != 0) return 1;
   if (lm_server_add_db(d.server, s->db, &s->server_id) != LM_OK) {
     fprintf(stderr, "FATAL: %s\n", lm_last_error());
     return 1;
@@ -366,10 +424,13 @@ int main(int argc, char **argv) {
     fprintf(stderr, "FATAL: %s\n", lm_last_error());
     return 1;
   }
+  // frames admitted beyond the ones on the device wait in pinned slots; a
+  // sender blocks when all are taken (the reference bounds its queue at 32
+  // frames, src/LaueMatchingGPUStream.cu:45, with one image per launch)
   lm_server_config cfg{};
-  cfg.max_batch = 16;
+  cfg.max_batch = kMaxBatch;
   cfg.workers = 2;
-  cfg.max_inflight = 32;  // MAX_QUEUE_SIZE, src/LaueMatchingGPUStream.cu:45
+  cfg.max_inflight = 2 * kMaxBatch;
   cfg.use_graphs = 1;
   if (lm_server_create(ctx, &cfg, &d.server) != LM_OK) {
     fprintf(stderr, "FATAL: %s\n", lm_last_error());

@@ -286,6 +286,12 @@ int lm_db_create(lm_ctx *ctx, const double *orients, uint64_t n, const int32_t *
   db->n = n;
   db->nhkls = nhkls;
   db->max_spots = max_spots;
+  // single-image layout: the image staged through LDS over 2^16-row
+  // super-tiles with 16-bit sums (2^15 rows and 32-bit counters when
+  // 255 * MaxNrLaueSpots does not fit 16 bits)
+  db->banded_tb = max_spots <= kSum16MaxSpots ? kSum16TileBits : 15;
+  db->banded_loads = 3;
+  db->scratch.ctx = ctx;  // direct calls' buffers count against the budget
   Geom &g = db->geom;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {

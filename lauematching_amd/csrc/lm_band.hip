@@ -984,7 +984,9 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   // aligned image; the caller takes the table walk otherwise)
   const bool s16 = b.tb >= kSum16TileBits;
   const int L = s16 && !(db->banded_loads == 3 || db->banded_loads >= 5) ? 3 : db->banded_loads;
-  const bool lds_walk = L == 3 || L >= 5;
+  // (2^13-row super-tiles have no chunk offsets: the LDS walk settings read
+  // them with the byte-gather walk)
+  const bool lds_walk = (L == 3 || L >= 5) && b.tb >= 14;
   if (s16 && (((uintptr_t)img & 15) || !sc.cands)) {
     set_error("2^16-row walk needs a 16-byte aligned image and a candidate list");
     return LM_ERR_INVALID;
@@ -1063,7 +1065,7 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   }
   // 4-byte gathers read whole words of the image: it must be 4-byte aligned
   // and its size a multiple of 4 (else the byte-gather walk is used)
-  int loads = lds_walk ? 0 : L;
+  int loads = (L == 3 || L >= 5) ? 0 : L;
   if (loads == 4 && (((uintptr_t)img & 3) || (npx & 3))) loads = 0;
   const int key = b.tb * 10 + loads;
 #define LM_SBW(TB, TH)                                                                           \

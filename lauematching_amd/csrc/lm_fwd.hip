@@ -47,6 +47,12 @@ struct DevGeom {
   double W, H;
   double Elo, Ehi;
   int nrPxX;
+  // division-free pre-test (maybe_spot): q_z >= 0 rejects when Elo > 0;
+  // energy bounds times 4 pi; detector-coordinate intervals (metres, before
+  // the division by z) widened by a margin, used when pre_px is set
+  int pre_sign, pre_px;
+  double eloC, ehiC;
+  double xlo, xhi, ylo, yhi;
 };
 
 // Pixel from a detector coordinate: the reference keeps (int)t within
@@ -82,6 +88,53 @@ __device__ __forceinline__ bool project64(const double *M, double h, double k, d
   double E = kHc * qlen / (kFourPi * (-u2));
   if (E < g.Elo || E > g.Ehi) return false;
   *pix = (uint32_t)ipy * (uint32_t)g.nrPxX + (uint32_t)ipx;
+  return true;
+}
+
+// A necessary condition for project64 to accept (h, k, l), with no division
+// or square root: false only when the reference rejects the pair.  q and
+// |q|^2 are computed exactly as project64 computes them (same operations,
+// same order), and each test carries a margin far above the rounding error
+// of either side:
+//  * q_z >= 0: q-hat_z has the sign of q_z, so E = hc|q| / (4 pi (-q-hat_z))
+//    is negative or infinite and E < Elo rejects (used when Elo > 0);
+//  * energy: E = hc |q|^2 / (4 pi (-q_z)) mathematically; the reference's E
+//    is within 7e-16 relative of it (six roundings), this side's products
+//    within 5e-16, so hc|q|^2 < Elo 4 pi (-q_z) (1 - 1e-12) means E < Elo,
+//    and likewise above Ehi;
+//  * detector side: K = |q|^2 k_f = (-2 q_z q_x, -2 q_z q_y, |q|^2 - 2 q_z^2)
+//    (|K| = |q|^2), lab z' = (rotT K)_z = |q|^2 z with an error below 20u
+//    |q|^2 on both sides (u = 2^-53); z' < -1e-12 |q|^2 means z < 0;
+//  * pixel bounds, when z' > 1e-3 |q|^2: the reference keeps -1 < t < N for
+//    t = (x P2 / z - P0) / px + (N-1)/2, i.e. x P2 / z inside an interval of
+//    detector coordinates; its X = x P2 / z is within |P2| 4.4e-9 of the exact
+//    one for z >= 1e-3, and the interval is widened by |P2| 1e-7 + 1e-6 |px|,
+//    so x' P2 outside z' [xlo, xhi] means t is outside (-1, N).
+// Non-finite values make every comparison false: the pair goes to project64.
+__device__ __forceinline__ bool maybe_spot(const double *M, double h, double k, double l,
+                                           const DevGeom &g) {
+  double q0 = 0, q1 = 0, q2 = 0;
+  q0 += M[0] * h; q0 += M[1] * k; q0 += M[2] * l;
+  q1 += M[3] * h; q1 += M[4] * k; q1 += M[5] * l;
+  q2 += M[6] * h; q2 += M[7] * k; q2 += M[8] * l;
+  if (g.pre_sign && q2 >= 0) return false;
+  const double ql2 = q0 * q0 + q1 * q1 + q2 * q2;
+  if (ql2 == 0) return false;
+  const double nq = -q2;
+  if (nq > 1e-150) {
+    const double A = kHc * ql2;
+    if (A < g.eloC * nq * (1.0 - 1e-12) || A > g.ehiC * nq * (1.0 + 1e-12)) return false;
+  }
+  const double t = -2.0 * q2;
+  const double K0 = t * q0, K1 = t * q1, K2 = __builtin_fma(t, q2, ql2);
+  const double zz = __builtin_fma(g.rotT[6], K0, __builtin_fma(g.rotT[7], K1, g.rotT[8] * K2));
+  if (zz < -1e-12 * ql2) return false;
+  if (g.pre_px && zz > 1e-3 * ql2) {
+    const double xx = __builtin_fma(g.rotT[0], K0, __builtin_fma(g.rotT[1], K1, g.rotT[2] * K2));
+    const double yy = __builtin_fma(g.rotT[3], K0, __builtin_fma(g.rotT[4], K1, g.rotT[5] * K2));
+    const double xp = xx * g.P[2], yp = yy * g.P[2];
+    if (xp < zz * g.xlo || xp > zz * g.xhi || yp < zz * g.ylo || yp > zz * g.yhi) return false;
+  }
   return true;
 }
 
@@ -228,15 +281,33 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
 #pragma unroll
     for (int i = 0; i < 9; ++i) M32[i] = (float)M[i];
   }
-  uint32_t nrec = 0;
+  uint32_t nrec = 0, nproj = 0;
   uint32_t *col = LDS_LIST ? lst + threadIdx.x : spill + o;
   const uint64_t colStride = LDS_LIST ? (uint64_t)kFwdBlock : (uint64_t)gridDim.x * kFwdBlock;
   int ns = live ? 0 : maxSpots;
   if (FROMKEEP) ns = live ? (int)counts[o] : 0;
-  for (int hi = 0; hi < (FROMKEEP ? 0 : nhkls); ++hi) {
+  // hkls in chunks of 64: first the division-free pre-test of every hkl of
+  // the chunk (wave-uniform loop, scalar hkl loads) marks the candidates in
+  // a 64-bit mask, then each lane projects only its own candidates, in hkl
+  // order (ctz), so the wave runs as many projections as its busiest lane
+  // has candidates instead of one per hkl; first unseen pixel still wins.
+  for (int h0 = 0; h0 < (FROMKEEP ? 0 : nhkls); h0 += 64) {
     if (__all(ns >= maxSpots)) break;
-    const int hI = hkls[3 * hi + 0], kI = hkls[3 * hi + 1], lI = hkls[3 * hi + 2];
+    const int hn = min(64, nhkls - h0);
+    uint64_t cand = 0;
     if (ns < maxSpots) {
+      for (int j = 0; j < hn; ++j) {
+        const int hi = h0 + j;
+        const double hd = (double)hkls[3 * hi + 0], kd = (double)hkls[3 * hi + 1],
+                     ld = (double)hkls[3 * hi + 2];
+        if (maybe_spot(M, hd, kd, ld, g)) cand |= 1ull << j;
+      }
+    }
+    while (cand != 0 && ns < maxSpots) {
+      const int hi = h0 + __builtin_ctzll(cand);
+      cand &= cand - 1;
+      ++nproj;
+      const int hI = hkls[3 * hi + 0], kI = hkls[3 * hi + 1], lI = hkls[3 * hi + 2];
       uint32_t pix = 0;
       bool ok;
       if (GUARD) {
@@ -265,11 +336,17 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
     if (live) counts[o] = (uint16_t)ns;
     if (keep && live)
       for (int s = 0; s < ns; ++s) keep[(uint64_t)s * n + o] = col[(uint64_t)s * colStride];
-    if (GUARD) {
-      uint32_t w = nrec;
+    // recomputed[0]: pairs recomputed in float64; recomputed[1]: pairs that
+    // passed the pre-test and were projected
+    uint32_t w = nrec, c = nproj;
 #pragma unroll
-      for (int d = 32; d > 0; d >>= 1) w += __shfl_xor(w, d);
-      if (__lane_id() == 0 && w) atomicAdd(recomputed, (unsigned long long)w);
+    for (int d = 32; d > 0; d >>= 1) {
+      w += __shfl_xor(w, d);
+      c += __shfl_xor(c, d);
+    }
+    if (__lane_id() == 0) {
+      if (w) atomicAdd(recomputed, (unsigned long long)w);
+      if (c) atomicAdd(recomputed + 1, (unsigned long long)c);
     }
     return;
   }
@@ -427,6 +504,22 @@ DevGeom dev_geom(const Geom &g) {
   d.Elo = g.Elo;
   d.Ehi = g.Ehi;
   d.nrPxX = g.nrPxX;
+  // pre-test constants (maybe_spot): products only, so each side's
+  // rounding is a few units of 2^-53 against margins of 1e-12 and more
+  d.pre_sign = g.Elo > 0;
+  d.eloC = g.Elo * kFourPi;
+  d.ehiC = g.Ehi * kFourPi;
+  auto interval = [&](double P0, double px, double half, double N, double *lo, double *hi) {
+    const double a = P0 + (-1.0 - half) * px, b = P0 + (N - half) * px;
+    const double m = std::fabs(g.P[2]) * 1e-7 + std::fabs(px) * 1e-6 +
+                     1e-12 * (std::fabs(a) + std::fabs(b));
+    *lo = std::min(a, b) - m;
+    *hi = std::max(a, b) + m;
+    return std::isfinite(*lo) && std::isfinite(*hi) && px != 0;
+  };
+  const bool okx = interval(g.P[0], g.pxX, d.halfX, d.W, &d.xlo, &d.xhi);
+  const bool oky = interval(g.P[1], g.pxY, d.halfY, d.H, &d.ylo, &d.yhi);
+  d.pre_px = okx && oky && std::isfinite(g.P[2]);
   return d;
 }
 
@@ -568,8 +661,8 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
       db->ctx->in_use -= keepBytes;
     }
   }
-  hipError_t e = hipMalloc(&d_rec, 8);
-  if (e == hipSuccess) e = hipMemsetAsync(d_rec, 0, 8, st);
+  hipError_t e = hipMalloc(&d_rec, 16);
+  if (e == hipSuccess) e = hipMemsetAsync(d_rec, 0, 16, st);
   if (e == hipSuccess && !ldsList)
     e = hipMalloc(&spill, chunkBlocks * kFwdBlock * (uint64_t)db->max_spots * 4);
   if (e != hipSuccess) {
@@ -639,9 +732,9 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
     }
   }
   (void)hipEventRecord(e1, st);
-  unsigned long long nrec = 0;
+  unsigned long long nrec[2] = {0, 0};
   e = hipStreamSynchronize(st);
-  if (e == hipSuccess) e = hipMemcpy(&nrec, d_rec, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nrec, d_rec, 16, hipMemcpyDeviceToHost);
   if (spill) (void)hipFree(spill);
   if (keep) {
     (void)hipFree(keep);
@@ -668,8 +761,9 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
     stats->n_orients = db->n;
     stats->n_spots = t.total;
     stats->n_pairs = db->n * (uint64_t)db->nhkls;
-    stats->n_recomputed = nrec;
+    stats->n_recomputed = nrec[0];
     stats->ms = ms;
+    stats->n_projected = nrec[1];
   }
   return LM_OK;
 }

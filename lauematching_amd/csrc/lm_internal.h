@@ -136,10 +136,20 @@ struct Scratch {
   };
   std::vector<Graph> graphs;
   uint64_t graph_launches = 0;
+  // Device bytes of the buffers above, charged to ctx's HBM budget beyond
+  // `prepaid` (bytes the owner already reserved, e.g. a serving worker's
+  // share); null ctx = not charged.
+  lm_ctx *ctx = nullptr;
+  uint64_t bytes = 0;
+  uint64_t prepaid = 0;
 };
 
 int scratch_init(Scratch &s);
 void scratch_release(Scratch &s);
+// Worst-case bytes of the scratch buffers of scoring calls on db at widths up
+// to max_batch, as first allocated (the hit buffer grows past its initial
+// 2^16 records per image only for calls with more hits, charged then).
+uint64_t scratch_bytes_for(const lm_db *db, int max_batch);
 int scratch_images(Scratch &s, uint64_t bytes);
 // chunk flag buffer of the single-image LDS walk (at least n words)
 int ensure_chunk_flags(Scratch &s, uint64_t n);
@@ -165,8 +175,8 @@ struct lm_db {
   lm::Table table;
   lm::Banded banded;            // single-image copy
   lm::Banded banded_b;          // batch copy (smaller super-tiles, B x T LDS counters)
-  int banded_tb = 14;           // super-tile bits for the banded copy (0 = no copy)
-  int banded_loads = 0;         // single-image walk: 0 scalar x8, 1 16-byte x2, 2 16-byte x4
+  int banded_tb = 16;           // super-tile bits for the banded copy (0 = no copy)
+  int banded_loads = 3;         // single-image walk (LM_OPT_BANDED_LOADS)
   int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
   double width_ms[9] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)

@@ -916,52 +916,64 @@ void drop_graphs(Scratch &sc) {
   sc.graphs.clear();
 }
 
-int ensure_hits(Scratch &sc, uint64_t need) {
-  if (sc.hit_cap >= need) return LM_OK;
-  drop_graphs(sc);
-  if (sc.hits) (void)hipFree(sc.hits);
-  sc.hits = nullptr;
-  sc.hit_cap = 0;
-  hipError_t e = hipMalloc(&sc.hits, need * 8);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit buffer)");
-  sc.hit_cap = need;
+}  // namespace
+
+// Moves one scratch buffer from old_b to new_b bytes in the scratch's total
+// and charges the part beyond `prepaid` to the context's HBM budget.
+int scratch_account(Scratch &sc, uint64_t old_b, uint64_t new_b) {
+  const uint64_t before = sc.bytes, after = sc.bytes - old_b + new_b;
+  if (sc.ctx) {
+    const uint64_t cb = before > sc.prepaid ? before - sc.prepaid : 0;
+    const uint64_t ca = after > sc.prepaid ? after - sc.prepaid : 0;
+    std::lock_guard<std::mutex> lk(sc.ctx->mu);
+    if (ca > cb && sc.ctx->in_use + (ca - cb) > sc.ctx->budget) {
+      set_error("HBM budget exceeded by scoring buffers");
+      return LM_ERR_BUDGET;
+    }
+    sc.ctx->in_use = sc.ctx->in_use + ca - cb;
+  }
+  sc.bytes = after;
   return LM_OK;
 }
 
-int ensure_vals(Scratch &sc, uint64_t need) {
-  if (sc.val_cap >= need) return LM_OK;
-  drop_graphs(sc);  // a captured single-image sequence may write vals
-  if (sc.vals) (void)hipFree(sc.vals);
-  sc.vals = nullptr;
-  sc.val_cap = 0;
-  hipError_t e = hipMalloc(&sc.vals, need * 8);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(hit values)");
-  sc.val_cap = need;
+namespace {
+
+// Replaces *buf (cap elements of `elem` bytes) by a buffer of `need`
+// elements: captured sequences that used the old one are dropped, the bytes
+// charged first (LM_ERR_BUDGET leaves the old buffer in place).
+template <typename T>
+int regrow(Scratch &sc, T **buf, uint64_t *cap, uint64_t need, uint64_t elem, const char *what) {
+  if (*cap >= need) return LM_OK;
+  int rc = scratch_account(sc, *cap * elem, need * elem);
+  if (rc) return rc;
+  drop_graphs(sc);
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  hipError_t e = hipMalloc((void **)buf, need * elem);
+  if (e != hipSuccess) {
+    (void)scratch_account(sc, need * elem, 0);
+    return hip_fail(e, what);
+  }
+  *cap = need;
   return LM_OK;
+}
+
+int ensure_hits(Scratch &sc, uint64_t need) {
+  return regrow(sc, &sc.hits, &sc.hit_cap, need, 8, "hipMalloc(hit buffer)");
+}
+
+// (a captured single-image sequence may write vals)
+int ensure_vals(Scratch &sc, uint64_t need) {
+  return regrow(sc, &sc.vals, &sc.val_cap, need, 8, "hipMalloc(hit values)");
 }
 
 int ensure_cands(Scratch &sc, uint64_t need) {
-  if (sc.cand_cap >= need) return LM_OK;
-  drop_graphs(sc);
-  if (sc.cands) (void)hipFree(sc.cands);
-  sc.cands = nullptr;
-  sc.cand_cap = 0;
-  hipError_t e = hipMalloc(&sc.cands, need * 8);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(candidate rows)");
-  sc.cand_cap = need;
-  return LM_OK;
+  return regrow(sc, &sc.cands, &sc.cand_cap, need, 8, "hipMalloc(candidate rows)");
 }
 
 int ensure_batch(Scratch &sc, uint64_t need) {
-  if (sc.batch_cap >= need) return LM_OK;
-  drop_graphs(sc);
-  if (sc.batch) (void)hipFree(sc.batch);
-  sc.batch = nullptr;
-  sc.batch_cap = 0;
-  hipError_t e = hipMalloc(&sc.batch, need);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(interleaved batch)");
-  sc.batch_cap = need;
-  return LM_OK;
+  return regrow(sc, &sc.batch, &sc.batch_cap, need, 1, "hipMalloc(interleaved batch)");
 }
 
 int launch_score(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
@@ -1141,6 +1153,7 @@ int scratch_init(Scratch &s) {
 }
 
 void scratch_release(Scratch &s) {
+  (void)scratch_account(s, s.bytes, 0);
   for (auto &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
   if (s.hits) (void)hipFree(s.hits);
   if (s.counters) (void)hipFree(s.counters);
@@ -1156,30 +1169,44 @@ void scratch_release(Scratch &s) {
   if (s.ev1) (void)hipEventDestroy(s.ev1);
   if (s.ew0) (void)hipEventDestroy(s.ew0);
   if (s.ew1) (void)hipEventDestroy(s.ew1);
+  lm_ctx *ctx = s.ctx;
+  const uint64_t prepaid = s.prepaid;
   s = Scratch{};
+  s.ctx = ctx;  // the owner may use the scratch again
+  s.prepaid = prepaid;
 }
 
 int ensure_chunk_flags(Scratch &s, uint64_t n) {
-  if (s.chunk_nz_cap >= n) return LM_OK;
-  drop_graphs(s);
-  if (s.chunk_nz) (void)hipFree(s.chunk_nz);
-  s.chunk_nz = nullptr;
-  s.chunk_nz_cap = 0;
-  hipError_t e = hipMalloc(&s.chunk_nz, std::max<uint64_t>(n, 1) * 4);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(chunk flags)");
-  s.chunk_nz_cap = n;
-  return LM_OK;
+  return regrow(s, &s.chunk_nz, &s.chunk_nz_cap, std::max<uint64_t>(n, 1), 4,
+                "hipMalloc(chunk flags)");
 }
 
 int scratch_images(Scratch &s, uint64_t bytes) {
   if (s.img_cap >= bytes) return LM_OK;
+  // (the staging buffer is never captured in a graph)
+  int rc = scratch_account(s, s.img_cap, bytes);
+  if (rc) return rc;
   if (s.img) (void)hipFree(s.img);
   s.img = nullptr;
   s.img_cap = 0;
   hipError_t e = hipMalloc(&s.img, bytes);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc(image staging)");
+  if (e != hipSuccess) {
+    (void)scratch_account(s, bytes, 0);
+    return hip_fail(e, "hipMalloc(image staging)");
+  }
   s.img_cap = bytes;
   return LM_OK;
+}
+
+uint64_t scratch_bytes_for(const lm_db *db, int max_batch) {
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  const int B = batch_width(std::min(std::max(max_batch, 1), kMaxBatch));
+  uint64_t b = (1ull << 16) * 8 * (uint64_t)B;              // hit records
+  b += (uint64_t)8 * (1ull << 22);                           // candidate rows (their cap)
+  b += (1ull << 16) * 8;                                     // float64 sums
+  if (B > 1) b += npx * (uint64_t)std::min(B, kMaxByteBatch);  // interleaved batch
+  b += ((uint64_t)db->banded.nchunks + 1) * 4;               // chunk flags
+  return b;
 }
 
 int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,

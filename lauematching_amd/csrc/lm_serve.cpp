@@ -61,6 +61,7 @@ struct lm_server {
   std::vector<int> free_slots;
   uint64_t slot_bytes = 0;
   uint64_t reserved = 0;  // device bytes reserved from the context budget
+  uint64_t scratch_prepaid = 0;  // per worker: scoring buffers within `reserved`
   std::deque<lm::Request> pending;
   std::unordered_map<uint64_t, lm::Done> done;
   uint64_t next_ticket = 1;
@@ -118,6 +119,10 @@ void worker_loop(lm_server *s, int wid) {
   if (hipStreamCreate(&st) != hipSuccess) return;
   Scratch sc;
   sc.use_graphs = s->cfg.use_graphs != 0;
+  // the worker's share of the reservation made at server start covers its
+  // buffers as first allocated; growth beyond it is charged to the budget
+  sc.ctx = s->ctx;
+  sc.prepaid = s->scratch_prepaid;
   uint8_t *stage = nullptr;
   uint64_t stage_cap = 0;
   uint64_t seen_graphs = 0;
@@ -260,9 +265,13 @@ int lm_server_add_db(lm_server *s, lm_db *db, int32_t *db_id) {
 }
 
 static int server_start(lm_server *s) {
-  // device reservation per worker: staging + interleave buffers for a full
-  // batch of the largest detector, plus hit buffers
-  const uint64_t per_worker = 2 * s->max_npx * s->cfg.max_batch + (64ull << 20);
+  // device reservation per worker: the staging buffer for a full batch of
+  // the largest detector plus the worst case, over the DBs, of the scoring
+  // buffers (hit records, candidate rows, interleaved batch, chunk flags)
+  uint64_t scratch = 0;
+  for (const lm_db *db : s->dbs) scratch = std::max(scratch, scratch_bytes_for(db, s->cfg.max_batch));
+  s->scratch_prepaid = scratch;
+  const uint64_t per_worker = s->max_npx * s->cfg.max_batch + scratch;
   const uint64_t need = per_worker * s->cfg.workers;
   {
     std::lock_guard<std::mutex> lk(s->ctx->mu);

@@ -25,66 +25,6 @@
 #define LMO_HC_KEVNM 1.2398419739
 #define LMO_EPS 1E-12
 
-static int key_is(const char *line, const char *key) {
-  return strncmp(line, key, strlen(key)) == 0;
-}
-
-/* src/LaueMatchingCPU.c:60-209.  Keys are matched as prefixes of the line in
- * the reference's order; the first matching key wins for a line. */
-int lmo_parse_params(const char *path, lmo_params *p) {
-  FILE *f = fopen(path, "r");
-  if (!f) return -1;
-  memset(p, 0, sizeof(*p));
-  p->maxNrSpots = 500;
-  p->minNrSpots = 5;
-  p->doFwd = 1;
-  p->sg = 225;
-  p->Elo = 5;
-  p->Ehi = 30;
-  p->minIntensity = 1000.0;
-  p->maxAngle = 2.0;
-  char line[1000], tag[1000];
-  while (fgets(line, sizeof line, f)) {
-    if (key_is(line, "LatticeParameter"))
-      sscanf(line, "%s %lf %lf %lf %lf %lf %lf", tag, &p->lat[0], &p->lat[1], &p->lat[2],
-             &p->lat[3], &p->lat[4], &p->lat[5]);
-    else if (key_is(line, "P_Array"))
-      sscanf(line, "%s %lf %lf %lf", tag, &p->P[0], &p->P[1], &p->P[2]);
-    else if (key_is(line, "R_Array"))
-      sscanf(line, "%s %lf %lf %lf", tag, &p->R[0], &p->R[1], &p->R[2]);
-    else if (key_is(line, "tol_c_over_a")) {
-    } else if (key_is(line, "PxX"))
-      sscanf(line, "%s %lf", tag, &p->pxX);
-    else if (key_is(line, "PxY"))
-      sscanf(line, "%s %lf", tag, &p->pxY);
-    else if (key_is(line, "Elo"))
-      sscanf(line, "%s %lf", tag, &p->Elo);
-    else if (key_is(line, "Ehi"))
-      sscanf(line, "%s %lf", tag, &p->Ehi);
-    else if (key_is(line, "DoFwd"))
-      sscanf(line, "%s %d", tag, &p->doFwd);
-    else if (key_is(line, "NrPxX"))
-      sscanf(line, "%s %d", tag, &p->nrPxX);
-    else if (key_is(line, "NrPxY"))
-      sscanf(line, "%s %d", tag, &p->nrPxY);
-    else if (key_is(line, "MaxNrLaueSpots"))
-      sscanf(line, "%s %d", tag, &p->maxNrSpots);
-    else if (key_is(line, "BatchSize")) {
-    } else if (key_is(line, "MinNrSpots"))
-      sscanf(line, "%s %d", tag, &p->minNrSpots);
-    else if (key_is(line, "SpaceGroup"))
-      sscanf(line, "%s %d", tag, &p->sg);
-    else if (key_is(line, "MinIntensity"))
-      sscanf(line, "%s %lf", tag, &p->minIntensity);
-    else if (key_is(line, "MaxAngle"))
-      sscanf(line, "%s %lf", tag, &p->maxAngle);
-    else if (key_is(line, "ForwardFile"))
-      sscanf(line, "%s %999s", tag, p->forwardFile);
-  }
-  fclose(f);
-  return (p->nrPxX > 0 && p->nrPxY > 0 && p->lat[0] != 0.0) ? 0 : -2;
-}
-
 /* src/LaueMatchingCPU.c:224-239.  Rodrigues formula; each entry keeps the
  * reference's association: ((1-c)*a)*b +/- s*v. */
 void lmo_detector_rotT(const double r[3], double rotT[3][3]) {
@@ -140,18 +80,6 @@ void lmo_calc_recip(const double lat[6], int sgnum, double recip[3][3]) {
   recip[0][2] = zero_small((A[1] * B[2] - A[2] * B[1]) * pv);
   recip[1][2] = zero_small((A[2] * B[0] - A[0] * B[2]) * pv);
   recip[2][2] = zero_small((A[0] * B[1] - A[1] * B[0]) * pv);
-}
-
-void lmo_geom_from_params(const lmo_params *p, lmo_geom *g) {
-  lmo_calc_recip(p->lat, p->sg, g->recip);
-  lmo_detector_rotT(p->R, g->rotT);
-  for (int i = 0; i < 3; i++) g->P[i] = p->P[i];
-  g->pxX = p->pxX;
-  g->pxY = p->pxY;
-  g->nrPxX = p->nrPxX;
-  g->nrPxY = p->nrPxY;
-  g->Elo = p->Elo;
-  g->Ehi = p->Ehi;
 }
 
 /* src/LaueMatchingHeaders.h:349-357 */

@@ -33,31 +33,11 @@ typedef struct {
   double Elo, Ehi;           /* energy window, inclusive                    */
 } lmo_geom;
 
-/* Parameter-file subset that drives the hot path
- * (src/LaueMatchingCPU.c:60-209; defaults at :60-70). */
-typedef struct {
-  double lat[6];
-  int sg;
-  double P[3], R[3];
-  double pxX, pxY;
-  int nrPxX, nrPxY;
-  double Elo, Ehi;
-  int maxNrSpots, minNrSpots, doFwd;
-  double minIntensity, maxAngle;
-  char forwardFile[1000];
-} lmo_params;
-
-/* src/LaueMatchingCPU.c:60-209: prefix-matched keys, C defaults. 0 = ok. */
-int lmo_parse_params(const char *path, lmo_params *p);
-
 /* src/LaueMatchingCPU.c:224-239: Rodrigues rotation from R_Array, transposed. */
 void lmo_detector_rotT(const double rArr[3], double rotT[3][3]);
 
 /* src/LaueMatchingHeaders.h:379-436 (calcRecipArray, zeroOut at :359). */
 void lmo_calc_recip(const double lat[6], int spaceGroup, double recip[3][3]);
-
-/* Fill a geometry block from parsed parameters (recip + rotT on the host). */
-void lmo_geom_from_params(const lmo_params *p, lmo_geom *g);
 
 /* src/LaueMatchingHeaders.h:349-357 (MatrixMultF33): res = m * n. */
 void lmo_matmul33(const double m[3][3], const double n[3][3], double res[3][3]);
@@ -95,6 +75,23 @@ long lmo_compare_u8(const uint16_t *table, size_t n, int maxNrSpots,
  * Writes up to cap entries; returns the number of hits found. */
 long lmo_collect(const double *matched, size_t n, uint64_t *rowNrs, double *mA,
                  long cap);
+
+/* ---- merge of duplicate candidates (lm_merge_oracle.c) ---------------------- */
+
+/* OrientMat2Quat, src/LaueMatchingHeaders.h:262-296 (row-major matrix). */
+void lmo_quat_from_matrix(const double m[9], double q[4]);
+
+/* GetMisOrientation, src/LaueMatchingHeaders.h:248-260, with the point group
+ * of space group sg (MakeSymmetries :182-220); degrees. */
+double lmo_misorientation_deg(const double q1[4], const double q2[4], int sg);
+
+/* mergeDuplicateOrientations, src/LaueMatchingHeaders.h:949-1065: the cap at
+ * the 50000 best-scored candidates (qsort), float32 misorientations, greedy
+ * clustering in list order with `<= maxAngle` and the strict `>` best-score
+ * rule.  Outputs hold min(nr, 50000) entries; returns the cluster count. */
+int lmo_merge(const double *orients, const uint64_t *rows, const double *scores, int nr,
+              double maxAngle, int sg, int nthreads, uint64_t *best_row, double *best_score,
+              int32_t *cluster_size);
 
 #ifdef __cplusplus
 }

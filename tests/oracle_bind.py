@@ -128,3 +128,46 @@ def collect(matched):
     mA = np.zeros(k)
     lib().lmo_collect(_p(m), m.size, _p(rows), _p(mA), k)
     return rows, mA
+
+
+def _merge_lib():
+    L = lib()
+    if not getattr(L, "_merge_ready", False):
+        vp = ctypes.c_void_p
+        L.lmo_quat_from_matrix.restype = None
+        L.lmo_quat_from_matrix.argtypes = [vp, vp]
+        L.lmo_misorientation_deg.restype = ctypes.c_double
+        L.lmo_misorientation_deg.argtypes = [vp, vp, ctypes.c_int]
+        L.lmo_merge.restype = ctypes.c_int
+        L.lmo_merge.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_int, vp, vp, vp]
+        L._merge_ready = True
+    return L
+
+
+def quat_from_matrix(m):
+    q = np.zeros(4)
+    _merge_lib().lmo_quat_from_matrix(_p(np.ascontiguousarray(m, dtype=np.float64).ravel()), _p(q))
+    return q
+
+
+def misorientation(q1, q2, sg=225):
+    return _merge_lib().lmo_misorientation_deg(_p(np.ascontiguousarray(q1, dtype=np.float64)),
+                                               _p(np.ascontiguousarray(q2, dtype=np.float64)), sg)
+
+
+def merge(orients, rows, scores, max_angle, sg=225, threads=8):
+    """Oracle merge: (best_row, best_score, cluster_size) per cluster."""
+    o = np.ascontiguousarray(orients, dtype=np.float64).reshape(-1, 9)
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    s = np.ascontiguousarray(scores, dtype=np.float64)
+    nr = int(r.size)
+    if nr and int(r.max()) >= o.shape[0]:
+        raise ValueError("candidate row outside the orientation DB")
+    cap = max(1, min(nr, 50000))
+    best = np.zeros(cap, np.uint64)
+    bs = np.zeros(cap)
+    cs = np.zeros(cap, np.int32)
+    k = _merge_lib().lmo_merge(_p(o), _p(r), _p(s), nr, max_angle, sg, threads, _p(best),
+                               _p(bs), _p(cs))
+    return best[:k], bs[:k], cs[:k]
