@@ -292,6 +292,7 @@ int lm_db_create(lm_ctx *ctx, const double *orients, uint64_t n, const int32_t *
   db->banded_tb = max_spots <= kSum16MaxSpots ? kSum16TileBits : 15;
   db->banded_loads = 3;
   db->scratch.ctx = ctx;  // direct calls' buffers count against the budget
+  db->hkls.assign(hkls, hkls + (size_t)nhkls * 3);
   Geom &g = db->geom;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
@@ -333,6 +334,7 @@ void lm_db_destroy(lm_db *db) {
   (void)hipSetDevice(db->ctx->device);
   (void)hipStreamSynchronize(db->ctx->stream);
   release_table(db);
+  if (db->d_fwd_plan) (void)hipFree(db->d_fwd_plan);
   if (db->d_orients) (void)hipFree(db->d_orients);
   if (db->d_hkls) (void)hipFree(db->d_hkls);
   scratch_release(db->scratch);

@@ -36,6 +36,10 @@ namespace {
 
 constexpr int kFwdBlock = 128;
 constexpr int kLdsSlots = 64;  // per-lane duplicate list kept in LDS when M <= 64
+// With the kept lists in device memory (count pass, M <= 64), the first 32
+// slots of each list stay in LDS and the rest go straight to the kept list:
+// half the LDS per wave, twice the waves per CU.
+constexpr int kLdsSlotsKeep = 32;
 constexpr double kHc = 1.2398419739;  // src/LaueMatchingHeaders.h:41
 constexpr double kFourPi = 4 * 3.14159265358979323846;
 
@@ -47,7 +51,7 @@ struct DevGeom {
   double W, H;
   double Elo, Ehi;
   int nrPxX;
-  // division-free pre-test (maybe_spot): q_z >= 0 rejects when Elo > 0;
+  // division-free pre-test (plan_bit): q_z >= 0 rejects when Elo > 0;
   // energy bounds times 4 pi; detector-coordinate intervals (metres, before
   // the division by z) widened by a margin, used when pre_px is set
   int pre_sign, pre_px;
@@ -91,17 +95,29 @@ __device__ __forceinline__ bool project64(const double *M, double h, double k, d
   return true;
 }
 
-// A necessary condition for project64 to accept (h, k, l), with no division
-// or square root: false only when the reference rejects the pair.  q and
-// |q|^2 are computed exactly as project64 computes them (same operations,
-// same order), and each test carries a margin far above the rounding error
-// of either side:
-//  * q_z >= 0: q-hat_z has the sign of q_z, so E = hc|q| / (4 pi (-q-hat_z))
-//    is negative or infinite and E < Elo rejects (used when Elo > 0);
-//  * energy: E = hc |q|^2 / (4 pi (-q_z)) mathematically; the reference's E
-//    is within 7e-16 relative of it (six roundings), this side's products
-//    within 5e-16, so hc|q|^2 < Elo 4 pi (-q_z) (1 - 1e-12) means E < Elo,
-//    and likewise above Ehi;
+// One pre-test evaluation of the plan (built on the host, fwd_plan): the hkl
+// as doubles and the candidate bit (position in its 64-hkl chunk) for each
+// sign of q_z.  A Friedel pair (h and -h in the same chunk) shares one
+// evaluation: q(-h) is -q(h) exactly (IEEE products and sums round
+// symmetrically), so |q|^2, |q_z| and K below are the same for both, and
+// only the member with q_z < 0 can be accepted when Elo > 0.  bpos = 64
+// (no bit) for an hkl without its partner in the chunk.
+struct FwdEval {
+  double h, k, l;
+  uint64_t bits;  // bit when q_z < 0 (low word) / when q_z > 0 (high word); 64 = none
+};
+
+// A necessary condition for project64 to accept the member of the pair that
+// the sign of q_z selects, with no division or square root: false only when
+// the reference rejects it.  q and |q|^2 are computed exactly as project64
+// computes them (same operations, same order), and each test carries a
+// margin far above the rounding error of either side:
+//  * sign: q-hat_z has the sign of q_z, so E = hc|q| / (4 pi (-q-hat_z)) is
+//    negative or infinite for q_z >= 0 and E < Elo rejects (when Elo > 0);
+//  * energy: E = hc |q|^2 / (4 pi |q_z|) mathematically; the reference's E is
+//    within 7e-16 relative of it (six roundings), this side's products within
+//    7e-16, so hc|q|^2 < Elo 4 pi |q_z| (1 - 1e-12) means E < Elo, and
+//    likewise above Ehi;
 //  * detector side: K = |q|^2 k_f = (-2 q_z q_x, -2 q_z q_y, |q|^2 - 2 q_z^2)
 //    (|K| = |q|^2), lab z' = (rotT K)_z = |q|^2 z with an error below 20u
 //    |q|^2 on both sides (u = 2^-53); z' < -1e-12 |q|^2 means z < 0;
@@ -110,32 +126,36 @@ __device__ __forceinline__ bool project64(const double *M, double h, double k, d
 //    detector coordinates; its X = x P2 / z is within |P2| 4.4e-9 of the exact
 //    one for z >= 1e-3, and the interval is widened by |P2| 1e-7 + 1e-6 |px|,
 //    so x' P2 outside z' [xlo, xhi] means t is outside (-1, N).
-// Non-finite values make every comparison false: the pair goes to project64.
-__device__ __forceinline__ bool maybe_spot(const double *M, double h, double k, double l,
-                                           const DevGeom &g) {
-  double q0 = 0, q1 = 0, q2 = 0;
-  q0 += M[0] * h; q0 += M[1] * k; q0 += M[2] * l;
-  q1 += M[3] * h; q1 += M[4] * k; q1 += M[5] * l;
-  q2 += M[6] * h; q2 += M[7] * k; q2 += M[8] * l;
-  if (g.pre_sign && q2 >= 0) return false;
+// Non-finite values make every rejecting comparison false: the pair goes to
+// project64.  Returns the candidate bit, or 64 for none.  Branch-free.
+// (project64 starts each component from 0: 0 + x differs from x only in the
+// sign of a zero, which changes no comparison below, so the sums start from
+// the first product here.)
+__device__ __forceinline__ int plan_bit(const double *M, const FwdEval &ev, const DevGeom &g) {
+  const double q0 = (M[0] * ev.h + M[1] * ev.k) + M[2] * ev.l;
+  const double q1 = (M[3] * ev.h + M[4] * ev.k) + M[5] * ev.l;
+  const double q2 = (M[6] * ev.h + M[7] * ev.k) + M[8] * ev.l;
   const double ql2 = q0 * q0 + q1 * q1 + q2 * q2;
-  if (ql2 == 0) return false;
-  const double nq = -q2;
-  if (nq > 1e-150) {
-    const double A = kHc * ql2;
-    if (A < g.eloC * nq * (1.0 - 1e-12) || A > g.ehiC * nq * (1.0 + 1e-12)) return false;
-  }
+  const double nq = __builtin_fabs(q2);
+  const double A = kHc * ql2;
+  bool rej = ql2 == 0;
+  // eloC = Elo 4 pi (1 - 1e-12), ehiC = Ehi 4 pi (1 + 1e-12), rounded once more
+  rej |= (nq > 1e-150) & ((A < g.eloC * nq) | (A > g.ehiC * nq));
   const double t = -2.0 * q2;
   const double K0 = t * q0, K1 = t * q1, K2 = __builtin_fma(t, q2, ql2);
   const double zz = __builtin_fma(g.rotT[6], K0, __builtin_fma(g.rotT[7], K1, g.rotT[8] * K2));
-  if (zz < -1e-12 * ql2) return false;
-  if (g.pre_px && zz > 1e-3 * ql2) {
-    const double xx = __builtin_fma(g.rotT[0], K0, __builtin_fma(g.rotT[1], K1, g.rotT[2] * K2));
-    const double yy = __builtin_fma(g.rotT[3], K0, __builtin_fma(g.rotT[4], K1, g.rotT[5] * K2));
-    const double xp = xx * g.P[2], yp = yy * g.P[2];
-    if (xp < zz * g.xlo || xp > zz * g.xhi || yp < zz * g.ylo || yp > zz * g.yhi) return false;
-  }
-  return true;
+  const double xx = __builtin_fma(g.rotT[0], K0, __builtin_fma(g.rotT[1], K1, g.rotT[2] * K2));
+  const double yy = __builtin_fma(g.rotT[3], K0, __builtin_fma(g.rotT[4], K1, g.rotT[5] * K2));
+  rej |= zz < -1e-12 * ql2;
+  const double xp = xx * g.P[2], yp = yy * g.P[2];
+  const bool px = (g.pre_px != 0) & (zz > 1e-3 * ql2);
+  rej |= px & ((xp < zz * g.xlo) | (xp > zz * g.xhi) | (yp < zz * g.ylo) | (yp > zz * g.yhi));
+  // q_z == 0 (or NaN): the sign test rejects both members when Elo > 0
+  const int bneg = (int)(uint32_t)ev.bits, bpos = (int)(uint32_t)(ev.bits >> 32);
+  const int bzero = g.pre_sign ? 64 : bneg;
+  int bit = q2 > 0 ? bpos : bzero;
+  bit = q2 < 0 ? bneg : bit;
+  return rej ? 64 : bit;
 }
 
 // Float32 geometry for the guarded path.
@@ -251,15 +271,18 @@ __device__ __forceinline__ int project32(const float *M, int hi, int ki, int li,
 // KEEP (count pass): the spot list of orientation o is also stored in
 // keep[s * n + o], so the write pass (FROMKEEP) copies it instead of
 // projecting again.
-template <bool WRITE, bool LDS_LIST, bool GUARD, bool FROMKEEP = false>
+// SLOTS: list slots per lane kept in LDS (0, 32 or 64); slots past them live
+// in `keep` (SLOTS = 32, count pass) or in `spill` (SLOTS = 0, M > 64).
+template <bool WRITE, int SLOTS, bool GUARD, bool FROMKEEP = false>
 __global__ void __launch_bounds__(kFwdBlock)
 fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__restrict__ hkls,
            int nhkls, double r00, double r01, double r02, double r10, double r11, double r12,
            double r20, double r21, double r22, DevGeom g, DevGeom32 g32, int maxSpots,
            int tileBits, uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
            uint32_t *__restrict__ entries, uint32_t *__restrict__ spill,
-           unsigned long long *__restrict__ recomputed, uint32_t *__restrict__ keep) {
-  __shared__ uint32_t lst[LDS_LIST ? kLdsSlots * kFwdBlock : 1];
+           unsigned long long *__restrict__ recomputed, uint32_t *__restrict__ keep,
+           const FwdEval *__restrict__ plan, const int4 *__restrict__ chunks, int nchunks) {
+  __shared__ uint32_t lst[SLOTS ? SLOTS * kFwdBlock : 1];
   __shared__ uint32_t scan[kFwdBlock];
   const uint64_t o = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
   const bool live = o < n;
@@ -282,30 +305,65 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
     for (int i = 0; i < 9; ++i) M32[i] = (float)M[i];
   }
   uint32_t nrec = 0, nproj = 0;
-  uint32_t *col = LDS_LIST ? lst + threadIdx.x : spill + o;
-  const uint64_t colStride = LDS_LIST ? (uint64_t)kFwdBlock : (uint64_t)gridDim.x * kFwdBlock;
+  uint32_t *const colL = lst + threadIdx.x;  // slot s at colL[s * kFwdBlock], s < SLOTS
+  uint32_t *const colG = SLOTS == 0 ? spill + o : keep + (uint64_t)SLOTS * n + o;
+  const uint64_t gStride = SLOTS == 0 ? (uint64_t)gridDim.x * kFwdBlock : n;
+  auto get = [&](int s) -> uint32_t {
+    return s < SLOTS ? colL[s * kFwdBlock] : colG[(uint64_t)(s - SLOTS) * gStride];
+  };
+  auto put = [&](int s, uint32_t v) {
+    if (s < SLOTS)
+      colL[s * kFwdBlock] = v;
+    else
+      colG[(uint64_t)(s - SLOTS) * gStride] = v;
+  };
   int ns = live ? 0 : maxSpots;
   if (FROMKEEP) ns = live ? (int)counts[o] : 0;
-  // hkls in chunks of 64: first the division-free pre-test of every hkl of
-  // the chunk (wave-uniform loop, scalar hkl loads) marks the candidates in
-  // a 64-bit mask, then each lane projects only its own candidates, in hkl
-  // order (ctz), so the wave runs as many projections as its busiest lane
-  // has candidates instead of one per hkl; first unseen pixel still wins.
-  for (int h0 = 0; h0 < (FROMKEEP ? 0 : nhkls); h0 += 64) {
+  // hkls in chunks of 64 (chunks[c] = first hkl, first and last+1 plan
+  // entry): first the division-free pre-test of the chunk's plan entries
+  // (wave-uniform loop, scalar loads) marks the candidates in a 64-bit mask,
+  // then each lane projects only its own candidates, in hkl order (ctz), so
+  // the wave runs as many projections as its busiest lane has candidates
+  // instead of one per hkl; first unseen pixel still wins.
+  // Chunks are taken four at a time (256 hkls): the pre-test fills four
+  // masks, then each lane walks the union of its four masks in hkl order, so
+  // the wave runs as many projections as its busiest lane has candidates in
+  // the 256 hkls, rather than the sum over chunks of each chunk's busiest.
+  constexpr int kSuper = 4;
+  for (int c0 = 0; c0 < (FROMKEEP ? 0 : nchunks); c0 += kSuper) {
     if (__all(ns >= maxSpots)) break;
-    const int hn = min(64, nhkls - h0);
-    uint64_t cand = 0;
-    if (ns < maxSpots) {
-      for (int j = 0; j < hn; ++j) {
-        const int hi = h0 + j;
-        const double hd = (double)hkls[3 * hi + 0], kd = (double)hkls[3 * hi + 1],
-                     ld = (double)hkls[3 * hi + 2];
-        if (maybe_spot(M, hd, kd, ld, g)) cand |= 1ull << j;
+    uint64_t cand[kSuper];
+    int base[kSuper];
+#pragma unroll
+    for (int k = 0; k < kSuper; ++k) {
+      cand[k] = 0;
+      base[k] = 0;
+      if (c0 + k < nchunks) {
+        const int4 ch = chunks[c0 + k];
+        base[k] = ch.x;
+        uint64_t m = 0;
+#pragma unroll 2
+        for (int e = ch.y; e < ch.z; ++e) {
+          const int bit = plan_bit(M, plan[e], g);
+          m |= bit < 64 ? 1ull << bit : 0ull;
+        }
+        cand[k] = ns < maxSpots ? m : 0ull;
       }
     }
-    while (cand != 0 && ns < maxSpots) {
-      const int hi = h0 + __builtin_ctzll(cand);
-      cand &= cand - 1;
+    for (;;) {
+      // this lane's next candidate: lowest bit of its first nonempty mask
+      int hi = -1;
+#pragma unroll
+      for (int k = kSuper - 1; k >= 0; --k)
+        if (cand[k] != 0) hi = base[k] + __builtin_ctzll(cand[k]);
+      bool taken = false;
+#pragma unroll
+      for (int k = 0; k < kSuper; ++k)
+        if (!taken && cand[k] != 0) {
+          cand[k] &= cand[k] - 1;
+          taken = true;
+        }
+      if (hi < 0 || ns >= maxSpots) break;
       ++nproj;
       const int hI = hkls[3 * hi + 0], kI = hkls[3 * hi + 1], lI = hkls[3 * hi + 2];
       uint32_t pix = 0;
@@ -324,9 +382,9 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
       if (ok) {
         bool dup = false;
         for (int s = 0; s < ns; ++s)
-          if (col[(uint64_t)s * colStride] == pix) { dup = true; break; }
+          if (get(s) == pix) { dup = true; break; }
         if (!dup) {
-          col[(uint64_t)ns * colStride] = pix;
+          put(ns, pix);
           ++ns;
         }
       }
@@ -335,7 +393,8 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   if (!WRITE) {
     if (live) counts[o] = (uint16_t)ns;
     if (keep && live)
-      for (int s = 0; s < ns; ++s) keep[(uint64_t)s * n + o] = col[(uint64_t)s * colStride];
+      for (int s = 0; s < (SLOTS == kLdsSlotsKeep ? min(ns, SLOTS) : ns); ++s)
+        keep[(uint64_t)s * n + o] = get(s);
     // recomputed[0]: pairs recomputed in float64; recomputed[1]: pairs that
     // passed the pre-test and were projected
     uint32_t w = nrec, c = nproj;
@@ -378,7 +437,7 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   if (live) {
     uint32_t *dst = entries + tileOff[tile] + blockBase + excl;
     for (int s = 0; s < c; ++s)
-      dst[s] = FROMKEEP ? keep[(uint64_t)s * n + o] : col[(uint64_t)s * colStride];
+      dst[s] = FROMKEEP ? keep[(uint64_t)s * n + o] : get(s);
   }
 }
 
@@ -504,11 +563,11 @@ DevGeom dev_geom(const Geom &g) {
   d.Elo = g.Elo;
   d.Ehi = g.Ehi;
   d.nrPxX = g.nrPxX;
-  // pre-test constants (maybe_spot): products only, so each side's
+  // pre-test constants (plan_bit): products only, so each side's
   // rounding is a few units of 2^-53 against margins of 1e-12 and more
   d.pre_sign = g.Elo > 0;
-  d.eloC = g.Elo * kFourPi;
-  d.ehiC = g.Ehi * kFourPi;
+  d.eloC = g.Elo * kFourPi * (1.0 - 1e-12);
+  d.ehiC = g.Ehi * kFourPi * (1.0 + 1e-12);
   auto interval = [&](double P0, double px, double half, double N, double *lo, double *hi) {
     const double a = P0 + (-1.0 - half) * px, b = P0 + (N - half) * px;
     const double m = std::fabs(g.P[2]) * 1e-7 + std::fabs(px) * 1e-6 +
@@ -539,6 +598,66 @@ DevGeom32 dev_geom32(const Geom &g) {
   d.P0px = (float)(std::fabs(g.P[0]) / g.pxX);
   d.P1px = (float)(std::fabs(g.P[1]) / g.pxY);
   return d;
+}
+
+// The pre-test plan: hkls in chunks of 64 in file order; within a chunk each
+// hkl is evaluated once, together with the first unpaired -hkl after it in
+// the same chunk when Elo > 0 (only the member with q_z < 0 can then be
+// accepted).  Uploaded once per DB: plan entries, then one int4 per chunk
+// (first hkl, first entry, last entry + 1, 0).
+int fwd_plan(lm_db *db, hipStream_t st) {
+  if (db->d_fwd_plan) return LM_OK;
+  const int n = db->nhkls;
+  const bool pair = db->geom.Elo > 0;
+  std::vector<FwdEval> ev;
+  std::vector<int4> ch;
+  for (int s = 0; s < n; s += 64) {
+    const int e = std::min(n, s + 64);
+    int4 c;
+    c.x = s;
+    c.y = (int)ev.size();
+    bool used[64] = {};
+    for (int i = s; i < e; ++i) {
+      if (used[i - s]) continue;
+      used[i - s] = true;
+      const int32_t *a = &db->hkls[(size_t)3 * i];
+      int partner = -1;
+      for (int j = i + 1; pair && j < e && partner < 0; ++j) {
+        const int32_t *b = &db->hkls[(size_t)3 * j];
+        if (!used[j - s] && b[0] == -a[0] && b[1] == -a[1] && b[2] == -a[2] &&
+            (a[0] | a[1] | a[2]) != 0)
+          partner = j;
+      }
+      if (partner >= 0) used[partner - s] = true;
+      FwdEval f;
+      f.h = a[0];
+      f.k = a[1];
+      f.l = a[2];
+      // without the sign test (Elo <= 0) the hkl is a candidate either way
+      const uint64_t bneg = (uint64_t)(i - s);
+      const uint64_t bpos = partner >= 0 ? (uint64_t)(partner - s) : (pair ? 64u : bneg);
+      f.bits = bneg | (bpos << 32);
+      ev.push_back(f);
+    }
+    c.z = (int)ev.size();
+    c.w = 0;
+    ch.push_back(c);
+  }
+  const size_t eb = ev.size() * sizeof(FwdEval), cb = ch.size() * sizeof(int4);
+  void *d = nullptr;
+  hipError_t e = hipMalloc(&d, std::max<size_t>(eb + cb, 64));
+  if (e == hipSuccess && eb) e = hipMemcpyAsync(d, ev.data(), eb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && cb)
+    e = hipMemcpyAsync((char *)d + eb, ch.data(), cb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    if (d) (void)hipFree(d);
+    return hip_fail(e, "forward plan upload");
+  }
+  db->d_fwd_plan = d;
+  db->fwd_plan_entries = (int)ev.size();
+  db->fwd_plan_chunks = (int)ch.size();
+  return LM_OK;
 }
 
 void free_table(lm_db *db) {
@@ -633,9 +752,14 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
   const bool guard = mode == LM_FWD_FP32_GUARD;
   hipStream_t st = db->ctx->stream;
   if (db->table.counts) free_table(db);
-  int rc = alloc_table_index(db);
+  int rc = fwd_plan(db, st);
+  if (rc) return rc;
+  rc = alloc_table_index(db);
   if (rc) return rc;
   Table &t = db->table;
+  const FwdEval *plan = (const FwdEval *)db->d_fwd_plan;
+  const int4 *chunks = (const int4 *)(plan + db->fwd_plan_entries);
+  const int nch = db->fwd_plan_chunks;
   const DevGeom g = dev_geom(db->geom);
   const DevGeom32 g32 = dev_geom32(db->geom);
   const double *R = db->geom.recip;
@@ -704,24 +828,31 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
 #define LM_FWD(W, L, G)                                                                         \
   hipLaunchKernelGGL((fwd_kernel<W, L, G>), dim3(blocks), dim3(kFwdBlock), 0, st, op, rows,      \
                      db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], \
-                     g, g32, db->max_spots, t.tile_bits, cp, tp, t.entries, spill, d_rec, kp)
+                     g, g32, db->max_spots, t.tile_bits, cp, tp, t.entries, spill, d_rec, kp,     \
+                     plan, chunks, nch)
       const int sel = (pass << 2) | (ldsList << 1) | (guard ? 1 : 0);
-      if (pass == 1 && keep) {
-        // copy the kept lists (no projection and no list: LDS_LIST and GUARD
+      if (pass == 0 && keep && ldsList) {
+        // count pass keeping the lists: 32 LDS slots, the rest in `keep`
+        if (guard)
+          LM_FWD(false, kLdsSlotsKeep, true);
+        else
+          LM_FWD(false, kLdsSlotsKeep, false);
+      } else if (pass == 1 && keep) {
+        // copy the kept lists (no projection and no list: SLOTS and GUARD
         // do not matter, false keeps the copy kernel free of LDS)
-        hipLaunchKernelGGL((fwd_kernel<true, false, false, true>), dim3(blocks), dim3(kFwdBlock), 0,
+        hipLaunchKernelGGL((fwd_kernel<true, 0, false, true>), dim3(blocks), dim3(kFwdBlock), 0,
                            st, op, rows, db->d_hkls, db->nhkls, R[0], R[1], R[2], R[3], R[4],
                            R[5], R[6], R[7], R[8], g, g32, db->max_spots, t.tile_bits, cp, tp,
-                           t.entries, spill, d_rec, kp);
+                           t.entries, spill, d_rec, kp, plan, chunks, nch);
       } else switch (sel) {
-        case 0: LM_FWD(false, false, false); break;
-        case 1: LM_FWD(false, false, true); break;
-        case 2: LM_FWD(false, true, false); break;
-        case 3: LM_FWD(false, true, true); break;
-        case 4: LM_FWD(true, false, false); break;
-        case 5: LM_FWD(true, false, true); break;
-        case 6: LM_FWD(true, true, false); break;
-        default: LM_FWD(true, true, true); break;
+        case 0: LM_FWD(false, 0, false); break;
+        case 1: LM_FWD(false, 0, true); break;
+        case 2: LM_FWD(false, kLdsSlots, false); break;
+        case 3: LM_FWD(false, kLdsSlots, true); break;
+        case 4: LM_FWD(true, 0, false); break;
+        case 5: LM_FWD(true, 0, true); break;
+        case 6: LM_FWD(true, kLdsSlots, false); break;
+        default: LM_FWD(true, kLdsSlots, true); break;
       }
 #undef LM_FWD
       e = hipGetLastError();

@@ -172,6 +172,9 @@ struct lm_db {
   lm::Geom geom{};
   double *d_orients = nullptr;  // n * 9
   int32_t *d_hkls = nullptr;    // nhkls * 3
+  std::vector<int32_t> hkls;    // host copy (nhkls * 3), for the forward plan
+  void *d_fwd_plan = nullptr;   // forward pre-test plan: entries, then chunks
+  int fwd_plan_entries = 0, fwd_plan_chunks = 0;
   lm::Table table;
   lm::Banded banded;            // single-image copy
   lm::Banded banded_b;          // batch copy (smaller super-tiles, B x T LDS counters)
