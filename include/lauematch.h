@@ -245,6 +245,23 @@ int lm_score_f64(lm_db *db, const double *img_f64, int32_t min_nr_spots, double 
 int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t min_nr_spots,
                  double min_intensity, void *stream, lm_hits *out);
 
+/* lm_score_dev with no allocation: no hipMalloc, hipFree or other allocating
+ * call, for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  No counterpart in the reference.
+ * Every device buffer of the call is carved from a fixed 17.3 MiB array that
+ * each GPU holds from the library's load until the process ends (hit records
+ * up to 65536 per call, candidate rows up to 2^21, detectors of at most 2^27
+ * pixels; beyond these the call fails with LM_ERR_OVERFLOW / LM_ERR_INVALID
+ * and lm_score_dev serves).  Calls that overlap on one GPU share that array
+ * and race: the caller serialises them.  Results go to the caller's host
+ * arrays ids/nspots/totint/score of cap entries, ascending ids, identical to
+ * lm_score_dev's; cap must hold the call's candidate records (else
+ * LM_ERR_OVERFLOW with *count = the number needed).  No kernel timing. */
+int lm_score_dev_noalloc(lm_db *db, const uint8_t *d_img_u8, double im_scale,
+                         int32_t min_nr_spots, double min_intensity, void *stream, uint64_t cap,
+                         uint64_t *ids, int32_t *nspots, double *totint, double *score,
+                         uint64_t *count, uint64_t *nr_results);
+
 /* Score nimg (1..256) host images against the DB in one pass over the table.
  * outs[i] receives image i's hits, identical to lm_score on that image. */
 int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,

@@ -121,6 +121,12 @@ SIGNATURES = {
     "lm_score_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                     ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
                                     ctypes.POINTER(LmHits)]),
+    "lm_score_dev_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                            ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.POINTER(ctypes.c_uint64)]),
     "lm_score_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_void_p), c_double_p, ctypes.c_int32,
                                       ctypes.c_double, ctypes.POINTER(LmHits)]),

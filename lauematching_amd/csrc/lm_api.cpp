@@ -170,6 +170,9 @@ int lm_ctx_create(int device, uint64_t budget, lm_ctx **out) {
   size_t freeb = 0, totalb = 0;
   e = hipMemGetInfo(&freeb, &totalb);
   if (e != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+  // address of this device's copy of the no-allocation scoring scratch
+  int rc = noalloc_init(device);
+  if (rc) return rc;
   lm_ctx *c = new (std::nothrow) lm_ctx;
   if (!c) {
     set_error("out of host memory");
@@ -515,6 +518,22 @@ int lm_score_dev(lm_db *db, const uint8_t *d_img_u8, double im_scale, int32_t mi
                  double min_intensity, void *stream, lm_hits *out) {
   return lm_score_batch_dev(db, 1, d_img_u8, &im_scale, min_nr_spots, min_intensity, stream,
                             out);
+}
+
+int lm_score_dev_noalloc(lm_db *db, const uint8_t *d_img_u8, double im_scale,
+                         int32_t min_nr_spots, double min_intensity, void *stream, uint64_t cap,
+                         uint64_t *ids, int32_t *nspots, double *totint, double *score,
+                         uint64_t *count, uint64_t *nr_results) {
+  if (!db) {
+    set_error("lm_score_dev_noalloc: null db");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  hipStream_t st = stream ? (hipStream_t)stream : db->ctx->stream;
+  return score_noalloc(db, d_img_u8, im_scale, min_nr_spots, min_intensity, st, cap, ids, nspots,
+                       totint, score, count, nr_results);
 }
 
 int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,

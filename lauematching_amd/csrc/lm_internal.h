@@ -252,6 +252,14 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
 int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
               int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *out);
 
+// Scoring without allocation (lm_score_dev_noalloc): every device buffer is
+// carved from a file-scope device array whose address noalloc_init (called
+// by lm_ctx_create with the device current) looks up once per device.
+int noalloc_init(int device);
+int score_noalloc(lm_db *db, const uint8_t *d_img, double im_scale, int min_sp, double min_int,
+                  hipStream_t st, uint64_t cap, uint64_t *ids, int32_t *nspots, double *totint,
+                  double *score, uint64_t *count, uint64_t *nr_results);
+
 int width_log2(int B);       // smallest k with 2^k >= B
 int batch_width(int nimg);   // kernel width for nimg images: 1, 4, 8, 16, 32, 64
 

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1283,6 +1284,43 @@ int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
   return LM_OK;
 }
 
+// The device predicate of a call: per image its thresholds, imScale and the
+// integer candidate tests the kernels apply; entries past nimg never pass.
+static void build_pred(Pred &pred, int nimg, const double *im_scales, const int *min_spots,
+                       const double *min_int) {
+  memset(&pred, 0, sizeof pred);
+  pred.nimg = nimg;
+  for (int b = 0; b < kMaxBatch; ++b) {
+    pred.cand_thr[b] = INFINITY;
+    pred.nib_sum[b] = pred.nib_cnt[b] = 0x7fff;
+  }
+  for (int b = 0; b < nimg; ++b) {
+    const double s = im_scales[b];
+    pred.min_spots[b] = min_spots[b];
+    pred.min_int[b] = min_int[b];
+    pred.scale[b] = s;
+    if (s == 1.0) {
+      // totInt == sum exactly: sum >= min_int decided on the device
+      pred.cand_thr[b] = min_int[b];
+      pred.exact[b >> 5] |= 1u << (b & 31);
+    } else {
+      // superset: the sequential float64 sum is within n*2^-53 relative of sum*s
+      pred.cand_thr[b] = min_int[b] <= 0 ? -INFINITY : min_int[b] / s * (1.0 - 1e-12);
+    }
+    pred.pk_thr[b] = pack_threshold(min_spots[b], pred.cand_thr[b]);
+    // 4-bit test: a sum >= cand_thr is an integer >= smin = ceil(cand_thr),
+    // and 17 * sum(q) >= sum, so sum(q) >= ceil(smin / 17); sums never
+    // exceed 255 * 511 < 2^17
+    const double ct = pred.cand_thr[b];
+    if (ct <= 0)
+      pred.nib_sum[b] = 0;
+    else if (ct < 131072.0)
+      pred.nib_sum[b] = (uint16_t)std::min<uint32_t>(((uint32_t)std::ceil(ct) + 16u) / 17u, 0x7fffu);
+    pred.nib_cnt[b] = (uint16_t)std::min(std::max(min_spots[b], 0), 0x7fff);
+    if (0 >= min_spots[b] && 0.0 >= min_int[b]) pred.zero_ok[b >> 5] |= 1u << (b & 31);
+  }
+}
+
 int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
                  const double *im_scales, const int *min_spots, const double *min_int,
                  hipStream_t st, lm_hits *outs) {
@@ -1324,37 +1362,7 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   };
   if (B > kMaxByteBatch && !nib) return halves();
   Pred &pred = *sc.pred_host;
-  memset(&pred, 0, sizeof pred);
-  pred.nimg = nimg;
-  for (int b = 0; b < kMaxBatch; ++b) {
-    pred.cand_thr[b] = INFINITY;
-    pred.nib_sum[b] = pred.nib_cnt[b] = 0x7fff;
-  }
-  for (int b = 0; b < nimg; ++b) {
-    const double s = im_scales[b];
-    pred.min_spots[b] = min_spots[b];
-    pred.min_int[b] = min_int[b];
-    pred.scale[b] = s;
-    if (s == 1.0) {
-      // totInt == sum exactly: sum >= min_int decided on the device
-      pred.cand_thr[b] = min_int[b];
-      pred.exact[b >> 5] |= 1u << (b & 31);
-    } else {
-      // superset: the sequential float64 sum is within n*2^-53 relative of sum*s
-      pred.cand_thr[b] = min_int[b] <= 0 ? -INFINITY : min_int[b] / s * (1.0 - 1e-12);
-    }
-    pred.pk_thr[b] = pack_threshold(min_spots[b], pred.cand_thr[b]);
-    // 4-bit test: a sum >= cand_thr is an integer >= smin = ceil(cand_thr),
-    // and 17 * sum(q) >= sum, so sum(q) >= ceil(smin / 17); sums never
-    // exceed 255 * 511 < 2^17
-    const double ct = pred.cand_thr[b];
-    if (ct <= 0)
-      pred.nib_sum[b] = 0;
-    else if (ct < 131072.0)
-      pred.nib_sum[b] = (uint16_t)std::min<uint32_t>(((uint32_t)std::ceil(ct) + 16u) / 17u, 0x7fffu);
-    pred.nib_cnt[b] = (uint16_t)std::min(std::max(min_spots[b], 0), 0x7fff);
-    if (0 >= min_spots[b] && 0.0 >= min_int[b]) pred.zero_ok[b >> 5] |= 1u << (b & 31);
-  }
+  build_pred(pred, nimg, im_scales, min_spots, min_int);
   hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "predicate upload");
   uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)B, 1u << 16);
@@ -1600,6 +1608,222 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   if (rc)
     for (int b = 0; b < nimg; ++b) lm_hits_free(&outs[b]);
   return rc;
+}
+
+// ---- scoring that allocates nothing -----------------------------------------
+//
+// One image scored with every device buffer carved from `scratch`, a
+// file-scope device array each GPU holds from module load until the process
+// ends; host code uses the address hipGetSymbolAddress gives for the current
+// device (noalloc_init, called by lm_ctx_create), never the array's name.
+// Calls on one GPU share the array: overlapping calls on one GPU race.
+
+// Layout (256-byte aligned pieces): counters | predicate | chunk flags |
+// float64 sums | hit records | candidate rows.
+constexpr uint64_t kNaCounters = 4096;
+constexpr uint64_t kNaPred = (sizeof(Pred) + 255) / 256 * 256;
+constexpr uint64_t kNaChunks = 1u << 16;     // chunk flags of the LDS walk
+constexpr uint64_t kNaHits = 1u << 16;       // hit records (and float64 sums)
+constexpr uint64_t kNaCands = 1u << 21;      // candidate rows of the 16-bit walk
+constexpr uint64_t kNaOffPred = kNaCounters;
+constexpr uint64_t kNaOffChunks = kNaOffPred + kNaPred;
+constexpr uint64_t kNaOffVals = kNaOffChunks + kNaChunks * 4;
+constexpr uint64_t kNaOffHits = kNaOffVals + kNaHits * 8;
+constexpr uint64_t kNaOffCands = kNaOffHits + kNaHits * 8;
+constexpr uint64_t kNaBytes = kNaOffCands + kNaCands * 8;  // about 17.3 MiB
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void *g_na_ptr[kNaMaxDevices] = {};
+
+// Image 0's entries of the predicate; every other image never passes.
+struct PredOne {
+  double min_int, cand_thr, scale;
+  int min_spots;
+  uint32_t exact, pk_thr, zero_ok;
+  uint32_t nib_sum, nib_cnt;
+};
+
+// Writes a whole Pred from the kernel argument (no host staging buffer).
+__global__ void pred_one_kernel(PredOne p, Pred *__restrict__ d) {
+  for (int b = threadIdx.x; b < kMaxBatch; b += blockDim.x) {
+    const bool z = b == 0;
+    d->min_int[b] = z ? p.min_int : 0.0;
+    d->cand_thr[b] = z ? p.cand_thr : INFINITY;
+    d->scale[b] = z ? p.scale : 0.0;
+    d->min_spots[b] = z ? p.min_spots : 0;
+    d->pk_thr[b] = z ? p.pk_thr : 0u;
+    d->nib_sum[b] = z ? (uint16_t)p.nib_sum : (uint16_t)0x7fff;
+    d->nib_cnt[b] = z ? (uint16_t)p.nib_cnt : (uint16_t)0x7fff;
+    if (b < kMaxBatch / 32) {
+      d->exact[b] = z ? p.exact : 0u;
+      d->zero_ok[b] = z ? p.zero_ok : 0u;
+    }
+  }
+  if (threadIdx.x == 0) d->nimg = 1;
+}
+
+// In-place heap sort of keys with their values (no allocation).
+void sort_pairs(uint64_t *key, double *val, uint64_t n) {
+  auto sift = [&](uint64_t i, uint64_t m) {
+    for (;;) {
+      uint64_t c = 2 * i + 1;
+      if (c >= m) return;
+      if (c + 1 < m && key[c + 1] > key[c]) ++c;
+      if (key[i] >= key[c]) return;
+      std::swap(key[i], key[c]);
+      std::swap(val[i], val[c]);
+      i = c;
+    }
+  };
+  for (uint64_t i = n / 2; i-- > 0;) sift(i, n);
+  for (uint64_t m = n; m > 1; --m) {
+    std::swap(key[0], key[m - 1]);
+    std::swap(val[0], val[m - 1]);
+    sift(0, m - 1);
+  }
+}
+
+}  // namespace
+
+int noalloc_init(int device) {
+  if (device < 0 || device >= kNaMaxDevices) {
+    set_error("device index out of range for the scratch table");
+    return LM_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr[device]) return LM_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_na_ptr[device] = ptr;
+  return LM_OK;
+}
+
+int score_noalloc(lm_db *db, const uint8_t *d_img, double im_scale, int min_sp, double min_int,
+                  hipStream_t st, uint64_t cap, uint64_t *ids, int32_t *nspots, double *totint,
+                  double *score, uint64_t *count, uint64_t *nr_results) {
+  const Table &t = db->table;
+  if (!t.ready) {
+    set_error("score: no table (run lm_db_forward or lm_db_import_table)");
+    return LM_ERR_STATE;
+  }
+  if (!d_img || !count || !nr_results || (cap && (!ids || !nspots || !totint || !score))) {
+    set_error("lm_score_dev_noalloc: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  if (!(im_scale > 0) || !std::isfinite(im_scale)) {
+    set_error("score: imScale must be positive and finite");
+    return LM_ERR_INVALID;
+  }
+  if (db->banded.ready && db->banded.chunk_off && (uint64_t)db->banded.nchunks > kNaChunks) {
+    set_error("lm_score_dev_noalloc: the detector has more than 65536 pixel chunks "
+              "(at most 2^27 pixels); use lm_score_dev");
+    return LM_ERR_INVALID;
+  }
+  uint8_t *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    base = (uint8_t *)g_na_ptr[db->ctx->device];
+  }
+  if (!base) {
+    set_error("lm_score_dev_noalloc: scratch address not set up (lm_ctx_create)");
+    return LM_ERR_STATE;
+  }
+  Scratch sc;  // views into `scratch`; no events, no graphs, not charged
+  sc.counters = (uint32_t *)base;
+  sc.pred = (Pred *)(base + kNaOffPred);
+  sc.chunk_nz = (uint32_t *)(base + kNaOffChunks);
+  sc.chunk_nz_cap = kNaChunks;
+  sc.vals = (double *)(base + kNaOffVals);
+  sc.val_cap = kNaHits;
+  sc.hits = (uint64_t *)(base + kNaOffHits);
+  sc.hit_cap = kNaHits;
+  sc.cands = (uint64_t *)(base + kNaOffCands);
+  sc.cand_cap = std::min<uint64_t>(cand_capacity(db), kNaCands);
+  {
+    Pred pred;
+    build_pred(pred, 1, &im_scale, &min_sp, &min_int);
+    PredOne p1;
+    p1.min_int = pred.min_int[0];
+    p1.cand_thr = pred.cand_thr[0];
+    p1.scale = pred.scale[0];
+    p1.min_spots = pred.min_spots[0];
+    p1.exact = pred.exact[0];
+    p1.pk_thr = pred.pk_thr[0];
+    p1.zero_ok = pred.zero_ok[0];
+    p1.nib_sum = pred.nib_sum[0];
+    p1.nib_cnt = pred.nib_cnt[0];
+    hipLaunchKernelGGL(pred_one_kernel, dim3(1), dim3(256), 0, st, p1, sc.pred);
+  }
+  const bool exact = im_scale == 1.0;
+  const bool sum16 = db->banded.ready && db->banded.tb >= kSum16TileBits;
+  int kind = (db->banded.ready && (!sum16 || (min_int > 0 && ((uintptr_t)d_img & 15) == 0)))
+                 ? 1 : 0;
+  uint32_t cnt[kCounterWords];
+  for (;;) {
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kCounterWords), 0, st, sc.counters,
+                       kCounterWords);
+    int rc = kind == 1 ? launch_banded(db, sc, d_img, kNaHits, st)
+                       : launch_score(db, sc, 1, d_img, kNaHits, st);
+    if (rc) return rc;
+    if (!exact) {
+      hipLaunchKernelGGL(exact_hits_kernel, dim3(64), dim3(256), 0, st, t.counts, t.tile_off,
+                         t.entries, sc.hits, kNaHits, sc.counters, d_img, sc.pred, sc.vals);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(cnt, sc.counters, sizeof cnt, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "score kernel");
+    if (kind == 1 && sum16 && cnt[kCandSlot] > sc.cand_cap) {
+      kind = 0;  // more candidates than the list holds: the table walk
+      continue;
+    }
+    break;
+  }
+  const uint64_t k = cnt[0];
+  if (k > kNaHits) {
+    set_error("lm_score_dev_noalloc: more than 65536 candidate hits; use lm_score_dev");
+    return LM_ERR_OVERFLOW;
+  }
+  if (k > cap) {
+    *count = k;
+    set_error("lm_score_dev_noalloc: the output arrays hold fewer entries than the "
+              "candidate hits (count holds the number needed)");
+    return LM_ERR_OVERFLOW;
+  }
+  hipError_t e = hipSuccess;
+  if (k) e = hipMemcpyAsync(ids, sc.hits, k * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && k && !exact)
+    e = hipMemcpyAsync(totint, sc.vals, k * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "hit copy");
+  // ascending rows (records are row << 26 | nspots << 17 | sum), then the
+  // reference predicate on the float64 sums, compacted in place
+  if (exact)
+    std::sort(ids, ids + k);
+  else
+    sort_pairs(ids, totint, k);
+  uint64_t m = 0;
+  for (uint64_t i = 0; i < k; ++i) {
+    const uint64_t rec = ids[i];
+    const double tot = exact ? (double)(rec & kSumMask) : totint[i];
+    if (!(tot >= min_int)) continue;
+    const int ns = (int)((uint32_t)(rec & 0x3ffffffull) >> kSumBits);
+    ids[m] = (rec >> 26) & 0xffffffffull;
+    nspots[m] = ns;
+    totint[m] = tot;
+    score[m] = tot * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
+    ++m;
+  }
+  *count = m;
+  *nr_results = (uint64_t)cnt[1] + m;
+  return LM_OK;
 }
 
 }  // namespace lm
