@@ -400,6 +400,7 @@ typedef struct lm_prep_result {
   int32_t n_kept;        /* components kept                                    */
   double im_scale;       /* imScale of the uint8 image                         */
   double ms;             /* device time of the whole pipeline                  */
+  uint64_t n_above;      /* pixels above the threshold (the ones it keeps)     */
 } lm_prep_result;
 /* raw and background: nrPxY x nrPxX float64 host images (background may be
  * NULL); the uint8 image goes to u8_host and/or d_u8 (a device buffer of the

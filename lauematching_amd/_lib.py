@@ -56,7 +56,8 @@ class LmPrepConfig(ctypes.Structure):
 class LmPrepResult(ctypes.Structure):
     _fields_ = [("threshold", ctypes.c_double), ("sigma", ctypes.c_double),
                 ("n_components", ctypes.c_int32), ("n_kept", ctypes.c_int32),
-                ("im_scale", ctypes.c_double), ("ms", ctypes.c_double)]
+                ("im_scale", ctypes.c_double), ("ms", ctypes.c_double),
+                ("n_above", ctypes.c_uint64)]
 
 
 class LmServerConfig(ctypes.Structure):

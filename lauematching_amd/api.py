@@ -378,7 +378,7 @@ def preprocess(ctx: "Context", raw: np.ndarray, background: np.ndarray | None = 
                               ctypes.byref(cfg), _ptr(u8), ctypes.c_void_p(d_u8 or None),
                               None if blurred is None else _ptr(blurred), ctypes.byref(res)))
     info = {"threshold": res.threshold, "sigma": res.sigma, "n_components": res.n_components,
-            "n_kept": res.n_kept, "ms": res.ms}
+            "n_kept": res.n_kept, "ms": res.ms, "n_above": res.n_above}
     if want_blurred:
         return u8, res.im_scale, info, blurred
     return u8, res.im_scale, info

@@ -128,14 +128,18 @@ bool recv_all(int fd, void *buf, size_t len) {
   return true;
 }
 
-// Quantize one frame (src/LaueMatchingCPU.c:355-377), submit it to the
-// server for DB k and queue it for post-processing.
+// Quantize one frame (src/LaueMatchingCPU.c:355-377) from its float64
+// pixels, submit it to the server for DB k and queue it for post-processing
+// (which uses the float32 copy, as the reference daemon's fit does).
+// have_f64: f64 already holds the frame's float64 pixels (wire v2 type 1);
+// otherwise they are the float32 pixels widened.
 bool submit_frame(Daemon *d, int k, uint32_t num, std::vector<float> &img,
-                  std::vector<double> &f64, std::vector<uint8_t> &u8) {
+                  std::vector<double> &f64, std::vector<uint8_t> &u8, bool have_f64 = false) {
   DbSlot &s = *d->dbs[k];
   f64.resize(s.npx);
   u8.resize(s.npx);
-  for (size_t i = 0; i < s.npx; ++i) f64[i] = img[i];
+  if (!have_f64)
+    for (size_t i = 0; i < s.npx; ++i) f64[i] = img[i];
   double scale = 1.0;
   lm_quantize_u8(f64.data(), s.npx, u8.data(), &scale);
   uint64_t ticket = 0;
@@ -201,16 +205,17 @@ void serve_v2(Daemon *d, int fd) {
     raw.resize(npx * kSize[type]);
     if (!recv_all(fd, raw.data(), raw.size())) break;
     img.resize(npx);
+    if (type == 1) f64.resize(npx);
     for (size_t i = 0; i < npx; ++i) {
       const uint8_t *q = raw.data() + i * kSize[type];
       switch (type) {
         case 0: { float v; memcpy(&v, q, 4); img[i] = v; break; }
-        case 1: { double v; memcpy(&v, q, 8); img[i] = (float)v; break; }
+        case 1: { double v; memcpy(&v, q, 8); f64[i] = v; img[i] = (float)v; break; }
         case 2: img[i] = (float)(q[0] | (q[1] << 8)); break;
         default: img[i] = (float)q[0]; break;
       }
     }
-    if (!submit_frame(d, k, num, img, f64, u8)) break;
+    if (!submit_frame(d, k, num, img, f64, u8, type == 1)) break;
   }
 }
 

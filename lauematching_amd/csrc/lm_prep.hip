@@ -76,13 +76,19 @@ __global__ void keys_kernel(const double *__restrict__ v, uint64_t npx, int mode
 }
 
 // 3. threshold and uint16 cast (C conversion through a 64-bit integer)
+// (*above counts the pixels > t, thresholds.py:29-32 keeps exactly those)
 __global__ void threshold_kernel(const double *__restrict__ v, uint64_t npx, double t,
-                                 uint16_t *__restrict__ u16) {
+                                 uint16_t *__restrict__ u16, unsigned long long *above) {
+  unsigned long long c = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const double x = v[i];
+    c += x > t;
     u16[i] = x > t ? (uint16_t)(int64_t)x : (uint16_t)0;
   }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+  if (__lane_id() == 0 && c) atomicAdd(above, c);
 }
 
 // 4. connected components
@@ -331,7 +337,7 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   if (e == hipSuccess) e = B.get(&d_v, npx);
   if (e == hipSuccess) e = B.get(&d_u16, npx);
   if (e == hipSuccess) e = B.get(&d_keys, npx);
-  if (e == hipSuccess) e = B.get(&d_cnt, 2);
+  if (e == hipSuccess) e = B.get(&d_cnt, 3);  // [2]: pixels above the threshold
   if (e == hipSuccess && !d_u8) e = B.get(&d_u8, npx);
   if (e == hipSuccess) e = hipEventCreate(&ev0);
   if (e == hipSuccess) e = hipEventCreate(&ev1);
@@ -403,7 +409,10 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
     return LM_ERR_INVALID;
   }
   res.threshold = t;
-  hipLaunchKernelGGL(threshold_kernel, dim3(g), dim3(kThreads), 0, st, d_v, npx, t, d_u16);
+  if (e == hipSuccess) e = hipMemsetAsync(d_cnt + 2, 0, 8, st);
+  if (e != hipSuccess) return hip_fail(e, "threshold");
+  hipLaunchKernelGGL(threshold_kernel, dim3(g), dim3(kThreads), 0, st, d_v, npx, t, d_u16,
+                     d_cnt + 2);
 
   // 4. components
   e = B.get(&d_p, npx);
@@ -415,10 +424,13 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   hipLaunchKernelGGL(cc_roots_kernel, dim3(g), dim3(kThreads), 0, st, d_p, npx, d_cid,
                      (unsigned int *)d_cnt);
   unsigned int ncomp = 0;
+  unsigned long long above = 0;
   e = hipMemcpyAsync(&ncomp, d_cnt, 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&above, d_cnt + 2, 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "components");
   res.n_components = (int32_t)ncomp;
+  res.n_above = above;
   std::vector<uint8_t> keep(ncomp ? ncomp : 1, 0);
   std::vector<std::pair<double, double>> centres;
   if (ncomp) {

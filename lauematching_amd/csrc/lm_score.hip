@@ -910,6 +910,10 @@ __global__ void zero_kernel(uint32_t *__restrict__ p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// Captured launch sequences kept per scratch (one per DB, source buffer,
+// width and capacity in use).
+constexpr size_t kMaxGraphs = 32;
+
 // Captured sequences reference the scratch buffers; drop them before any of
 // those buffers is reallocated.
 void drop_graphs(Scratch &sc) {
@@ -1419,7 +1423,10 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (kind == 2) return launch_banded_batch(db, sc, B, img, cp, st);
     const int r = kind == 1 ? launch_banded(db, sc, img, cp, st) : launch_score(db, sc, B, img, cp, st);
     if (r || !dev_exact) return r;
-    hipLaunchKernelGGL(exact_hits_kernel, dim3(64), dim3(256), 0, st, t.counts, t.tile_off,
+    // one thread per record the buffer can hold (the count is only known on
+    // the device; a captured sequence keeps this grid for its capacity)
+    const unsigned xg = (unsigned)std::min<uint64_t>((cp + 255) / 256, 65535);
+    hipLaunchKernelGGL(exact_hits_kernel, dim3(xg), dim3(256), 0, st, t.counts, t.tile_off,
                        t.entries, sc.hits, cp, sc.counters, img, sc.pred, sc.vals);
     const hipError_t le = hipGetLastError();
     return le == hipSuccess ? LM_OK : hip_fail(le, "exact pass launch");
@@ -1428,11 +1435,31 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (!sc.use_graphs) return sequence(cp);
     const int gkind = kind + (dev_exact ? 16 : 0);
     hipGraphExec_t exec = nullptr;
-    for (auto &g : sc.graphs)
-      if (g.db == db && g.generation == db->generation && g.src == d_imgs &&
-          g.hits == sc.hits && g.width == B && g.cap_per == cp && g.kind == gkind)
+    // sequences captured on an older table or layout of this DB can never
+    // be replayed again: drop them; the hit is moved to the back (recently
+    // used), and past kMaxGraphs the least recently used is dropped
+    for (size_t i = 0; i < sc.graphs.size();) {
+      if (sc.graphs[i].db == db && sc.graphs[i].generation != db->generation) {
+        (void)hipGraphExecDestroy(sc.graphs[i].exec);
+        sc.graphs.erase(sc.graphs.begin() + i);
+      } else {
+        ++i;
+      }
+    }
+    for (size_t i = 0; i < sc.graphs.size(); ++i) {
+      const auto &g = sc.graphs[i];
+      if (g.db == db && g.src == d_imgs && g.hits == sc.hits && g.width == B && g.cap_per == cp &&
+          g.kind == gkind) {
         exec = g.exec;
+        std::rotate(sc.graphs.begin() + i, sc.graphs.begin() + i + 1, sc.graphs.end());
+        break;
+      }
+    }
     if (!exec) {
+      if (sc.graphs.size() >= kMaxGraphs) {
+        (void)hipGraphExecDestroy(sc.graphs.front().exec);
+        sc.graphs.erase(sc.graphs.begin());
+      }
       hipGraph_t graph = nullptr;
       hipError_t ge = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
       if (ge != hipSuccess) return hip_fail(ge, "hipStreamBeginCapture");

@@ -106,3 +106,23 @@ def test_preprocessed_image_scores_from_device(ctx):
     np.testing.assert_array_equal(a.ids, b.ids)
     np.testing.assert_array_equal(a.score, b.score)
     db.close()
+
+
+@pytest.mark.parametrize("method", ["adaptive", "percentile", "fixed"])
+def test_preprocess_threshold_matches_reference_goldens(ctx, method):
+    """The device threshold and the count of pixels it keeps, on the seeded
+    frames of the reference's threshold characterization, equal the
+    reference's goldens (and the oracle's threshold bit for bit)."""
+    from test_prep_goldens import FRAMES, METHODS, frame, golden
+
+    want = golden(method)
+    kw = METHODS[method]
+    for name, fkw in FRAMES.items():
+        img = frame(**fkw)
+        _say(f"preprocess golden frame {name} {method}")
+        u8, scale, info = api.preprocess(ctx, img, None, kw["method"], kw["fixed_value"],
+                                         kw["percentile"], 10, 0.2, 513.0, 0.4)
+        t = po.threshold_value(img, kw["method"], kw["fixed_value"], kw["percentile"])
+        assert info["threshold"] == t, (name, info["threshold"], t)
+        assert round(info["threshold"], 6) == want[name]["thresh"], name
+        assert info["n_above"] == want[name]["nonzero"], name

@@ -139,8 +139,9 @@ def _v2_frame(num, db, img, dtype):
 @pytest.mark.gpu
 def test_stream_daemon_two_dbs_and_wire_v2(tmp_path):
     """Two DBs (different detectors and orientation sets) in one daemon: v1
-    frames to DB 0, v2 frames (float64, uint16, uint8, float32) to both; a
-    malformed v2 header closes only its own connection.  Every result row
+    frames to DB 0, v2 frames (float64 -- with values float32 cannot hold --,
+    uint16, uint8, float32) to both; a malformed v2 header closes only its
+    own connection.  Every result row
     equals the single-image program's for the same image and DB, in
     solutions.txt (DB 0) and solutions_db1.txt (DB 1)."""
     import signal
@@ -168,9 +169,16 @@ def test_stream_daemon_two_dbs_and_wire_v2(tmp_path):
         return np.floor(im * (60000.0 / im.max())) if ints == 16 else (
             np.floor(im * (250.0 / im.max())) if ints == 8 else im.astype(np.float32))
 
+    # a float64 frame float32 cannot hold: integers 1..254 lowered by 1e-9
+    # (maximum exactly 255, so imScale is 1), which truncate one level lower
+    # in float64 than after a float32 rounding
+    f64 = synth.synth_image(p0, hk, O0[rng.choice(len(O0), 3, replace=False)], seed=2)
+    f64 = np.floor(f64 * (255.0 / f64.max()))
+    f64 = f64 - 1e-9 * ((f64 > 0) & (f64 < 255))
+    assert not np.array_equal(f64.astype(np.float32).astype(np.float64), f64)
     sends = [  # (wire, number, db, image, dtype)
         ("v1", 1, 0, img0.astype(np.float32), np.float32),
-        ("v2", 2, 0, frame_for(p0, O0, 2, 0).astype(np.float64), np.float64),
+        ("v2", 2, 0, f64, np.float64),
         ("v2", 3, 1, frame_for(p1, O1, 3, 16), np.uint16),
         ("v2", 4, 1, frame_for(p1, O1, 4, 0), np.float32),
         ("v2", 5, 0, frame_for(p0, O0, 5, 8), np.uint8),
@@ -227,7 +235,8 @@ def test_stream_daemon_two_dbs_and_wire_v2(tmp_path):
     for wire, num, db, im, dt in sends:
         base = tmp_path if db == 0 else d1
         p = p0 if db == 0 else p1
-        synth.write_image(str(base / "image.bin"), im.astype(np.float32).astype(np.float64))
+        synth.write_image(str(base / "image.bin"),
+                          im if dt == np.float64 else im.astype(np.float32).astype(np.float64))
         p.write(str(base / "params.txt"))
         r = subprocess.run([EXE, str(base / "params.txt"), str(base / "orients.bin"),
                             str(base / "hkls.csv"), str(base / "image.bin"), "4"],
@@ -272,3 +281,97 @@ def test_end_to_end_dofwd1_then_dofwd0(tmp_path):
     np.testing.assert_array_equal(sc, mA)
     assert "file was found. Will not do forward simulation." in out
     assert f"Initial solutions: {nres} " in out
+
+
+@pytest.mark.gpu
+def test_missing_forward_file_runs_the_simulation(tmp_path):
+    """DoFwd=0 with no ForwardFile: the program switches to simulation mode
+    (src/LaueMatchingCPU.c:422-433), writes the file and gives the DoFwd=1
+    results."""
+    p, hk, O, img = _case(tmp_path)
+    p.DoFwd = 0
+    assert not (tmp_path / "fwd.bin").exists()
+    out = _run(tmp_path, p)
+    assert "Could not read the forward simulation file. Running in simulation mode!" in out
+    table, matched, nres = ob.forward(O, hk, ob.geom(p), p.MaxNrLaueSpots, img=img.ravel(),
+                                      min_spots=p.MinNrSpots, min_int=p.MinIntensity)
+    np.testing.assert_array_equal(
+        np.fromfile(tmp_path / "fwd.bin", dtype=np.uint16).reshape(table.shape), table)
+    rows, mA = ob.collect(matched)
+    ids, sc = _hits_file(tmp_path / "image.bin.hits.txt")
+    np.testing.assert_array_equal(ids, rows)
+    np.testing.assert_array_equal(sc, mA)
+
+
+@pytest.mark.gpu
+def test_truncated_forward_file_is_an_error(tmp_path):
+    """A ForwardFile shorter than N x (1 + 2M) uint16 records ends the run
+    with an error instead of scoring past the end of the data (the
+    reference would read beyond it)."""
+    p, hk, O, img = _case(tmp_path)
+    _run(tmp_path, p)  # DoFwd=1 writes the file
+    full = (tmp_path / "fwd.bin").read_bytes()
+    (tmp_path / "fwd.bin").write_bytes(full[: len(full) // 2 + 7])
+    p.DoFwd = 0
+    p.write(str(tmp_path / "params.txt"))
+    r = subprocess.run([EXE, str(tmp_path / "params.txt"), str(tmp_path / "orients.bin"),
+                        str(tmp_path / "hkls.csv"), str(tmp_path / "image.bin"), "4"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1
+    assert "Failed to read forward simulation file" in r.stdout
+
+
+@pytest.mark.gpu
+def test_stream_daemon_survives_broken_clients(tmp_path):
+    """A client that closes in the middle of a frame's pixels, one that sends
+    only a partial header, and one that connects and leaves: none of their
+    partial frames is scored, the daemon keeps serving (a complete frame on
+    a new connection gets its rows) and exits cleanly on SIGTERM
+    (src/LaueMatchingGPUStream.cu:295-383)."""
+    import signal
+    import socket
+    import struct
+    import time
+
+    p, hk, O, img = _case(tmp_path)
+    p.extra = {"MinGoodSpots": p.MinNrSpots, "ResultDir": str(tmp_path / "res")}
+    p.DoFwd = 1
+    p.write(str(tmp_path / "params.txt"))
+    port = _free_port()
+    env = dict(os.environ, LM_STREAM_PORT=str(port), LM_STREAM_AUTOTUNE="0")
+    exe = os.path.join(ROOT, "bin", "LaueMatchingGPUStream")
+    proc = subprocess.Popen([exe, str(tmp_path / "params.txt"), str(tmp_path / "orients.bin"),
+                             str(tmp_path / "hkls.csv"), "4"], stdout=subprocess.PIPE,
+                            stderr=subprocess.STDOUT, text=True, env=env)
+    payload = img.astype(np.float32).tobytes()
+    try:
+        deadline = time.time() + 120
+        while time.time() < deadline:
+            try:
+                socket.create_connection(("127.0.0.1", port), timeout=2).close()  # leaves
+                break
+            except OSError:
+                assert proc.poll() is None, proc.stdout.read()
+                time.sleep(0.3)
+        s1 = socket.create_connection(("127.0.0.1", port), timeout=2)
+        s1.sendall(struct.pack("<H", 11) + payload[: len(payload) // 3])  # mid-payload
+        s1.close()
+        s2 = socket.create_connection(("127.0.0.1", port), timeout=2)
+        s2.sendall(b"\x0c")  # one byte of the 2-byte header
+        s2.close()
+        s3 = socket.create_connection(("127.0.0.1", port), timeout=2)
+        s3.sendall(struct.pack("<H", 13) + payload)  # a complete frame
+        s3.close()
+        time.sleep(2.0)
+        proc.send_signal(signal.SIGTERM)
+        out, _ = proc.communicate(timeout=120)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+    assert proc.returncode == 0, out
+    assert "exited cleanly" in out
+    assert "Received image 11" not in out and "Received image 12" not in out
+    assert "Received image 13" in out
+    rows = [l.rstrip("\n").split("\t") for l in open(tmp_path / "res" / "solutions.txt")][1:]
+    nums = {int(r[0]) for r in rows}
+    assert nums == {13}
