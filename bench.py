@@ -120,7 +120,7 @@ def make_batch_images(count, n, db_seed, seed=1234, workers=16):
         return list(ex.map(_batch_image, jobs))
 
 
-def run_batches(db, batch_imgs, widths, p, steps, warmup, parity_cases):
+def run_batches(db, batch_imgs, widths, p, steps, warmup, parity_cases, spots):
     """Scores batches of B images in one pass over the table; returns per-B
     rates, checks each batched image against scoring it alone and adds three
     images per width to the oracle parity cases."""
@@ -156,12 +156,26 @@ def run_batches(db, batch_imgs, widths, p, steps, warmup, parity_cases):
         for i in sorted({0, B // 2, B - 1}):
             parity_cases.append((f"batch{B}_img{i}", u8s[i], scales[i], p.MinNrSpots,
                                  p.MinIntensity, hs[i]))
-        out.append({"images": B, "ms_per_batch": ms, "kernel_ms": float(np.mean(kms)),
-                    "pairs_per_s": db.n * B / (ms / 1e3), "bit_exact_vs_single": bool(ok),
+        kms_m = float(np.mean(kms))
+        # algorithmic bytes: the table once, the B images once, 16 B per hit;
+        # the walks over the orientation-major table fetch one 128-byte line
+        # of the interleaved images per spot (the line-fetch bound)
+        alg = 2.0 * db.n + 4.0 * spots + B * float(u8s[0].size) + 16.0 * sum(
+            int(h.nr_results) for h in hs)
+        lines = 128.0 * spots
+        out.append({"images": B, "ms_per_batch": ms, "kernel_ms": kms_m,
+                    "pairs_per_s": db.n * B / (ms / 1e3),
+                    "algorithmic_GBps": alg / (kms_m / 1e3) / 1e9,
+                    "algorithmic_frac_of_hbm": alg / (kms_m / 1e3) / HBM_PEAK,
+                    "line_fetch_GBps": lines / (kms_m / 1e3) / 1e9,
+                    "line_fetch_frac_of_hbm": lines / (kms_m / 1e3) / HBM_PEAK,
+                    "line_fetch_floor_ms": lines / HBM_PEAK * 1e3,
+                    "bit_exact_vs_single": bool(ok),
                     "images_with_all_grains_recovered": int(found),
                     "hits_per_image": [int(len(h.ids)) for h in hs]})
         log(f"batch B={B}: {ms:.2f} ms/batch, {out[-1]['pairs_per_s']:.3e} pairs/s, "
-            f"exact={ok}, grains recovered in {found}/{B}")
+            f"{out[-1]['algorithmic_GBps']:.0f} GB/s algorithmic, line fetches "
+            f"{out[-1]['line_fetch_GBps']:.0f} GB/s, exact={ok}, grains recovered in {found}/{B}")
         del dev
     return out
 
@@ -215,7 +229,7 @@ def poisson_run(srv, reqs, rate, duration, seed):
 
     srv.reset_stats()
     rng = np.random.default_rng(seed)
-    nreq = max(20, int(rate * duration))
+    nreq = max(12 if rate < 5 else 20, int(rate * duration))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     picks = rng.integers(0, len(reqs), nreq)
     tickets = queue.Queue()
@@ -465,6 +479,166 @@ def config4(ctx, O, n, steps, batch_imgs):
     return res
 
 
+def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=512, conns=8):
+    """bin/LaueMatchingGPUStream -- the drop-in daemon the reference's senders
+    talk to -- on the same 100M-orientation DB, fed wire-v1 frames (uint16
+    image number + float32 pixels, the reference's format) over TCP from
+    `conns` connections: Poisson arrivals at each rate, then a closed loop of
+    `closed` frames sent back to back.  Latencies are the daemon's own: to the
+    compact hit list (frame received -> quantized, queued, scored) and end to
+    end (frame received -> merge, fit and result rows written)."""
+    import queue
+    import re
+    import shutil
+    import signal
+    import socket
+    import struct
+    import tempfile
+    import threading
+
+    from lauematching_amd import synth
+
+    work = tempfile.mkdtemp(prefix="lm_bench_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    out = {"connections": conns, "frames": "wire v1, float32 2048x2048 (16 MiB)"}
+    proc = None
+    try:
+        t0 = time.time()
+        O.tofile(os.path.join(work, "orients.bin"))
+        synth.write_hkl_file(os.path.join(work, "hkls.csv"), hk)
+        pp = synth.baseline_params(MaxNrLaueSpots=p.MaxNrLaueSpots, MinNrSpots=p.MinNrSpots,
+                                   MinIntensity=p.MinIntensity, DoFwd=1, ForwardFile="")
+        pp.extra = {"MinGoodSpots": p.MinNrSpots, "ResultDir": os.path.join(work, "res")}
+        pp.write(os.path.join(work, "params.txt"))
+        with socket.socket() as so:
+            so.bind(("127.0.0.1", 0))
+            port = so.getsockname()[1]
+        exe = os.path.join(ROOT, "bin", "LaueMatchingGPUStream")
+        env = dict(os.environ, LM_STREAM_PORT=str(port))
+        proc = subprocess.Popen([exe, os.path.join(work, "params.txt"),
+                                 os.path.join(work, "orients.bin"), os.path.join(work, "hkls.csv"),
+                                 "16"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                text=True, env=env, bufsize=1)
+        gpu, total, tail = {}, {}, []
+        ready = threading.Event()
+        rx_g = re.compile(r"\[Image (\d+)\] GPU: .*hit list ([0-9.]+) ms after arrival")
+        rx_t = re.compile(r"\[Image (\d+)\] Total: ([0-9.]+) ms after arrival")
+        lock = threading.Lock()
+
+        def reader():
+            for ln in proc.stdout:
+                now = time.perf_counter()
+                tail.append(ln.rstrip())
+                del tail[:-40]
+                if "Waiting for images" in ln:
+                    ready.set()
+                elif "Autotuned" in ln:
+                    out["autotune_line"] = ln.strip()
+                elif "Init time" in ln:
+                    out["init_s"] = float(ln.split()[-2])
+                m = rx_g.search(ln)
+                if m:
+                    with lock:
+                        gpu[int(m.group(1))] = (float(m.group(2)), now)
+                m = rx_t.search(ln)
+                if m:
+                    with lock:
+                        total[int(m.group(1))] = (float(m.group(2)), now)
+            ready.set()
+
+        rt = threading.Thread(target=reader, daemon=True)
+        rt.start()
+        if not ready.wait(600) or proc.poll() is not None:
+            raise RuntimeError("daemon did not start: " + " | ".join(tail[-8:]))
+        out["startup_s"] = time.time() - t0
+        log(f"daemon ready in {out['startup_s']:.1f} s ({out.get('autotune_line', '')})")
+        payloads = [memoryview(im.astype(np.float32).tobytes()) for im in frames]
+        socks = [socket.create_connection(("127.0.0.1", port), timeout=30) for _ in range(conns)]
+        qs = [queue.Queue() for _ in range(conns)]
+
+        def sender(k):
+            while True:
+                item = qs[k].get()
+                if item is None:
+                    return
+                num, pl = item
+                socks[k].sendall(struct.pack("<H", num))
+                socks[k].sendall(pl)
+
+        ths = [threading.Thread(target=sender, args=(k,), daemon=True) for k in range(conns)]
+        for th in ths:
+            th.start()
+        nxt = [1]
+
+        def wait_for(nums, limit):
+            end = time.time() + limit
+            while time.time() < end:
+                with lock:
+                    if all(x in total for x in nums):
+                        return True
+                time.sleep(0.01)
+            return False
+
+        def stats(nums):
+            with lock:
+                g = np.array([gpu[x][0] for x in nums if x in gpu])
+                t = np.array([total[x][0] for x in nums if x in total])
+            return {"p50_ms": float(np.percentile(g, 50)), "p99_ms": float(np.percentile(g, 99)),
+                    "end_to_end_p50_ms": float(np.percentile(t, 50)),
+                    "end_to_end_p99_ms": float(np.percentile(t, 99))}
+
+        rng = np.random.default_rng(77)
+        rows = []
+        for r in rates:
+            nreq = max(8 if r < 5 else 20, int(r * duration))
+            arr = np.cumsum(rng.exponential(1.0 / r, nreq))
+            nums = list(range(nxt[0], nxt[0] + nreq))
+            nxt[0] += nreq
+            t0 = time.perf_counter()
+            for j, num in enumerate(nums):
+                dt = t0 + arr[j] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                qs[j % conns].put((num, payloads[num % len(payloads)]))
+            if not wait_for(nums, 120):
+                raise RuntimeError("daemon results missing at rate %s" % r)
+            row = {"rate_images_per_s": r, "requests": nreq, **stats(nums)}
+            rows.append(row)
+            log(f"daemon {r}/s: to hit list p50 {row['p50_ms']:.1f} p99 {row['p99_ms']:.1f} ms, "
+                f"end to end p50 {row['end_to_end_p50_ms']:.1f} p99 {row['end_to_end_p99_ms']:.1f} ms")
+        out["per_rate"] = rows
+        nums = list(range(nxt[0], nxt[0] + closed))
+        nxt[0] += closed
+        t0 = time.perf_counter()
+        for j, num in enumerate(nums):
+            qs[j % conns].put((num, payloads[num % len(payloads)]))
+        if not wait_for(nums, 300):
+            raise RuntimeError("daemon closed-loop results missing")
+        with lock:
+            tg = max(gpu[x][1] for x in nums) - t0
+            tt = max(total[x][1] for x in nums) - t0
+        out["closed_loop"] = {"images": closed, "images_per_s_to_hit_list": closed / tg,
+                              "images_per_s_end_to_end": closed / tt, **stats(nums)}
+        log(f"daemon closed loop: {closed / tg:.0f} img/s to the hit list, {closed / tt:.0f} img/s "
+            f"end to end")
+        for q in qs:
+            q.put(None)
+        for th in ths:
+            th.join()
+        for so in socks:
+            so.close()
+        proc.send_signal(signal.SIGTERM)
+        proc.wait(timeout=120)
+        rt.join(timeout=10)
+        out["exit_code"] = proc.returncode
+        out["final_line"] = next((l for l in tail if l.startswith("Processed")), None)
+    finally:
+        if proc is not None and proc.poll() is None:
+            proc.kill()
+            proc.wait()
+        shutil.rmtree(work, ignore_errors=True)
+    return out
+
+
 def kernel_name(db):
     """Name of the single-image kernel the DB's layout options select, with
     every template argument as rocprofv3 prints it (the LDS walk's last
@@ -512,7 +686,7 @@ def main():
                          "table (0 = none) instead of the autotuned one")
     ap.add_argument("--loads", type=int, default=0,
                     help="with --tile-bits: the single-image walk (LM_OPT_BANDED_LOADS)")
-    ap.add_argument("--batch", type=str, default="64,128,256",
+    ap.add_argument("--batch", type=str, default="8,16,32,64,128,256",
                     help="comma-separated batch widths to measure as well (one pass over the "
                          "table per batch), e.g. 8,16,32,64,128; empty = none")
     ap.add_argument("--no-autotune", action="store_true",
@@ -523,13 +697,17 @@ def main():
                     help="skip the duplicate-merge and preprocessing timings")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32+guard rebuild of the table")
-    ap.add_argument("--config4", action="store_true",
-                    help="also run the |hkl|<=10, 5-40 keV precision configuration")
-    ap.add_argument("--serve", action="store_true",
-                    help="also run the full serving configuration (two crystals x two "
-                         "detectors resident, Poisson rates 5..200 images/s)")
-    ap.add_argument("--no-latency", action="store_true",
-                    help="skip the default single-DB serving latency measurement")
+    ap.add_argument("--no-config4", action="store_true",
+                    help="skip the |hkl|<=10, 5-40 keV precision configuration (table "
+                         "resident vs regenerated per batch of 32)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving configuration (two crystals x two detectors "
+                         "resident, 70/30 detector mix, Poisson rates 1..200 images/s and "
+                         "overload, closed loop)")
+    ap.add_argument("--latency", action="store_true",
+                    help="also measure serving latency on the single 100M FCC DB alone")
+    ap.add_argument("--no-daemon", action="store_true",
+                    help="skip bin/LaueMatchingGPUStream fed over TCP")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -537,6 +715,9 @@ def main():
         sys.exit(2)
     widths = [int(b) for b in args.batch.split(",") if b]
     batch_imgs = []
+    args.config4 = not args.no_config4
+    args.serve = not args.no_serve
+    args.no_latency = not args.latency
     need = max(widths + ([32] if args.config4 else []), default=0)
     if need:
         # worker processes are started before anything touches the GPU
@@ -545,7 +726,7 @@ def main():
         log(f"generated {len(batch_imgs)} batch images in {time.time() - t0:.1f} s")
     n_hex = args.orients // 2
     serve_pool = []
-    if args.serve or not args.no_latency:
+    if args.serve or not args.no_latency or not args.no_daemon:
         combos = [("fcc", 2048, args.orients, args.seed)]
         if args.serve:
             combos += [("fcc", 1024, args.orients, args.seed),
@@ -700,7 +881,7 @@ def main():
     if widths:
         db.set_use_graphs(True)
         batched = run_batches(db, batch_imgs, widths, p, max(3, args.steps // 2), 1,
-                              parity_cases)
+                              parity_cases, st.n_spots)
         db.set_use_graphs(False)
 
     latency = None
@@ -733,7 +914,8 @@ def main():
             u8_, s_ = api.quantize_u8(img_)
             reps = 7 if d == 2048 else 3
             reqs += [(keys.index((c, d)), u8_, s_)] * reps
-        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [50, 200, 800, 1600], 3.0, 2048)
+        serving = run_serving(ctx, [dbs[k] for k in keys], reqs, [1, 20, 100, 200, 800, 1600],
+                              2.0, 2048)
         serving["dbs"] = [{"crystal": c, "detector": d, "orientations": dbs[(c, d)].n}
                           for c, d in keys]
         serving["hbm_in_use_gb"] = ctx.bytes_in_use() / 1e9
@@ -746,12 +928,35 @@ def main():
     if args.config4:
         c4 = config4(ctx, O, n, max(3, args.steps // 2), batch_imgs)
 
+    # the forward simulation against the FP64 VALU peak: 65 ops per
+    # (orientation, hkl) pair and 45 per orientation, divisions and square
+    # roots counted as one (the reference's per-pair work)
+    fwd_ops = 65.0 * n * len(hk) + 45.0 * n
+    forward = {"ms": st.ms, "ops": fwd_ops, "tflops": fwd_ops / (st.ms / 1e3) / 1e12,
+               "peak_tflops_fp64": 78.6, "frac": fwd_ops / (st.ms / 1e3) / 78.6e12,
+               "pairs": int(st.n_pairs), "projected_pairs": int(st.n_projected),
+               "spots": int(st.n_spots)}
+    log(f"forward simulation: {forward['tflops']:.1f} TFLOP/s by the reference's op count "
+        f"({100 * forward['frac']:.1f}% of FP64 peak); {st.n_projected / st.n_pairs:.3%} of "
+        f"pairs projected past the pre-test")
+
     cpu, parity = None, None
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
         cpu = cpu_baseline(db, u8, im_scale, p.MinNrSpots, p.MinIntensity, sample)
         log(f"cpu baseline {cpu['value']:.3e} pairs/s ({cpu['cores']} cores)")
         parity = oracle_parity(db, parity_cases)
+
+    daemon = None
+    if not args.no_daemon:
+        # the drop-in daemon on its own DB: free this process's table first
+        db.close()
+        dframes = [img_ for c, d, img_ in serve_pool if (c, d) == ("fcc", 2048)]
+        try:
+            daemon = daemon_leg(O, hk, p, dframes)
+        except Exception as ex:  # reported in the line, the rest stands
+            daemon = {"error": str(ex)}
+            log(f"daemon leg failed: {ex}")
 
     line = {
         "metric": "orient×image pairs/s at 100M orients + p50/p99 image latency vs plain GPU",
@@ -801,8 +1006,19 @@ def main():
                            "plain_closed_loop_images_per_s":
                                latency["plain"]["closed_loop"]["images_per_s"],
                            "detail": latency}
+    line["forward"] = forward
     if serving:
+        serving["per_rate"] = [
+            {"rate_images_per_s": b["rate"], "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
+             "plain_p50_ms": pl["p50_ms"], "plain_p99_ms": pl["p99_ms"]}
+            for b, pl in zip(serving["batched"]["poisson"], serving["plain"]["poisson"])]
+        serving["closed_loop_images_per_s"] = serving["batched"]["closed_loop"]["images_per_s"]
+        serving["plain_closed_loop_images_per_s"] = serving["plain"]["closed_loop"]["images_per_s"]
+        serving["closed_loop_vs_plain"] = (serving["closed_loop_images_per_s"] /
+                                           serving["plain_closed_loop_images_per_s"])
         line["serving"] = serving
+    if daemon:
+        line["daemon"] = daemon
     if precision:
         line["precision"] = precision
     line["nonunit_scale"] = nonunit

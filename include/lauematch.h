@@ -113,6 +113,10 @@ int lm_geom_init(const double lattice[6], int32_t space_group, const double P[3]
 /* uint8 quantization of a float64 image, src/LaueMatchingCPU.c:355-377.
  * Writes out[npx] and *im_scale (= max/255, or 1 when max <= 0). */
 int lm_quantize_u8(const double *img, uint64_t npx, uint8_t *out, double *im_scale);
+/* The same quantization of a float32 image (the streaming daemon's frames,
+ * src/LaueMatchingGPUStream.cu:299-368): identical to lm_quantize_u8 on the
+ * pixels widened to float64, without the float64 copy. */
+int lm_quantize_u8_f32(const float *img, uint64_t npx, uint8_t *out, double *im_scale);
 
 /* ---- orientation DB + spot table ------------------------------------------------ */
 

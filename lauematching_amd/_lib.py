@@ -122,6 +122,8 @@ SIGNATURES = {
     "lm_score_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                     ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
                                     ctypes.POINTER(LmHits)]),
+    "lm_quantize_u8_f32": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                          c_double_p]),
     "lm_score_dev_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                             ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
                                             ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,

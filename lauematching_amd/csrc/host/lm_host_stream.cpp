@@ -82,8 +82,9 @@ struct Job {
   uint32_t image_num;
   int db;
   uint64_t ticket;
-  std::vector<float> image;
-  double t_recv;
+  std::vector<float> image;  // the frame's pixels (for the fit), from the pool
+  double t_arrival;          // frame fully received
+  double t_submit;           // quantized, handed to the server
 };
 
 // One DB of the daemon: its parameter file, orientations, hkls, table and
@@ -108,6 +109,24 @@ struct Daemon {
   std::deque<Job> jobs;
   int open_clients = 0;
   std::vector<std::thread> handlers;
+  // frame buffers handed from the connection handlers to the result loop and
+  // back, so a frame is received into a buffer that the fit then reads
+  std::mutex pool_mu;
+  std::vector<std::vector<float>> pool;
+  std::vector<float> take_buffer(size_t npx) {
+    std::lock_guard<std::mutex> lk(pool_mu);
+    std::vector<float> b;
+    if (!pool.empty()) {
+      b = std::move(pool.back());
+      pool.pop_back();
+    }
+    b.resize(npx);
+    return b;
+  }
+  void give_buffer(std::vector<float> &&b) {
+    std::lock_guard<std::mutex> lk(pool_mu);
+    if (pool.size() < 64) pool.push_back(std::move(b));
+  }
 };
 
 bool recv_all(int fd, void *buf, size_t len) {
@@ -128,20 +147,21 @@ bool recv_all(int fd, void *buf, size_t len) {
   return true;
 }
 
-// Quantize one frame (src/LaueMatchingCPU.c:355-377) from its float64
-// pixels, submit it to the server for DB k and queue it for post-processing
-// (which uses the float32 copy, as the reference daemon's fit does).
-// have_f64: f64 already holds the frame's float64 pixels (wire v2 type 1);
-// otherwise they are the float32 pixels widened.
-bool submit_frame(Daemon *d, int k, uint32_t num, std::vector<float> &img,
-                  std::vector<double> &f64, std::vector<uint8_t> &u8, bool have_f64 = false) {
+// Quantize one frame (src/LaueMatchingCPU.c:355-377) -- from its float64
+// pixels when the wire carried them (f64 non-null), else from the float32
+// pixels, which is the same arithmetic -- submit it to the server for DB k and
+// queue it for post-processing (which reads the float32 pixels, as the
+// reference daemon's fit does).  img moves into the job.
+bool submit_frame(Daemon *d, int k, uint32_t num, std::vector<float> &img, const double *f64,
+                  std::vector<uint8_t> &u8, double t_arrival) {
   DbSlot &s = *d->dbs[k];
-  f64.resize(s.npx);
   u8.resize(s.npx);
-  if (!have_f64)
-    for (size_t i = 0; i < s.npx; ++i) f64[i] = img[i];
   double scale = 1.0;
-  lm_quantize_u8(f64.data(), s.npx, u8.data(), &scale);
+  if (f64)
+    lm_quantize_u8(f64, s.npx, u8.data(), &scale);
+  else
+    lm_quantize_u8_f32(img.data(), s.npx, u8.data(), &scale);
+  const double t_submit = omp_get_wtime();
   uint64_t ticket = 0;
   if (lm_server_submit(d->server, s.server_id, u8.data(), scale, s.p.min_nr_spots,
                        s.p.min_intensity, &ticket) != LM_OK) {
@@ -152,7 +172,7 @@ bool submit_frame(Daemon *d, int k, uint32_t num, std::vector<float> &img,
   fflush(stdout);
   {
     std::lock_guard<std::mutex> lk(d->mu);
-    d->jobs.push_back(Job{num, k, ticket, img, omp_get_wtime()});
+    d->jobs.push_back(Job{num, k, ticket, std::move(img), t_arrival, t_submit});
   }
   d->cv.notify_one();
   return true;
@@ -162,24 +182,25 @@ bool submit_frame(Daemon *d, int k, uint32_t num, std::vector<float> &img,
 // (scripts/laue_image_server.py:217-219, src/LaueMatchingGPUStream.cu:295-383)
 void serve_v1(Daemon *d, int fd) {
   const size_t npx = d->dbs[0]->npx;
-  std::vector<float> buf(npx);
-  std::vector<double> f64;
   std::vector<uint8_t> u8;
   printf("Client handler started (socket %d), wire_payload=%zu bytes\n", fd, npx * 4);
   fflush(stdout);
   while (g_running) {
     uint8_t hdr[2];
     if (!recv_all(fd, hdr, 2)) break;
-    if (!recv_all(fd, buf.data(), npx * 4)) break;
+    std::vector<float> buf = d->take_buffer(npx);
+    if (!recv_all(fd, buf.data(), npx * 4)) {
+      d->give_buffer(std::move(buf));
+      break;
+    }
     const uint16_t num = (uint16_t)(hdr[0] | (hdr[1] << 8));
-    if (!submit_frame(d, 0, num, buf, f64, u8)) break;
+    if (!submit_frame(d, 0, num, buf, nullptr, u8, omp_get_wtime())) break;
   }
 }
 
 // v2: per-frame header (see the top of this file)
 void serve_v2(Daemon *d, int fd) {
   std::vector<uint8_t> raw;
-  std::vector<float> img;
   std::vector<double> f64;
   std::vector<uint8_t> u8;
   printf("Client handler started (socket %d), wire v2\n", fd);
@@ -204,7 +225,8 @@ void serve_v2(Daemon *d, int fd) {
     const size_t npx = (size_t)W * H;
     raw.resize(npx * kSize[type]);
     if (!recv_all(fd, raw.data(), raw.size())) break;
-    img.resize(npx);
+    const double t_arrival = omp_get_wtime();
+    std::vector<float> img = d->take_buffer(npx);
     if (type == 1) f64.resize(npx);
     for (size_t i = 0; i < npx; ++i) {
       const uint8_t *q = raw.data() + i * kSize[type];
@@ -215,7 +237,7 @@ void serve_v2(Daemon *d, int fd) {
         default: img[i] = (float)q[0]; break;
       }
     }
-    if (!submit_frame(d, k, num, img, f64, u8, type == 1)) break;
+    if (!submit_frame(d, k, num, img, type == 1 ? f64.data() : nullptr, u8, t_arrival)) break;
   }
 }
 
@@ -492,8 +514,11 @@ int main(int argc, char **argv) {
       fprintf(stderr, "[Image %u] scoring failed: %s\n", job.image_num, lm_last_error());
       continue;
     }
-    printf("[Image %u] GPU: %.1f ms (queue + batch), %llu matches\n", job.image_num, lat,
-           (unsigned long long)hits.nr_results);
+    // latencies from the frame's arrival: the hit list is ready when the
+    // server completed it (quantization + the server's queue and batch)
+    printf("[Image %u] GPU: %.1f ms (queue + batch), %llu matches; hit list %.1f ms after "
+           "arrival\n", job.image_num, lat, (unsigned long long)hits.nr_results,
+           (job.t_submit - job.t_arrival) * 1e3 + lat);
     if (hits.nr_results == 0) {
       printf("[Image %u] No matches, skipping fitting.\n", job.image_num);
     } else {
@@ -512,8 +537,10 @@ int main(int argc, char **argv) {
       const int ns = append_solutions(in, s.sol, s.spots);
       printf("[Image %u] %d unique orientations found\n", job.image_num, ns);
     }
-    printf("[Image %u] Total: %.1f ms\n", job.image_num, (omp_get_wtime() - job.t_recv) * 1e3);
+    printf("[Image %u] Total: %.1f ms after arrival\n", job.image_num,
+           (omp_get_wtime() - job.t_arrival) * 1e3);
     fflush(stdout);
+    d.give_buffer(std::move(job.image));
     lm_hits_free(&hits);
     ++processed;
   }

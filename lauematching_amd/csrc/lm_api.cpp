@@ -249,6 +249,31 @@ int lm_quantize_u8(const double *img, uint64_t npx, uint8_t *out, double *im_sca
   return LM_OK;
 }
 
+int lm_quantize_u8_f32(const float *img, uint64_t npx, uint8_t *out, double *im_scale) {
+  if ((!img || !out) && npx) {
+    set_error("lm_quantize_u8_f32: null buffer");
+    return LM_ERR_INVALID;
+  }
+  // every float is a double exactly: the same operations as lm_quantize_u8
+  // on the widened pixels, without the float64 copy
+  double mx = 0;
+  for (uint64_t i = 0; i < npx; ++i)
+    if ((double)img[i] > mx) mx = img[i];
+  const double scale = (mx > 0) ? mx / 255.0 : 1.0;
+  const double inv = (mx > 0) ? 255.0 / mx : 0.0;
+  for (uint64_t i = 0; i < npx; ++i) {
+    const double x = img[i];
+    if (x > 0) {
+      const double v = x * inv;
+      out[i] = (v >= 255.0) ? 255 : (v < 1.0) ? 1 : (uint8_t)v;
+    } else {
+      out[i] = 0;
+    }
+  }
+  if (im_scale) *im_scale = scale;
+  return LM_OK;
+}
+
 int lm_db_create(lm_ctx *ctx, const double *orients, uint64_t n, const int32_t *hkls,
                  int32_t nhkls, const lm_geom *geom, int32_t max_spots, lm_db **out) {
   int rc = check_ctx(ctx);

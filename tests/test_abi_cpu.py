@@ -74,6 +74,26 @@ def test_quantize_bit_identical_to_oracle():
         np.testing.assert_array_equal(a, b)
 
 
+def test_quantize_f32_equals_quantize_of_widened_pixels():
+    """lm_quantize_u8_f32 (the daemon's float32 frames) gives the oracle's
+    quantization of the same pixels widened to float64."""
+    import ctypes
+    L = _lib.lib()
+    rng = np.random.default_rng(9)
+    for scale in (1.0, 0.37, 1234.5, 3e-30):
+        img = (rng.gamma(0.3, 20.0, 100000) * scale).astype(np.float32)
+        img[rng.random(img.size) < 0.2] = 0
+        img[rng.random(img.size) < 0.01] = -1
+        img[:3] = [np.nan, np.inf, -np.inf] if scale == 1.0 else img[:3]
+        out = np.zeros(img.size, np.uint8)
+        s = ctypes.c_double()
+        _lib.check(L.lm_quantize_u8_f32(img.ctypes.data, img.size, out.ctypes.data,
+                                        ctypes.byref(s)))
+        b, sb = ob.quantize(img.astype(np.float64))
+        assert s.value == sb
+        np.testing.assert_array_equal(out, b)
+
+
 def test_invalid_geometry_rejected():
     with pytest.raises(_lib.LaueMatchError):
         api.Geometry((0, 0, 0, 90, 90, 90), 225, (0, 0, .5), (0.1, 0, 0), 1e-4, 1e-4, 64, 64, 5, 30)
