@@ -626,6 +626,50 @@ def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=5
             th.join()
         for so in socks:
             so.close()
+        # the same closed loop with wire-v2 uint8 frames (4 MiB, a quarter of
+        # the float32 bytes; the frames quantize to the same image)
+        from lauematching_amd import api
+        u8p = []
+        for im in frames:
+            u8_, _ = api.quantize_u8(im)
+            H, W = im.shape
+            u8p.append((struct.pack("<HHIII", 0, 3, W, H, 0), memoryview(u8_.tobytes())))
+        socks = [socket.create_connection(("127.0.0.1", port + 1), timeout=30)
+                 for _ in range(conns)]
+        qs = [queue.Queue() for _ in range(conns)]
+
+        def sender2(k):
+            while True:
+                item = qs[k].get()
+                if item is None:
+                    return
+                num, (hdr, pl) = item
+                socks[k].sendall(struct.pack("<II", 0x32464D4C, num) + hdr)
+                socks[k].sendall(pl)
+
+        ths = [threading.Thread(target=sender2, args=(k,), daemon=True) for k in range(conns)]
+        for th in ths:
+            th.start()
+        nums = list(range(nxt[0], nxt[0] + closed))
+        nxt[0] += closed
+        t0 = time.perf_counter()
+        for j, num in enumerate(nums):
+            qs[j % conns].put((num, u8p[num % len(u8p)]))
+        if not wait_for(nums, 300):
+            raise RuntimeError("daemon closed-loop (uint8 frames) results missing")
+        with lock:
+            tg = max(gpu[x][1] for x in nums) - t0
+            tt = max(total[x][1] for x in nums) - t0
+        out["closed_loop_u8_frames"] = {"images": closed, "images_per_s_to_hit_list": closed / tg,
+                                        "images_per_s_end_to_end": closed / tt, **stats(nums)}
+        log(f"daemon closed loop (wire v2 uint8 frames): {closed / tg:.0f} img/s to the hit list, "
+            f"{closed / tt:.0f} img/s end to end")
+        for q in qs:
+            q.put(None)
+        for th in ths:
+            th.join()
+        for so in socks:
+            so.close()
         proc.send_signal(signal.SIGTERM)
         proc.wait(timeout=120)
         rt.join(timeout=10)
