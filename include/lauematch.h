@@ -178,6 +178,15 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * list holds) 128 images take the uint8 walks and wider batches are scored
  * as two calls of at most 128.  Default 0x180. */
 #define LM_OPT_BATCH_NIBBLE 7
+/* LM_OPT_TABLE_ON_HOST (read only): 1 when the spot table did not fit the
+ * context's HBM budget and is kept in host memory (compact: counts and 4
+ * bytes per spot); every scoring call then copies it to the device in chunks
+ * of at most 4 GB and half the free budget and scores each chunk, with the
+ * same results (src/LaueMatchingGPU.cu:660-749 streams the ForwardFile the
+ * same way).  Such a DB cannot be autotuned, served by lm_server or scored
+ * by lm_score_dev_noalloc.  A banded copy that does not fit is left out
+ * (scoring reads the orientation-major table). */
+#define LM_OPT_TABLE_ON_HOST 8
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 

@@ -191,6 +191,12 @@ class OrientationDB:
         check(lib().lm_db_get_option(self._h, int(option), ctypes.byref(v)))
         return v.value
 
+    @property
+    def table_on_host(self) -> bool:
+        """True when the table did not fit the HBM budget and is kept in host
+        memory, streamed to the device in chunks by every scoring call."""
+        return self.get_option(8) == 1
+
     def banded_layout(self):
         """(super-tile bits, walk) of the single-image layout in use."""
         return self.get_option(1), self.get_option(5)

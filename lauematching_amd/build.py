@@ -31,7 +31,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
 
 LIB_SOURCES = ["lm_api.cpp", "lm_fwd.hip", "lm_score.hip", "lm_band.hip", "lm_merge.cpp",
                "lm_merge_dev.hip", "lm_prep.hip",
-               "lm_serve.cpp", "lm_tune.cpp"]
+               "lm_serve.cpp", "lm_tune.cpp", "lm_hosttable.cpp"]
 HOST_COMMON = ["host/lm_params.cpp", "host/lm_hostio.cpp", "host/lm_postproc.cpp"]
 HOST_PROGRAMS = {"LaueMatchingGPU": ["host/lm_host_gpu.cpp", *HOST_COMMON],
                  "LaueMatchingGPUStream": ["host/lm_host_stream.cpp", *HOST_COMMON]}

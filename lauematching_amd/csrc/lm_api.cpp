@@ -362,6 +362,7 @@ void lm_db_destroy(lm_db *db) {
   (void)hipSetDevice(db->ctx->device);
   (void)hipStreamSynchronize(db->ctx->stream);
   release_table(db);
+  release_host_table(db);
   if (db->d_fwd_plan) (void)hipFree(db->d_fwd_plan);
   if (db->d_orients) (void)hipFree(db->d_orients);
   if (db->d_hkls) (void)hipFree(db->d_hkls);
@@ -375,7 +376,10 @@ void lm_db_destroy(lm_db *db) {
 
 uint64_t lm_db_size(const lm_db *db) { return db ? db->n : 0; }
 uint64_t lm_db_table_bytes(const lm_db *db) { return db && db->table.ready ? db->table.bytes : 0; }
-uint64_t lm_db_table_spots(const lm_db *db) { return db && db->table.ready ? db->table.total : 0; }
+uint64_t lm_db_table_spots(const lm_db *db) {
+  if (!db) return 0;
+  return db->table.ready ? db->table.total : (db->htable.ready ? db->htable.total : 0);
+}
 
 int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats) {
   if (!db) {
@@ -385,7 +389,11 @@ int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats) {
   int rc = check_ctx(db->ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
-  return forward_build(db, mode, stats);
+  release_host_table(db);
+  rc = forward_build(db, mode, stats);
+  // a table larger than the HBM budget stays on the host, scored in chunks
+  if (rc == LM_ERR_BUDGET) rc = forward_host(db, mode, stats);
+  return rc;
 }
 
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
@@ -474,6 +482,7 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
     case LM_OPT_BATCH_WIDTHS: *value = db->batch_widths; return LM_OK;
     case LM_OPT_USE_GRAPHS: *value = db->scratch.use_graphs ? 1 : 0; return LM_OK;
     case LM_OPT_BANDED_LOADS: *value = db->banded_loads; return LM_OK;
+    case LM_OPT_TABLE_ON_HOST: *value = db->htable.ready ? 1 : 0; return LM_OK;
     case LM_OPT_BATCH_GROUPED: *value = db->group_widths; return LM_OK;
     case LM_OPT_BATCH_NIBBLE: *value = db->nib_widths; return LM_OK;
     default: set_error("unknown option"); return LM_ERR_INVALID;
@@ -495,7 +504,9 @@ int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
   if (!db->table.ready) {
-    set_error("lm_db_autotune: no table (run lm_db_forward or lm_db_import_table first)");
+    set_error(db->htable.ready
+                  ? "lm_db_autotune: the table is kept on the host (larger than the HBM budget)"
+                  : "lm_db_autotune: no table (run lm_db_forward or lm_db_import_table first)");
     return LM_ERR_STATE;
   }
   return autotune(db, images, nimg, max_batch, reps, out);
@@ -509,6 +520,10 @@ int lm_db_table_checksum(lm_db *db, uint64_t *sum) {
   int rc = check_ctx(db->ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
+  if (db->htable.ready) {
+    *sum = checksum_host(db);
+    return LM_OK;
+  }
   return table_checksum(db, sum);
 }
 
@@ -520,7 +535,10 @@ int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count) {
   int rc = check_ctx(db->ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
-  return import_table(db, records, count);
+  release_host_table(db);
+  rc = import_table(db, records, count);
+  if (rc == LM_ERR_BUDGET) rc = import_host(db, records, count);
+  return rc;
 }
 
 int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out) {
@@ -531,6 +549,13 @@ int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out)
   int rc = check_ctx(db->ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
+  if (db->htable.ready) {
+    if (first > db->n || count > db->n - first || (count && !out)) {
+      set_error("export_table: rows out of range");
+      return LM_ERR_INVALID;
+    }
+    return export_host(db, first, count, out);
+  }
   return export_table(db, first, count, out);
 }
 
@@ -556,6 +581,11 @@ int lm_score_dev_noalloc(lm_db *db, const uint8_t *d_img_u8, double im_scale,
   int rc = check_ctx(db->ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(db->mu);
+  if (db->htable.ready) {
+    set_error("lm_score_dev_noalloc: the table is kept on the host (larger than the HBM "
+              "budget); use lm_score_dev");
+    return LM_ERR_STATE;
+  }
   hipStream_t st = stream ? (hipStream_t)stream : db->ctx->stream;
   return score_noalloc(db, d_img_u8, im_scale, min_nr_spots, min_intensity, st, cap, ids, nspots,
                        totint, score, count, nr_results);

@@ -944,7 +944,7 @@ void release_banded(lm_db *db) {
   release_one(db, db->banded_b);
 }
 
-int build_banded(lm_db *db) {
+int build_banded(lm_db *db, bool best_effort) {
   release_banded(db);
   if (db->banded_tb != 0 && (db->banded_tb < 13 || db->banded_tb > kSum16TileBits)) {
     set_error("banded super-tile bits must be 0 or 13..16");
@@ -959,7 +959,11 @@ int build_banded(lm_db *db) {
     return LM_ERR_INVALID;
   }
   int rc = build_one(db, db->banded, db->banded_tb, db->banded_tb >= 14);
+  // without room for a copy (best effort), scoring reads the orientation-major
+  // table instead, which gives the same results
+  if (rc == LM_ERR_BUDGET && best_effort) rc = LM_OK;
   if (rc == LM_OK) rc = build_one(db, db->banded_b, db->batch_tb, false);
+  if (rc == LM_ERR_BUDGET && best_effort) rc = LM_OK;
   if (rc) release_banded(db);
   return rc;
 }

@@ -600,12 +600,14 @@ DevGeom32 dev_geom32(const Geom &g) {
   return d;
 }
 
+}  // namespace
+
 // The pre-test plan: hkls in chunks of 64 in file order; within a chunk each
 // hkl is evaluated once, together with the first unpaired -hkl after it in
 // the same chunk when Elo > 0 (only the member with q_z < 0 can then be
 // accepted).  Uploaded once per DB: plan entries, then one int4 per chunk
 // (first hkl, first entry, last entry + 1, 0).
-int fwd_plan(lm_db *db, hipStream_t st) {
+int ensure_fwd_plan(lm_db *db, hipStream_t st) {
   if (db->d_fwd_plan) return LM_OK;
   const int n = db->nhkls;
   const bool pair = db->geom.Elo > 0;
@@ -659,6 +661,8 @@ int fwd_plan(lm_db *db, hipStream_t st) {
   db->fwd_plan_chunks = (int)ch.size();
   return LM_OK;
 }
+
+namespace {
 
 void free_table(lm_db *db) {
   release_banded(db);
@@ -752,7 +756,7 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
   const bool guard = mode == LM_FWD_FP32_GUARD;
   hipStream_t st = db->ctx->stream;
   if (db->table.counts) free_table(db);
-  int rc = fwd_plan(db, st);
+  int rc = ensure_fwd_plan(db, st);
   if (rc) return rc;
   rc = alloc_table_index(db);
   if (rc) return rc;
@@ -883,7 +887,7 @@ int forward_build(lm_db *db, int mode, lm_fwd_stats *stats) {
     return rc;
   }
   t.ready = true;
-  rc = build_banded(db);
+  rc = build_banded(db, true);
   if (rc) {
     free_table(db);
     return rc;
@@ -955,7 +959,7 @@ int import_table(lm_db *db, const uint16_t *records, uint64_t count) {
     return rc;
   }
   t.ready = true;
-  rc = build_banded(db);
+  rc = build_banded(db, true);
   if (rc) {
     free_table(db);
     return rc;

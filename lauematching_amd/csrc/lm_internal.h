@@ -154,6 +154,21 @@ int scratch_images(Scratch &s, uint64_t bytes);
 // chunk flag buffer of the single-image LDS walk (at least n words)
 int ensure_chunk_flags(Scratch &s, uint64_t n);
 
+// Spot table kept in host memory when the HBM budget cannot hold it
+// (lm_hosttable.cpp): compact counts, per-tile offsets and pixel entries,
+// streamed through one device staging buffer per scoring call.
+struct HostTable {
+  bool ready = false;
+  std::vector<uint16_t> counts;    // n + 64
+  std::vector<uint64_t> tile_off;  // ntiles + 1, absolute
+  std::vector<uint32_t> entries;   // total
+  uint64_t total = 0;
+  uint16_t *d_counts = nullptr;    // staging of one chunk (charged to the budget)
+  uint64_t *d_tile_off = nullptr;
+  uint32_t *d_entries = nullptr;
+  uint64_t cap_tiles = 0, cap_entries = 0, bytes = 0;
+};
+
 }  // namespace lm
 
 struct lm_ctx {
@@ -187,6 +202,7 @@ struct lm_db {
   uint32_t nib_widths = 0x180;  // bit log2(B): batches of width B (128, 256) use 4-bit bounds
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
+  lm::HostTable htable;         // the table when it does not fit the budget
   std::mutex mu;
 };
 
@@ -230,8 +246,27 @@ int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, 
                   double *s);
 int table_checksum(lm_db *db, uint64_t *sum);
 
+// lm_fwd.hip: the forward pre-test plan of the DB's hkls (built once)
+int ensure_fwd_plan(lm_db *db, hipStream_t st);
+
+// lm_hosttable.cpp: tables larger than the HBM budget, kept on the host and
+// scored in chunks (forward_host / import_host are taken when the resident
+// build returns LM_ERR_BUDGET)
+int forward_host(lm_db *db, int mode, lm_fwd_stats *stats);
+int import_host(lm_db *db, const uint16_t *records, uint64_t count);
+int export_host(lm_db *db, uint64_t first, uint64_t count, uint16_t *out);
+uint64_t checksum_host(const lm_db *db);
+void release_host_table(lm_db *db);
+int score_images_host(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
+                      const double *im_scales, const int *min_spots, const double *min_int,
+                      hipStream_t st, lm_hits *outs);
+int score_seq_host(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
+                   int min_nr_spots, double min_intensity, hipStream_t st, lm_hits *out);
+
 // lm_band.hip
-int build_banded(lm_db *db);
+// best_effort: a copy the HBM budget cannot hold is left out (scoring then
+// reads the orientation-major table) instead of failing.
+int build_banded(lm_db *db, bool best_effort = false);
 void release_banded(lm_db *db);
 int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipStream_t st);
 // Candidate list of the 16-bit-sum walk: rows / 16, at least 4096, at most

@@ -1217,6 +1217,8 @@ uint64_t scratch_bytes_for(const lm_db *db, int max_batch) {
 int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
               int min_nr_spots, double min_intensity, hipStream_t st, lm_hits *out) {
   const Table &t = db->table;
+  if (!t.ready && db->htable.ready)
+    return score_seq_host(db, sc, d_img, f64, scale, min_nr_spots, min_intensity, st, out);
   if (!t.ready) {
     set_error("score: no table (run lm_db_forward or lm_db_import_table)");
     return LM_ERR_STATE;
@@ -1329,6 +1331,8 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
                  const double *im_scales, const int *min_spots, const double *min_int,
                  hipStream_t st, lm_hits *outs) {
   const Table &t = db->table;
+  if (!t.ready && db->htable.ready)
+    return score_images_host(db, sc, nimg, d_imgs, im_scales, min_spots, min_int, st, outs);
   if (!t.ready) {
     set_error("score: no table (run lm_db_forward or lm_db_import_table)");
     return LM_ERR_STATE;
