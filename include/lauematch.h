@@ -187,6 +187,12 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * by lm_score_dev_noalloc.  A banded copy that does not fit is left out
  * (scoring reads the orientation-major table). */
 #define LM_OPT_TABLE_ON_HOST 8
+/* LM_OPT_BATCH_WIDTH_TILE_BITS (set only): value = (log2 B) << 8 | bits gives
+ * batches of width B (4..64) their own batch copy of 2^bits-row super-tiles
+ * (9..13, B x 2^bits <= 32768; 0 = none), so different widths can keep
+ * different copies (the autotuner sets these); LM_OPT_BATCH_TILE_BITS sets
+ * every fitting width at once. */
+#define LM_OPT_BATCH_WIDTH_TILE_BITS 9
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
@@ -213,6 +219,8 @@ typedef struct lm_tune_report {
   uint32_t batch_widths_mask; /* chosen LM_OPT_BATCH_WIDTHS                       */
   uint32_t grouped_mask;      /* chosen LM_OPT_BATCH_GROUPED                      */
   uint32_t nibble_mask;       /* chosen LM_OPT_BATCH_NIBBLE                       */
+  int32_t batch_tile_bits_w[7]; /* per width: super-tile bits of the batch copy it
+                                   keeps (0 = a table walk)                        */
 } lm_tune_report;
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out);

@@ -163,6 +163,11 @@ class OrientationDB:
         """Super-tile size (log2, 9..13) of the batch banded copy; 0 drops it."""
         check(lib().lm_db_set_option(self._h, 2, int(bits)))
 
+    def set_batch_width_tile_bits(self, width: int, bits: int) -> None:
+        """Give batches of `width` images (4..64) their own batch copy of
+        2^bits-row super-tiles (9..13; 0 = none)."""
+        check(lib().lm_db_set_option(self._h, 9, (int(width).bit_length() - 1) << 8 | int(bits)))
+
     def set_batch_widths(self, mask: int) -> None:
         """Bit k set: batches of width 2**k read the batch banded copy."""
         check(lib().lm_db_set_option(self._h, 3, int(mask)))
@@ -233,6 +238,7 @@ class OrientationDB:
             "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
             "batch_widths_mask": r.batch_widths_mask, "grouped_mask": r.grouped_mask,
             "nibble_mask": r.nibble_mask,
+            "batch_tile_bits_per_width": {B: r.batch_tile_bits_w[i] for i, B in enumerate(widths)},
         }
 
     def forward(self, mode: int = LM_FWD_FP64) -> LmFwdStats:

@@ -423,6 +423,21 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
       return LM_ERR_INVALID;
     }
     db->batch_tb = (int)value;
+    // every width the copy's LDS counters hold (B x 2^bits <= 32768) takes it
+    for (int k = 0; k < 9; ++k)
+      db->width_tb[k] = (value && k >= 2 && k <= 6 && (1ll << k) * (1ll << value) <= 32768)
+                            ? (int8_t)value : (int8_t)0;
+    if (db->table.ready) return build_banded(db);
+    return LM_OK;
+  }
+  if (option == LM_OPT_BATCH_WIDTH_TILE_BITS) {
+    const int64_t k = value >> 8, tb = value & 0xff;
+    if (k < 2 || k > 6 || (tb != 0 && (tb < 9 || tb > 13 || (1ll << k) * (1ll << tb) > 32768))) {
+      set_error("LM_OPT_BATCH_WIDTH_TILE_BITS: (log2 width 2..6) << 8 | bits (0, or 9..13 "
+                "with width x 2^bits <= 32768)");
+      return LM_ERR_INVALID;
+    }
+    db->width_tb[k] = (int8_t)tb;
     if (db->table.ready) return build_banded(db);
     return LM_OK;
   }

@@ -941,7 +941,7 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
 void release_banded(lm_db *db) {
   ++db->generation;  // any captured launch that read the old table is stale
   release_one(db, db->banded);
-  release_one(db, db->banded_b);
+  for (Banded &b : db->banded_bk) release_one(db, b);
 }
 
 int build_banded(lm_db *db, bool best_effort) {
@@ -954,16 +954,23 @@ int build_banded(lm_db *db, bool best_effort) {
     set_error("2^16-row super-tiles keep 16-bit sums: MaxNrLaueSpots must be at most 257");
     return LM_ERR_INVALID;
   }
-  if (db->batch_tb != 0 && (db->batch_tb < 9 || db->batch_tb > 13)) {
-    set_error("batch super-tile bits must be 0 or 9..13");
-    return LM_ERR_INVALID;
-  }
+  for (const int8_t tb : db->width_tb)
+    if (tb != 0 && (tb < 9 || tb > 13)) {
+      set_error("batch super-tile bits must be 0 or 9..13");
+      return LM_ERR_INVALID;
+    }
   int rc = build_one(db, db->banded, db->banded_tb, db->banded_tb >= 14);
   // without room for a copy (best effort), scoring reads the orientation-major
   // table instead, which gives the same results
   if (rc == LM_ERR_BUDGET && best_effort) rc = LM_OK;
-  if (rc == LM_OK) rc = build_one(db, db->banded_b, db->batch_tb, false);
-  if (rc == LM_ERR_BUDGET && best_effort) rc = LM_OK;
+  // one batch copy per super-tile size some width takes
+  for (int tb = 9; tb <= 13 && rc == LM_OK; ++tb) {
+    bool used = false;
+    for (const int8_t w : db->width_tb) used |= w == tb;
+    if (!used) continue;
+    rc = build_one(db, db->banded_bk[tb - 9], tb, false);
+    if (rc == LM_ERR_BUDGET && best_effort) rc = LM_OK;
+  }
   if (rc) release_banded(db);
   return rc;
 }
@@ -1102,13 +1109,16 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
 }
 
 bool banded_batch_fits(const lm_db *db, int B) {
-  const Banded &b = db->banded_b;
-  return b.ready && B >= 4 && B <= 64 && (uint64_t)B * (1ull << b.tb) <= 32768;
+  if (B < 4 || B > 64) return false;
+  const int tb = db->width_tb[width_log2(B)];
+  if (tb < 9 || tb > 13) return false;
+  const Banded &b = db->banded_bk[tb - 9];
+  return b.ready && (uint64_t)B * (1ull << b.tb) <= 32768;
 }
 
 int launch_banded_batch(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
                         hipStream_t st) {
-  const Banded &b = db->banded_b;
+  const Banded &b = db->banded_bk[db->width_tb[width_log2(B)] - 9];
   const Table &t = db->table;
   const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
   const unsigned threads = (1u << b.tb) < 1024 ? (1u << b.tb) : 1024;

@@ -192,10 +192,11 @@ struct lm_db {
   int fwd_plan_entries = 0, fwd_plan_chunks = 0;
   lm::Table table;
   lm::Banded banded;            // single-image copy
-  lm::Banded banded_b;          // batch copy (smaller super-tiles, B x T LDS counters)
+  lm::Banded banded_bk[5];      // batch copies of 2^9..2^13-row super-tiles (B x T LDS counters)
   int banded_tb = 16;           // super-tile bits for the banded copy (0 = no copy)
   int banded_loads = 3;         // single-image walk (LM_OPT_BANDED_LOADS)
-  int batch_tb = 0;             // super-tile bits of the batch copy (0 = no copy)
+  int batch_tb = 0;             // LM_OPT_BATCH_TILE_BITS: the copy every width takes by default
+  int8_t width_tb[9] = {};      // per width 2^k: super-tile bits of its batch copy (0 = none)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy
   double width_ms[9] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
   uint32_t group_widths = 0xc0; // bit log2(B): table batches of width B use grouped lanes

@@ -63,9 +63,16 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
   const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths,
                  saved_nib = db->nib_widths;
+  int8_t saved_wtb[9];
+  memcpy(saved_wtb, db->width_tb, sizeof saved_wtb);
+  auto set_all_widths = [&](int tb) {  // every width that tb's counters hold
+    for (int k = 0; k < 9; ++k)
+      db->width_tb[k] = (tb && k >= 2 && k <= 6 && (1 << k) * (1 << tb) <= 32768) ? (int8_t)tb : 0;
+  };
   auto restore = [&](int code) {
     db->banded_tb = saved_tb;
     db->batch_tb = saved_btb;
+    memcpy(db->width_tb, saved_wtb, sizeof saved_wtb);
     db->batch_widths = saved_mask;
     db->banded_loads = saved_loads;
     db->group_widths = saved_group;
@@ -75,6 +82,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   };
   // single image: the table, then 2^13 .. 2^16 super-tiles x entry loads
   db->batch_tb = 0;
+  set_all_widths(0);
   db->banded_tb = 0;
   rc = build_banded(db);
   if (rc) return restore(rc);
@@ -111,6 +119,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   rep->n_widths = nw;
   db->banded_tb = 0;  // room for the batch copies
   db->batch_tb = 0;
+  set_all_widths(0);
   rc = build_banded(db);
   if (rc) return restore(rc);
   // the table walks: one lane per orientation (up to 128 images), grouped
@@ -162,6 +171,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->batch_widths = 0x7f;
   for (int tb = 9; tb <= 13; ++tb) {
     db->batch_tb = tb;
+    set_all_widths(tb);
     rc = build_banded(db);
     if (rc) return restore(rc);
     for (int w = 0; w < nw; ++w) {
@@ -171,29 +181,26 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
       if (rc) return restore(rc);
     }
   }
-  // batch copy: the size with the least per-image time summed over the
-  // widths (each width taking the faster of the copy and the table); none if
-  // the table alone is as fast
-  double best_cost = 0;
-  for (int w = 0; w < nw; ++w) best_cost += tbl[w] / rep->widths[w];
-  int best_btb = 0;
-  for (int tb = 9; tb <= 13; ++tb) {
-    double cost = 0;
-    for (int w = 0; w < nw; ++w) {
-      const double ms = rep->ms_banded[w][tb - 9];
-      cost += (ms > 0 ? std::min(ms, tbl[w]) : tbl[w]) / rep->widths[w];
-    }
-    if (cost < best_cost) {
-      best_cost = cost;
-      best_btb = tb;
-    }
-  }
+  // per width: the fastest batch copy when it beats the table walks (each
+  // width may keep a different super-tile size; the copies are built once)
   uint32_t mask = 0;
-  if (best_btb)
-    for (int w = 0; w < nw; ++w) {
-      const double ms = rep->ms_banded[w][best_btb - 9];
-      if (ms > 0 && ms < tbl[w]) mask |= 1u << width_log2(rep->widths[w]);
+  int8_t wtb[9] = {};
+  for (int w = 0; w < nw; ++w) {
+    const int k = width_log2(rep->widths[w]);
+    double best = tbl[w];
+    for (int tb = 9; tb <= 13; ++tb) {
+      const double ms = rep->ms_banded[w][tb - 9];
+      if (ms > 0 && ms < best) {
+        best = ms;
+        wtb[k] = (int8_t)tb;
+      }
     }
+    rep->batch_tile_bits_w[w] = wtb[k];
+    if (wtb[k]) mask |= 1u << k;
+  }
+  int best_btb = 0;  // reported: the copy of the narrowest width that keeps one
+  for (int w = nw - 1; w >= 0; --w)
+    if (wtb[width_log2(rep->widths[w])]) best_btb = wtb[width_log2(rep->widths[w])];
   rep->grouped_mask = gmask;
   rep->nibble_mask = nmask;
   rep->banded_tile_bits = best_tb;
@@ -203,13 +210,14 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->banded_tb = best_tb;
   db->banded_loads = best_loads;
   db->batch_tb = best_btb;
+  memcpy(db->width_tb, wtb, sizeof wtb);
   db->batch_widths = mask;
   // cost table for the server's batch-size choice
   for (double &v : db->width_ms) v = 0;
   db->width_ms[0] = best1;
   for (int w = 0; w < nw; ++w) {
     const int k = width_log2(rep->widths[w]);
-    db->width_ms[k] = ((mask >> k) & 1) ? rep->ms_banded[w][best_btb - 9] : tbl[w];
+    db->width_ms[k] = ((mask >> k) & 1) ? rep->ms_banded[w][wtb[k] - 9] : tbl[w];
   }
   rc = build_banded(db);  // also bumps the generation: no stale captured launch
   if (rc) return restore(rc);

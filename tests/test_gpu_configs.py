@@ -124,7 +124,7 @@ def batch_imgs(cfg0):
 
 
 @pytest.mark.parametrize("B", [8, 16, 32, 64])
-@pytest.mark.parametrize("walk", ["table", "banded", "autotuned"])
+@pytest.mark.parametrize("walk", ["table", "banded", "per_width", "autotuned"])
 def test_config2_batches_vs_oracle(cfg0, batch_imgs, B, walk):
     p, hk, O, want, grains, img, db = cfg0
     if db.table_spots == 0:
@@ -136,6 +136,12 @@ def test_config2_batches_vs_oracle(cfg0, batch_imgs, B, walk):
     elif walk == "banded":
         bits = {8: 12, 16: 11, 32: 10, 64: 9}[B]
         db.set_batch_tile_bits(bits)
+        db.set_batch_widths(0x7f)
+    elif walk == "per_width":
+        # every width its own copy at once (2^12, 2^11, 2^10, 2^9 rows)
+        db.set_batch_tile_bits(0)
+        for w, bits in ((8, 12), (16, 11), (32, 10), (64, 9)):
+            db.set_batch_width_tile_bits(w, bits)
         db.set_batch_widths(0x7f)
     else:
         db.autotune([u for u, _, _ in imgs[:3]], max_batch=64, reps=1)

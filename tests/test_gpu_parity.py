@@ -617,8 +617,12 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
         for B, row in rep["ms_batch"].items():
             assert row["table"] > 0
             use = (rep["batch_widths_mask"] >> (B.bit_length() - 1)) & 1
-            if use:
-                assert row[f"banded{rep['batch_tile_bits']}"] < row["table"]
+            wtb = rep["batch_tile_bits_per_width"][B]
+            assert bool(use) == bool(wtb)
+            if use:  # the width's own copy, the fastest of its copies, beats the table
+                assert row[f"banded{wtb}"] < row["table"]
+                assert row[f"banded{wtb}"] == min(v for k, v in row.items()
+                                                  if k.startswith("banded"))
         got = db.score_batch(samples * 8, [1.0] * 16, 5, 500.0)
         for i in (0, 1):
             matched, ns, tot, nres = ob.compare(table, 64, samples[i], p.NrPxX, 1.0, 5, 500.0)
