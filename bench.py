@@ -479,14 +479,105 @@ def config4(ctx, O, n, steps, batch_imgs):
     return res
 
 
-def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=512, conns=8):
+def host_cpus():
+    """CPUs this process may run on, and the cgroup CPU quota if one is set."""
+    out = {"affinity": len(os.sched_getaffinity(0))}
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        out["cgroup_quota_cpus"] = None if q == "max" else int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    return out
+
+
+def tcp_sink_rate(items, conns, count):
+    """Frames per second that `conns` sender threads (the daemon legs' own
+    senders: one queue and thread per connection, sendall of each part of a
+    frame) push over loopback TCP into receivers that discard them -- the
+    ingest ceiling of a closed loop of `count` such frames.  items: frames as
+    tuples of byte parts.  Returns (frames/s, cores busy in this process)."""
+    import queue
+    import socket
+    import threading
+
+    srv = socket.socket()
+    srv.bind(("127.0.0.1", 0))
+    srv.listen(conns)
+    port = srv.getsockname()[1]
+    sizes = [sum(len(x) for x in it) for it in items]
+    expect = [0] * conns
+    for j in range(count):
+        expect[j % conns] += sizes[j % len(items)]
+    ends = []
+    lock = threading.Lock()
+
+    def receiver(c):
+        who = bytearray(4)
+        c.recv_into(who, 4, socket.MSG_WAITALL)
+        left = expect[int.from_bytes(who, "little")]
+        buf = bytearray(1 << 24)
+        while left:
+            n = c.recv_into(buf, min(len(buf), left))
+            if n == 0:
+                break
+            left -= n
+        with lock:
+            ends.append(time.perf_counter())
+        c.close()
+
+    socks = [socket.create_connection(("127.0.0.1", port), timeout=60) for _ in range(conns)]
+    rts = []
+    for _ in range(conns):
+        c, _ = srv.accept()
+        rts.append(threading.Thread(target=receiver, args=(c,), daemon=True))
+        rts[-1].start()
+    for k, s in enumerate(socks):
+        s.sendall(k.to_bytes(4, "little"))
+    qs = [queue.Queue() for _ in range(conns)]
+
+    def sender(k):
+        while True:
+            item = qs[k].get()
+            if item is None:
+                return
+            for part in item:
+                socks[k].sendall(part)
+
+    ths = [threading.Thread(target=sender, args=(k,), daemon=True) for k in range(conns)]
+    for th in ths:
+        th.start()
+    c0 = os.times()
+    t0 = time.perf_counter()
+    for j in range(count):
+        qs[j % conns].put(items[j % len(items)])
+    for q in qs:
+        q.put(None)
+    for th in ths:
+        th.join()
+    for th in rts:
+        th.join(timeout=120)
+    for s in socks:
+        s.close()
+    srv.close()
+    c1 = os.times()
+    if len(ends) != conns:
+        return None, None
+    wall = max(ends) - t0
+    return count / wall, (c1.user + c1.system - c0.user - c0.system) / wall
+
+
+def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=2048, warm=512,
+               conns=8):
     """bin/LaueMatchingGPUStream -- the drop-in daemon the reference's senders
     talk to -- on the same 100M-orientation DB, fed wire-v1 frames (uint16
     image number + float32 pixels, the reference's format) over TCP from
-    `conns` connections: Poisson arrivals at each rate, then a closed loop of
-    `closed` frames sent back to back.  Latencies are the daemon's own: to the
-    compact hit list (frame received -> quantized, queued, scored) and end to
-    end (frame received -> merge, fit and result rows written)."""
+    `conns` connections: Poisson arrivals at each rate, then closed loops
+    (`warm` untimed frames, then `closed` timed frames sent back to back) of
+    wire-v1 float32 and wire-v2 uint8 frames, then the same senders into
+    receivers that discard the frames (the loopback TCP ceiling).  Latencies
+    are the daemon's own: to the compact hit list (frame received ->
+    quantized, queued, scored) and end to end (frame received -> merge, fit
+    and result rows written)."""
     import queue
     import re
     import shutil
@@ -522,6 +613,9 @@ def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=5
         ready = threading.Event()
         rx_g = re.compile(r"\[Image (\d+)\] GPU: .*hit list ([0-9.]+) ms after arrival")
         rx_t = re.compile(r"\[Image (\d+)\] Total: ([0-9.]+) ms after arrival")
+        rx_r = re.compile(r"Received image (\d+) .*receive ([0-9.]+) ms, queued ([0-9.]+) ms, "
+                          r"quantize ([0-9.]+) ms, hand-over ([0-9.]+) ms")
+        stage = {}
         lock = threading.Lock()
 
         def reader():
@@ -543,6 +637,10 @@ def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=5
                 if m:
                     with lock:
                         total[int(m.group(1))] = (float(m.group(2)), now)
+                m = rx_r.search(ln)
+                if m:
+                    with lock:
+                        stage[int(m.group(1))] = [float(m.group(k)) for k in range(2, 6)]
             ready.set()
 
         rt = threading.Thread(target=reader, daemon=True)
@@ -582,10 +680,27 @@ def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=5
             with lock:
                 g = np.array([gpu[x][0] for x in nums if x in gpu])
                 t = np.array([total[x][0] for x in nums if x in total])
+                st = np.array([stage[x] for x in nums if x in stage]).reshape(-1, 4)
             return {"p50_ms": float(np.percentile(g, 50)), "p99_ms": float(np.percentile(g, 99)),
                     "end_to_end_p50_ms": float(np.percentile(t, 50)),
-                    "end_to_end_p99_ms": float(np.percentile(t, 99))}
+                    "end_to_end_p99_ms": float(np.percentile(t, 99)),
+                    "host_stage_mean_ms": dict(zip(("receive", "queued", "quantize", "hand_over"),
+                                                   st.mean(axis=0).tolist()))}
 
+        def cpu_now():
+            """CPU seconds so far of the daemon and of this process (the
+            senders and the stdout reader)."""
+            with open(f"/proc/{proc.pid}/stat") as f:
+                fields = f.read().rsplit(")", 1)[1].split()
+            tick = os.sysconf("SC_CLK_TCK")
+            t = os.times()
+            return (int(fields[11]) + int(fields[12])) / tick, t.user + t.system
+
+        def cores_used(c0, wall):
+            c1 = cpu_now()
+            return {"daemon_cores": (c1[0] - c0[0]) / wall, "client_cores": (c1[1] - c0[1]) / wall}
+
+        out["host_cpus"] = host_cpus()
         rng = np.random.default_rng(77)
         rows = []
         for r in rates:
@@ -606,20 +721,48 @@ def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=5
             log(f"daemon {r}/s: to hit list p50 {row['p50_ms']:.1f} p99 {row['p99_ms']:.1f} ms, "
                 f"end to end p50 {row['end_to_end_p50_ms']:.1f} p99 {row['end_to_end_p99_ms']:.1f} ms")
         out["per_rate"] = rows
-        nums = list(range(nxt[0], nxt[0] + closed))
-        nxt[0] += closed
-        t0 = time.perf_counter()
-        for j, num in enumerate(nums):
-            qs[j % conns].put((num, payloads[num % len(payloads)]))
-        if not wait_for(nums, 300):
-            raise RuntimeError("daemon closed-loop results missing")
-        with lock:
-            tg = max(gpu[x][1] for x in nums) - t0
-            tt = max(total[x][1] for x in nums) - t0
-        out["closed_loop"] = {"images": closed, "images_per_s_to_hit_list": closed / tg,
-                              "images_per_s_end_to_end": closed / tt, **stats(nums)}
-        log(f"daemon closed loop: {closed / tg:.0f} img/s to the hit list, {closed / tt:.0f} img/s "
-            f"end to end")
+
+        def closed_loop(key, label, put):
+            """`warm` frames sent back to back (untimed: the daemon's frame
+            buffers and the server's batch widths warm up), then `closed`
+            frames timed: the rate over the whole loop (including the
+            pipeline's fill and drain) and over the middle half of the hit
+            lists (the sustained rate)."""
+            warm_nums = list(range(nxt[0], nxt[0] + warm))
+            nxt[0] += warm
+            for j, num in enumerate(warm_nums):
+                put(j, num)
+            if not wait_for(warm_nums, 300):
+                raise RuntimeError(f"daemon {label} warm-up results missing")
+            nums = list(range(nxt[0], nxt[0] + closed))
+            nxt[0] += closed
+            c0 = cpu_now()
+            t0 = time.perf_counter()
+            for j, num in enumerate(nums):
+                put(j, num)
+            if not wait_for(nums, 300):
+                raise RuntimeError(f"daemon {label} results missing")
+            wall = time.perf_counter() - t0
+            with lock:
+                tg = sorted(gpu[x][1] for x in nums)
+                tt = max(total[x][1] for x in nums) - t0
+            lo, hi = len(tg) // 4, 3 * len(tg) // 4
+            res = {"images": closed, "warmup_images": warm,
+                   "images_per_s_to_hit_list": closed / (tg[-1] - t0),
+                   "sustained_images_per_s_to_hit_list": (hi - lo) / (tg[hi] - tg[lo]),
+                   "images_per_s_end_to_end": closed / tt, **stats(nums),
+                   **cores_used(c0, wall)}
+            out[key] = res
+            log(f"daemon closed loop ({label}): {res['images_per_s_to_hit_list']:.0f} img/s to the "
+                f"hit list over {closed} frames, {res['sustained_images_per_s_to_hit_list']:.0f} "
+                f"img/s sustained (middle half), {res['images_per_s_end_to_end']:.0f} img/s end "
+                f"to end; p50 {res['p50_ms']:.1f} ms; host stages per frame (ms) "
+                f"{ {k: round(v, 2) for k, v in res['host_stage_mean_ms'].items()} }; cores busy: "
+                f"daemon {res['daemon_cores']:.1f}, senders + reader {res['client_cores']:.1f} "
+                f"(host {out['host_cpus']})")
+
+        closed_loop("closed_loop", "wire v1 float32 frames",
+                    lambda j, num: qs[j % conns].put((num, payloads[num % len(payloads)])))
         for q in qs:
             q.put(None)
         for th in ths:
@@ -650,31 +793,32 @@ def daemon_leg(O, hk, p, frames, rates=(1, 20, 100, 200), duration=3.0, closed=5
         ths = [threading.Thread(target=sender2, args=(k,), daemon=True) for k in range(conns)]
         for th in ths:
             th.start()
-        nums = list(range(nxt[0], nxt[0] + closed))
-        nxt[0] += closed
-        t0 = time.perf_counter()
-        for j, num in enumerate(nums):
-            qs[j % conns].put((num, u8p[num % len(u8p)]))
-        if not wait_for(nums, 300):
-            raise RuntimeError("daemon closed-loop (uint8 frames) results missing")
-        with lock:
-            tg = max(gpu[x][1] for x in nums) - t0
-            tt = max(total[x][1] for x in nums) - t0
-        out["closed_loop_u8_frames"] = {"images": closed, "images_per_s_to_hit_list": closed / tg,
-                                        "images_per_s_end_to_end": closed / tt, **stats(nums)}
-        log(f"daemon closed loop (wire v2 uint8 frames): {closed / tg:.0f} img/s to the hit list, "
-            f"{closed / tt:.0f} img/s end to end")
+        closed_loop("closed_loop_u8_frames", "wire v2 uint8 frames",
+                    lambda j, num: qs[j % conns].put((num, u8p[num % len(u8p)])))
         for q in qs:
             q.put(None)
         for th in ths:
             th.join()
         for so in socks:
             so.close()
+        # the same senders and frames into receivers that discard them
+        ceil_v1, cores_v1 = tcp_sink_rate([(struct.pack("<H", 1), pl) for pl in payloads], conns,
+                                          closed)
+        ceil_u8, cores_u8 = tcp_sink_rate([(struct.pack("<II", 0x32464D4C, 1) + h, pl)
+                                           for h, pl in u8p], conns, closed)
+        out["tcp_ingest_ceiling"] = {"v1_float32_images_per_s": ceil_v1,
+                                     "v2_uint8_images_per_s": ceil_u8,
+                                     "v1_cores": cores_v1, "v2_cores": cores_u8}
+        log(f"loopback TCP ingest ceiling ({conns} connections, same senders, frames discarded): "
+            f"{ceil_v1:.0f} img/s float32 frames ({cores_v1:.1f} cores), {ceil_u8:.0f} img/s "
+            f"uint8 frames ({cores_u8:.1f} cores)")
         proc.send_signal(signal.SIGTERM)
         proc.wait(timeout=120)
         rt.join(timeout=10)
         out["exit_code"] = proc.returncode
         out["final_line"] = next((l for l in tail if l.startswith("Processed")), None)
+        out["host_time_line"] = next((l for l in tail if l.startswith("Host time")), None)
+        log(f"daemon: {out['final_line']}; {out['host_time_line']}")
     finally:
         if proc is not None and proc.poll() is None:
             proc.kill()

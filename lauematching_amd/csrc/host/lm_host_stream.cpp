@@ -43,7 +43,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -82,7 +84,8 @@ struct Job {
   uint32_t image_num;
   int db;
   uint64_t ticket;
-  std::vector<float> image;  // the frame's pixels (for the fit), from the pool
+  int type;                  // pixel type of raw: 0 float32, 1 float64, 2 uint16, 3 uint8
+  std::vector<uint8_t> raw;  // the frame's pixels as received (for the fit), from the pool
   double t_arrival;          // frame fully received
   double t_submit;           // quantized, handed to the server
 };
@@ -101,6 +104,20 @@ struct DbSlot {
   FILE *sol = nullptr, *spots = nullptr;
 };
 
+// A scored frame with hits, waiting for the merge and fit.  seq orders the
+// result rows: they are appended in the order the frames were submitted.
+struct PostJob {
+  uint64_t seq;
+  Job job;
+  lm_hits hits;
+};
+
+struct PostDone {
+  Job job;
+  int nsol;
+  std::string sol, spots;
+};
+
 struct Daemon {
   std::vector<std::unique_ptr<DbSlot>> dbs;
   lm_server *server = nullptr;
@@ -108,31 +125,96 @@ struct Daemon {
   std::condition_variable cv;
   std::deque<Job> jobs;
   int open_clients = 0;
+  // received frames waiting for a quantize worker; `unsubmitted` counts them
+  // (under mu) until their job is queued or dropped
+  struct Frame {
+    int db;
+    uint32_t num;
+    int type;
+    std::vector<uint8_t> raw;
+    double t_arrival;
+    double recv_ms;
+  };
+  std::mutex in_mu;
+  std::condition_variable in_cv;
+  std::deque<Frame> intake;
+  bool in_closed = false;
+  int unsubmitted = 0;
   std::vector<std::thread> handlers;
-  // frame buffers handed from the connection handlers to the result loop and
-  // back, so a frame is received into a buffer that the fit then reads
+  // Frame buffers handed from the connection handlers to the fit and back.
+  // They are kept for reuse (no allocation, zero fill or page faults per
+  // frame) up to pool_bytes in all; a handler that needs more waits until a
+  // frame is done, which holds the sender back as the reference's bounded
+  // frame queue does (src/LaueMatchingGPUStream.cu:45).  Frames stay in
+  // their buffer from arrival until the fit, so the frames in flight -- the
+  // rate times that time -- must fit: the default 8 GiB holds the server's
+  // 512 admitted frames of a 2048 x 2048 float32 detector
+  // (LM_STREAM_FRAME_POOL_GB overrides it).
+  size_t pool_bytes = size_t(8) << 30;
   std::mutex pool_mu;
-  std::vector<std::vector<float>> pool;
-  std::vector<float> take_buffer(size_t npx) {
-    std::lock_guard<std::mutex> lk(pool_mu);
-    std::vector<float> b;
-    if (!pool.empty()) {
-      b = std::move(pool.back());
-      pool.pop_back();
+  std::condition_variable pool_cv;
+  std::vector<std::vector<uint8_t>> pool;  // free buffers
+  size_t held = 0;                         // capacity of every buffer, free or in use
+  std::vector<uint8_t> take_buffer(size_t bytes) {
+    std::unique_lock<std::mutex> lk(pool_mu);
+    for (;;) {
+      // the smallest free buffer that holds the frame; one more than twice
+      // its size is only taken when no new buffer fits under the cap
+      int best = -1;
+      for (int i = 0; i < (int)pool.size(); ++i)
+        if (pool[i].capacity() >= bytes &&
+            (best < 0 || pool[i].capacity() < pool[best].capacity()))
+          best = i;
+      const bool room = held + bytes <= pool_bytes || held == 0;
+      if (best >= 0 && (pool[best].capacity() <= 2 * bytes || !room)) {
+        std::vector<uint8_t> b = std::move(pool[best]);
+        pool.erase(pool.begin() + best);
+        b.resize(bytes);
+        return b;
+      }
+      if (room) {
+        held += bytes;
+        lk.unlock();
+        std::vector<uint8_t> b(bytes);
+        return b;
+      }
+      int small = -1;  // a free buffer too small for the frame: make room
+      for (int i = 0; i < (int)pool.size(); ++i)
+        if (pool[i].capacity() < bytes) small = i;
+      if (small >= 0) {
+        held -= pool[small].capacity();
+        pool.erase(pool.begin() + small);
+        continue;
+      }
+      pool_cv.wait(lk);
     }
-    b.resize(npx);
-    return b;
   }
-  void give_buffer(std::vector<float> &&b) {
-    std::lock_guard<std::mutex> lk(pool_mu);
-    if (pool.size() < 64) pool.push_back(std::move(b));
+  void give_buffer(std::vector<uint8_t> &&b) {
+    {
+      std::lock_guard<std::mutex> lk(pool_mu);
+      if (b.capacity()) pool.push_back(std::move(b));
+    }
+    pool_cv.notify_one();
   }
+  // host time spent per stage over all frames (ns): payload receive (from
+  // its header on, so it includes waiting for the sender), quantization,
+  // hand-over to the server (the copy into its admission slot)
+  std::atomic<int64_t> ns_recv{0}, ns_quant{0}, ns_submit{0};
+  std::atomic<uint64_t> frames{0};
+  // merge + fit workers and the in-order writer of their rows
+  std::mutex post_mu;
+  std::condition_variable post_cv;
+  std::deque<PostJob> post;
+  bool post_closed = false;
+  std::mutex out_mu;
+  std::map<uint64_t, PostDone> done;
+  uint64_t next_out = 0;
 };
 
 bool recv_all(int fd, void *buf, size_t len) {
   size_t got = 0;
   while (got < len) {
-    const ssize_t n = recv(fd, (char *)buf + got, len - got, 0);
+    const ssize_t n = recv(fd, (char *)buf + got, len - got, MSG_WAITALL);
     if (n < 0) {
       if (errno == EINTR) continue;
       if (errno == EAGAIN || errno == EWOULDBLOCK) {
@@ -147,62 +229,143 @@ bool recv_all(int fd, void *buf, size_t len) {
   return true;
 }
 
-// Quantize one frame (src/LaueMatchingCPU.c:355-377) -- from its float64
-// pixels when the wire carried them (f64 non-null), else from the float32
-// pixels, which is the same arithmetic -- submit it to the server for DB k and
-// queue it for post-processing (which reads the float32 pixels, as the
-// reference daemon's fit does).  img moves into the job.
-bool submit_frame(Daemon *d, int k, uint32_t num, std::vector<float> &img, const double *f64,
-                  std::vector<uint8_t> &u8, double t_arrival) {
+constexpr size_t kPixelBytes[4] = {4, 8, 2, 1};
+
+// The uint8 quantization of a frame (src/LaueMatchingCPU.c:355-377) from its
+// pixels as received.  Float frames go through lm_quantize_u8(_f32).  An
+// integer frame maps each value v to the level the quantizer gives v when
+// the frame's largest value is mx; that map is the quantization of the
+// values 0..mx themselves (largest mx), so one call of the library's
+// quantizer on them gives the table and the scale.  Returns true, with u8
+// left unwritten, for a uint8 frame whose table is the identity (largest
+// value 255, or none above 0): its levels are its own bytes.
+bool quantize_frame(int type, const uint8_t *raw, size_t npx, uint8_t *u8, double *scale) {
+  if (type == 0) {
+    lm_quantize_u8_f32((const float *)raw, npx, u8, scale);
+    return false;
+  }
+  if (type == 1) {
+    lm_quantize_u8((const double *)raw, npx, u8, scale);
+    return false;
+  }
+  uint32_t mx = 0;
+  if (type == 2) {
+    const uint16_t *v = (const uint16_t *)raw;
+    for (size_t i = 0; i < npx; ++i) mx = v[i] > mx ? v[i] : mx;
+  } else {
+    for (size_t i = 0; i < npx; ++i) mx = raw[i] > mx ? raw[i] : mx;
+  }
+  std::vector<float> vals(mx + 1);
+  for (uint32_t v = 0; v <= mx; ++v) vals[v] = (float)v;
+  std::vector<uint8_t> lut(mx + 1);
+  lm_quantize_u8_f32(vals.data(), mx + 1, lut.data(), scale);
+  if (type == 2) {
+    const uint16_t *v = (const uint16_t *)raw;
+    for (size_t i = 0; i < npx; ++i) u8[i] = lut[v[i]];
+    return false;
+  }
+  bool identity = true;
+  for (uint32_t v = 0; v <= mx; ++v) identity = identity && lut[v] == v;
+  if (identity) return true;
+  for (size_t i = 0; i < npx; ++i) u8[i] = lut[raw[i]];
+  return false;
+}
+
+// Quantize one frame, submit it to the server for DB k and queue it for the
+// hit list and post-processing.  raw moves into the job.
+bool submit_frame(Daemon *d, int k, uint32_t num, int type, std::vector<uint8_t> &raw,
+                  std::vector<uint8_t> &u8, double t_arrival, double recv_ms) {
   DbSlot &s = *d->dbs[k];
   u8.resize(s.npx);
   double scale = 1.0;
-  if (f64)
-    lm_quantize_u8(f64, s.npx, u8.data(), &scale);
-  else
-    lm_quantize_u8_f32(img.data(), s.npx, u8.data(), &scale);
+  const double t_q0 = omp_get_wtime();
+  const bool same = quantize_frame(type, raw.data(), s.npx, u8.data(), &scale);
   const double t_submit = omp_get_wtime();
+  d->ns_quant += (int64_t)((t_submit - t_q0) * 1e9);
   uint64_t ticket = 0;
-  if (lm_server_submit(d->server, s.server_id, u8.data(), scale, s.p.min_nr_spots,
+  if (lm_server_submit(d->server, s.server_id, same ? raw.data() : u8.data(), scale,
+                       s.p.min_nr_spots,
                        s.p.min_intensity, &ticket) != LM_OK) {
     fprintf(stderr, "submit failed for image %u: %s\n", num, lm_last_error());
+    d->give_buffer(std::move(raw));
+    {
+      std::lock_guard<std::mutex> lk(d->mu);
+      --d->unsubmitted;
+    }
+    d->cv.notify_one();
     return false;
   }
-  printf("Received image %u for DB %d (%zu pixels)\n", num, k, s.npx);
+  const double t_handed = omp_get_wtime();
+  d->ns_submit += (int64_t)((t_handed - t_submit) * 1e9);
+  ++d->frames;
+  printf("Received image %u for DB %d (%zu bytes wire, %zu pixels; receive %.2f ms, queued "
+         "%.2f ms, quantize %.2f ms, hand-over %.2f ms)\n", num, k, raw.size(), s.npx, recv_ms,
+         (t_q0 - t_arrival) * 1e3, (t_submit - t_q0) * 1e3, (t_handed - t_submit) * 1e3);
   fflush(stdout);
   {
     std::lock_guard<std::mutex> lk(d->mu);
-    d->jobs.push_back(Job{num, k, ticket, std::move(img), t_arrival, t_submit});
+    d->jobs.push_back(Job{num, k, ticket, type, std::move(raw), t_arrival, t_submit});
+    --d->unsubmitted;
   }
   d->cv.notify_one();
   return true;
+}
+
+// Hands a received frame to the quantize workers, so that the connection
+// goes back to receiving at once.
+void queue_frame(Daemon *d, int k, uint32_t num, int type, std::vector<uint8_t> &&raw,
+                 double t_arrival, double recv_ms) {
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    ++d->unsubmitted;
+  }
+  {
+    std::lock_guard<std::mutex> lk(d->in_mu);
+    d->intake.push_back(Daemon::Frame{k, num, type, std::move(raw), t_arrival, recv_ms});
+  }
+  d->in_cv.notify_one();
+}
+
+// Quantize worker: quantizes received frames and submits them to the server.
+void quantize_worker(Daemon *d) {
+  std::vector<uint8_t> u8;
+  for (;;) {
+    Daemon::Frame f;
+    {
+      std::unique_lock<std::mutex> lk(d->in_mu);
+      d->in_cv.wait(lk, [&] { return !d->intake.empty() || d->in_closed; });
+      if (d->intake.empty()) return;
+      f = std::move(d->intake.front());
+      d->intake.pop_front();
+    }
+    submit_frame(d, f.db, f.num, f.type, f.raw, u8, f.t_arrival, f.recv_ms);
+  }
 }
 
 // v1: uint16 image number + float32 pixels of DB 0's detector
 // (scripts/laue_image_server.py:217-219, src/LaueMatchingGPUStream.cu:295-383)
 void serve_v1(Daemon *d, int fd) {
   const size_t npx = d->dbs[0]->npx;
-  std::vector<uint8_t> u8;
   printf("Client handler started (socket %d), wire_payload=%zu bytes\n", fd, npx * 4);
   fflush(stdout);
   while (g_running) {
     uint8_t hdr[2];
     if (!recv_all(fd, hdr, 2)) break;
-    std::vector<float> buf = d->take_buffer(npx);
+    const double t0 = omp_get_wtime();
+    std::vector<uint8_t> buf = d->take_buffer(npx * 4);
     if (!recv_all(fd, buf.data(), npx * 4)) {
       d->give_buffer(std::move(buf));
       break;
     }
+    const double t1 = omp_get_wtime();
+    d->ns_recv += (int64_t)((t1 - t0) * 1e9);
     const uint16_t num = (uint16_t)(hdr[0] | (hdr[1] << 8));
-    if (!submit_frame(d, 0, num, buf, nullptr, u8, omp_get_wtime())) break;
+    queue_frame(d, 0, num, 0, std::move(buf), t1, (t1 - t0) * 1e3);
   }
 }
 
 // v2: per-frame header (see the top of this file)
 void serve_v2(Daemon *d, int fd) {
-  std::vector<uint8_t> raw;
-  std::vector<double> f64;
-  std::vector<uint8_t> u8;
   printf("Client handler started (socket %d), wire v2\n", fd);
   fflush(stdout);
   while (g_running) {
@@ -215,7 +378,6 @@ void serve_v2(Daemon *d, int fd) {
     const uint32_t magic = u32(0), num = u32(4), W = u32(12), H = u32(16), resv = u32(20);
     const int k = h[8] | (h[9] << 8);
     const int type = h[10] | (h[11] << 8);
-    static const size_t kSize[4] = {4, 8, 2, 1};
     if (magic != kV2Magic || resv != 0 || type > 3 || k >= (int)d->dbs.size() ||
         (int64_t)W != d->dbs[k]->p.nrPxX || (int64_t)H != d->dbs[k]->p.nrPxY) {
       fprintf(stderr, "wire v2: bad frame header (magic %08x, DB %d, type %d, %u x %u); "
@@ -223,21 +385,15 @@ void serve_v2(Daemon *d, int fd) {
       break;
     }
     const size_t npx = (size_t)W * H;
-    raw.resize(npx * kSize[type]);
-    if (!recv_all(fd, raw.data(), raw.size())) break;
-    const double t_arrival = omp_get_wtime();
-    std::vector<float> img = d->take_buffer(npx);
-    if (type == 1) f64.resize(npx);
-    for (size_t i = 0; i < npx; ++i) {
-      const uint8_t *q = raw.data() + i * kSize[type];
-      switch (type) {
-        case 0: { float v; memcpy(&v, q, 4); img[i] = v; break; }
-        case 1: { double v; memcpy(&v, q, 8); f64[i] = v; img[i] = (float)v; break; }
-        case 2: img[i] = (float)(q[0] | (q[1] << 8)); break;
-        default: img[i] = (float)q[0]; break;
-      }
+    const double t0 = omp_get_wtime();
+    std::vector<uint8_t> buf = d->take_buffer(npx * kPixelBytes[type]);
+    if (!recv_all(fd, buf.data(), buf.size())) {
+      d->give_buffer(std::move(buf));
+      break;
     }
-    if (!submit_frame(d, k, num, img, type == 1 ? f64.data() : nullptr, u8, t_arrival)) break;
+    const double t1 = omp_get_wtime();
+    d->ns_recv += (int64_t)((t1 - t0) * 1e9);
+    queue_frame(d, k, num, type, std::move(buf), t1, (t1 - t0) * 1e3);
   }
 }
 
@@ -480,6 +636,11 @@ int main(int argc, char **argv) {
   if (lfd < 0 || lfd2 < 0) return 1;
   g_listen_fd = lfd;
   g_listen_fd2 = lfd2;
+  if (const char *pe = getenv("LM_STREAM_FRAME_POOL_GB"))
+    if (atof(pe) > 0) d.pool_bytes = (size_t)(atof(pe) * (double)(1ull << 30));
+  const int nQuant = std::max(2, std::min(8, numProcs / 2));
+  std::vector<std::thread> quantizers;
+  for (int i = 0; i < nQuant; ++i) quantizers.emplace_back(quantize_worker, &d);
   std::thread acceptor(accept_loop, &d, lfd, false);
   std::thread acceptor2(accept_loop, &d, lfd2, true);
   const DbSlot &d0 = *d.dbs[0];
@@ -494,24 +655,107 @@ int main(int argc, char **argv) {
          d0.npx * 4 / 1e6, cfg.max_batch, cfg.workers, omp_get_wtime() - t_init);
   fflush(stdout);
 
-  uint64_t processed = 0;
+  // Merge + fit workers: the rows are appended in submission order by
+  // whichever worker completes the next one.  A frame that is alone in the
+  // post-processing stage is formatted by a team of threads (its latency);
+  // under load each worker formats its own frame on one thread, so no
+  // OpenMP team is started whose idle threads would spin on the cores the
+  // connection handlers need.
+  const int nPost = std::min(8, numProcs);
+  const int teamThreads = std::min(4, numProcs);
+  int postActive = 0;
+  auto post_worker = [&] {
+    for (;;) {
+      PostJob pj;
+      int threads = 1;
+      {
+        std::unique_lock<std::mutex> lk(d.post_mu);
+        d.post_cv.wait(lk, [&] { return !d.post.empty() || d.post_closed; });
+        if (d.post.empty()) return;
+        pj = std::move(d.post.front());
+        d.post.pop_front();
+        if (postActive == 0 && d.post.empty()) threads = teamThreads;
+        ++postActive;
+      }
+      DbSlot &s = *d.dbs[pj.job.db];
+      const size_t npx = s.npx;
+      std::vector<float> conv;
+      PostInput in;
+      if (pj.job.type == 0) {
+        in.image_f = (const float *)pj.job.raw.data();
+      } else if (pj.job.type == 1) {
+        in.image = (const double *)pj.job.raw.data();
+      } else {
+        conv.resize(npx);
+        const uint8_t *r = pj.job.raw.data();
+        if (pj.job.type == 2)
+          for (size_t i = 0; i < npx; ++i) conv[i] = ((const uint16_t *)r)[i];
+        else
+          for (size_t i = 0; i < npx; ++i) conv[i] = r[i];
+        in.image_f = conv.data();
+      }
+      in.orients = (const double *)s.orient.data;
+      in.n_orients = s.n;
+      in.hits = &pj.hits;
+      in.nr_results = pj.hits.nr_results;
+      in.params = &s.fitp;
+      in.hkls = s.hkls.data();
+      in.nhkls = s.nhkls;
+      in.num_procs = threads;
+      in.ctx = ctx;
+      in.image_nr = (int)pj.job.image_num;
+      PostDone r;
+      r.nsol = format_solutions(in, &r.sol, &r.spots);
+      lm_hits_free(&pj.hits);
+      {
+        std::lock_guard<std::mutex> lk(d.post_mu);
+        --postActive;
+      }
+      r.job = std::move(pj.job);
+      std::lock_guard<std::mutex> lk(d.out_mu);
+      d.done.emplace(pj.seq, std::move(r));
+      for (auto it = d.done.find(d.next_out); it != d.done.end();
+           it = d.done.find(d.next_out)) {
+        PostDone &o = it->second;
+        DbSlot &t = *d.dbs[o.job.db];
+        fputs(o.spots.c_str(), t.spots);
+        fputs(o.sol.c_str(), t.sol);
+        fflush(t.sol);
+        fflush(t.spots);
+        if (o.nsol >= 0)
+          printf("[Image %u] %d unique orientations found\n", o.job.image_num, o.nsol);
+        printf("[Image %u] Total: %.1f ms after arrival\n", o.job.image_num,
+               (omp_get_wtime() - o.job.t_arrival) * 1e3);
+        fflush(stdout);
+        d.give_buffer(std::move(o.job.raw));
+        d.done.erase(it);
+        ++d.next_out;
+      }
+    }
+  };
+  std::vector<std::thread> posters;
+  for (int i = 0; i < nPost; ++i) posters.emplace_back(post_worker);
+
+  // Hit lists in submission order: each is reported when the server completes
+  // it; frames with matches go on to the merge and fit.
+  uint64_t processed = 0, seq = 0;
   for (;;) {
     Job job;
     {
       std::unique_lock<std::mutex> lk(d.mu);
       d.cv.wait_for(lk, std::chrono::milliseconds(100), [&] { return !d.jobs.empty(); });
       if (d.jobs.empty()) {
-        if (!g_running && d.open_clients == 0) break;
+        if (!g_running && d.open_clients == 0 && d.unsubmitted == 0) break;
         continue;
       }
       job = std::move(d.jobs.front());
       d.jobs.pop_front();
     }
-    DbSlot &s = *d.dbs[job.db];
     lm_hits hits{};
     double lat = 0;
     if (lm_server_wait(d.server, job.ticket, &hits, &lat) != LM_OK) {
       fprintf(stderr, "[Image %u] scoring failed: %s\n", job.image_num, lm_last_error());
+      d.give_buffer(std::move(job.raw));
       continue;
     }
     // latencies from the frame's arrival: the hit list is ready when the
@@ -519,31 +763,29 @@ int main(int argc, char **argv) {
     printf("[Image %u] GPU: %.1f ms (queue + batch), %llu matches; hit list %.1f ms after "
            "arrival\n", job.image_num, lat, (unsigned long long)hits.nr_results,
            (job.t_submit - job.t_arrival) * 1e3 + lat);
+    ++processed;
     if (hits.nr_results == 0) {
       printf("[Image %u] No matches, skipping fitting.\n", job.image_num);
-    } else {
-      PostInput in;
-      in.orients = (const double *)s.orient.data;
-      in.n_orients = s.n;
-      in.hits = &hits;
-      in.nr_results = hits.nr_results;
-      in.params = &s.fitp;
-      in.hkls = s.hkls.data();
-      in.nhkls = s.nhkls;
-      in.image_f = job.image.data();
-      in.num_procs = numProcs;
-      in.ctx = ctx;
-      in.image_nr = (int)job.image_num;
-      const int ns = append_solutions(in, s.sol, s.spots);
-      printf("[Image %u] %d unique orientations found\n", job.image_num, ns);
+      printf("[Image %u] Total: %.1f ms after arrival\n", job.image_num,
+             (omp_get_wtime() - job.t_arrival) * 1e3);
+      fflush(stdout);
+      lm_hits_free(&hits);
+      d.give_buffer(std::move(job.raw));
+      continue;
     }
-    printf("[Image %u] Total: %.1f ms after arrival\n", job.image_num,
-           (omp_get_wtime() - job.t_arrival) * 1e3);
     fflush(stdout);
-    d.give_buffer(std::move(job.image));
-    lm_hits_free(&hits);
-    ++processed;
+    {
+      std::lock_guard<std::mutex> lk(d.post_mu);
+      d.post.push_back(PostJob{seq++, std::move(job), hits});
+    }
+    d.post_cv.notify_one();
   }
+  {
+    std::lock_guard<std::mutex> lk(d.post_mu);
+    d.post_closed = true;
+  }
+  d.post_cv.notify_all();
+  for (auto &t : posters) t.join();
   printf("\nShutting down...\n");
   shutdown(lfd, SHUT_RDWR);
   shutdown(lfd2, SHUT_RDWR);
@@ -552,11 +794,21 @@ int main(int argc, char **argv) {
   acceptor.join();
   acceptor2.join();
   for (auto &t : d.handlers) t.join();
+  {
+    std::lock_guard<std::mutex> lk(d.in_mu);
+    d.in_closed = true;
+  }
+  d.in_cv.notify_all();
+  for (auto &t : quantizers) t.join();
   lm_server_stats ss{};
   lm_server_get_stats(d.server, &ss);
   printf("Processed %llu images; latency p50 %.1f ms, p99 %.1f ms; %.1f images/s; "
          "%.2f images per launch\n", (unsigned long long)processed, ss.p50_ms, ss.p99_ms,
          ss.images_per_s, ss.mean_batch);
+  if (d.frames)
+    printf("Host time per frame: receive %.2f ms (from its header), quantize %.2f ms, "
+           "hand-over to the server %.2f ms\n", d.ns_recv / 1e6 / d.frames,
+           d.ns_quant / 1e6 / d.frames, d.ns_submit / 1e6 / d.frames);
   lm_server_destroy(d.server);
   for (auto &s : d.dbs) {
     fclose(s->sol);

@@ -180,7 +180,7 @@ Geo geo_from(const LmParams &p) {
   return g;
 }
 
-int emit(const PostInput &in, FILE *sol, FILE *spots) {
+int emit(const PostInput &in, std::string &solOut, std::string &spotsOut) {
   const LmParams &p = *in.params;
   const lm_hits &h = *in.hits;
   // The merge sees nrResults entries: the collected hits, then zeros for
@@ -278,8 +278,8 @@ int emit(const PostInput &in, FILE *sol, FILE *spots) {
     }
   }
   for (int s = 0; s < nsol; ++s) {
-    fputs(spotLines[s].c_str(), spots);
-    fputs(solLines[s].c_str(), sol);
+    spotsOut += spotLines[s];
+    solOut += solLines[s];
   }
   return nsol;
 }
@@ -316,15 +316,15 @@ int write_solutions(const PostInput &in, const char *sol_path, const char *spots
     return -1;
   }
   write_solution_headers(sol, spots, false);
-  const int n = emit(in, sol, spots);
+  std::string solRows, spotRows;
+  const int n = emit(in, solRows, spotRows);
+  fputs(spotRows.c_str(), spots);
+  fputs(solRows.c_str(), sol);
   fclose(spots);
   fclose(sol);
   return n;
 }
 
-int append_solutions(const PostInput &in, FILE *sol, FILE *spots) {
-  const int n = emit(in, sol, spots);
-  fflush(sol);
-  fflush(spots);
-  return n;
+int format_solutions(const PostInput &in, std::string *sol_rows, std::string *spot_rows) {
+  return emit(in, *sol_rows, *spot_rows);
 }

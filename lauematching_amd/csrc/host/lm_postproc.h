@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <string>
 
 #include "lauematch.h"
 #include "lm_params.h"
@@ -38,8 +39,10 @@ struct PostInput {
 // number of unique orientations, or -1 if a file cannot be opened.
 int write_solutions(const PostInput &in, const char *sol_path, const char *spots_path);
 
-// Merge + append rows to already-open files (streaming daemon).
-int append_solutions(const PostInput &in, FILE *sol, FILE *spots);
+// Merge + the rows of both files as text, in grain order, for the caller to
+// append (streaming daemon: several frames are formatted at once and written
+// in order).  Returns the number of unique orientations, or -1.
+int format_solutions(const PostInput &in, std::string *sol_rows, std::string *spot_rows);
 
 // Header lines of the two output files (src/LaueMatchingCPU.c:753-770).
 void write_solution_headers(FILE *sol, FILE *spots, bool image_column);

@@ -7,6 +7,7 @@
 // quantization (src/LaueMatchingCPU.c:355-377).
 
 #include <hip/hip_runtime.h>
+#include <immintrin.h>
 
 #include <cmath>
 #include <cstdio>
@@ -141,6 +142,130 @@ int score_uniform(lm_db *db, int nimg, const uint8_t *d, const double *scales, i
   return score_images(db, db->scratch, nimg, d, scales, sp, mi, st, outs);
 }
 
+// The uint8 quantization (src/LaueMatchingCPU.c:355-377): mx = the largest
+// pixel (strict >, from 0), then x > 0 -> v = x * (255 / mx) clamped to
+// [1, 255] and truncated, else 0.  The scalar loops are the definition.  The
+// AVX2 loops compute the same IEEE operations four pixels at a time: the
+// maximum as (x > m) ? x : m (what vmaxps/vmaxpd compute, NaN skipped like
+// the scalar test), v = x * inv, min(v, 255) then max(., 1), truncation, and
+// a mask for x > 0.  They are taken only when mx is finite: then v is finite
+// wherever x > 0 and the two agree on every input, NaN, signed zeros and
+// subnormals included (with mx = +inf the scalar code truncates a NaN).
+template <class T>
+double max_scalar(const T *p, uint64_t n) {
+  double mx = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    if ((double)p[i] > mx) mx = p[i];
+  return mx;
+}
+
+template <class T>
+void map_scalar(const T *p, uint64_t n, double inv, uint8_t *out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const double x = p[i];
+    if (x > 0) {
+      const double v = x * inv;
+      out[i] = (v >= 255.0) ? 255 : (v < 1.0) ? 1 : (uint8_t)v;
+    } else {
+      out[i] = 0;
+    }
+  }
+}
+
+__attribute__((target("avx2"))) double max_avx2(const float *p, uint64_t n) {
+  __m256 m0 = _mm256_setzero_ps(), m1 = m0, m2 = m0, m3 = m0;
+  uint64_t i = 0;
+  for (; i + 32 <= n; i += 32) {
+    m0 = _mm256_max_ps(_mm256_loadu_ps(p + i), m0);
+    m1 = _mm256_max_ps(_mm256_loadu_ps(p + i + 8), m1);
+    m2 = _mm256_max_ps(_mm256_loadu_ps(p + i + 16), m2);
+    m3 = _mm256_max_ps(_mm256_loadu_ps(p + i + 24), m3);
+  }
+  alignas(32) float lane[32];
+  _mm256_store_ps(lane, m0);
+  _mm256_store_ps(lane + 8, m1);
+  _mm256_store_ps(lane + 16, m2);
+  _mm256_store_ps(lane + 24, m3);
+  double mx = max_scalar(lane, 32);
+  const double t = max_scalar(p + i, n - i);
+  return t > mx ? t : mx;
+}
+
+__attribute__((target("avx2"))) double max_avx2(const double *p, uint64_t n) {
+  __m256d m0 = _mm256_setzero_pd(), m1 = m0, m2 = m0, m3 = m0;
+  uint64_t i = 0;
+  for (; i + 16 <= n; i += 16) {
+    m0 = _mm256_max_pd(_mm256_loadu_pd(p + i), m0);
+    m1 = _mm256_max_pd(_mm256_loadu_pd(p + i + 4), m1);
+    m2 = _mm256_max_pd(_mm256_loadu_pd(p + i + 8), m2);
+    m3 = _mm256_max_pd(_mm256_loadu_pd(p + i + 12), m3);
+  }
+  alignas(32) double lane[16];
+  _mm256_store_pd(lane, m0);
+  _mm256_store_pd(lane + 4, m1);
+  _mm256_store_pd(lane + 8, m2);
+  _mm256_store_pd(lane + 12, m3);
+  double mx = max_scalar(lane, 16);
+  const double t = max_scalar(p + i, n - i);
+  return t > mx ? t : mx;
+}
+
+// four pixels (as doubles) -> four int32 levels
+__attribute__((target("avx2"))) inline __m128i level4(__m256d x, __m256d inv) {
+  __m256d v = _mm256_mul_pd(x, inv);
+  v = _mm256_max_pd(_mm256_min_pd(v, _mm256_set1_pd(255.0)), _mm256_set1_pd(1.0));
+  v = _mm256_and_pd(v, _mm256_cmp_pd(x, _mm256_setzero_pd(), _CMP_GT_OQ));
+  return _mm256_cvttpd_epi32(v);
+}
+
+__attribute__((target("avx2"))) inline void store16(uint8_t *o, __m128i a, __m128i b, __m128i c,
+                                                   __m128i d) {
+  const __m128i w = _mm_packus_epi16(_mm_packus_epi32(a, b), _mm_packus_epi32(c, d));
+  _mm_storeu_si128((__m128i *)o, w);
+}
+
+__attribute__((target("avx2"))) void map_avx2(const float *p, uint64_t n, double inv,
+                                              uint8_t *out) {
+  const __m256d vi = _mm256_set1_pd(inv);
+  uint64_t i = 0;
+  for (; i + 16 <= n; i += 16) {
+    const __m256 a = _mm256_loadu_ps(p + i), b = _mm256_loadu_ps(p + i + 8);
+    store16(out + i, level4(_mm256_cvtps_pd(_mm256_castps256_ps128(a)), vi),
+            level4(_mm256_cvtps_pd(_mm256_extractf128_ps(a, 1)), vi),
+            level4(_mm256_cvtps_pd(_mm256_castps256_ps128(b)), vi),
+            level4(_mm256_cvtps_pd(_mm256_extractf128_ps(b, 1)), vi));
+  }
+  map_scalar(p + i, n - i, inv, out + i);
+}
+
+__attribute__((target("avx2"))) void map_avx2(const double *p, uint64_t n, double inv,
+                                              uint8_t *out) {
+  const __m256d vi = _mm256_set1_pd(inv);
+  uint64_t i = 0;
+  for (; i + 16 <= n; i += 16)
+    store16(out + i, level4(_mm256_loadu_pd(p + i), vi), level4(_mm256_loadu_pd(p + i + 4), vi),
+            level4(_mm256_loadu_pd(p + i + 8), vi), level4(_mm256_loadu_pd(p + i + 12), vi));
+  map_scalar(p + i, n - i, inv, out + i);
+}
+
+bool have_avx2() {
+  static const bool yes = __builtin_cpu_supports("avx2");
+  return yes;
+}
+
+template <class T>
+void quantize(const T *img, uint64_t npx, uint8_t *out, double *im_scale) {
+  const bool vec = have_avx2();
+  const double mx = vec ? max_avx2(img, npx) : max_scalar(img, npx);
+  const double scale = (mx > 0) ? mx / 255.0 : 1.0;
+  const double inv = (mx > 0) ? 255.0 / mx : 0.0;
+  if (vec && mx < HUGE_VAL)
+    map_avx2(img, npx, inv, out);
+  else
+    map_scalar(img, npx, inv, out);
+  if (im_scale) *im_scale = scale;
+}
+
 }  // namespace
 }  // namespace lm
 
@@ -232,20 +357,7 @@ int lm_quantize_u8(const double *img, uint64_t npx, uint8_t *out, double *im_sca
     set_error("lm_quantize_u8: null buffer");
     return LM_ERR_INVALID;
   }
-  double mx = 0;
-  for (uint64_t i = 0; i < npx; ++i)
-    if (img[i] > mx) mx = img[i];
-  const double scale = (mx > 0) ? mx / 255.0 : 1.0;
-  const double inv = (mx > 0) ? 255.0 / mx : 0.0;
-  for (uint64_t i = 0; i < npx; ++i) {
-    if (img[i] > 0) {
-      const double v = img[i] * inv;
-      out[i] = (v >= 255.0) ? 255 : (v < 1.0) ? 1 : (uint8_t)v;
-    } else {
-      out[i] = 0;
-    }
-  }
-  if (im_scale) *im_scale = scale;
+  quantize(img, npx, out, im_scale);
   return LM_OK;
 }
 
@@ -256,21 +368,7 @@ int lm_quantize_u8_f32(const float *img, uint64_t npx, uint8_t *out, double *im_
   }
   // every float is a double exactly: the same operations as lm_quantize_u8
   // on the widened pixels, without the float64 copy
-  double mx = 0;
-  for (uint64_t i = 0; i < npx; ++i)
-    if ((double)img[i] > mx) mx = img[i];
-  const double scale = (mx > 0) ? mx / 255.0 : 1.0;
-  const double inv = (mx > 0) ? 255.0 / mx : 0.0;
-  for (uint64_t i = 0; i < npx; ++i) {
-    const double x = img[i];
-    if (x > 0) {
-      const double v = x * inv;
-      out[i] = (v >= 255.0) ? 255 : (v < 1.0) ? 1 : (uint8_t)v;
-    } else {
-      out[i] = 0;
-    }
-  }
-  if (im_scale) *im_scale = scale;
+  quantize(img, npx, out, im_scale);
   return LM_OK;
 }
 

@@ -94,6 +94,40 @@ def test_quantize_f32_equals_quantize_of_widened_pixels():
         np.testing.assert_array_equal(out, b)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_quantize_level_boundaries_and_ragged_lengths(dtype):
+    """Pixels on and one ulp either side of every level boundary x = k * mx /
+    255 (the 1 and 255 clamps included), subnormals, signed zeros, NaN and
+    -inf, at every length 0..47 (the vector loops' tails) and in full: both
+    quantizers are byte-equal to the oracle."""
+    import ctypes
+    L = _lib.lib()
+    rng = np.random.default_rng(12)
+    tiny = np.finfo(dtype).smallest_subnormal
+    for mx in (1000.0, 3.0, 255.0, 7.3e-3):
+        mx = float(dtype(mx))
+        base = np.arange(0, 257, dtype=np.float64) * (mx / 255.0)
+        vals = np.concatenate([base, np.nextafter(base, np.inf), np.nextafter(base, -np.inf)])
+        vals = vals.astype(dtype)
+        vals = vals[vals <= mx]
+        special = np.array([0.0, -0.0, np.nan, -np.inf, -1.0, tiny, -tiny, mx], dtype)
+        img = np.concatenate([vals, special])
+        img = img[rng.permutation(img.size)]
+        for n in list(range(48)) + [img.size]:
+            x = np.ascontiguousarray(img[:n])
+            want, sw = ob.quantize(x.astype(np.float64))
+            if dtype == np.float64:
+                got, sg = api.quantize_u8(x)
+            else:
+                got = np.zeros(n, np.uint8)
+                s = ctypes.c_double()
+                _lib.check(L.lm_quantize_u8_f32(x.ctypes.data, n, got.ctypes.data,
+                                                ctypes.byref(s)))
+                sg = s.value
+            assert sg == sw, (mx, n)
+            np.testing.assert_array_equal(got, want, err_msg=f"mx={mx} n={n}")
+
+
 def test_invalid_geometry_rejected():
     with pytest.raises(_lib.LaueMatchError):
         api.Geometry((0, 0, 0, 90, 90, 90), 225, (0, 0, .5), (0.1, 0, 0), 1e-4, 1e-4, 64, 64, 5, 30)

@@ -164,10 +164,11 @@ def test_stream_daemon_two_dbs_and_wire_v2(tmp_path):
     p1.write(str(d1 / "params.txt"))
     rng = np.random.default_rng(5)
 
-    def frame_for(p, O, seed, ints):
+    def frame_for(p, O, seed, top):
+        # top 0: float32 pixels; else integers 0..top (uint8 frames: 250 goes
+        # through the daemon's level table, 255 is its identity case)
         im = synth.synth_image(p, hk, O[rng.choice(len(O), 3, replace=False)], seed=seed)
-        return np.floor(im * (60000.0 / im.max())) if ints == 16 else (
-            np.floor(im * (250.0 / im.max())) if ints == 8 else im.astype(np.float32))
+        return im.astype(np.float32) if top == 0 else np.floor(im * (top / im.max()))
 
     # a float64 frame float32 cannot hold: integers 1..254 lowered by 1e-9
     # (maximum exactly 255, so imScale is 1), which truncate one level lower
@@ -179,9 +180,10 @@ def test_stream_daemon_two_dbs_and_wire_v2(tmp_path):
     sends = [  # (wire, number, db, image, dtype)
         ("v1", 1, 0, img0.astype(np.float32), np.float32),
         ("v2", 2, 0, f64, np.float64),
-        ("v2", 3, 1, frame_for(p1, O1, 3, 16), np.uint16),
+        ("v2", 3, 1, frame_for(p1, O1, 3, 60000), np.uint16),
         ("v2", 4, 1, frame_for(p1, O1, 4, 0), np.float32),
-        ("v2", 5, 0, frame_for(p0, O0, 5, 8), np.uint8),
+        ("v2", 5, 0, frame_for(p0, O0, 5, 250), np.uint8),
+        ("v2", 6, 0, frame_for(p0, O0, 6, 255), np.uint8),
     ]
     port = _free_port()
     env = dict(os.environ, LM_STREAM_PORT=str(port))
