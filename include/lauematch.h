@@ -302,6 +302,23 @@ int lm_score_batch_each(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
 int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,
                        int32_t min_nr_spots, double min_intensity, void *stream, lm_hits *outs);
 
+/* On-the-fly scoring: the forward simulation of src/LaueMatchingCPU.c:591-667
+ * (mode LM_FWD_FP64 or LM_FWD_FP32_GUARD) and the compare of :668-689 in one
+ * pass, with no spot table read or written -- each orientation's spot list
+ * is projected and scored against the images as its spots are found.  Images
+ * as in lm_score_batch_dev (1..256, device-resident, back to back), with
+ * thresholds per image; groups of up to 64 images share one projection pass.
+ * Each image's hits equal those lm_score_batch_dev returns from the table
+ * lm_db_forward(mode) builds (no table is needed: the DB may have none).
+ * stats (may be NULL): pairs, projected and recomputed pairs summed over the
+ * passes, n_spots the spots listed per pass, ms the device time.  Needs
+ * MaxNrLaueSpots <= 64.  The reference reads a precomputed ForwardFile
+ * instead (src/LaueMatchingGPU.cu:660-749); no counterpart there. */
+int lm_score_fwd_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs,
+                           const double *im_scales, const int32_t *min_nr_spots,
+                           const double *min_intensity, int32_t mode, void *stream,
+                           lm_hits *outs, lm_fwd_stats *stats);
+
 void lm_hits_free(lm_hits *h);
 
 /* ---- serving: continuous batching over resident DBs ---------------------------------- */

@@ -141,6 +141,11 @@ SIGNATURES = {
     "lm_score_batch_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                           c_double_p, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.POINTER(LmHits)]),
+    "lm_score_fwd_batch_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                              c_double_p, ctypes.POINTER(ctypes.c_int32),
+                                              c_double_p, ctypes.c_int32, ctypes.c_void_p,
+                                              ctypes.POINTER(LmHits),
+                                              ctypes.POINTER(LmFwdStats)]),
     "lm_hits_free": (None, [ctypes.POINTER(LmHits)]),
     "lm_preprocess": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_void_p, ctypes.c_void_p,

@@ -329,6 +329,26 @@ class OrientationDB:
                                        ctypes.c_void_p(stream), hs))
         return [Hits._take(hs[i]) for i in range(nimg)]
 
+    def score_fwd_batch_dev(self, dev_ptr: int, nimg: int, im_scales, min_nr_spots,
+                            min_intensity, mode: int = LM_FWD_FP64, stream: int = 0):
+        """On-the-fly scoring (lm_score_fwd_batch_dev): nimg device images
+        back to back, thresholds per image (sequences, or one value for all);
+        every orientation is projected in the same pass, no table is read.
+        Returns (hits per image, LmFwdStats)."""
+        if np.isscalar(min_nr_spots):
+            min_nr_spots = [min_nr_spots] * nimg
+        if np.isscalar(min_intensity):
+            min_intensity = [min_intensity] * nimg
+        sc = (ctypes.c_double * nimg)(*[float(s) for s in im_scales])
+        sp = (ctypes.c_int32 * nimg)(*[int(v) for v in min_nr_spots])
+        mi = (ctypes.c_double * nimg)(*[float(v) for v in min_intensity])
+        hs = (LmHits * nimg)()
+        st = LmFwdStats()
+        check(lib().lm_score_fwd_batch_dev(self._h, nimg, ctypes.c_void_p(dev_ptr), sc, sp, mi,
+                                           int(mode), ctypes.c_void_p(stream), hs,
+                                           ctypes.byref(st)))
+        return [Hits._take(hs[i]) for i in range(nimg)], st
+
 
 def merge_candidates(orients, rows, scores, max_angle: float, space_group: int,
                      ctx: "Context | None" = None, threads: int = 8):

@@ -783,6 +783,24 @@ int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const dou
   return score_uniform(db, nimg, d_imgs, im_scales, min_nr_spots, min_intensity, st, outs);
 }
 
+int lm_score_fwd_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs,
+                           const double *im_scales, const int32_t *min_nr_spots,
+                           const double *min_intensity, int32_t mode, void *stream,
+                           lm_hits *outs, lm_fwd_stats *stats) {
+  if (!db || !d_imgs || nimg < 1 || nimg > kMaxBatch || !im_scales || !min_nr_spots ||
+      !min_intensity || !outs) {
+    set_error("lm_score_fwd_batch_dev: bad arguments");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(db->ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(db->mu);
+  hipStream_t st = stream ? (hipStream_t)stream : db->ctx->stream;
+  std::vector<int> sp(min_nr_spots, min_nr_spots + nimg);
+  return score_images_fwd(db, db->scratch, nimg, d_imgs, im_scales, sp.data(), min_intensity,
+                          mode, st, outs, stats);
+}
+
 int lm_selftest_fp64(lm_ctx *ctx, const double *a, const double *b, uint64_t n, double *q,
                      double *s) {
   int rc = check_ctx(ctx);

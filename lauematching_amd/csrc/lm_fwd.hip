@@ -27,6 +27,7 @@
 #include <cstring>
 #include <vector>
 
+#include "lm_gather.h"
 #include "lm_internal.h"
 
 #pragma clang fp contract(off)
@@ -57,42 +58,49 @@ struct DevGeom {
   int pre_sign, pre_px;
   double eloC, ehiC;
   double xlo, xhi, ylo, yhi;
+  // cone pre-test (cone_bit), used when cone != 0: the lab direction c of
+  // the detector's cone, the metric G0 = recip^T recip every orientation's
+  // M^T M must match within tolG (then |q|^2 = h^T G0 h up to the per-hkl
+  // margins of the plan)
+  int cone;
+  double cLab[3];
+  double G0[6];  // (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+  double tolG;
 };
 
-// Pixel from a detector coordinate: the reference keeps (int)t within
-// [0, N-1] (src/LaueMatchingCPU.c:627-632), i.e. exactly -1 < t < N.
-__device__ __forceinline__ bool to_pixel(double t, double n, int *ip) {
-  if (!(t > -1.0 && t < n)) return false;
-  *ip = (int)t;
-  return true;
-}
-
-// src/LaueMatchingCPU.c:604-636 for one (orientation, hkl).
+// src/LaueMatchingCPU.c:604-636 for one (orientation, hkl).  Every value is
+// computed as the reference computes it (same operations, same order); the
+// reference's early exits become one conjunction at the end, so a wave runs
+// the sequence without divergent branches (a rejected pair's later values
+// may be inf or NaN and are then discarded).  The tests: qlen != 0, z > 0,
+// -1 < t < N for both pixel coordinates (the (int) truncation kept within
+// [0, N-1]), and not (E < Elo or E > Ehi).
 __device__ __forceinline__ bool project64(const double *M, double h, double k, double l,
                                           const DevGeom &g, uint32_t *pix) {
   double q0 = 0, q1 = 0, q2 = 0;
   q0 += M[0] * h; q0 += M[1] * k; q0 += M[2] * l;
   q1 += M[3] * h; q1 += M[4] * k; q1 += M[5] * l;
   q2 += M[6] * h; q2 += M[7] * k; q2 += M[8] * l;
-  double qlen = __builtin_sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-  if (qlen == 0) return false;
-  double u0 = q0 / qlen, u1 = q1 / qlen, u2 = q2 / qlen;
-  double td = 2 * u2;
-  double k0 = 0 - td * u0, k1 = 0 - td * u1, k2 = 1.0 - td * u2;
+  const double qlen = __builtin_sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+  const double u0 = q0 / qlen, u1 = q1 / qlen, u2 = q2 / qlen;
+  const double td = 2 * u2;
+  const double k0 = 0 - td * u0, k1 = 0 - td * u1, k2 = 1.0 - td * u2;
   double x = 0, y = 0, z = 0;
   x += g.rotT[0] * k0; x += g.rotT[1] * k1; x += g.rotT[2] * k2;
   y += g.rotT[3] * k0; y += g.rotT[4] * k1; y += g.rotT[5] * k2;
   z += g.rotT[6] * k0; z += g.rotT[7] * k1; z += g.rotT[8] * k2;
-  if (z <= 0) return false;
-  double X = x * g.P[2] / z;
-  double Y = y * g.P[2] / z;
-  int ipx, ipy;
-  if (!to_pixel((X - g.P[0]) / g.pxX + g.halfX, g.W, &ipx)) return false;
-  if (!to_pixel((Y - g.P[1]) / g.pxY + g.halfY, g.H, &ipy)) return false;
-  double E = kHc * qlen / (kFourPi * (-u2));
-  if (E < g.Elo || E > g.Ehi) return false;
+  const double X = x * g.P[2] / z;
+  const double Y = y * g.P[2] / z;
+  const double tx = (X - g.P[0]) / g.pxX + g.halfX;
+  const double ty = (Y - g.P[1]) / g.pxY + g.halfY;
+  const double E = kHc * qlen / (kFourPi * (-u2));
+  const bool ok = (qlen != 0) & (z > 0) & (tx > -1.0) & (tx < g.W) & (ty > -1.0) & (ty < g.H) &
+                  !(E < g.Elo) & !(E > g.Ehi);
+  // (int) of a value inside (-1, N) truncates toward zero as the reference's
+  // cast does; rejected pairs convert 0
+  const int ipx = (int)(ok ? tx : 0.0), ipy = (int)(ok ? ty : 0.0);
   *pix = (uint32_t)ipy * (uint32_t)g.nrPxX + (uint32_t)ipx;
-  return true;
+  return ok;
 }
 
 // One pre-test evaluation of the plan (built on the host, fwd_plan): the hkl
@@ -105,7 +113,44 @@ __device__ __forceinline__ bool project64(const double *M, double h, double k, d
 struct FwdEval {
   double h, k, l;
   uint64_t bits;  // bit when q_z < 0 (low word) / when q_z > 0 (high word); 64 = none
+  // cone_bit bounds of this hkl: |q_z| in [ahi, alo], q_z (q.c) <= dcone
+  double alo, ahi, dcone;
+  double pad;     // 64 bytes: the int4 chunk table after the entries stays aligned
 };
+
+// The cone pre-test: a cheaper necessary condition than plan_bit, taken for
+// the waves whose 64 orientations all have M^T M = G0 within tolG (any
+// rotation of the lattice does; other matrices take plan_bit).  Then
+// |q|^2 = h^T G0 h is a per-hkl constant up to a margin, and, with
+// q = M (h, k, l):
+//  * energy: the reference's E = hc|q|^2 / (4 pi |q_z|) to 1e-15 relative,
+//    so E in [Elo, Ehi] needs |q_z| in [ahi, alo], bounds computed on the
+//    host from |q|^2 with margins of 1e-9 relative and 1e-9 |q| absolute
+//    (q_z is within a few 2^-53 |q| of the reference's);
+//  * detector: every direction that lands inside the detector rectangle
+//    (pixel coordinates -1 < t < N, the reference's test) lies within the
+//    cone of half-angle g around the lab direction c, g the largest angle
+//    between c and the rectangle's corners (the angle to c is quasiconvex
+//    over the detector plane while g < 90 deg, so its maximum over the
+//    rectangle is at a corner); cos g carries a margin of 1e-6.  With
+//    k_f = e_z - 2 q_z q / |q|^2 (unit), k_f . c >= cos g reads
+//    q_z (q . c) <= (c_z - cos g) |q|^2 / 2 = dcone (widened by the |q|^2
+//    margin and 1e-9 |q|^2);
+//  * sign: as in plan_bit, only the Friedel member with q_z < 0 can be
+//    accepted (Elo > 0), and q_z = 0 is rejected by both.
+// qc = q . c comes from v = M^T c, computed once per orientation.
+// Returns the candidate bit, or 64 for none.  Branch-free.
+__device__ __forceinline__ int cone_bit(const double *M, const double (&v)[3],
+                                        const FwdEval &ev) {
+  const double qz = __builtin_fma(M[8], ev.l, __builtin_fma(M[7], ev.k, M[6] * ev.h));
+  const double qc = __builtin_fma(v[2], ev.l, __builtin_fma(v[1], ev.k, v[0] * ev.h));
+  const double a = __builtin_fabs(qz);
+  const bool rej = (a > ev.alo) | (a < ev.ahi) | (qz * qc > ev.dcone);
+  const int bneg = (int)(uint32_t)ev.bits, bpos = (int)(uint32_t)(ev.bits >> 32);
+  int bit = qz > 0 ? bpos : 64;
+  bit = qz < 0 ? bneg : bit;
+  return rej ? 64 : bit;
+}
 
 // A necessary condition for project64 to accept the member of the pair that
 // the sign of q_z selects, with no division or square root: false only when
@@ -264,6 +309,176 @@ __device__ __forceinline__ int project32(const float *M, int hi, int ki, int li,
   return 1;
 }
 
+// thisOrient = orientation x recip (MatrixMultF33, src/LaueMatchingHeaders.h:
+// 349-357: (a*b + c*d) + e*f), its float copy for the guarded path, and the
+// cone pre-test's v = M^T c.  Returns true when M^T M = G0 within tolG (a
+// rotation of the lattice: the cone pre-test applies; NaN fails).
+template <bool GUARD>
+__device__ __forceinline__ bool orient_load(const double *__restrict__ orients, uint64_t o,
+                                            const double (&R)[9], const DevGeom &g,
+                                            double (&M)[9], float (&M32)[9], double (&cv)[3]) {
+  double t[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) t[i] = orients[o * 9 + i];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    M[r * 3 + 0] = t[r * 3 + 0] * R[0] + t[r * 3 + 1] * R[3] + t[r * 3 + 2] * R[6];
+    M[r * 3 + 1] = t[r * 3 + 0] * R[1] + t[r * 3 + 1] * R[4] + t[r * 3 + 2] * R[7];
+    M[r * 3 + 2] = t[r * 3 + 0] * R[2] + t[r * 3 + 1] * R[5] + t[r * 3 + 2] * R[8];
+  }
+  if (GUARD) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) M32[i] = (float)M[i];
+  }
+  cv[0] = cv[1] = cv[2] = 0;
+  if (!g.cone) return false;
+  const double G[6] = {
+      __builtin_fma(M[6], M[6], __builtin_fma(M[3], M[3], M[0] * M[0])),
+      __builtin_fma(M[7], M[7], __builtin_fma(M[4], M[4], M[1] * M[1])),
+      __builtin_fma(M[8], M[8], __builtin_fma(M[5], M[5], M[2] * M[2])),
+      __builtin_fma(M[6], M[7], __builtin_fma(M[3], M[4], M[0] * M[1])),
+      __builtin_fma(M[6], M[8], __builtin_fma(M[3], M[5], M[0] * M[2])),
+      __builtin_fma(M[7], M[8], __builtin_fma(M[4], M[5], M[1] * M[2]))};
+  bool fast = true;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) fast &= __builtin_fabs(G[i] - g.G0[i]) <= g.tolG;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    cv[j] = __builtin_fma(M[6 + j], g.cLab[2], __builtin_fma(M[3 + j], g.cLab[1], M[j] * g.cLab[0]));
+  return fast;
+}
+
+// The spot list of one orientation (src/LaueMatchingCPU.c:604-667): hkls in
+// file order, each accepted pixel not yet listed is appended, then
+// spot(pix) is called, until ns reaches maxSpots.  List slot s < SLOTS is
+// colL[s * kFwdBlock] (LDS), slot s >= SLOTS is colG[(s - SLOTS) * gStride].
+// A lane enters with ns = maxSpots to take no part.
+//
+// hkls in chunks of at most 64 (chunks[c] = first hkl, first and last+1
+// plan entry): first the division-free pre-test of the chunk's plan entries
+// (wave-uniform loop, scalar loads) marks the candidates in a 64-bit mask,
+// then each lane projects only its own candidates, in hkl order (ctz), so
+// the wave runs as many projections as its busiest lane has candidates
+// instead of one per hkl; first unseen pixel still wins.  Chunks are taken
+// four at a time (up to 256 hkls): the pre-test fills four masks, then each
+// lane walks the union of its four masks in hkl order, so the wave runs as
+// many projections as its busiest lane has candidates in those hkls, rather
+// than the sum over chunks of each chunk's busiest.  The next candidate's
+// hkl is loaded while the current one is projected.  useCone
+// (wave-uniform) selects cone_bit over plan_bit.  GUARD: float32
+// projection, float64 recompute of undecided pairs (counted in nrec); nproj
+// counts the projected pairs.
+template <bool GUARD, int SLOTS, class Spot>
+__device__ __forceinline__ void walk_hkls(const double (&M)[9], const float (&M32)[9],
+                                          const double (&cv)[3], bool useCone, const DevGeom &g,
+                                          const DevGeom32 &g32, const int32_t *__restrict__ hkls,
+                                          const FwdEval *__restrict__ plan,
+                                          const int4 *__restrict__ chunks, int nchunks,
+                                          int maxSpots, int &ns, uint32_t &nrec, uint32_t &nproj,
+                                          uint32_t *colL, uint32_t *colG, uint64_t gStride,
+                                          Spot spot) {
+  constexpr int kSuper = 4;
+  for (int c0 = 0; c0 < nchunks; c0 += kSuper) {
+    if (__all(ns >= maxSpots)) break;
+    uint64_t cand[kSuper];
+    int base[kSuper];
+#pragma unroll
+    for (int k = 0; k < kSuper; ++k) {
+      cand[k] = 0;
+      base[k] = 0;
+      if (c0 + k < nchunks) {
+        const int4 ch = chunks[c0 + k];
+        base[k] = ch.x;
+        uint64_t m = 0;
+        if (useCone) {
+#pragma unroll 4
+          for (int e = ch.y; e < ch.z; ++e) {
+            const int bit = cone_bit(M, cv, plan[e]);
+            m |= bit < 64 ? 1ull << bit : 0ull;
+          }
+        } else {
+#pragma unroll 2
+          for (int e = ch.y; e < ch.z; ++e) {
+            const int bit = plan_bit(M, plan[e], g);
+            m |= bit < 64 ? 1ull << bit : 0ull;
+          }
+        }
+        cand[k] = ns < maxSpots ? m : 0ull;
+      }
+    }
+    // this lane's next candidate (hkl index, or -1): the lowest bit of its
+    // first nonempty mask, which is then cleared; selects, no branches
+    auto pop = [&]() -> int {
+      const bool n0 = cand[0] != 0, n1 = cand[1] != 0, n2 = cand[2] != 0;
+      const uint64_t c = n0 ? cand[0] : n1 ? cand[1] : n2 ? cand[2] : cand[3];
+      const int b = n0 ? base[0] : n1 ? base[1] : n2 ? base[2] : base[3];
+      const uint64_t r = c & (c - 1);
+      cand[0] = n0 ? r : cand[0];
+      cand[1] = !n0 && n1 ? r : cand[1];
+      cand[2] = !n0 && !n1 && n2 ? r : cand[2];
+      cand[3] = !n0 && !n1 && !n2 ? r : cand[3];
+      return c ? b + (int)__builtin_ctzll(c) : -1;
+    };
+    int hi = pop();
+    int hI = 0, kI = 0, lI = 0;
+    if (hi >= 0) {
+      hI = hkls[3 * hi + 0];
+      kI = hkls[3 * hi + 1];
+      lI = hkls[3 * hi + 2];
+    }
+    while (hi >= 0 && ns < maxSpots) {
+      const int hn = pop();
+      const int h0 = hI, k0 = kI, l0 = lI;
+      if (hn >= 0) {
+        hI = hkls[3 * hn + 0];
+        kI = hkls[3 * hn + 1];
+        lI = hkls[3 * hn + 2];
+      }
+      ++nproj;
+      uint32_t pix = 0;
+      bool ok;
+      if (GUARD) {
+        const int v = project32(M32, h0, k0, l0, g32, (uint32_t)g.nrPxX, &pix);
+        if (v == 2) {
+          ok = project64(M, (double)h0, (double)k0, (double)l0, g, &pix);
+          ++nrec;
+        } else {
+          ok = v == 1;
+        }
+      } else {
+        ok = project64(M, (double)h0, (double)k0, (double)l0, g, &pix);
+      }
+      if (ok) {
+        // duplicate test over the whole list, LDS slots four at a time
+        bool dup = false;
+        const int nl = SLOTS ? min(ns, SLOTS) : 0;
+        int s = 0;
+        for (; s + 4 <= nl; s += 4)
+          dup |= (colL[s * kFwdBlock] == pix) | (colL[(s + 1) * kFwdBlock] == pix) |
+                 (colL[(s + 2) * kFwdBlock] == pix) | (colL[(s + 3) * kFwdBlock] == pix);
+        for (; s < nl; ++s) dup |= colL[s * kFwdBlock] == pix;
+        for (s = SLOTS; s < ns; ++s) dup |= colG[(uint64_t)(s - SLOTS) * gStride] == pix;
+        if (!dup) {
+          if (ns < SLOTS)
+            colL[ns * kFwdBlock] = pix;
+          else
+            colG[(uint64_t)(ns - SLOTS) * gStride] = pix;
+          spot(pix);
+          ++ns;
+        }
+      }
+      hi = hn;
+    }
+  }
+}
+
+// Sum of v over the wave, added to *dst by lane 0.
+__device__ __forceinline__ void wave_add(uint32_t v, unsigned long long *dst) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  if (__lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
 // One lane per orientation.  WRITE=false: counts[o] = number of spots.
 // WRITE=true: spots of orientation o go to entries[tile_off[tile] + in-tile
 // prefix of counts], in spot order.  GUARD: float32 projection, float64
@@ -286,24 +501,14 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   __shared__ uint32_t scan[kFwdBlock];
   const uint64_t o = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
   const bool live = o < n;
-  double M[9];
-  if (live && !FROMKEEP) {
-    double t[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) t[i] = orients[o * 9 + i];
-    // MatrixMultF33 (src/LaueMatchingHeaders.h:349-357): (a*b + c*d) + e*f
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      M[r * 3 + 0] = t[r * 3 + 0] * r00 + t[r * 3 + 1] * r10 + t[r * 3 + 2] * r20;
-      M[r * 3 + 1] = t[r * 3 + 0] * r01 + t[r * 3 + 1] * r11 + t[r * 3 + 2] * r21;
-      M[r * 3 + 2] = t[r * 3 + 0] * r02 + t[r * 3 + 1] * r12 + t[r * 3 + 2] * r22;
-    }
-  }
+  const double R[9] = {r00, r01, r02, r10, r11, r12, r20, r21, r22};
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, cv[3] = {0, 0, 0};
   float M32[9];
-  if (GUARD) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) M32[i] = (float)M[i];
-  }
+  // the cone pre-test when every orientation of the wave is a rotation of
+  // the lattice, else plan_bit
+  bool fast = !live;
+  if (live && !FROMKEEP) fast = orient_load<GUARD>(orients, o, R, g, M, M32, cv);
+  const bool useCone = g.cone && !FROMKEEP && __all(fast);
   uint32_t nrec = 0, nproj = 0;
   uint32_t *const colL = lst + threadIdx.x;  // slot s at colL[s * kFwdBlock], s < SLOTS
   uint32_t *const colG = SLOTS == 0 ? spill + o : keep + (uint64_t)SLOTS * n + o;
@@ -311,85 +516,12 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
   auto get = [&](int s) -> uint32_t {
     return s < SLOTS ? colL[s * kFwdBlock] : colG[(uint64_t)(s - SLOTS) * gStride];
   };
-  auto put = [&](int s, uint32_t v) {
-    if (s < SLOTS)
-      colL[s * kFwdBlock] = v;
-    else
-      colG[(uint64_t)(s - SLOTS) * gStride] = v;
-  };
   int ns = live ? 0 : maxSpots;
-  if (FROMKEEP) ns = live ? (int)counts[o] : 0;
-  // hkls in chunks of 64 (chunks[c] = first hkl, first and last+1 plan
-  // entry): first the division-free pre-test of the chunk's plan entries
-  // (wave-uniform loop, scalar loads) marks the candidates in a 64-bit mask,
-  // then each lane projects only its own candidates, in hkl order (ctz), so
-  // the wave runs as many projections as its busiest lane has candidates
-  // instead of one per hkl; first unseen pixel still wins.
-  // Chunks are taken four at a time (256 hkls): the pre-test fills four
-  // masks, then each lane walks the union of its four masks in hkl order, so
-  // the wave runs as many projections as its busiest lane has candidates in
-  // the 256 hkls, rather than the sum over chunks of each chunk's busiest.
-  constexpr int kSuper = 4;
-  for (int c0 = 0; c0 < (FROMKEEP ? 0 : nchunks); c0 += kSuper) {
-    if (__all(ns >= maxSpots)) break;
-    uint64_t cand[kSuper];
-    int base[kSuper];
-#pragma unroll
-    for (int k = 0; k < kSuper; ++k) {
-      cand[k] = 0;
-      base[k] = 0;
-      if (c0 + k < nchunks) {
-        const int4 ch = chunks[c0 + k];
-        base[k] = ch.x;
-        uint64_t m = 0;
-#pragma unroll 2
-        for (int e = ch.y; e < ch.z; ++e) {
-          const int bit = plan_bit(M, plan[e], g);
-          m |= bit < 64 ? 1ull << bit : 0ull;
-        }
-        cand[k] = ns < maxSpots ? m : 0ull;
-      }
-    }
-    for (;;) {
-      // this lane's next candidate: lowest bit of its first nonempty mask
-      int hi = -1;
-#pragma unroll
-      for (int k = kSuper - 1; k >= 0; --k)
-        if (cand[k] != 0) hi = base[k] + __builtin_ctzll(cand[k]);
-      bool taken = false;
-#pragma unroll
-      for (int k = 0; k < kSuper; ++k)
-        if (!taken && cand[k] != 0) {
-          cand[k] &= cand[k] - 1;
-          taken = true;
-        }
-      if (hi < 0 || ns >= maxSpots) break;
-      ++nproj;
-      const int hI = hkls[3 * hi + 0], kI = hkls[3 * hi + 1], lI = hkls[3 * hi + 2];
-      uint32_t pix = 0;
-      bool ok;
-      if (GUARD) {
-        const int v = project32(M32, hI, kI, lI, g32, (uint32_t)g.nrPxX, &pix);
-        if (v == 2) {
-          ok = project64(M, (double)hI, (double)kI, (double)lI, g, &pix);
-          ++nrec;
-        } else {
-          ok = v == 1;
-        }
-      } else {
-        ok = project64(M, (double)hI, (double)kI, (double)lI, g, &pix);
-      }
-      if (ok) {
-        bool dup = false;
-        for (int s = 0; s < ns; ++s)
-          if (get(s) == pix) { dup = true; break; }
-        if (!dup) {
-          put(ns, pix);
-          ++ns;
-        }
-      }
-    }
-  }
+  if (FROMKEEP)
+    ns = live ? (int)counts[o] : 0;
+  else
+    walk_hkls<GUARD, SLOTS>(M, M32, cv, useCone, g, g32, hkls, plan, chunks, nchunks, maxSpots,
+                            ns, nrec, nproj, colL, colG, gStride, [](uint32_t) {});
   if (!WRITE) {
     if (live) counts[o] = (uint16_t)ns;
     if (keep && live)
@@ -397,16 +529,8 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
         keep[(uint64_t)s * n + o] = get(s);
     // recomputed[0]: pairs recomputed in float64; recomputed[1]: pairs that
     // passed the pre-test and were projected
-    uint32_t w = nrec, c = nproj;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-      w += __shfl_xor(w, d);
-      c += __shfl_xor(c, d);
-    }
-    if (__lane_id() == 0) {
-      if (w) atomicAdd(recomputed, (unsigned long long)w);
-      if (c) atomicAdd(recomputed + 1, (unsigned long long)c);
-    }
+    wave_add(nrec, recomputed);
+    wave_add(nproj, recomputed + 1);
     return;
   }
   // In-tile offset of this orientation: counts of the tile's rows before this
@@ -439,6 +563,138 @@ fwd_kernel(const double *__restrict__ orients, uint64_t n, const int32_t *__rest
     for (int s = 0; s < c; ++s)
       dst[s] = FROMKEEP ? keep[(uint64_t)s * n + o] : get(s);
   }
+}
+
+// On-the-fly scoring (lm_score_fwd_batch_dev): each orientation's spot list
+// is projected as the count pass projects it and then scored against B
+// interleaved images, so no table is read or written.  The accumulators and
+// the hit test are those of the table walk (score_kernel), so each image's
+// hit records are the ones the table would give.  M <= 64: list slots 0..31
+// in LDS, slots 32.. in spill[(s - 32) * (grid rows) + row in the launch].
+// stats[0] pairs recomputed in float64, [1] pairs projected, [2] spots
+// listed.
+template <int B, bool GUARD>
+__global__ void __launch_bounds__(kFwdBlock)
+fwd_score_kernel(const double *__restrict__ orients, uint64_t n, uint64_t row0,
+                 const int32_t *__restrict__ hkls, double r00, double r01, double r02, double r10,
+                 double r11, double r12, double r20, double r21, double r22, DevGeom g,
+                 DevGeom32 g32, int maxSpots, const FwdEval *__restrict__ plan,
+                 const int4 *__restrict__ chunks, int nchunks, const uint8_t *__restrict__ img,
+                 const Pred *__restrict__ pred, uint64_t *__restrict__ hits, uint64_t capPer,
+                 uint32_t *__restrict__ counters, unsigned long long *__restrict__ stats,
+                 uint32_t *__restrict__ spill) {
+  __shared__ uint32_t lst[kLdsSlotsKeep * kFwdBlock];
+  const uint64_t local = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
+  const uint64_t o = row0 + local;
+  const bool live = o < n;
+  const int lane = __lane_id();
+  const double R[9] = {r00, r01, r02, r10, r11, r12, r20, r21, r22};
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, cv[3] = {0, 0, 0};
+  float M32[9];
+  bool fast = !live;
+  if (live) fast = orient_load<GUARD>(orients, o, R, g, M, M32, cv);
+  const bool useCone = g.cone && __all(fast);
+  uint32_t nrec = 0, nproj = 0;
+  uint32_t *const colL = lst + threadIdx.x;
+  uint32_t *const colG = spill + local;
+  const uint64_t gStride = (uint64_t)gridDim.x * kFwdBlock;
+  auto get = [&](int s) -> uint32_t {
+    return s < kLdsSlotsKeep ? colL[s * kFwdBlock] : colG[(uint64_t)(s - kLdsSlotsKeep) * gStride];
+  };
+  int ns = live ? 0 : maxSpots;
+  uint32_t acc1 = 0;
+  Swar<B < 4 ? 4 : B> a;
+#pragma unroll
+  for (int j = 0; j < (B < 4 ? 1 : B / 4); ++j) a.s02[j] = a.s13[j] = a.cn[j] = 0;
+  walk_hkls<GUARD, kLdsSlotsKeep>(M, M32, cv, useCone, g, g32, hkls, plan, chunks, nchunks,
+                                  maxSpots, ns, nrec, nproj, colL, colG, gStride,
+                                  [](uint32_t) {});
+  // the listed pixels of the B images, read after the walk with four
+  // gathers in flight (not one stalled gather per accepted spot)
+  auto add = [&](const uint32_t (&w)[PxW<B>::W]) {
+    if constexpr (B == 1)
+      acc1 += w[0] ? (1u << kSumBits) + w[0] : 0u;
+    else
+      swar_px<B>(w, a);
+  };
+  const int nl = live ? ns : 0;
+  int s = 0;
+  for (; s + 4 <= nl; s += 4) {
+    uint32_t w[4][PxW<B>::W];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) load_px<B>(img, get(s + q), w[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) add(w[q]);
+  }
+  for (; s < nl; ++s) {
+    uint32_t w[PxW<B>::W];
+    load_px<B>(img, get(s), w);
+    add(w);
+  }
+  wave_add(nrec, stats);
+  wave_add(nproj, stats + 1);
+  wave_add(live ? (uint32_t)ns : 0u, stats + 2);
+  // src/LaueMatchingCPU.c:683-689 per image, as score_kernel decides it
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    uint32_t accb;
+    if constexpr (B == 1)
+      accb = acc1;
+    else
+      accb = swar_acc<B>(a, b);
+    const uint32_t nsb = accb >> kSumBits;
+    const uint32_t sum = accb & kSumMask;
+    bool cand = false, zero_pass = false;
+    if (live && b < pred->nimg) {
+      if (nsb == 0)
+        zero_pass = (0 >= pred->min_spots[b]) && (0.0 >= pred->min_int[b]);
+      else
+        cand = ((int)nsb >= pred->min_spots[b]) && ((double)sum >= pred->cand_thr[b]);
+    }
+    const uint64_t zm = __ballot(zero_pass);
+    if (zm && lane == 0) atomicAdd(&counters[2 * b + 1], (uint32_t)__popcll(zm));
+    append(cand, (o << 26) | (uint64_t)accb, hits + (uint64_t)b * capPer, &counters[2 * b],
+           capPer);
+  }
+}
+
+// Float64 in-order sums (src/LaueMatchingCPU.c:674-681) of the candidate
+// rows of images with imScale != 1 on the on-the-fly path: each lane lists
+// its row's spots again (float64: the guarded projection lists the same
+// spots) and sums image b's pixels in list order.  xids[i] = b << 32 | row;
+// the images lie back to back, npx pixels each.
+__global__ void __launch_bounds__(kFwdBlock)
+fwd_exact_kernel(const double *__restrict__ orients, const int32_t *__restrict__ hkls,
+                 double r00, double r01, double r02, double r10, double r11, double r12,
+                 double r20, double r21, double r22, DevGeom g, DevGeom32 g32, int maxSpots,
+                 const FwdEval *__restrict__ plan, const int4 *__restrict__ chunks, int nchunks,
+                 const uint64_t *__restrict__ xids, uint64_t ncand,
+                 const uint8_t *__restrict__ imgs, uint64_t npx, const Pred *__restrict__ pred,
+                 double *__restrict__ out) {
+  __shared__ uint32_t lst[kLdsSlots * kFwdBlock];
+  const uint64_t i = (uint64_t)blockIdx.x * kFwdBlock + threadIdx.x;
+  const bool live = i < ncand;
+  const uint64_t x = live ? xids[i] : 0;
+  const uint32_t b = (uint32_t)(x >> 32);
+  const uint64_t o = x & 0xffffffffull;
+  const double R[9] = {r00, r01, r02, r10, r11, r12, r20, r21, r22};
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, cv[3] = {0, 0, 0};
+  float M32[9];
+  bool fast = !live;
+  if (live) fast = orient_load<false>(orients, o, R, g, M, M32, cv);
+  const bool useCone = g.cone && __all(fast);
+  uint32_t nrec = 0, nproj = 0;
+  int ns = live ? 0 : maxSpots;
+  const double scale = live ? pred->scale[b] : 0.0;
+  const uint8_t *const im = imgs + (uint64_t)b * npx;
+  double tot = 0;
+  walk_hkls<false, kLdsSlots>(M, M32, cv, useCone, g, g32, hkls, plan, chunks, nchunks, maxSpots,
+                              ns, nrec, nproj, lst + threadIdx.x, nullptr, 0,
+                              [&](uint32_t pix) {
+                                const uint8_t raw = im[pix];
+                                if (raw > 0) tot += (double)raw * scale;
+                              });
+  if (live) out[i] = tot;
 }
 
 // Per-tile sums of the counts.
@@ -550,6 +806,71 @@ __global__ void selftest_kernel(const double *__restrict__ a, const double *__re
   s[i] = __builtin_sqrt(a[i]);
 }
 
+// Host constants of the cone pre-test (cone_bit); ok = false leaves every
+// wave on plan_bit.
+struct ConeSetup {
+  bool ok = false;
+  double cLab[3] = {0, 0, 0};
+  double cosg = 0;   // cosine of the cone's half-angle, margin included
+  double G0[6] = {0, 0, 0, 0, 0, 0};
+  double G0max = 0, tolG = 0;
+};
+
+ConeSetup cone_setup(const Geom &g) {
+  ConeSetup cs;
+  const double P2 = g.P[2];
+  if (!(g.Elo > 0) || !std::isfinite(g.Ehi) || !(g.Ehi >= g.Elo) || !std::isfinite(P2) ||
+      P2 == 0 || !std::isfinite(g.P[0]) || !std::isfinite(g.P[1]) || !(g.pxX != 0) ||
+      !(g.pxY != 0) || !std::isfinite(g.pxX) || !std::isfinite(g.pxY))
+    return cs;
+  // the detector rotation must be orthogonal: k_f . c = (rotT k_f) . c_det
+  // is then the cosine between the lab direction and c
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int r = 0; r < 3; ++r) s += g.rotT[r * 3 + i] * g.rotT[r * 3 + j];
+      if (!(std::fabs(s - (i == j ? 1.0 : 0.0)) <= 1e-12)) return cs;
+    }
+  // detector-frame directions (x/z, y/z, 1) of the rectangle's corners at
+  // pixel coordinates -1 and N (x P2 / z = P0 + (t - (N-1)/2) px)
+  double cd[4][3], c[3] = {0, 0, 0};
+  int k = 0;
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy, ++k) {
+      const double tx = ix ? (double)g.nrPxX : -1.0, ty = iy ? (double)g.nrPxY : -1.0;
+      const double X = g.P[0] + (tx - 0.5 * (g.nrPxX - 1)) * g.pxX;
+      const double Y = g.P[1] + (ty - 0.5 * (g.nrPxY - 1)) * g.pxY;
+      const double v[3] = {X / P2, Y / P2, 1.0};
+      const double nv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      for (int j = 0; j < 3; ++j) {
+        cd[k][j] = v[j] / nv;
+        c[j] += cd[k][j];
+      }
+    }
+  const double nc = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  if (!(nc > 0)) return cs;
+  double cosg = 1.0;
+  for (int j = 0; j < 3; ++j) c[j] /= nc;
+  for (k = 0; k < 4; ++k) cosg = std::min(cosg, cd[k][0] * c[0] + cd[k][1] * c[1] + cd[k][2] * c[2]);
+  // below 84 deg half-angle (the corner argument needs < 90 deg)
+  if (!(cosg > 0.1)) return cs;
+  cs.cosg = cosg - 1e-6;
+  for (int i = 0; i < 3; ++i)
+    cs.cLab[i] = g.rotT[0 * 3 + i] * c[0] + g.rotT[1 * 3 + i] * c[1] + g.rotT[2 * 3 + i] * c[2];
+  const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  for (int e = 0; e < 6; ++e) {
+    long double s = 0;
+    for (int r = 0; r < 3; ++r)
+      s += (long double)g.recip[r * 3 + ij[e][0]] * (long double)g.recip[r * 3 + ij[e][1]];
+    cs.G0[e] = (double)s;
+    cs.G0max = std::max(cs.G0max, std::fabs(cs.G0[e]));
+  }
+  if (!(cs.G0max > 0) || !std::isfinite(cs.G0max)) return cs;
+  cs.tolG = 1e-9 * cs.G0max;
+  cs.ok = true;
+  return cs;
+}
+
 DevGeom dev_geom(const Geom &g) {
   DevGeom d{};
   for (int i = 0; i < 9; ++i) d.rotT[i] = g.rotT[i];
@@ -579,6 +900,11 @@ DevGeom dev_geom(const Geom &g) {
   const bool okx = interval(g.P[0], g.pxX, d.halfX, d.W, &d.xlo, &d.xhi);
   const bool oky = interval(g.P[1], g.pxY, d.halfY, d.H, &d.ylo, &d.yhi);
   d.pre_px = okx && oky && std::isfinite(g.P[2]);
+  const ConeSetup cs = cone_setup(g);
+  d.cone = cs.ok && d.pre_sign;
+  for (int i = 0; i < 3; ++i) d.cLab[i] = cs.cLab[i];
+  for (int i = 0; i < 6; ++i) d.G0[i] = cs.G0[i];
+  d.tolG = cs.tolG;
   return d;
 }
 
@@ -611,10 +937,42 @@ int ensure_fwd_plan(lm_db *db, hipStream_t st) {
   if (db->d_fwd_plan) return LM_OK;
   const int n = db->nhkls;
   const bool pair = db->geom.Elo > 0;
+  const ConeSetup cs = cone_setup(db->geom);
+  static_assert(sizeof(FwdEval) == 64, "plan entries keep the chunk table 16-byte aligned");
   std::vector<FwdEval> ev;
   std::vector<int4> ch;
-  for (int s = 0; s < n; s += 64) {
-    const int e = std::min(n, s + 64);
+  // Friedel mates in file order (the first unpaired -h after h), so that
+  // chunks can end where no pair is split: a chunk is the longest run of at
+  // most 64 hkls that splits no pair, or 64 hkls when every cut would
+  std::vector<int> mate(n, -1);
+  if (pair) {
+    std::vector<bool> taken(n, false);
+    for (int i = 0; i < n; ++i) {
+      if (taken[i]) continue;
+      taken[i] = true;
+      const int32_t *a = &db->hkls[(size_t)3 * i];
+      if ((a[0] | a[1] | a[2]) == 0) continue;
+      for (int j = i + 1; j < n; ++j) {
+        const int32_t *b = &db->hkls[(size_t)3 * j];
+        if (!taken[j] && b[0] == -a[0] && b[1] == -a[1] && b[2] == -a[2]) {
+          mate[i] = j;
+          mate[j] = i;
+          taken[j] = true;
+          break;
+        }
+      }
+    }
+  }
+  for (int s = 0; s < n;) {
+    int e = std::min(n, s + 64);
+    for (int cut = e; cut > s && e < n; --cut) {
+      bool split = false;
+      for (int i = s; i < cut && !split; ++i) split = mate[i] >= cut;
+      if (!split) {
+        e = cut;
+        break;
+      }
+    }
     int4 c;
     c.x = s;
     c.y = (int)ev.size();
@@ -639,11 +997,37 @@ int ensure_fwd_plan(lm_db *db, hipStream_t st) {
       const uint64_t bneg = (uint64_t)(i - s);
       const uint64_t bpos = partner >= 0 ? (uint64_t)(partner - s) : (pair ? 64u : bneg);
       f.bits = bneg | (bpos << 32);
+      f.alo = f.ahi = f.dcone = f.pad = 0;
+      if (cs.ok) {
+        // |q|^2 = h^T G0 h of a rotation of the lattice; eps covers
+        // M^T M - G0 up to tolG per element: |h^T dG h| <= tolG (|h|_1)^2
+        const long double hv[3] = {(long double)a[0], (long double)a[1], (long double)a[2]};
+        const long double Gf[3][3] = {{cs.G0[0], cs.G0[3], cs.G0[4]},
+                                      {cs.G0[3], cs.G0[1], cs.G0[5]},
+                                      {cs.G0[4], cs.G0[5], cs.G0[2]}};
+        long double q2 = 0;
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) q2 += hv[r] * Gf[r][c] * hv[c];
+        const double ql2 = (double)q2;
+        if (!(ql2 > 0)) {
+          f.alo = -1.0;  // |q| = 0: rejected (qlen == 0)
+        } else {
+          const double L1 = std::fabs((double)a[0]) + std::fabs((double)a[1]) +
+                            std::fabs((double)a[2]);
+          const double eps = cs.tolG * L1 * L1 / ql2 + 1e-12;
+          const double ql = std::sqrt(ql2);
+          f.alo = kHc * ql2 * (1 + eps) / (kFourPi * db->geom.Elo) * (1 + 1e-9) + 1e-9 * ql;
+          f.ahi = kHc * ql2 * (1 - eps) / (kFourPi * db->geom.Ehi) * (1 - 1e-9) - 1e-9 * ql;
+          const double kappa = cs.cLab[2] - cs.cosg;
+          f.dcone = 0.5 * kappa * ql2 * (1 + (kappa > 0 ? eps : -eps)) + 1e-9 * ql2;
+        }
+      }
       ev.push_back(f);
     }
     c.z = (int)ev.size();
     c.w = 0;
     ch.push_back(c);
+    s = e;
   }
   const size_t eb = ev.size() * sizeof(FwdEval), cb = ch.size() * sizeof(int4);
   void *d = nullptr;
@@ -1045,6 +1429,77 @@ int selftest_fp64(hipStream_t st, const double *a, const double *b, uint64_t n, 
   (void)hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "selftest_fp64");
   return LM_OK;
+}
+
+
+int launch_fwd_score(lm_db *db, int B, bool guard, const uint8_t *img, const Pred *pred,
+                     uint64_t *hits, uint64_t capPer, uint32_t *counters,
+                     unsigned long long *d_stats, uint32_t *spill, uint64_t spillRows,
+                     hipStream_t st) {
+  if (db->max_spots > kLdsSlots) {
+    set_error("on-the-fly scoring keeps at most 64 spots per orientation (MaxNrLaueSpots)");
+    return LM_ERR_INVALID;
+  }
+  int rc = ensure_fwd_plan(db, st);
+  if (rc) return rc;
+  const FwdEval *plan = (const FwdEval *)db->d_fwd_plan;
+  const int4 *chunks = (const int4 *)(plan + db->fwd_plan_entries);
+  const int nch = db->fwd_plan_chunks;
+  const DevGeom g = dev_geom(db->geom);
+  const DevGeom32 g32 = dev_geom32(db->geom);
+  const double *R = db->geom.recip;
+  // 32M orientations per launch, or as many as the spill of list slots
+  // 32..M-1 holds (M > 32)
+  const uint64_t chunkRows = db->max_spots > kLdsSlotsKeep
+                                 ? spillRows / kFwdBlock * kFwdBlock
+                                 : (uint64_t)(1u << 18) * kFwdBlock;
+  if (chunkRows == 0 || (db->max_spots > kLdsSlotsKeep && !spill)) {
+    set_error("on-the-fly scoring: no spill buffer for list slots past 32");
+    return LM_ERR_INVALID;
+  }
+  for (uint64_t row0 = 0; row0 < db->n; row0 += chunkRows) {
+    const uint64_t rows = std::min<uint64_t>(db->n - row0, chunkRows);
+    const unsigned blocks = (unsigned)((rows + kFwdBlock - 1) / kFwdBlock);
+#define LM_FS(BB, GG)                                                                           \
+  hipLaunchKernelGGL((fwd_score_kernel<BB, GG>), dim3(blocks), dim3(kFwdBlock), 0, st,          \
+                     db->d_orients, db->n, row0, db->d_hkls, R[0], R[1], R[2], R[3], R[4], R[5], \
+                     R[6], R[7], R[8], g, g32, db->max_spots, plan, chunks, nch, img, pred, hits, \
+                     capPer, counters, d_stats, spill)
+#define LM_FS2(BB) \
+  if (guard) LM_FS(BB, true); else LM_FS(BB, false)
+    switch (B) {
+      case 1: LM_FS2(1); break;
+      case 4: LM_FS2(4); break;
+      case 8: LM_FS2(8); break;
+      case 16: LM_FS2(16); break;
+      case 32: LM_FS2(32); break;
+      case 64: LM_FS2(64); break;
+      default: set_error("on-the-fly scoring: unsupported batch width"); return LM_ERR_INVALID;
+    }
+#undef LM_FS2
+#undef LM_FS
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "on-the-fly scoring launch");
+  }
+  return LM_OK;
+}
+
+int launch_fwd_exact(lm_db *db, const uint64_t *xids, uint64_t k, const uint8_t *imgs,
+                     uint64_t npx, const Pred *pred, double *out, hipStream_t st) {
+  if (k == 0) return LM_OK;
+  int rc = ensure_fwd_plan(db, st);
+  if (rc) return rc;
+  const FwdEval *plan = (const FwdEval *)db->d_fwd_plan;
+  const int4 *chunks = (const int4 *)(plan + db->fwd_plan_entries);
+  const DevGeom g = dev_geom(db->geom);
+  const DevGeom32 g32 = dev_geom32(db->geom);
+  const double *R = db->geom.recip;
+  hipLaunchKernelGGL(fwd_exact_kernel, dim3((unsigned)((k + kFwdBlock - 1) / kFwdBlock)),
+                     dim3(kFwdBlock), 0, st, db->d_orients, db->d_hkls, R[0], R[1], R[2], R[3],
+                     R[4], R[5], R[6], R[7], R[8], g, g32, db->max_spots, plan, chunks,
+                     db->fwd_plan_chunks, xids, k, imgs, npx, pred, out);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LM_OK : hip_fail(e, "on-the-fly exact pass launch");
 }
 
 }  // namespace lm

@@ -114,6 +114,8 @@ struct Scratch {
   uint64_t batch_cap = 0;
   uint8_t *img = nullptr;        // staging of host images
   uint64_t img_cap = 0;
+  uint32_t *fwd_spill = nullptr; // on-the-fly scoring: list slots past 32 (words)
+  uint64_t fwd_spill_cap = 0;
   Pred *pred = nullptr;          // device copy of the predicate
   Pred *pred_host = nullptr;     // pinned host copy
   // pinned host copies of the counters and of the first hit records of a
@@ -249,6 +251,21 @@ int table_checksum(lm_db *db, uint64_t *sum);
 
 // lm_fwd.hip: the forward pre-test plan of the DB's hkls (built once)
 int ensure_fwd_plan(lm_db *db, hipStream_t st);
+// lm_fwd.hip: on-the-fly scoring, no table: every orientation projected
+// (float64, or float32 with the guard band) and scored against B = 1, 4, 8,
+// 16, 32 or 64 interleaved images (img) with the table walks' accumulators,
+// predicate and hit records; d_stats[0..2] += pairs recomputed in float64,
+// pairs projected, spots listed.  MaxNrLaueSpots <= 64.
+// List slots 32..M-1 go to spill (at least spillRows * (M - 32) words;
+// launches then take at most spillRows orientations).
+int launch_fwd_score(lm_db *db, int B, bool guard, const uint8_t *img, const Pred *pred,
+                     uint64_t *hits, uint64_t capPer, uint32_t *counters,
+                     unsigned long long *d_stats, uint32_t *spill, uint64_t spillRows,
+                     hipStream_t st);
+// lm_fwd.hip: float64 in-order sums of k candidates (image << 32 | row) of
+// images laid out back to back, each row's spots listed again.
+int launch_fwd_exact(lm_db *db, const uint64_t *xids, uint64_t k, const uint8_t *imgs,
+                     uint64_t npx, const Pred *pred, double *out, hipStream_t st);
 
 // lm_hosttable.cpp: tables larger than the HBM budget, kept on the host and
 // scored in chunks (forward_host / import_host are taken when the resident
@@ -287,6 +304,13 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
 // (f64 = image of doubles; otherwise uint8 values times scale).
 int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
               int min_nr_spots, double min_intensity, hipStream_t stream, lm_hits *out);
+// Images scored with every orientation projected on the fly (no table;
+// mode LM_FWD_FP64 or LM_FWD_FP32_GUARD), groups of at most 64 images per
+// projection pass; the same hits as score_images on the table that
+// lm_db_forward(mode) builds.
+int score_images_fwd(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
+                     const double *im_scales, const int *min_spots, const double *min_int,
+                     int mode, hipStream_t stream, lm_hits *outs, lm_fwd_stats *stats);
 
 // Scoring without allocation (lm_score_dev_noalloc): every device buffer is
 // carved from a file-scope device array whose address noalloc_init (called

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <vector>
 
+#include "lm_gather.h"
 #include "lm_internal.h"
 
 namespace lm {
@@ -43,8 +44,6 @@ namespace {
 
 constexpr int kTile = 256;          // orientations per workgroup (= 1 << tile bits)
 constexpr int kStageWords = 12288;  // 48 KB of staged entries per workgroup
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Packed integer form of "nspots >= min_spots && (double)sum >= thr" for
 // nspots in 1..255 and integer sums below 2^23: the smallest passing nspots
@@ -66,39 +65,6 @@ static uint32_t pack_threshold(int min_spots, double thr) {
 
 __host__ __device__ __forceinline__ bool bit(const Pred &p, int b) {
   return (p.exact[b >> 5] >> (b & 31)) & 1u;
-}
-
-// Words holding one pixel of B interleaved images.
-template <int B>
-struct PxW {
-  static constexpr int W = B < 4 ? 1 : B / 4;
-};
-
-// One gather per spot serves all B images: B bytes at img + pix*B
-// (1 byte, 4 bytes, 8 bytes or B/16 16-byte loads).
-template <int B>
-__device__ __forceinline__ void load_px(const uint8_t *__restrict__ img, uint32_t pix,
-                                        uint32_t (&w)[PxW<B>::W]) {
-  if constexpr (B == 1) {
-    w[0] = img[pix];
-  } else if constexpr (B == 4) {
-    w[0] = *(const uint32_t *)(img + (uint64_t)pix * 4);
-  } else if constexpr (B == 8) {
-    const uint2 v = *(const uint2 *)(img + (uint64_t)pix * 8);
-    w[0] = v.x;
-    w[1] = v.y;
-  } else {
-    static_assert(B % 16 == 0, "batch width must be 1, 4, 8 or a multiple of 16");
-    const u32x4 *p = (const u32x4 *)(img + (uint64_t)pix * B);
-#pragma unroll
-    for (int q = 0; q < B / 16; ++q) {
-      const u32x4 v = p[q];
-      w[4 * q + 0] = v.x;
-      w[4 * q + 1] = v.y;
-      w[4 * q + 2] = v.z;
-      w[4 * q + 3] = v.w;
-    }
-  }
 }
 
 template <int B>
@@ -145,30 +111,6 @@ __device__ __forceinline__ void accum_spots(const uint8_t *__restrict__ img, con
   }
 }
 
-// Packed (SWAR) accumulation of four images per 32-bit word: for the word w
-// holding pixel bytes x0..x3 of images 4j..4j+3, s02 gathers x0 and x2 in two
-// 16-bit sums, s13 gathers x1 and x3, and cn counts the nonzero bytes in four
-// 8-bit fields.  Valid while a row has at most 255 spots (8-bit counts; sums
-// stay below 255 * 255 < 2^16).  A byte x is nonzero exactly when bit 7 of
-// ((x & 0x7f) + 0x7f) | x is set.
-template <int B>
-struct Swar {
-  static constexpr int J = B / 4;
-  uint32_t s02[J], s13[J], cn[J];
-};
-
-template <int B>
-__device__ __forceinline__ void swar_px(const uint32_t (&w)[PxW<B>::W], Swar<B> &a) {
-#pragma unroll
-  for (int j = 0; j < B / 4; ++j) {
-    const uint32_t x = w[j];
-    a.s02[j] += x & 0x00ff00ffu;
-    a.s13[j] += (x >> 8) & 0x00ff00ffu;
-    const uint32_t nz = ((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x;
-    a.cn[j] += (nz >> 7) & 0x01010101u;
-  }
-}
-
 template <int B>
 __device__ __forceinline__ void swar_spots(const uint8_t *__restrict__ img, const uint32_t *src,
                                            uint32_t c, Swar<B> &a) {
@@ -185,33 +127,6 @@ __device__ __forceinline__ void swar_spots(const uint8_t *__restrict__ img, cons
     uint32_t w[PxW<B>::W];
     load_px<B>(img, src[s], w);
     swar_px<B>(w, a);
-  }
-}
-
-// Image b's packed accumulator (nspots << 17 | sum) from the SWAR fields.
-template <int B>
-__device__ __forceinline__ uint32_t swar_acc(const Swar<B> &a, int b) {
-  const int j = b >> 2, q = b & 3;
-  const uint32_t s = (q & 1) ? a.s13[j] : a.s02[j];
-  const uint32_t sum = (q & 2) ? (s >> 16) : (s & 0xffffu);
-  const uint32_t ns = (a.cn[j] >> (8 * q)) & 0xffu;
-  return (ns << kSumBits) | sum;
-}
-
-// Wave-aggregated append: one atomic per wave, lanes write at base + rank.
-__device__ __forceinline__ void append(bool flag, uint64_t rec, uint64_t *__restrict__ out,
-                                       uint32_t *__restrict__ counter, uint64_t cap) {
-  const uint64_t m = __ballot(flag);
-  if (m == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  if (flag) {
-    const uint64_t below = m & ((1ull << lane) - 1ull);
-    const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(below);
-    if (pos < cap) out[pos] = rec;
   }
 }
 
@@ -1167,6 +1082,7 @@ void scratch_release(Scratch &s) {
   if (s.vals) (void)hipFree(s.vals);
   if (s.batch) (void)hipFree(s.batch);
   if (s.img) (void)hipFree(s.img);
+  if (s.fwd_spill) (void)hipFree(s.fwd_spill);
   if (s.pred) (void)hipFree(s.pred);
   if (s.pred_host) (void)hipHostFree(s.pred_host);
   if (s.host_cnt) (void)hipHostFree(s.host_cnt);
@@ -1639,6 +1555,196 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   if (rc)
     for (int b = 0; b < nimg; ++b) lm_hits_free(&outs[b]);
   return rc;
+}
+
+// ---- on-the-fly scoring: no table ------------------------------------------
+//
+// nimg device images (back to back) scored by projecting every orientation
+// in the same launch (launch_fwd_score), in groups of at most 64 images, one
+// projection pass per group; then the table path's hit handling: counters,
+// packed records sorted per image, the float64 in-order re-sums of the
+// candidates of images with imScale != 1 (their spots listed again by
+// launch_fwd_exact), and the final predicate.
+int score_images_fwd(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
+                     const double *im_scales, const int *min_spots, const double *min_int,
+                     int mode, hipStream_t st, lm_hits *outs, lm_fwd_stats *stats) {
+  if (nimg < 1 || nimg > kMaxBatch || !d_imgs || !im_scales || !min_spots || !min_int || !outs) {
+    set_error("score_fwd: need 1..256 images and their thresholds");
+    return LM_ERR_INVALID;
+  }
+  if (mode != LM_FWD_FP64 && mode != LM_FWD_FP32_GUARD) {
+    set_error("score_fwd: unknown forward mode");
+    return LM_ERR_INVALID;
+  }
+  for (int b = 0; b < nimg; ++b)
+    if (!(im_scales[b] > 0) || !std::isfinite(im_scales[b])) {
+      set_error("score_fwd: imScale must be positive and finite");
+      return LM_ERR_INVALID;
+    }
+  const uint64_t npx = (uint64_t)db->geom.nrPxX * db->geom.nrPxY;
+  constexpr int kGroup = 64;
+  if (nimg > kGroup) {
+    lm_fwd_stats acc{}, one{};
+    for (int b0 = 0; b0 < nimg; b0 += kGroup) {
+      const int k = std::min(kGroup, nimg - b0);
+      const int r = score_images_fwd(db, sc, k, d_imgs + (uint64_t)b0 * npx, im_scales + b0,
+                                     min_spots + b0, min_int + b0, mode, st, outs + b0, &one);
+      if (r) {
+        for (int b = 0; b < b0; ++b) lm_hits_free(&outs[b]);
+        return r;
+      }
+      acc.n_orients = one.n_orients;
+      acc.n_spots = one.n_spots;
+      acc.n_pairs += one.n_pairs;
+      acc.n_recomputed += one.n_recomputed;
+      acc.n_projected += one.n_projected;
+      acc.ms += one.ms;
+    }
+    if (stats) *stats = acc;
+    return LM_OK;
+  }
+  int rc = scratch_init(sc);
+  if (rc) return rc;
+  const int B = batch_width(nimg);
+  Pred &pred = *sc.pred_host;
+  build_pred(pred, nimg, im_scales, min_spots, min_int);
+  hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "predicate upload");
+  uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)B, 1u << 16);
+  if (B > 1) {
+    rc = ensure_batch(sc, npx * B);
+    if (rc) return rc;
+  }
+  // the three statistics counters live in the (otherwise unused) candidate list
+  rc = ensure_cands(sc, 4);
+  if (rc) return rc;
+  unsigned long long *d_stats = (unsigned long long *)sc.cands;
+  // list slots 32..M-1 of 4M orientations per launch (512 MiB at M = 64)
+  constexpr uint64_t kSpillRows = 1u << 22;
+  const uint64_t spillRows = std::min<uint64_t>(kSpillRows, db->n);
+  if (db->max_spots > 32) {
+    rc = regrow(sc, &sc.fwd_spill, &sc.fwd_spill_cap,
+                (spillRows + 256) * (uint64_t)(db->max_spots - 32), 4,
+                "hipMalloc(on-the-fly list spill)");
+    if (rc) return rc;
+  }
+  const uint8_t *img = B > 1 ? sc.batch : d_imgs;
+  std::vector<uint32_t> cnt(kCounterWords);
+  uint64_t hstat[3] = {0, 0, 0};
+  float ms = 0;
+  bool grown = false;
+  for (;;) {
+    rc = ensure_hits(sc, capPer * (uint64_t)B);
+    if (rc) return rc;
+    (void)hipEventRecord(sc.ev0, st);
+    if (B > 1) {
+      rc = launch_interleave(B, d_imgs, sc.pred, npx, sc.batch, st);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(kCounterWords), 0, st, sc.counters,
+                       kCounterWords);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, (uint32_t *)sc.cands, 6);
+    rc = launch_fwd_score(db, B, mode == LM_FWD_FP32_GUARD, img, sc.pred, sc.hits, capPer,
+                          sc.counters, d_stats, sc.fwd_spill, spillRows + 256, st);
+    if (rc) return rc;
+    (void)hipEventRecord(sc.ev1, st);
+    e = hipMemcpyAsync(sc.host_cnt, sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hstat, d_stats, 24, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "on-the-fly scoring");
+    memcpy(cnt.data(), sc.host_cnt, kCounterWords * 4);
+    (void)hipEventElapsedTime(&ms, sc.ev0, sc.ev1);
+    uint64_t need = 0;
+    for (int b = 0; b < nimg; ++b) need = std::max<uint64_t>(need, cnt[2 * b]);
+    if (need <= capPer) break;
+    if (grown) {
+      set_error("score_fwd: hit list overflow");
+      return LM_ERR_OVERFLOW;
+    }
+    grown = true;
+    capPer = need + need / 4 + 1024;  // rerun once with a buffer that fits
+  }
+  std::vector<uint64_t> first(nimg + 1, 0);
+  for (int b = 0; b < nimg; ++b) first[b + 1] = first[b] + cnt[2 * b];
+  std::vector<uint64_t> all(first[nimg]);
+  if (first[nimg]) {
+    rc = ensure_vals(sc, first[nimg]);
+    if (rc) return rc;
+    uint64_t *packed = (uint64_t *)sc.vals;
+    hipLaunchKernelGGL(pack_hits_kernel, dim3(nimg), dim3(256), 0, st, sc.hits, capPer,
+                       sc.counters, nimg, packed);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(all.data(), packed, first[nimg] * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "hit copy");
+  }
+  for (int b = 0; b < nimg; ++b) std::sort(all.begin() + first[b], all.begin() + first[b + 1]);
+  // images with imScale != 1: float64 in-order sums of their candidates
+  std::vector<uint64_t> xids;
+  std::vector<uint64_t> xfirst(nimg + 1, 0);
+  for (int b = 0; b < nimg; ++b) {
+    if (!bit(pred, b))
+      for (uint64_t i = first[b]; i < first[b + 1]; ++i)
+        xids.push_back(((uint64_t)b << 32) | ((all[i] >> 26) & 0xffffffffull));
+    xfirst[b + 1] = xids.size();
+  }
+  std::vector<double> xtot(xids.size());
+  if (!xids.empty()) {
+    const uint64_t k = xids.size();
+    rc = ensure_vals(sc, k);
+    if (rc) return rc;
+    // (the hit buffer holds at least k records and is free again)
+    e = hipMemcpyAsync(sc.hits, xids.data(), k * 8, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "candidate upload");
+    rc = launch_fwd_exact(db, sc.hits, k, d_imgs, npx, sc.pred, sc.vals, st);
+    if (rc) return rc;
+    e = hipMemcpyAsync(xtot.data(), sc.vals, k * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "on-the-fly exact pass");
+  }
+  for (int b = 0; b < nimg; ++b) memset(&outs[b], 0, sizeof(lm_hits));
+  for (int b = 0; b < nimg && rc == LM_OK; ++b) {
+    lm_hits &h = outs[b];
+    h.kernel_ms = ms;
+    h.walk_ms = ms;
+    const uint64_t k = cnt[2 * b];
+    const uint64_t *rec = all.data() + first[b];
+    std::vector<double> tot(k);
+    for (uint64_t i = 0; i < k; ++i)
+      tot[i] = bit(pred, b) ? (double)(rec[i] & kSumMask) : xtot[xfirst[b] + i];
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < k; ++i)
+      if (tot[i] >= min_int[b]) ++m;
+    h.nr_results = (uint64_t)cnt[2 * b + 1] + m;
+    h.count = m;
+    rc = fill_hits(h, m);
+    if (rc) break;
+    uint64_t j = 0;
+    for (uint64_t i = 0; i < k; ++i) {
+      if (!(tot[i] >= min_int[b])) continue;
+      const uint32_t packed = (uint32_t)(rec[i] & 0x3ffffffull);
+      const int ns = (int)(packed >> kSumBits);
+      h.ids[j] = (rec[i] >> 26) & 0xffffffffull;
+      h.nspots[j] = ns;
+      h.totint[j] = tot[i];
+      h.score[j] = tot[i] * std::sqrt((double)ns);  // src/LaueMatchingCPU.c:688
+      ++j;
+    }
+  }
+  if (rc) {
+    for (int b = 0; b < nimg; ++b) lm_hits_free(&outs[b]);
+    return rc;
+  }
+  if (stats) {
+    stats->n_orients = db->n;
+    stats->n_spots = hstat[2];
+    stats->n_pairs = db->n * (uint64_t)db->nhkls;
+    stats->n_recomputed = hstat[0];
+    stats->n_projected = hstat[1];
+    stats->ms = ms;
+  }
+  return LM_OK;
 }
 
 // ---- scoring that allocates nothing -----------------------------------------

@@ -1,0 +1,170 @@
+"""GPU parity of the forward pre-tests and of on-the-fly scoring.
+
+* The cone pre-test (rotations of the lattice) and its fallbacks (matrices
+  that are not rotations, geometries where the cone does not apply) give
+  tables byte-identical to the oracle's ForwardFile records.
+* lm_score_fwd_batch_dev projects every orientation while it scores, with no
+  table: each image's hits (ids, spot counts, float64 sums and scores,
+  nrResults) equal the table path's and the oracle's, in float64 and in
+  float32 with the guard band, for batch widths 1..70 with per-image
+  thresholds and scales.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from lauematching_amd import _lib, api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return api.Context(0)
+
+
+def _db(ctx, p, O, hk):
+    return api.OrientationDB(ctx, O, hk, api.Geometry.from_params(p), p.MaxNrLaueSpots)
+
+
+FALLBACK_GEOMS = {
+    # corners 85 deg off the detector axis: the cone pre-test is not used
+    "wide": dict(P_Array=(0.0, 0.0, 0.025)),
+    # no lower energy bound: no sign test, no Friedel pairing, no cone
+    "elo0": dict(Elo=0.0),
+    # detector behind the sample plane (P2 < 0), off-centre, tilted
+    "behind": dict(P_Array=(0.01, -0.02, -0.4), R_Array=(-1.1, -1.3, -1.2)),
+}
+
+
+@pytest.mark.parametrize("geo", list(FALLBACK_GEOMS))
+@pytest.mark.parametrize("mode", [api.LM_FWD_FP64, api.LM_FWD_FP32_GUARD])
+def test_forward_pretest_geometries(ctx, geo, mode):
+    p = synth.baseline_params(MaxNrLaueSpots=64, **FALLBACK_GEOMS[geo])
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.random_orientations(12000, seed=21)
+    _say(f"forward pre-test geo={geo} mode={mode}")
+    db = _db(ctx, p, O, hk)
+    db.forward(mode)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    np.testing.assert_array_equal(db.export_table(), want)
+
+
+@pytest.mark.parametrize("mode", [api.LM_FWD_FP64, api.LM_FWD_FP32_GUARD])
+def test_forward_matrices_that_are_not_rotations(ctx, mode):
+    """Rows scaled by 1 + 1e-6 or 0.97, or sheared, fail the M^T M check and
+    send their wave to the full pre-test; rows of the same wave that are
+    rotations, and whole waves of rotations, keep the cone pre-test.  Every
+    row's spot list equals the oracle's."""
+    p = synth.baseline_params(MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.random_orientations(16384, seed=22).reshape(-1, 9).copy()
+    rng = np.random.default_rng(5)
+    O[64:96] *= 1 + 1e-6                      # half a wave, barely off
+    O[1000:1003] *= 0.97                      # three rows of another wave
+    sh = O[5000:5064].reshape(-1, 3, 3)       # a whole sheared wave
+    sh[:, 0, 1] += 0.01
+    O[5000:5064] = sh.reshape(-1, 9)
+    O[9000] = rng.normal(size=9)               # an arbitrary matrix
+    _say(f"forward non-rotations mode={mode}")
+    db = _db(ctx, p, O, hk)
+    db.forward(mode)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    np.testing.assert_array_equal(db.export_table(), want)
+
+
+def _frames(p, hk, O, n, seed):
+    """n uint8 frames of 1..4 grains of the DB (imScale 1 and != 1), plus a
+    blank one when n > 2."""
+    rng = np.random.default_rng(seed)
+    imgs, scales = [], []
+    for i in range(n):
+        if n > 2 and i == n - 1:
+            imgs.append(np.zeros(p.NrPxX * p.NrPxY, np.uint8))
+            scales.append(1.0)
+            continue
+        k = int(rng.integers(1, 5))
+        g = rng.choice(O.shape[0], k, replace=False)
+        img = synth.synth_image(p, hk, O[g], seed=seed + i)
+        if i % 3 == 1:  # a detector frame whose maximum is not 255
+            img = img * 0.613 + (img > 0) * 0.05
+        u8, s = api.quantize_u8(img)
+        imgs.append(u8.ravel())
+        scales.append(s)
+    return imgs, scales
+
+
+def _same(a, b):
+    assert a.nr_results == b.nr_results
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.nspots, b.nspots)
+    np.testing.assert_array_equal(a.totint.view(np.uint64), b.totint.view(np.uint64))
+    np.testing.assert_array_equal(a.score.view(np.uint64), b.score.view(np.uint64))
+
+
+@pytest.fixture(scope="module")
+def fly(ctx):
+    p = synth.baseline_params(MaxNrLaueSpots=64, NrPxX=512, NrPxY=512, PxX=0.0008, PxY=0.0008)
+    hk = synth.fcc_hkls(8, "box")
+    O = synth.random_orientations(24000, seed=23)
+    imgs, scales = _frames(p, hk, O, 70, seed=40)
+    table, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    return p, hk, O, imgs, scales, table
+
+
+@pytest.mark.parametrize("nimg", [1, 3, 8, 16, 33, 64, 70])
+@pytest.mark.parametrize("mode", [api.LM_FWD_FP64, api.LM_FWD_FP32_GUARD])
+def test_score_fwd_equals_table_and_oracle(ctx, fly, nimg, mode):
+    import torch
+
+    p, hk, O, imgs, scales, table = fly
+    rng = np.random.default_rng(nimg)
+    ims = imgs[:nimg - 1] + [imgs[-1]] if nimg > 1 else imgs[:1]
+    sc = scales[:nimg - 1] + [scales[-1]] if nimg > 1 else scales[:1]
+    msp = [int(v) for v in rng.integers(3, 8, nimg)]
+    mint = [float(v) for v in rng.uniform(200, 900, nimg)]
+    if nimg >= 8:
+        msp[2], mint[2] = 0, 0.0      # every row passes, zero-spot rows counted
+        msp[4], mint[4] = 5, -1.0     # MinIntensity <= 0
+    _say(f"score on the fly nimg={nimg} mode={mode}")
+    db = _db(ctx, p, O, hk)
+    dev = torch.from_numpy(np.stack(ims)).to("cuda:0")
+    torch.cuda.synchronize()
+    got, st = db.score_fwd_batch_dev(dev.data_ptr(), nimg, sc, msp, mint, mode)
+    assert st.n_pairs == O.shape[0] * len(hk) * ((nimg + 63) // 64)
+    assert st.n_spots == int(table[:, 0].astype(np.int64).sum())
+    if mode == api.LM_FWD_FP32_GUARD:
+        assert st.n_recomputed > 0
+    # the table path on the same DB, then the oracle for every image
+    db.forward(api.LM_FWD_FP64)
+    want = db.score_batch_each(ims, sc, msp, mint)
+    for b in range(nimg):
+        _same(got[b], want[b])
+        matched, _, _, nres = ob.compare(table, 64, ims[b].reshape(p.NrPxY, p.NrPxX), p.NrPxX,
+                                         sc[b], msp[b], mint[b])
+        rows, mA = ob.collect(matched)
+        assert got[b].nr_results == nres
+        np.testing.assert_array_equal(got[b].ids, rows)
+        np.testing.assert_array_equal(got[b].score.view(np.uint64), mA.view(np.uint64))
+    assert sum(len(h.ids) for h in got) > 0
+    db.close()
+
+
+def test_score_fwd_rejects_more_than_64_spots(ctx, fly):
+    import torch
+
+    p, hk, O, imgs, scales, _ = fly
+    p2 = synth.baseline_params(MaxNrLaueSpots=65, NrPxX=512, NrPxY=512, PxX=0.0008,
+                               PxY=0.0008)
+    db = _db(ctx, p2, O[:1000], hk)
+    dev = torch.from_numpy(imgs[0]).to("cuda:0")
+    torch.cuda.synchronize()
+    with pytest.raises(_lib.LaueMatchError):
+        db.score_fwd_batch_dev(dev.data_ptr(), 1, [1.0], [5], [500.0])
