@@ -331,7 +331,11 @@ def run_serving(ctx, dbs, reqs, rates, duration, nmax):
         for t in [srv.submit(*rq[k % len(rq)], 5, 1000.0) for k in range(3)]:
             srv.wait(t)
         runs = [poisson_run(srv, rq, r, duration, 100 + j) for j, r in enumerate(rates)]
-        cl = closed_loop(srv, rq, nmax, window=2 * kw["max_batch"] if kw["max_batch"] > 1 else 128)
+        # closed loop: two full batches outstanding for one DB, one per DB
+        # when there are more (requests split over the DBs; the plain server
+        # takes one image per launch whatever the window)
+        mb = kw["max_batch"]
+        cl = closed_loop(srv, rq, nmax, window=max(2 * mb, mb * len(dbs)) if mb > 1 else 128)
         st = srv.stats()
         srv.close()
         res[name] = {"poisson": runs, "closed_loop": cl, "graph_launches": st["graph_launches"]}

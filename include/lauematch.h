@@ -193,6 +193,12 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * different copies (the autotuner sets these); LM_OPT_BATCH_TILE_BITS sets
  * every fitting width at once. */
 #define LM_OPT_BATCH_WIDTH_TILE_BITS 9
+/* LM_OPT_BATCH_SUM16: bit k set (k = 2..6) = batches of width 2^k read their
+ * batch copy with 16-bit sums, two images per LDS counter word, so the copy
+ * may have B x 2^bits <= 65536 (set this before LM_OPT_BATCH_WIDTH_TILE_BITS);
+ * the (row, image) pairs whose sum can pass are recounted exactly from the
+ * table.  Calls where some MinIntensity <= 0 read the table instead. */
+#define LM_OPT_BATCH_SUM16 10
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
@@ -221,6 +227,9 @@ typedef struct lm_tune_report {
   uint32_t nibble_mask;       /* chosen LM_OPT_BATCH_NIBBLE                       */
   int32_t batch_tile_bits_w[7]; /* per width: super-tile bits of the batch copy it
                                    keeps (0 = a table walk)                        */
+  double ms_banded16[7][5];   /* per width: batch copy 2^9..2^13 read with 16-bit
+                                 sums (0 = no fit)                               */
+  uint32_t sum16_mask;        /* chosen LM_OPT_BATCH_SUM16                        */
 } lm_tune_report;
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out);

@@ -81,7 +81,8 @@ class LmTuneReport(ctypes.Structure):
                 ("banded_tile_bits", ctypes.c_int32), ("banded_loads", ctypes.c_int32),
                 ("batch_tile_bits", ctypes.c_int32), ("batch_widths_mask", ctypes.c_uint32),
                 ("grouped_mask", ctypes.c_uint32), ("nibble_mask", ctypes.c_uint32),
-                ("batch_tile_bits_w", ctypes.c_int32 * 7)]
+                ("batch_tile_bits_w", ctypes.c_int32 * 7),
+                ("ms_banded16", (ctypes.c_double * 5) * 7), ("sum16_mask", ctypes.c_uint32)]
 
 
 _lib = None

@@ -191,6 +191,13 @@ class OrientationDB:
         clear: 128 images take the uint8 walks, wider batches two calls."""
         check(lib().lm_db_set_option(self._h, 7, int(mask)))
 
+    def set_batch_sum16(self, mask: int) -> None:
+        """Bit k set (k = 2..6): batches of width 2**k read their batch copy
+        with 16-bit sums (two images per LDS word, copies with
+        width x 2^bits <= 65536) and recount the candidates exactly; set it
+        before set_batch_width_tile_bits."""
+        check(lib().lm_db_set_option(self._h, 10, int(mask)))
+
     def get_option(self, option: int) -> int:
         v = ctypes.c_int64()
         check(lib().lm_db_get_option(self._h, int(option), ctypes.byref(v)))
@@ -233,11 +240,13 @@ class OrientationDB:
                              **({"grouped": r.ms_grouped[i]} if r.ms_grouped[i] > 0 else {}),
                              **({"nibble": r.ms_nibble[i]} if r.ms_nibble[i] > 0 else {}),
                              **{f"banded{tb}": r.ms_banded[i][tb - 9]
-                                for tb in range(9, 14) if r.ms_banded[i][tb - 9] > 0}}
+                                for tb in range(9, 14) if r.ms_banded[i][tb - 9] > 0},
+                             **{f"banded{tb}_sum16": r.ms_banded16[i][tb - 9]
+                                for tb in range(9, 14) if r.ms_banded16[i][tb - 9] > 0}}
                          for i, B in enumerate(widths)},
             "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
             "batch_widths_mask": r.batch_widths_mask, "grouped_mask": r.grouped_mask,
-            "nibble_mask": r.nibble_mask,
+            "nibble_mask": r.nibble_mask, "sum16_mask": r.sum16_mask,
             "batch_tile_bits_per_width": {B: r.batch_tile_bits_w[i] for i, B in enumerate(widths)},
         }
 

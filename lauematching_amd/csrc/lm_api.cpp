@@ -530,9 +530,10 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
   }
   if (option == LM_OPT_BATCH_WIDTH_TILE_BITS) {
     const int64_t k = value >> 8, tb = value & 0xff;
-    if (k < 2 || k > 6 || (tb != 0 && (tb < 9 || tb > 13 || (1ll << k) * (1ll << tb) > 32768))) {
+    const int64_t lim = (k >= 2 && k <= 6 && ((db->batch16_widths >> k) & 1)) ? 65536 : 32768;
+    if (k < 2 || k > 6 || (tb != 0 && (tb < 9 || tb > 13 || (1ll << k) * (1ll << tb) > lim))) {
       set_error("LM_OPT_BATCH_WIDTH_TILE_BITS: (log2 width 2..6) << 8 | bits (0, or 9..13 "
-                "with width x 2^bits <= 32768)");
+                "with width x 2^bits <= 32768, or <= 65536 with 16-bit sums)");
       return LM_ERR_INVALID;
     }
     db->width_tb[k] = (int8_t)tb;
@@ -570,6 +571,15 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     ++db->generation;
     return LM_OK;
   }
+  if (option == LM_OPT_BATCH_SUM16) {
+    if (value < 0 || (value & ~(int64_t)0x7c)) {
+      set_error("LM_OPT_BATCH_SUM16 is a mask of bits 2..6 (batches of 4..64 images)");
+      return LM_ERR_INVALID;
+    }
+    db->batch16_widths = (uint32_t)value;
+    ++db->generation;
+    return LM_OK;
+  }
   if (option == LM_OPT_BATCH_NIBBLE) {
     if (value < 0 || value > 0x180 || (value & 0x7f)) {
       set_error("LM_OPT_BATCH_NIBBLE is a mask of bits 7 and 8 (128 and 256 images)");
@@ -598,6 +608,7 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
     case LM_OPT_TABLE_ON_HOST: *value = db->htable.ready ? 1 : 0; return LM_OK;
     case LM_OPT_BATCH_GROUPED: *value = db->group_widths; return LM_OK;
     case LM_OPT_BATCH_NIBBLE: *value = db->nib_widths; return LM_OK;
+    case LM_OPT_BATCH_SUM16: *value = db->batch16_widths; return LM_OK;
     default: set_error("unknown option"); return LM_ERR_INVALID;
   }
 }

@@ -853,6 +853,113 @@ score_band_batch_kernel(const uint32_t *__restrict__ bentries,
   }
 }
 
+// The batch walk with 16-bit sums: one LDS word holds the sums of two
+// images -- images 4j and 4j+2 in word 2j, images 4j+1 and 4j+3 in word
+// 2j+1, as the interleaved pixel word of images 4j..4j+3 splits with the
+// masks 0x00ff00ff and 0x00ff00ff << 8 -- so a spot costs B/2 LDS adds
+// instead of B, and B images fit 2^tb rows with B * 2^tb <= 65536 (twice
+// the rows of score_band_batch_kernel).  The sums are exact (255 *
+// MaxNrLaueSpots < 2^16: a half-word never carries into its neighbour); a
+// (row, image) pair whose sum reaches max(ceil(cand_thr), MinNrSpots, 1)
+// (each nonzero pixel adds at least 1) can pass and is listed as
+// b << 32 | row for the exact recount from the orientation-major table
+// (nib_recount_kernel), which also counts its spots.  Needs MinIntensity > 0
+// for every image (else every row with a spot is a candidate).
+template <int TB, int B, int U>
+__global__ void __launch_bounds__((1 << TB) < 1024 ? (1 << TB) : 1024)
+score_band_batch16_kernel(const uint32_t *__restrict__ bentries,
+                          const uint64_t *__restrict__ tileOff,
+                          const uint32_t *__restrict__ bandOff, int nbands, uint64_t n,
+                          uint64_t nsuper, const uint8_t *__restrict__ img,
+                          const Pred *__restrict__ pred, uint64_t *__restrict__ cands,
+                          uint64_t candCap, uint32_t *__restrict__ counters) {
+  constexpr int T = 1 << TB;
+  constexpr int THREADS = T < 1024 ? T : 1024;
+  constexpr int PB = 32 - TB;
+  constexpr uint32_t PMASK = (1u << PB) - 1u;
+  constexpr uint32_t STEP = THREADS * U;
+  constexpr int NP = B / 2;  // counter words per row
+  static_assert(B >= 4 && B % 4 == 0, "four images per pixel word");
+  static_assert(NP * T * 4 <= 131072, "LDS counters exceed 128 KiB");
+  __shared__ uint32_t cnt[NP * T];
+  const int tid = threadIdx.x, lane = __lane_id();
+  const int nimg = pred->nimg;
+  for (uint64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
+    for (int i = tid; i < NP * T; i += THREADS) cnt[i] = 0;
+    __syncthreads();
+    const uint64_t first = t << TB;
+    const uint32_t *src = bentries + tileOff[first >> kTileBits];
+    const uint32_t *bo = bandOff + t * (uint64_t)(nbands + 1);
+    const uint32_t total = bo[nbands];
+    int band = 0;
+    uint32_t bend = bo[1];
+    uint32_t e[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      e[k] = i < total ? __builtin_nontemporal_load(src + i) : 0u;
+    }
+    for (uint32_t g0 = 0; g0 < total; g0 += STEP) {
+      uint32_t w[U][B / 4];
+      uint32_t row[U];
+      bool ok[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = g0 + k * THREADS + tid;
+        row[k] = e[k] >> PB;
+        ok[k] = i < total;
+        if (ok[k]) {
+          while (i >= bend) {
+            ++band;
+            bend = bo[band + 1];
+          }
+          gather_b<B>(img, ((uint64_t)band << PB) | (e[k] & PMASK), w[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = g0 + STEP + k * THREADS + tid;
+        e[k] = i < total ? __builtin_nontemporal_load(src + i) : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        if (!ok[k]) continue;
+#pragma unroll
+        for (int j = 0; j < B / 4; ++j) {
+          const uint32_t v = w[k][j];
+          const uint32_t lo = v & 0x00ff00ffu, hi = (v >> 8) & 0x00ff00ffu;
+          if (lo) atomicAdd(&cnt[(2 * j) * T + row[k]], lo);
+          if (hi) atomicAdd(&cnt[(2 * j + 1) * T + row[k]], hi);
+        }
+      }
+    }
+    __syncthreads();
+    for (int rr = tid; rr < T; rr += THREADS) {
+      const uint64_t o = first + rr;
+      const bool live = o < n;
+      for (int b = 0; b < nimg; ++b) {
+        const uint32_t word = cnt[(2 * (b >> 2) + (b & 1)) * T + rr];
+        const uint32_t sum = (b & 2) ? word >> 16 : word & 0xffffu;
+        const uint32_t pk = pred->pk_thr[b];
+        const uint32_t nthr = (pk >> 23) > 1u ? (pk >> 23) : 1u, sthr = pk & 0x7fffffu;
+        const bool cand = live && sum >= nthr && sum >= sthr;
+        const uint64_t m = __ballot(cand);
+        if (m) {
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          uint32_t base = 0;
+          if (lane == leader) base = atomicAdd(&counters[kCandSlot], (uint32_t)__popcll(m));
+          base = __shfl(base, leader);
+          if (cand) {
+            const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (pos < candCap) cands[pos] = ((uint64_t)b << 32) | o;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 int reserve_bytes(lm_db *db, uint64_t bytes) {
   std::lock_guard<std::mutex> lk(db->ctx->mu);
   if (db->ctx->in_use + bytes > db->ctx->budget) {
@@ -1113,7 +1220,52 @@ bool banded_batch_fits(const lm_db *db, int B) {
   const int tb = db->width_tb[width_log2(B)];
   if (tb < 9 || tb > 13) return false;
   const Banded &b = db->banded_bk[tb - 9];
-  return b.ready && (uint64_t)B * (1ull << b.tb) <= 32768;
+  // 32-bit counters: B * 2^tb <= 32768; 16-bit sums: B * 2^tb <= 65536
+  const uint64_t lim = ((db->batch16_widths >> width_log2(B)) & 1) ? 65536 : 32768;
+  return b.ready && (uint64_t)B * (1ull << b.tb) <= lim;
+}
+
+bool banded_batch16(const lm_db *db, int B) {
+  return banded_batch_fits(db, B) && ((db->batch16_widths >> width_log2(B)) & 1);
+}
+
+int launch_banded_batch16(lm_db *db, Scratch &sc, int B, const uint8_t *img, hipStream_t st) {
+  const Banded &b = db->banded_bk[db->width_tb[width_log2(B)] - 9];
+  const Table &t = db->table;
+  const unsigned grid = (unsigned)std::min<uint64_t>(b.nsuper, 1u << 20);
+  const unsigned threads = (1u << b.tb) < 1024 ? (1u << b.tb) : 1024;
+#define LM_B16(TB, BB)                                                                           \
+  hipLaunchKernelGGL((score_band_batch16_kernel<TB, BB, 4>), dim3(grid), dim3(threads), 0, st,   \
+                     b.entries, t.tile_off, b.band_off, b.nbands, db->n, b.nsuper, img, sc.pred, \
+                     sc.cands, sc.cand_cap, sc.counters)
+  switch (b.tb * 100 + B) {
+    case 1304: LM_B16(13, 4); break;
+    case 1308: LM_B16(13, 8); break;
+    case 1204: LM_B16(12, 4); break;
+    case 1208: LM_B16(12, 8); break;
+    case 1216: LM_B16(12, 16); break;
+    case 1104: LM_B16(11, 4); break;
+    case 1108: LM_B16(11, 8); break;
+    case 1116: LM_B16(11, 16); break;
+    case 1132: LM_B16(11, 32); break;
+    case 1004: LM_B16(10, 4); break;
+    case 1008: LM_B16(10, 8); break;
+    case 1016: LM_B16(10, 16); break;
+    case 1032: LM_B16(10, 32); break;
+    case 1064: LM_B16(10, 64); break;
+    case 904: LM_B16(9, 4); break;
+    case 908: LM_B16(9, 8); break;
+    case 916: LM_B16(9, 16); break;
+    case 932: LM_B16(9, 32); break;
+    case 964: LM_B16(9, 64); break;
+    default:
+      set_error("no 16-bit batch kernel for this batch width and super-tile size");
+      return LM_ERR_INVALID;
+  }
+#undef LM_B16
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "16-bit batch kernel launch");
+  return LM_OK;
 }
 
 int launch_banded_batch(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,

@@ -203,6 +203,7 @@ struct lm_db {
   double width_ms[9] = {};      // autotuned device ms per launch at width 2^k (0 = unknown)
   uint32_t group_widths = 0xc0; // bit log2(B): table batches of width B use grouped lanes
   uint32_t nib_widths = 0x180;  // bit log2(B): batches of width B (128, 256) use 4-bit bounds
+  uint32_t batch16_widths = 0;  // bit log2(B): the batch copy read with 16-bit sums
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   lm::HostTable htable;         // the table when it does not fit the budget
@@ -291,6 +292,10 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
 // 2^22 (32 MiB); a call with more candidates reruns over the table.
 uint64_t cand_capacity(const lm_db *db);
 bool banded_batch_fits(const lm_db *db, int B);
+// Width B reads its batch copy with 16-bit sums (LM_OPT_BATCH_SUM16):
+// candidates to cands (b << 32 | row) for the exact recount.
+bool banded_batch16(const lm_db *db, int B);
+int launch_banded_batch16(lm_db *db, Scratch &sc, int B, const uint8_t *img, hipStream_t st);
 int launch_banded_batch(lm_db *db, Scratch &sc, int B, const uint8_t *img, uint64_t capPer,
                         hipStream_t st);
 

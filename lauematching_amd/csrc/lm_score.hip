@@ -1291,7 +1291,8 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   if (e != hipSuccess) return hip_fail(e, "predicate upload");
   uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)B, 1u << 16);
   // kernel: 0 lane-per-orientation over the table, 1 single-image banded
-  // copy, 2 batch banded copy
+  // copy, 2 batch banded copy, 3 4-bit bounds over the table, 4 batch
+  // banded copy with 16-bit sums (3 and 4 recount their candidates)
   int kind = 0;
   // the 16-bit-sum walk keeps rows whose sum reaches MinIntensity as
   // candidates: with MinIntensity <= 0 that is every row, so the table walk
@@ -1310,7 +1311,18 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     rc = ensure_chunk_flags(sc, (uint64_t)db->banded.nchunks);
     if (rc) return rc;
   }
-  if (B > 1 && banded_batch_fits(db, B) && ((db->batch_widths >> width_log2(B)) & 1)) kind = 2;
+  if (B > 1 && banded_batch_fits(db, B) && ((db->batch_widths >> width_log2(B)) & 1)) {
+    // 4: the batch copy with 16-bit sums and the exact recount of its
+    // candidates, when every MinIntensity > 0 (else the table walk)
+    bool pos = true;
+    for (int b = 0; b < nimg; ++b) pos &= min_int[b] > 0;
+    kind = banded_batch16(db, B) ? (pos ? 4 : 0) : 2;
+  }
+  if (kind == 4) {
+    rc = ensure_cands(sc, std::min<uint64_t>(
+                              1u << 22, std::max<uint64_t>(cand_capacity(db), db->n * B / 64)));
+    if (rc) return rc;
+  }
   // 3: 4-bit bounds of 128 / 256 images over the table plus the exact recount
   if (nib) {
     kind = 3;
@@ -1341,6 +1353,15 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
                        kCounterWords);
     if (kind == 3) return launch_nibble_score(db, sc, B, d_imgs, npx, cp, st);
     if (kind == 2) return launch_banded_batch(db, sc, B, img, cp, st);
+    if (kind == 4) {
+      const int r = launch_banded_batch16(db, sc, B, img, st);
+      if (r) return r;
+      hipLaunchKernelGGL(nib_recount_kernel, dim3(512), dim3(256), 0, st, t.counts, t.tile_off,
+                         t.entries, sc.cands, sc.cand_cap, d_imgs, npx, sc.pred, sc.hits, cp,
+                         sc.counters);
+      const hipError_t le = hipGetLastError();
+      return le == hipSuccess ? LM_OK : hip_fail(le, "batch recount launch");
+    }
     const int r = kind == 1 ? launch_banded(db, sc, img, cp, st) : launch_score(db, sc, B, img, cp, st);
     if (r || !dev_exact) return r;
     // one thread per record the buffer can hold (the count is only known on
@@ -1432,6 +1453,10 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
       kind = 0;  // more candidates than the list holds: rerun over the table
       continue;
     }
+    if (kind == 4 && cnt[kCandSlot] > sc.cand_cap) {
+      kind = 0;  // more candidates than the list holds: rerun over the table
+      continue;
+    }
     if (kind == 3 && cnt[kCandSlot] > sc.cand_cap) {
       // the same for the 4-bit walk: uint8 walks (two calls above 128 images)
       if (B > kMaxByteBatch) return halves();
@@ -1510,9 +1535,10 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (e == hipSuccess) {
       // image b's bytes: separate images for the 4-bit walk and single
       // images, interleaved for the uint8 batch walks
-      const uint8_t *xi = kind == 3 ? d_imgs : img;
-      const uint32_t stride = kind == 3 ? 1u : (uint32_t)B;
-      const uint64_t bstep = kind == 3 ? npx : (B > 1 ? 1u : 0u);
+      const bool sep = kind == 3 || kind == 4;
+      const uint8_t *xi = sep ? d_imgs : img;
+      const uint32_t stride = sep ? 1u : (uint32_t)B;
+      const uint64_t bstep = sep ? npx : (B > 1 ? 1u : 0u);
       hipLaunchKernelGGL(exact_kernel, dim3((unsigned)((k + 127) / 128)), dim3(128), 0, st,
                          t.counts, t.tile_off, t.entries, sc.hits, k, xi, stride, bstep, sc.pred,
                          sc.vals);

@@ -6,7 +6,8 @@
 //     pixels with an exact recount of the candidates (128, 256),
 //   - the single-image banded copy (super-tiles of 2^13 .. 2^16 rows),
 //   - the batch banded copy (super-tiles of 2^9..2^13 rows; a batch of width
-//     B fits when B * 2^tb <= 32768, its LDS counters filling <= 128 KiB).
+//     B fits when B * 2^tb <= 32768, its LDS counters filling <= 128 KiB, or
+//     B * 2^tb <= 65536 with 16-bit sums and an exact recount).
 // Which one is fastest depends on the image density, the table's spots per
 // orientation and the batch width, so the tuner times each candidate on the
 // caller's sample images (device time of the whole launch sequence, best of
@@ -62,12 +63,14 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
   const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
   const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths,
-                 saved_nib = db->nib_widths;
+                 saved_nib = db->nib_widths, saved_s16 = db->batch16_widths;
   int8_t saved_wtb[9];
   memcpy(saved_wtb, db->width_tb, sizeof saved_wtb);
-  auto set_all_widths = [&](int tb) {  // every width that tb's counters hold
+  // every width that tb's counters hold: 32-bit counters B x 2^tb <= 32768,
+  // 16-bit sums B x 2^tb <= 65536
+  auto set_all_widths = [&](int tb, int lim = 32768) {
     for (int k = 0; k < 9; ++k)
-      db->width_tb[k] = (tb && k >= 2 && k <= 6 && (1 << k) * (1 << tb) <= 32768) ? (int8_t)tb : 0;
+      db->width_tb[k] = (tb && k >= 2 && k <= 6 && (1 << k) * (1 << tb) <= lim) ? (int8_t)tb : 0;
   };
   auto restore = [&](int code) {
     db->banded_tb = saved_tb;
@@ -77,6 +80,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     db->banded_loads = saved_loads;
     db->group_widths = saved_group;
     db->nib_widths = saved_nib;
+    db->batch16_widths = saved_s16;
     const int r = build_banded(db);
     return code ? code : r;
   };
@@ -119,6 +123,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   rep->n_widths = nw;
   db->banded_tb = 0;  // room for the batch copies
   db->batch_tb = 0;
+  db->batch16_widths = 0;
   set_all_widths(0);
   rc = build_banded(db);
   if (rc) return restore(rc);
@@ -171,6 +176,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->batch_widths = 0x7f;
   for (int tb = 9; tb <= 13; ++tb) {
     db->batch_tb = tb;
+    db->batch16_widths = 0;
     set_all_widths(tb);
     rc = build_banded(db);
     if (rc) return restore(rc);
@@ -180,24 +186,45 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
       rc = time_batch(db, d, B, reps, &rep->ms_banded[w][tb - 9]);
       if (rc) return restore(rc);
     }
+    // the same copy read with 16-bit sums (twice the rows per width)
+    db->batch16_widths = 0x7c;
+    set_all_widths(tb, 65536);
+    ++db->generation;
+    for (int w = 0; w < nw; ++w) {
+      const int B = rep->widths[w];
+      if (!banded_batch16(db, B)) continue;
+      rc = time_batch(db, d, B, reps, &rep->ms_banded16[w][tb - 9]);
+      if (rc) return restore(rc);
+    }
+    db->batch16_widths = 0;
   }
   // per width: the fastest batch copy when it beats the table walks (each
-  // width may keep a different super-tile size; the copies are built once)
-  uint32_t mask = 0;
+  // width may keep a different super-tile size and counter width; the copies
+  // are built once)
+  uint32_t mask = 0, s16 = 0;
   int8_t wtb[9] = {};
+  std::vector<double> wms(nw, 0.0);
   for (int w = 0; w < nw; ++w) {
     const int k = width_log2(rep->widths[w]);
     double best = tbl[w];
-    for (int tb = 9; tb <= 13; ++tb) {
-      const double ms = rep->ms_banded[w][tb - 9];
-      if (ms > 0 && ms < best) {
-        best = ms;
-        wtb[k] = (int8_t)tb;
+    wms[w] = tbl[w];
+    for (int tb = 9; tb <= 13; ++tb)
+      for (int v = 0; v < 2; ++v) {
+        const double ms = v ? rep->ms_banded16[w][tb - 9] : rep->ms_banded[w][tb - 9];
+        if (ms > 0 && ms < best) {
+          best = ms;
+          wms[w] = ms;
+          wtb[k] = (int8_t)tb;
+          if (v)
+            s16 |= 1u << k;
+          else
+            s16 &= ~(1u << k);
+        }
       }
-    }
     rep->batch_tile_bits_w[w] = wtb[k];
     if (wtb[k]) mask |= 1u << k;
   }
+  rep->sum16_mask = s16;
   int best_btb = 0;  // reported: the copy of the narrowest width that keeps one
   for (int w = nw - 1; w >= 0; --w)
     if (wtb[width_log2(rep->widths[w])]) best_btb = wtb[width_log2(rep->widths[w])];
@@ -212,13 +239,11 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->batch_tb = best_btb;
   memcpy(db->width_tb, wtb, sizeof wtb);
   db->batch_widths = mask;
+  db->batch16_widths = s16;
   // cost table for the server's batch-size choice
   for (double &v : db->width_ms) v = 0;
   db->width_ms[0] = best1;
-  for (int w = 0; w < nw; ++w) {
-    const int k = width_log2(rep->widths[w]);
-    db->width_ms[k] = ((mask >> k) & 1) ? rep->ms_banded[w][wtb[k] - 9] : tbl[w];
-  }
+  for (int w = 0; w < nw; ++w) db->width_ms[width_log2(rep->widths[w])] = wms[w];
   rc = build_banded(db);  // also bumps the generation: no stale captured launch
   if (rc) return restore(rc);
   return LM_OK;

@@ -591,6 +591,56 @@ def test_batch_banded_copy_matches_oracle(scored_case, tb, nimg):
         np.testing.assert_array_equal(got[i].nspots, ns[rows.astype(np.int64)])
 
 
+@pytest.mark.parametrize("tb,nimg,low", [(12, 16, False), (13, 8, False), (11, 32, False),
+                                         (10, 64, False), (9, 64, False), (13, 5, False),
+                                         (12, 9, False), (12, 16, True)])
+def test_batch_sum16_copy_matches_oracle(scored_case, tb, nimg, low):
+    """Batches read from the batch copy with 16-bit sums (two images per LDS
+    word, twice the rows) and an exact recount of the candidates: every image
+    bit-exact against the oracle, with per-image thresholds, imScale != 1,
+    blank images, a MinIntensity <= 0 call (read from the table instead) and,
+    with low thresholds, more candidates than the list holds (rerun over the
+    table)."""
+    p, hk, O, grains, img, table, db = scored_case
+    rng = np.random.default_rng(300 + nimg + tb)
+    base, _ = api.quantize_u8(img)
+    imgs, scales, msp, mint = [], [], [], []
+    for i in range(nimg):
+        if i % 5 == 4:
+            u8, s = api.quantize_u8(img * 0.37 + 0.02 * (img > 0))
+        elif i % 5 == 2:
+            u8, s = np.zeros_like(base), 1.0
+        else:
+            u8, s = (base if i == 0 else base[rng.permutation(base.size)]), 1.0
+        imgs.append(u8)
+        scales.append(s)
+        msp.append(1 if low else int(rng.integers(3, 7)))
+        mint.append(1.0 if low else float(rng.uniform(300, 900)) * s)
+    k = (nimg - 1).bit_length() if nimg > 4 else 2
+    _say(f"batch sum16 tb={tb} nimg={nimg} low={low}")
+    db.set_batch_sum16(1 << k)
+    db.set_batch_width_tile_bits(1 << k, tb)
+    db.set_batch_widths(1 << k)
+    try:
+        got = db.score_batch_each(imgs, scales, msp, mint)
+        mint0 = list(mint)
+        mint0[1] = -1.0  # one image with MinIntensity <= 0: the table walk
+        got0 = db.score_batch_each(imgs, scales, msp, mint0)
+    finally:
+        db.set_batch_sum16(0)
+        db.set_batch_width_tile_bits(1 << k, 0)
+        db.set_batch_widths(0x7F)
+    for gg, mi in ((got, mint), (got0, mint0)):
+        for i in range(nimg):
+            matched, ns, tot, nres = ob.compare(table, 64, imgs[i], p.NrPxX, scales[i], msp[i],
+                                                mi[i])
+            rows, mA = ob.collect(matched)
+            assert gg[i].nr_results == nres, (i, msp[i], mi[i])
+            np.testing.assert_array_equal(gg[i].ids, rows)
+            np.testing.assert_array_equal(gg[i].score, mA)
+            np.testing.assert_array_equal(gg[i].nspots, ns[rows.astype(np.int64)])
+
+
 def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
     """The autotuner times every layout, keeps the fastest and leaves the DB
     scoring bit-exactly (single image and batches) with what it chose."""
@@ -620,9 +670,10 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
             wtb = rep["batch_tile_bits_per_width"][B]
             assert bool(use) == bool(wtb)
             if use:  # the width's own copy, the fastest of its copies, beats the table
-                assert row[f"banded{wtb}"] < row["table"]
-                assert row[f"banded{wtb}"] == min(v for k, v in row.items()
-                                                  if k.startswith("banded"))
+                s16 = (rep["sum16_mask"] >> (B.bit_length() - 1)) & 1
+                key = f"banded{wtb}" + ("_sum16" if s16 else "")
+                assert row[key] < row["table"]
+                assert row[key] == min(v for k, v in row.items() if k.startswith("banded"))
         got = db.score_batch(samples * 8, [1.0] * 16, 5, 500.0)
         for i in (0, 1):
             matched, ns, tot, nres = ob.compare(table, 64, samples[i], p.NrPxX, 1.0, 5, 500.0)
@@ -632,6 +683,7 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
                 np.testing.assert_array_equal(g.ids, rows)
                 np.testing.assert_array_equal(g.score, mA)
     finally:
+        db.set_batch_sum16(0)
         db.set_batch_tile_bits(0)
         db.set_batch_widths(0x7F)
         db.set_banded_tile_bits(14)
