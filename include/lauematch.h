@@ -199,6 +199,11 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * the (row, image) pairs whose sum can pass are recounted exactly from the
  * table.  Calls where some MinIntensity <= 0 read the table instead. */
 #define LM_OPT_BATCH_SUM16 10
+/* LM_OPT_BATCH_SPLIT: bit k set (k = 3..6) = a batch of width 2^k (more than
+ * 2^(k-1) images) is scored as two calls of at most 2^(k-1) images, when the
+ * narrower width's layout serves two calls faster than one (the autotuner
+ * sets it from its timings). */
+#define LM_OPT_BATCH_SPLIT 11
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
@@ -230,6 +235,7 @@ typedef struct lm_tune_report {
   double ms_banded16[7][5];   /* per width: batch copy 2^9..2^13 read with 16-bit
                                  sums (0 = no fit)                               */
   uint32_t sum16_mask;        /* chosen LM_OPT_BATCH_SUM16                        */
+  uint32_t split_mask;        /* chosen LM_OPT_BATCH_SPLIT                        */
 } lm_tune_report;
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out);

@@ -82,7 +82,8 @@ class LmTuneReport(ctypes.Structure):
                 ("batch_tile_bits", ctypes.c_int32), ("batch_widths_mask", ctypes.c_uint32),
                 ("grouped_mask", ctypes.c_uint32), ("nibble_mask", ctypes.c_uint32),
                 ("batch_tile_bits_w", ctypes.c_int32 * 7),
-                ("ms_banded16", (ctypes.c_double * 5) * 7), ("sum16_mask", ctypes.c_uint32)]
+                ("ms_banded16", (ctypes.c_double * 5) * 7), ("sum16_mask", ctypes.c_uint32),
+                ("split_mask", ctypes.c_uint32)]
 
 
 _lib = None

@@ -198,6 +198,11 @@ class OrientationDB:
         before set_batch_width_tile_bits."""
         check(lib().lm_db_set_option(self._h, 10, int(mask)))
 
+    def set_batch_split(self, mask: int) -> None:
+        """Bit k set (k = 3..6): batches of width 2**k are scored as two calls
+        of at most 2**(k-1) images."""
+        check(lib().lm_db_set_option(self._h, 11, int(mask)))
+
     def get_option(self, option: int) -> int:
         v = ctypes.c_int64()
         check(lib().lm_db_get_option(self._h, int(option), ctypes.byref(v)))
@@ -247,6 +252,7 @@ class OrientationDB:
             "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
             "batch_widths_mask": r.batch_widths_mask, "grouped_mask": r.grouped_mask,
             "nibble_mask": r.nibble_mask, "sum16_mask": r.sum16_mask,
+            "split_mask": r.split_mask,
             "batch_tile_bits_per_width": {B: r.batch_tile_bits_w[i] for i, B in enumerate(widths)},
         }
 

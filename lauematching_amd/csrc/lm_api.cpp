@@ -580,6 +580,15 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     ++db->generation;
     return LM_OK;
   }
+  if (option == LM_OPT_BATCH_SPLIT) {
+    if (value < 0 || (value & ~(int64_t)0x78)) {
+      set_error("LM_OPT_BATCH_SPLIT is a mask of bits 3..6 (batches of 8..64 images)");
+      return LM_ERR_INVALID;
+    }
+    db->split_widths = (uint32_t)value;
+    ++db->generation;
+    return LM_OK;
+  }
   if (option == LM_OPT_BATCH_NIBBLE) {
     if (value < 0 || value > 0x180 || (value & 0x7f)) {
       set_error("LM_OPT_BATCH_NIBBLE is a mask of bits 7 and 8 (128 and 256 images)");
@@ -609,6 +618,7 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
     case LM_OPT_BATCH_GROUPED: *value = db->group_widths; return LM_OK;
     case LM_OPT_BATCH_NIBBLE: *value = db->nib_widths; return LM_OK;
     case LM_OPT_BATCH_SUM16: *value = db->batch16_widths; return LM_OK;
+    case LM_OPT_BATCH_SPLIT: *value = db->split_widths; return LM_OK;
     default: set_error("unknown option"); return LM_ERR_INVALID;
   }
 }

@@ -1274,17 +1274,27 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   for (int b = 0; b < nimg && nib; ++b)
     if (!(min_int[b] > 0)) nib = false;
   // more images than the uint8 walks take, without the 4-bit walk: two calls
-  auto halves = [&]() -> int {
-    const int h = kMaxByteBatch;
+  // (and widths the autotuner found cheaper as two calls of half the width)
+  auto halves = [&](int h = kMaxByteBatch) -> int {
     int r = score_images(db, sc, h, d_imgs, im_scales, min_spots, min_int, st, outs);
     if (r) return r;
     r = score_images(db, sc, nimg - h, d_imgs + (uint64_t)h * npx, im_scales + h, min_spots + h,
                      min_int + h, st, outs + h);
-    if (r)
+    if (r) {
       for (int b = 0; b < h; ++b) lm_hits_free(&outs[b]);
-    return r;
+      return r;
+    }
+    // every image reports the device time of both calls
+    const double k = outs[0].kernel_ms + outs[h].kernel_ms, w = outs[0].walk_ms + outs[h].walk_ms;
+    for (int b = 0; b < nimg; ++b) {
+      outs[b].kernel_ms = k;
+      outs[b].walk_ms = w;
+    }
+    return LM_OK;
   };
   if (B > kMaxByteBatch && !nib) return halves();
+  if (B >= 8 && B <= 64 && ((db->split_widths >> width_log2(B)) & 1) && nimg > B / 2)
+    return halves(B / 2);
   Pred &pred = *sc.pred_host;
   build_pred(pred, nimg, im_scales, min_spots, min_int);
   hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);

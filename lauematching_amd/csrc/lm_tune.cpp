@@ -63,7 +63,8 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
   const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
   const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths,
-                 saved_nib = db->nib_widths, saved_s16 = db->batch16_widths;
+                 saved_nib = db->nib_widths, saved_s16 = db->batch16_widths,
+                 saved_split = db->split_widths;
   int8_t saved_wtb[9];
   memcpy(saved_wtb, db->width_tb, sizeof saved_wtb);
   // every width that tb's counters hold: 32-bit counters B x 2^tb <= 32768,
@@ -81,6 +82,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     db->group_widths = saved_group;
     db->nib_widths = saved_nib;
     db->batch16_widths = saved_s16;
+    db->split_widths = saved_split;
     const int r = build_banded(db);
     return code ? code : r;
   };
@@ -124,6 +126,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->banded_tb = 0;  // room for the batch copies
   db->batch_tb = 0;
   db->batch16_widths = 0;
+  db->split_widths = 0;
   set_all_widths(0);
   rc = build_banded(db);
   if (rc) return restore(rc);
@@ -225,6 +228,18 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     if (wtb[k]) mask |= 1u << k;
   }
   rep->sum16_mask = s16;
+  // a width served faster by two calls of half the images (the narrower
+  // width's layout staying in L2 where the wider one's does not)
+  uint32_t split = 0;
+  for (int w = 1; w < nw; ++w) {
+    const int B = rep->widths[w];
+    if (B < 8 || B > 64 || rep->widths[w - 1] * 2 != B) continue;
+    if (2.0 * wms[w - 1] < wms[w]) {
+      wms[w] = 2.0 * wms[w - 1];
+      split |= 1u << width_log2(B);
+    }
+  }
+  rep->split_mask = split;
   int best_btb = 0;  // reported: the copy of the narrowest width that keeps one
   for (int w = nw - 1; w >= 0; --w)
     if (wtb[width_log2(rep->widths[w])]) best_btb = wtb[width_log2(rep->widths[w])];
@@ -240,6 +255,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   memcpy(db->width_tb, wtb, sizeof wtb);
   db->batch_widths = mask;
   db->batch16_widths = s16;
+  db->split_widths = split;
   // cost table for the server's batch-size choice
   for (double &v : db->width_ms) v = 0;
   db->width_ms[0] = best1;

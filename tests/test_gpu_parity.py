@@ -641,6 +641,30 @@ def test_batch_sum16_copy_matches_oracle(scored_case, tb, nimg, low):
             np.testing.assert_array_equal(gg[i].nspots, ns[rows.astype(np.int64)])
 
 
+@pytest.mark.parametrize("nimg", [13, 40])
+def test_batch_split_matches_oracle(scored_case, nimg):
+    """A width scored as two calls of half the width (LM_OPT_BATCH_SPLIT):
+    every image bit-exact against the oracle, per-image thresholds kept."""
+    p, hk, O, grains, img, table, db = scored_case
+    rng = np.random.default_rng(500 + nimg)
+    base, _ = api.quantize_u8(img)
+    imgs = [base] + [base[rng.permutation(base.size)] for _ in range(nimg - 1)]
+    msp = [int(v) for v in rng.integers(3, 7, nimg)]
+    mint = [float(v) for v in rng.uniform(300, 900, nimg)]
+    _say(f"batch split nimg={nimg}")
+    db.set_batch_split(0x78)
+    try:
+        got = db.score_batch_each(imgs, [1.0] * nimg, msp, mint)
+    finally:
+        db.set_batch_split(0)
+    for i in range(nimg):
+        matched, ns, tot, nres = ob.compare(table, 64, imgs[i], p.NrPxX, 1.0, msp[i], mint[i])
+        rows, mA = ob.collect(matched)
+        assert got[i].nr_results == nres, i
+        np.testing.assert_array_equal(got[i].ids, rows)
+        np.testing.assert_array_equal(got[i].score, mA)
+
+
 def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
     """The autotuner times every layout, keeps the fastest and leaves the DB
     scoring bit-exactly (single image and batches) with what it chose."""
@@ -684,6 +708,7 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
                 np.testing.assert_array_equal(g.score, mA)
     finally:
         db.set_batch_sum16(0)
+        db.set_batch_split(0)
         db.set_batch_tile_bits(0)
         db.set_batch_widths(0x7F)
         db.set_banded_tile_bits(14)
