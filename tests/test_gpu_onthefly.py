@@ -168,3 +168,46 @@ def test_score_fwd_rejects_more_than_64_spots(ctx, fly):
     torch.cuda.synchronize()
     with pytest.raises(_lib.LaueMatchError):
         db.score_fwd_batch_dev(dev.data_ptr(), 1, [1.0], [5], [500.0])
+
+
+@pytest.mark.parametrize("M", [64, 40])
+@pytest.mark.parametrize("mode", [api.LM_FWD_FP64, api.LM_FWD_FP32_GUARD])
+def test_dense_spot_lists_table_and_on_the_fly(ctx, M, mode):
+    """|hkl| <= 10 box and 2-100 keV: 31 spots per orientation on average,
+    44% of the rows above 32 (list slots past 32 live in global memory in
+    the count pass and in on-the-fly scoring) and, with M = 40, rows capped
+    at M.  The table equals the oracle's; on-the-fly hits equal the
+    oracle's on the same table."""
+    import torch
+
+    p = synth.baseline_params(MaxNrLaueSpots=M, Elo=2.0, Ehi=100.0)
+    hk = synth.fcc_hkls(10, "box")
+    O = synth.random_orientations(6000, seed=31)
+    want, _, _ = ob.forward(O, hk, ob.geom(p), M)
+    counts = want[:, 0].astype(np.int64)
+    assert (counts > 32).mean() > 0.2 and (M == 64 or (counts == M).any())
+    _say(f"dense spot lists M={M} mode={mode}")
+    db = _db(ctx, p, O, hk)
+    rng = np.random.default_rng(M)
+    imgs, scales = [], []
+    for i in range(5):
+        img = synth.synth_image(p, hk, O[rng.choice(len(O), 2, replace=False)], seed=70 + i)
+        if i == 3:
+            img = img * 0.58 + (img > 0) * 0.03
+        u8, s = api.quantize_u8(img)
+        imgs.append(u8.ravel())
+        scales.append(s)
+    dev = torch.from_numpy(np.stack(imgs)).to("cuda:0")
+    torch.cuda.synchronize()
+    got, st = db.score_fwd_batch_dev(dev.data_ptr(), 5, scales, 5, 800.0, mode)
+    assert st.n_spots == int(counts.sum())
+    db.forward(mode)
+    np.testing.assert_array_equal(db.export_table(), want)
+    for b in range(5):
+        matched, ns, tot, nres = ob.compare(want, M, imgs[b], p.NrPxX, scales[b], 5, 800.0)
+        rows, mA = ob.collect(matched)
+        assert got[b].nr_results == nres
+        np.testing.assert_array_equal(got[b].ids, rows)
+        np.testing.assert_array_equal(got[b].nspots, ns[rows.astype(np.int64)])
+        np.testing.assert_array_equal(got[b].score.view(np.uint64), mA.view(np.uint64))
+    assert sum(len(h.ids) for h in got) > 0
