@@ -57,6 +57,23 @@ def cpu_baseline(db, u8, im_scale, min_sp, min_int, sample, budget_s=12.0):
                       f"{t_cpu:.1f} s"}
 
 
+def cpu_forward_baseline(O, hk, p, rows):
+    """The CPU oracle's forward simulation (C + OpenMP restatement of the
+    reference DoFwd=1 loop, every pair projected) on all host cores over the
+    first `rows` orientations of the same DB."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    ob.forward(O[:rows], hk, ob.geom(p), p.MaxNrLaueSpots, threads=cores)
+    dt = time.perf_counter() - t0
+    out = {"pairs_per_s": rows * len(hk) / dt, "cores": cores, "kind": "port",
+           "sample": f"first {rows} orientations x {len(hk)} hkls, {dt:.1f} s"}
+    log(f"cpu forward simulation {out['pairs_per_s']:.3e} pairs/s ({cores} cores, {rows} rows)")
+    return out
+
+
 def oracle_parity(db, cases, chunk=10_000_000):
     """Every GPU result of the run against the CPU oracle over the WHOLE
     table: the table is exported in chunks of `chunk` rows (ForwardFile
@@ -488,6 +505,14 @@ def config4(ctx, O, n, steps, batch_imgs):
             db.score_fwd_batch_dev(dev.data_ptr(), B, scales, 5, 1000.0, api.LM_FWD_FP32_GUARD)
         torch.cuda.synchronize()
         ms_regen = (time.perf_counter() - t0) / max(1, steps // 3) * 1e3
+        # the same on the fly in float64 (no guard band needed)
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps // 3)):
+            g64, _ = db.score_fwd_batch_dev(dev.data_ptr(), B, scales, 5, 1000.0, api.LM_FWD_FP64)
+        torch.cuda.synchronize()
+        ms_regen64 = (time.perf_counter() - t0) / max(1, steps // 3) * 1e3
+        same &= all(g.nr_results == w.nr_results and np.array_equal(g.ids, w.ids) and
+                    np.array_equal(g.score, w.score) for g, w in zip(g64, want))
         res.update({"batch": B, "resident_ms_per_batch": ms_res,
                     "resident_pairs_per_s": n * B / (ms_res / 1e3),
                     "regenerated_ms_per_batch": ms_regen,
@@ -495,6 +520,7 @@ def config4(ctx, O, n, steps, batch_imgs):
                     "regenerated_kernel_ms": fst.ms,
                     "regenerated_recompute_fraction": fst.n_recomputed / fst.n_pairs,
                     "regenerated_equals_resident": bool(same),
+                    "regenerated_fp64_ms_per_batch": ms_regen64,
                     "rebuild_table_then_score_ms": ms_rebuild})
     log(f"config4: {res}")
     db.close()
@@ -1072,6 +1098,28 @@ def main():
     copy_gbps = 2.0 * (1 << 30) / (copy_ms / 1e3) / 1e9
     del csrc, cdst
     log(f"plain 1 GiB device copy: {copy_ms:.3f} ms, {copy_gbps:.0f} GB/s (read + write)")
+    # host-to-device rate from pinned memory (the serving closed loop uploads
+    # every image), and the headline step with the image handed over in host
+    # memory (upload included; reported beside `value`, never as it)
+    hsrc = torch.empty(1 << 28, dtype=torch.uint8, pin_memory=True)
+    hdst = torch.empty(1 << 28, dtype=torch.uint8, device="cuda:0")
+    hdst.copy_(hsrc, non_blocking=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        hdst.copy_(hsrc, non_blocking=True)
+    e1.record()
+    torch.cuda.synchronize()
+    h2d_gbps = (1 << 28) / (e0.elapsed_time(e1) / 5 / 1e3) / 1e9
+    del hsrc, hdst
+    for _ in range(args.warmup):
+        db.score(u8, im_scale, p.MinNrSpots, p.MinIntensity)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        db.score(u8, im_scale, p.MinNrSpots, p.MinIntensity)
+    host_step_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    log(f"pinned host-to-device copy {h2d_gbps:.1f} GB/s; step with the image in host memory "
+        f"{host_step_ms:.3f} ms (upload included)")
 
     # algorithmic bytes of one compare launch (per orientation 2 + 4 n_o bytes,
     # the image once, 16 bytes per hit)
@@ -1160,6 +1208,8 @@ def main():
         sample = min(args.cpu_sample, n)
         cpu = cpu_baseline(db, u8, im_scale, p.MinNrSpots, p.MinIntensity, sample)
         log(f"cpu baseline {cpu['value']:.3e} pairs/s ({cpu['cores']} cores)")
+        cpu["forward"] = cpu_forward_baseline(O, hk, p, 1_000_000)
+        forward["cpu_port_pairs_per_s"] = cpu["forward"]["pairs_per_s"]
         parity = oracle_parity(db, parity_cases)
 
     daemon = None
@@ -1196,7 +1246,9 @@ def main():
                    "oracle_parity_rows": parity["rows"] if parity else 0,
                    "oracle_parity_all_equal": parity["all_equal"] if parity else None,
                    "forward_ms": st.ms, "kernel_ms": kernel_ms, "walk_ms": walk_ms,
-                   "hbm_copy_GBps": copy_gbps,
+                   "hbm_copy_GBps": copy_gbps, "h2d_pinned_GBps": h2d_gbps,
+                   "image_resident_in_hbm": True,
+                   "host_image_ms_per_step": host_step_ms,
                    "single_kernel": single_kernel, "graph_replay": bool(args.graphs)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
