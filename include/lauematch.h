@@ -429,10 +429,11 @@ double lm_misorientation_deg(const double q1[4], const double q2[4], int32_t spa
  * removed; the Gaussian sigma max(0.25 * min(nearest centre-of-mass
  * distance, distance * tan(orientation_spacing) / px), 1) (3 with fewer than
  * two components); a float64 Gaussian blur (radius int(4 sigma + 0.5),
- * reflecting edges) and the uint8 quantization of lm_quantize_u8.  The
- * background median filter of the reference is not computed here: pass the
- * background (or NULL for none).  No counterpart runs on the GPU in the
- * reference. */
+ * reflecting edges) and the uint8 quantization of lm_quantize_u8.  With a
+ * NULL background the background is lm_median_background of the raw image
+ * with cfg's filter_radius and median_passes (preprocess.py:255-266: no
+ * background file, so it is computed; zero when either is <= 0).  No
+ * counterpart runs on the GPU in the reference. */
 #define LM_THRESH_ADAPTIVE 0
 #define LM_THRESH_PERCENTILE 1
 #define LM_THRESH_FIXED 2
@@ -445,6 +446,8 @@ typedef struct lm_prep_config {
   double px_x;                 /* pixel size (mm), detector distance (mm) and the   */
   double distance;             /* DB's orientation spacing (degrees) bound sigma    */
   double orientation_spacing;
+  int32_t filter_radius;       /* background median filter size (0: none)          */
+  int32_t median_passes;       /* and passes, used when background is NULL          */
 } lm_prep_config;
 typedef struct lm_prep_result {
   double threshold;      /* threshold applied                                  */
@@ -461,6 +464,22 @@ typedef struct lm_prep_result {
 int lm_preprocess(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *raw,
                   const double *background, const lm_prep_config *cfg, uint8_t *u8_host,
                   void *d_u8, double *blurred, lm_prep_result *res);
+
+/* The median-filter background of a raw frame (compute_background,
+ * laue_index/preprocess.py:43-64) on the device, as the reference computes
+ * it without diplib: scipy.ndimage.median_filter(bg, size=filter_radius)
+ * applied median_passes times (:61-65), i.e. per pixel the element of rank
+ * filter_radius^2 / 2 (0-based, rounded down) of the filter_radius x
+ * filter_radius window whose first row and column lie filter_radius / 2
+ * before the pixel, edges extended by half-sample reflection.  Zeros when
+ * filter_radius <= 0 or median_passes <= 0 (:53-54).  image and background
+ * are nr_px_y x nr_px_x float64 host images; ms (optional) receives the
+ * device time.  Limits: filter_radius <= 255, at most 2^24 pixels.  diplib's
+ * MedianFilter (:56-59), the reference's first choice when it is installed,
+ * uses a circular window instead and is not provided. */
+int lm_median_background(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *image,
+                         int32_t filter_radius, int32_t median_passes, double *background,
+                         double *ms);
 
 /* ---- diagnostics ------------------------------------------------------------------ */
 

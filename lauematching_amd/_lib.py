@@ -50,7 +50,8 @@ class LmPrepConfig(ctypes.Structure):
     _fields_ = [("threshold_method", ctypes.c_int32), ("threshold_value", ctypes.c_double),
                 ("threshold_percentile", ctypes.c_double), ("min_area", ctypes.c_int32),
                 ("px_x", ctypes.c_double), ("distance", ctypes.c_double),
-                ("orientation_spacing", ctypes.c_double)]
+                ("orientation_spacing", ctypes.c_double), ("filter_radius", ctypes.c_int32),
+                ("median_passes", ctypes.c_int32)]
 
 
 class LmPrepResult(ctypes.Structure):
@@ -154,6 +155,9 @@ SIGNATURES = {
                                      ctypes.POINTER(LmPrepConfig), ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(LmPrepResult)]),
+    "lm_median_background": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "lm_merge_candidates_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,

@@ -399,11 +399,14 @@ PREP_METHODS = {"adaptive": 0, "percentile": 1, "fixed": 2, "otsu": 3}
 def preprocess(ctx: "Context", raw: np.ndarray, background: np.ndarray | None = None,
                method: str = "adaptive", fixed_value: float = 0.0, percentile: float = 90.0,
                min_area: int = 10, pixel_size: float = 0.2, distance: float = 0.513,
-               orient_spacing: float = 0.4, d_u8: int = 0, want_blurred: bool = False):
+               orient_spacing: float = 0.4, d_u8: int = 0, want_blurred: bool = False,
+               filter_radius: int = 0, median_passes: int = 1):
     """Image preprocessing on the device (preprocess_image,
     laue_index/preprocess.py:221-337, enhancement off): background
     subtraction, threshold, uint16 cast, 8-connected components with the
-    small ones removed, Gaussian blur, uint8 quantization.  Returns (u8 image,
+    small ones removed, Gaussian blur, uint8 quantization.  Without a
+    background and with filter_radius > 0 the background is the median
+    filter of median_background (preprocess.py:255-266).  Returns (u8 image,
     imScale, info dict[, blurred float64 image]); with d_u8 (a device pointer
     of npx bytes) the uint8 image is also left there for scoring."""
     img = np.ascontiguousarray(raw, dtype=np.float64)
@@ -417,7 +420,7 @@ def preprocess(ctx: "Context", raw: np.ndarray, background: np.ndarray | None = 
             raise ValueError("background must have the image's shape")
     cfg = _lib.LmPrepConfig(PREP_METHODS[method], float(fixed_value), float(percentile),
                             int(min_area), float(pixel_size), float(distance),
-                            float(orient_spacing))
+                            float(orient_spacing), int(filter_radius), int(median_passes))
     u8 = np.zeros((H, W), np.uint8)
     blurred = np.zeros((H, W)) if want_blurred else None
     res = _lib.LmPrepResult()
@@ -429,6 +432,23 @@ def preprocess(ctx: "Context", raw: np.ndarray, background: np.ndarray | None = 
     if want_blurred:
         return u8, res.im_scale, info, blurred
     return u8, res.im_scale, info
+
+
+def median_background(ctx: "Context", image: np.ndarray, filter_radius: int = 101,
+                      median_passes: int = 1):
+    """compute_background (laue_index/preprocess.py:43-64) on the device:
+    scipy.ndimage.median_filter(size=filter_radius) applied median_passes
+    times, zeros when either is <= 0.  Returns (background float64 image,
+    device ms)."""
+    img = np.ascontiguousarray(image, dtype=np.float64)
+    if img.ndim != 2:
+        raise ValueError("image must be a 2-D (NrPxY, NrPxX) array")
+    H, W = img.shape
+    out = np.empty((H, W), np.float64)
+    ms = ctypes.c_double()
+    check(lib().lm_median_background(ctx.handle, W, H, _ptr(img), int(filter_radius),
+                                     int(median_passes), _ptr(out), ctypes.byref(ms)))
+    return out, ms.value
 
 
 def quantize_u8(img: np.ndarray):

@@ -30,7 +30,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-I", INCLUDE, "-I", CSRC]
 
 LIB_SOURCES = ["lm_api.cpp", "lm_fwd.hip", "lm_score.hip", "lm_band.hip", "lm_merge.cpp",
-               "lm_merge_dev.hip", "lm_prep.hip",
+               "lm_merge_dev.hip", "lm_prep.hip", "lm_median.hip",
                "lm_serve.cpp", "lm_tune.cpp", "lm_hosttable.cpp"]
 HOST_COMMON = ["host/lm_params.cpp", "host/lm_hostio.cpp", "host/lm_postproc.cpp"]
 HOST_PROGRAMS = {"LaueMatchingGPU": ["host/lm_host_gpu.cpp", *HOST_COMMON],

@@ -886,11 +886,59 @@ int lm_preprocess(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *r
     set_error("lm_preprocess: threshold method or percentile out of range");
     return LM_ERR_INVALID;
   }
+  if (!background && cfg->filter_radius > 0 && cfg->median_passes > 0 &&
+      (cfg->filter_radius > kMaxMedianSize ||
+       (uint64_t)nr_px_x * (uint64_t)nr_px_y > (1ull << 24))) {
+    set_error("lm_preprocess: median background needs filter_radius <= 255 and at most 2^24 "
+              "pixels");
+    return LM_ERR_INVALID;
+  }
   int rc = check_ctx(ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(ctx->mu);
   return preprocess(ctx, nr_px_x, nr_px_y, raw, background, *cfg, u8_host, (uint8_t *)d_u8,
                     blurred, *res);
+}
+
+int lm_median_background(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *image,
+                         int32_t filter_radius, int32_t median_passes, double *background,
+                         double *ms) {
+  if (!ctx || !image || !background || nr_px_x < 1 || nr_px_y < 1 ||
+      (uint64_t)nr_px_x * (uint64_t)nr_px_y > (1ull << 24) || filter_radius > kMaxMedianSize) {
+    set_error("lm_median_background: bad arguments (at most 2^24 pixels, filter_radius <= 255)");
+    return LM_ERR_INVALID;
+  }
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const uint64_t npx = (uint64_t)nr_px_x * (uint64_t)nr_px_y;
+  hipStream_t st = ctx->stream;
+  double *d_img = nullptr, *d_bg = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t e = hipMalloc(&d_img, npx * 8);
+  if (e == hipSuccess) e = hipMalloc(&d_bg, npx * 8);
+  if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_img, image, npx * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+  rc = e == hipSuccess ? LM_OK : hip_fail(e, "lm_median_background");
+  if (rc == LM_OK)
+    rc = median_background_dev(st, nr_px_x, nr_px_y, d_img, filter_radius, median_passes, d_bg);
+  if (rc == LM_OK) {
+    e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(background, d_bg, npx * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float t = 0;
+    if (e == hipSuccess && ms) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+    if (ms) *ms = t;
+    if (e != hipSuccess) rc = hip_fail(e, "lm_median_background");
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  (void)hipFree(d_img);
+  (void)hipFree(d_bg);
+  return rc;
 }
 
 void lm_quat_from_matrix(const double m[9], double q[4]) { quat_from_matrix(m, q); }

@@ -339,6 +339,13 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
                const lm_prep_config &cfg, uint8_t *u8_host, uint8_t *d_u8, double *blurred,
                lm_prep_result &res);
 
+// lm_median.hip: scipy.ndimage.median_filter(size=size), `passes` times, of
+// the W x H float64 device image d_img into d_bg (zeros when size <= 0 or
+// passes <= 0); synchronizes st before returning.
+constexpr int kMaxMedianSize = 255;
+int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int size,
+                          int passes, double *d_bg);
+
 // lm_merge.cpp
 void quat_from_matrix(const double m[9], double q[4]);
 double misorientation_deg(const double a[4], const double b[4], int space_group);

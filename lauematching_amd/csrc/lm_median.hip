@@ -1,0 +1,425 @@
+// lm_median.hip - the median-filter background of the preprocessing
+// (compute_background, laue_index/preprocess.py:43-64), on the device.
+//
+// Without diplib the reference filters with
+// scipy.ndimage.median_filter(bg, size=filter_radius), median_passes times
+// (preprocess.py:61-65): a square window of filter_radius x filter_radius
+// pixels whose first row and column lie filter_radius // 2 before the output
+// pixel, the image extended by half-sample reflection (scipy's 'reflect',
+// ... c b a | a b c ... | c b a ..., periodic with period 2n when the window
+// is wider than the image), and as output the element of 0-based rank
+// (filter_radius^2) // 2 of the window's values (no averaging for even
+// sizes).  This file computes exactly that.
+//
+// 1. Dense ranks.  The float64 values are sorted (radix sort of an
+//    order-preserving 64-bit key, with the pixel index as payload); equal
+//    values share one rank, so the image becomes a uint32 rank image over
+//    D distinct values plus a table of those values.  The median of the
+//    ranks is the rank of the median, and a pass's output holds ranks of
+//    the same table, so every pass works on ranks and the table maps the
+//    last one back to float64.
+// 2. The running median.  One workgroup per output column (or column
+//    segment) slides the window down: per row it removes the leaving row's
+//    filter_radius ranks from a histogram in LDS, adds the entering row's,
+//    and moves the median bin up or down from where it was (Huang's running
+//    median), one wavefront scanning 64 bins per step with prefix sums.
+//    The histogram holds 16-bit counts, two per LDS word, over the rank's
+//    top bits: all of it when D <= 65536 (the integer-count frames of an
+//    X-ray detector), else its top 16 bits, and then the rank's low bits
+//    come from a second histogram of the window's elements in the current
+//    coarse bin, updated with the window and rebuilt from it when the
+//    median moves to another coarse bin.
+//
+// Limits: filter_radius <= 255 (a window of at most 65025 elements keeps
+// every count in 16 bits) and at most 2^24 pixels (so the low bits are at
+// most 8).  NaN values are not ordered (the reference's filters leave
+// them undefined as well).
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "lm_internal.h"
+
+namespace lm {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMedThreads = 256;
+
+unsigned grid_for(uint64_t n) {
+  return (unsigned)std::min<uint64_t>((n + kThreads - 1) / kThreads, 1u << 20);
+}
+
+// float64 -> 64-bit key with the values' order (+0 for both zeros)
+__device__ __forceinline__ uint64_t order_key(double v) {
+  uint64_t b = v == 0.0 ? 0ull : (uint64_t)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
+__device__ __forceinline__ double key_value(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+__global__ void key_kernel(const double *__restrict__ v, uint64_t npx, uint64_t *__restrict__ keys,
+                           uint32_t *__restrict__ idx) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    keys[i] = order_key(v[i]);
+    idx[i] = (uint32_t)i;
+  }
+}
+
+// 1 where a sorted key starts a run of equal keys
+__global__ void head_kernel(const uint64_t *__restrict__ sk, uint64_t n, uint32_t *__restrict__ f) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (uint64_t)gridDim.x * blockDim.x)
+    f[j] = (j == 0 || sk[j] != sk[j - 1]) ? 1u : 0u;
+}
+
+// rank image and the table of distinct values (scan = inclusive sum of heads)
+__global__ void rank_kernel(const uint64_t *__restrict__ sk, const uint32_t *__restrict__ sidx,
+                            const uint32_t *__restrict__ scan, uint64_t n,
+                            uint32_t *__restrict__ rank, double *__restrict__ table) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = scan[j] - 1;
+    rank[sidx[j]] = r;
+    if (j == 0 || sk[j] != sk[j - 1]) table[r] = key_value(sk[j]);
+  }
+}
+
+__global__ void unrank_kernel(const uint32_t *__restrict__ rank, const double *__restrict__ table,
+                              uint64_t npx, double *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = table[rank[i]];
+}
+
+// scipy's 'reflect' extension: half-sample symmetric, period 2n
+__device__ __forceinline__ int reflect(int i, int n) {
+  const int p = 2 * n;
+  i %= p;
+  if (i < 0) i += p;
+  return i < n ? i : p - 1 - i;
+}
+
+// Wave-uniform searches over counts h(0..n-1) for the bin m holding the
+// element of rank k: below = sum of h(j < m) on entry and exit, and on exit
+// below <= k < below + h(m).  Every loop ends: move_up stops at the last
+// bin, move_down at bin 0.
+template <class F>
+__device__ void move_up(const F &h, uint32_t n, uint32_t k, uint32_t &m, uint32_t &below) {
+  const int lane = __lane_id();
+  while (m < n) {
+    const uint32_t bin = m + lane;
+    const uint32_t v = bin < n ? h(bin) : 0u;
+    uint32_t p = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(p, d);
+      if (lane >= d) p += t;
+    }
+    const uint64_t hit = __ballot(below + p > k);
+    if (hit) {
+      const int l = __builtin_ctzll(hit);
+      below += __shfl(p, l) - __shfl(v, l);
+      m += l;
+      return;
+    }
+    below += __shfl(p, 63);
+    m += 64;
+  }
+  m = n - 1;  // not reached while the counts sum to more than k
+}
+
+template <class F>
+__device__ void move_down(const F &h, uint32_t k, uint32_t &m, uint32_t &below) {
+  const int lane = __lane_id();
+  while (m > 0) {
+    const int bin = (int)m - 64 + lane;
+    const uint32_t v = bin >= 0 ? h((uint32_t)bin) : 0u;
+    uint32_t s = v;  // sum of v over lanes >= this one
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_down(s, d);
+      if (lane + d < 64) s += t;
+    }
+    const uint64_t hit = __ballot(s >= below - k);
+    if (hit) {
+      const int l = 63 - __builtin_clzll(hit);
+      below -= __shfl(s, l);
+      m = (uint32_t)((int)m - 64 + l);
+      return;
+    }
+    below -= __shfl(s, 0);
+    m = m > 64 ? m - 64 : 0;
+  }
+  below = 0;  // not reached while below counts the bins under m
+}
+
+// One workgroup per (column, row segment).  NBITS: histogram bins 2^NBITS
+// over rank >> shift; FINE: resolve the low `shift` bits of the rank.  The
+// rows a segment touches are reflected once into an LDS table; window scans
+// give each thread one column (of 128 or 256) and stride over rows; the
+// update loads run kDepth rows ahead of the histogram updates.
+constexpr int kMaxSegRows = 1024;
+constexpr int kDepth = 4;
+
+template <int NBITS, bool FINE>
+__global__ void __launch_bounds__(kMedThreads)
+median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, int W, int H,
+                     int s, int seg_rows, int shift) {
+  constexpr uint32_t NB = 1u << NBITS;
+  __shared__ uint32_t hist[NB / 2];
+  __shared__ uint32_t fine[FINE ? 256 : 1];
+  __shared__ uint32_t part[kMedThreads];
+  __shared__ int rowtab[kMaxSegRows + 256];
+  __shared__ int delta[kMedThreads / 64];
+  __shared__ uint32_t st_m, st_below, st_fm;
+
+  uint32_t b = blockIdx.x;
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);  // a band per XCD
+  const int x = (int)(b % (uint32_t)W);
+  const int y0 = (int)(b / (uint32_t)W) * seg_rows;
+  if (y0 >= H) return;  // an empty last segment (whole workgroup)
+  const int y1 = min(H, y0 + seg_rows);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lo = s >> 1;
+  const uint32_t k = (uint32_t)(s * s) >> 1;
+  const uint32_t fmask = (1u << shift) - 1u;
+  const auto hcount = [&](uint32_t j) { return (hist[j >> 1] >> ((j & 1u) << 4)) & 0xffffu; };
+  const auto fcount = [&](uint32_t j) { return fine[j]; };
+
+  // rowtab[i] = image row of window row y0 - lo + i
+  for (int i = tid; i < y1 - y0 + s; i += kMedThreads) rowtab[i] = reflect(y0 - lo + i, H);
+  for (uint32_t i = tid; i < NB / 2; i += kMedThreads) hist[i] = 0;
+  if (FINE)
+    for (uint32_t i = tid; i < 256; i += kMedThreads) fine[i] = 0;
+  // window scans: this thread's column sdx and first row sdy
+  const int cw = s <= 128 ? 128 : 256, rw = kMedThreads / cw;
+  const int sdx = tid % cw, sdy = tid / cw;
+  const bool scol = sdx < s;
+  const size_t wcol = scol ? (size_t)reflect(x - lo + sdx, W) : 0;
+  __syncthreads();
+  // the window of row y0
+  if (scol)
+    for (int dy = sdy; dy < s; dy += rw) {
+      const uint32_t v = in[(size_t)rowtab[dy] * W + wcol] >> shift;
+      atomicAdd(&hist[v >> 1], 1u << ((v & 1u) << 4));
+    }
+  __syncthreads();
+  // its median bin: per-thread sums of NB / 256 bins, then bins within one
+  constexpr uint32_t CH = NB >= kMedThreads ? NB / kMedThreads : 1u;
+  {
+    uint32_t sum = 0;
+    if ((uint32_t)tid * CH < NB)
+      for (uint32_t j = 0; j < CH; ++j) sum += hcount(tid * CH + j);
+    part[tid] = sum;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    uint32_t pm = 0, pb = 0;
+    move_up([&](uint32_t j) { return part[j]; }, (uint32_t)kMedThreads, k, pm, pb);
+    uint32_t m = pm * CH, below = pb;
+    move_up(hcount, NB, k, m, below);
+    if (lane == 0) {
+      st_m = m;
+      st_below = below;
+      st_fm = 0xffffffffu;
+    }
+  }
+  __syncthreads();
+
+  // this thread's updates per row: slots j = tid and tid + 256 of the s
+  // removals (j < s) and s additions
+  size_t col[2];
+  bool act[2], add[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int j = tid + u * kMedThreads;
+    act[u] = j < 2 * s;
+    add[u] = j >= s;
+    col[u] = act[u] ? (size_t)reflect(x - lo + (add[u] ? j - s : j), W) : 0;
+  }
+  // entering row y: window row y - 1 - lo leaves, y - 1 - lo + s enters
+  const auto load_step = [&](int y, uint32_t (&v)[2]) {
+    const int rl = rowtab[y - 1 - y0], re = rowtab[y - 1 - y0 + s];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (act[u]) v[u] = in[(size_t)(add[u] ? re : rl) * W + col[u]];
+  };
+  const auto update = [&](const uint32_t (&v)[2]) {
+    const uint32_t m = st_m, fm = st_fm;
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (!act[u]) continue;
+      const uint32_t r = v[u], bin = r >> shift;
+      const uint32_t inc = 1u << ((bin & 1u) << 4);
+      atomicAdd(&hist[bin >> 1], add[u] ? inc : 0u - inc);
+      if (bin < m) c += add[u] ? 1 : -1;
+      if (FINE && bin == fm) atomicAdd(&fine[r & fmask], add[u] ? 1u : 0xffffffffu);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
+    if (lane == 0) delta[wv] = c;
+    __syncthreads();
+  };
+  // the median of row y (its window counted, delta[] holding the change
+  // of the count below the previous median when y > y0)
+  const auto finish = [&](int y) {
+    if (wv == 0) {
+      uint32_t m = st_m;
+      uint32_t below = st_below;
+      if (y > y0) {
+        int dsum = 0;
+#pragma unroll
+        for (int w = 0; w < kMedThreads / 64; ++w) dsum += delta[w];
+        below = (uint32_t)((int)below + dsum);
+      }
+      if (below > k)
+        move_down(hcount, k, m, below);
+      else if (below + hcount(m) <= k)
+        move_up(hcount, NB, k, m, below);
+      if (lane == 0) {
+        st_m = m;
+        st_below = below;
+        if (!FINE) out[(size_t)y * W + x] = m;
+      }
+    }
+    __syncthreads();
+    if (FINE) {
+      const uint32_t m = st_m;
+      if (m != st_fm) {  // rebuild the low-bit counts of coarse bin m
+        for (uint32_t i = tid; i < 256; i += kMedThreads) fine[i] = 0;
+        __syncthreads();
+        if (scol)
+          for (int dy = sdy; dy < s; dy += rw) {
+            const uint32_t r = in[(size_t)rowtab[y - y0 + dy] * W + wcol];
+            if ((r >> shift) == m) atomicAdd(&fine[r & fmask], 1u);
+          }
+        __syncthreads();
+        if (tid == 0) st_fm = m;
+      }
+      if (wv == 0) {
+        uint32_t f = 0, fb = 0;
+        move_up(fcount, fmask + 1u, k - st_below, f, fb);
+        if (lane == 0) out[(size_t)y * W + x] = (m << shift) | f;
+      }
+      __syncthreads();
+    }
+  };
+
+  uint32_t ring[kDepth][2];
+#pragma unroll
+  for (int q = 0; q < kDepth; ++q)
+    if (y0 + 1 + q < y1) load_step(y0 + 1 + q, ring[q]);
+  finish(y0);
+  for (int yb = y0 + 1; yb < y1; yb += kDepth) {
+#pragma unroll
+    for (int q = 0; q < kDepth; ++q) {
+      const int y = yb + q;
+      if (y < y1) {
+        update(ring[q]);
+        if (y + kDepth < y1) load_step(y + kDepth, ring[q]);
+        finish(y);
+      }
+    }
+  }
+}
+
+template <int NBITS, bool FINE>
+void launch_median(hipStream_t st, const uint32_t *in, uint32_t *out, int W, int H, int s,
+                   int seg_rows, int nseg, int shift) {
+  hipLaunchKernelGGL((median_column_kernel<NBITS, FINE>), dim3((unsigned)(W * nseg)),
+                     dim3(kMedThreads), 0, st, in, out, W, H, s, seg_rows, shift);
+}
+
+}  // namespace
+
+int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int size,
+                          int passes, double *d_bg) {
+  const uint64_t npx = (uint64_t)W * H;
+  if (W < 1 || H < 1 || npx > (1ull << 24) || size > kMaxMedianSize) {
+    set_error("median background: image over 2^24 pixels or filter size over 255");
+    return LM_ERR_INVALID;
+  }
+  if (size <= 0 || passes <= 0) {  // compute_background returns zeros
+    const hipError_t e = hipMemsetAsync(d_bg, 0, npx * 8, st);
+    return e == hipSuccess ? LM_OK : hip_fail(e, "median background");
+  }
+  // one allocation, carved: keys, sorted keys, table (8 B per pixel), index,
+  // sorted index, scan, two rank images (4 B per pixel), sort/scan storage
+  size_t sort_bytes = 0, scan_bytes = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+      nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr,
+      (uint32_t *)nullptr, (int)npx, 0, 64, st);
+  if (e == hipSuccess)
+    e = hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, (uint32_t *)nullptr,
+                                         (uint32_t *)nullptr, (int)npx, st);
+  if (e != hipSuccess) return hip_fail(e, "median background buffers");
+  const auto up = [](uint64_t n) { return (n + 255) & ~(uint64_t)255; };
+  const uint64_t b8 = up(npx * 8), b4 = up(npx * 4);
+  uint8_t *base = nullptr;
+  e = hipMalloc(&base, 3 * b8 + 5 * b4 + up(std::max(sort_bytes, scan_bytes)));
+  if (e != hipSuccess) return hip_fail(e, "median background buffers");
+  struct Free {
+    void *p;
+    ~Free() { (void)hipFree(p); }
+  } free_base{base};
+  uint64_t *keys = (uint64_t *)base, *skeys = (uint64_t *)(base + b8);
+  double *table = (double *)(base + 2 * b8);
+  uint32_t *idx = (uint32_t *)(base + 3 * b8), *sidx = (uint32_t *)(base + 3 * b8 + b4);
+  uint32_t *scan = (uint32_t *)(base + 3 * b8 + 2 * b4);
+  uint32_t *ra = (uint32_t *)(base + 3 * b8 + 3 * b4), *rb = (uint32_t *)(base + 3 * b8 + 4 * b4);
+  void *tmp = base + 3 * b8 + 5 * b4;
+  const unsigned g = grid_for(npx);
+  hipLaunchKernelGGL(key_kernel, dim3(g), dim3(kThreads), 0, st, d_img, npx, keys, idx);
+  e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, skeys, idx, sidx, (int)npx, 0, 64,
+                                         st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(head_kernel, dim3(g), dim3(kThreads), 0, st, skeys, npx, ra);
+    e = hipcub::DeviceScan::InclusiveSum(tmp, scan_bytes, ra, scan, (int)npx, st);
+  }
+  uint32_t D = 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(rank_kernel, dim3(g), dim3(kThreads), 0, st, skeys, sidx, scan, npx, ra,
+                       table);
+    e = hipMemcpyAsync(&D, scan + npx - 1, 4, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "median background ranks");
+  int R = 1;  // bits of the largest rank D - 1 (at least 1)
+  while (R < 32 && ((uint64_t)(D - 1) >> R)) ++R;
+  const int shift = std::max(0, R - 16);
+  // row segments: enough workgroups to fill the chip, each several windows tall
+  const int want = (4096 + W - 1) / W;
+  const int max_seg = std::max(1, H / std::max(2 * size, 64));
+  const int nseg = std::max({1, std::min(want, max_seg), (H + kMaxSegRows - 1) / kMaxSegRows});
+  const int seg_rows = (H + nseg - 1) / nseg;
+  uint32_t *src = ra, *dst = rb;
+  for (int p = 0; p < passes; ++p) {
+    if (shift > 0)
+      launch_median<16, true>(st, src, dst, W, H, size, seg_rows, nseg, shift);
+    else if (R <= 8)
+      launch_median<8, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
+    else if (R <= 10)
+      launch_median<10, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
+    else if (R <= 12)
+      launch_median<12, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
+    else if (R <= 14)
+      launch_median<14, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
+    else
+      launch_median<16, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
+    std::swap(src, dst);
+  }
+  hipLaunchKernelGGL(unrank_kernel, dim3(g), dim3(kThreads), 0, st, src, table, npx, d_bg);
+  e = hipStreamSynchronize(st);  // the buffers are freed on return
+  return e == hipSuccess ? LM_OK : hip_fail(e, "median background");
+}
+
+}  // namespace lm

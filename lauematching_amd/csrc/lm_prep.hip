@@ -4,7 +4,8 @@
 // src/LaueMatchingCPU.c:355-377).
 //
 // Pipeline (one call, one stream):
-//   1. v = max(raw - background, 0)                      (preprocess.py:263)
+//   1. v = max(raw - background, 0)                      (preprocess.py:263);
+//      without a background, the median-filter background of lm_median.hip
 //   2. threshold t: fixed, numpy-linear percentile of all pixels, or the
 //      adaptive noise floor median(nz) + 4 * 1.4826 * MAD(nz), >= 1
 //      (thresholds.py:36-47); order statistics come from a radix sort of the
@@ -332,8 +333,9 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   unsigned long long *d_cnt = nullptr;
   uint8_t *d_u8 = d_u8_out;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const bool median_bg = !background && cfg.filter_radius > 0 && cfg.median_passes > 0;
   hipError_t e = B.get(&d_raw, npx);
-  if (e == hipSuccess && background) e = B.get(&d_bg, npx);
+  if (e == hipSuccess && (background || median_bg)) e = B.get(&d_bg, npx);
   if (e == hipSuccess) e = B.get(&d_v, npx);
   if (e == hipSuccess) e = B.get(&d_u16, npx);
   if (e == hipSuccess) e = B.get(&d_keys, npx);
@@ -355,9 +357,14 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   } evg{ev0, ev1};
   const unsigned g = blocks_for(npx);
   e = hipMemcpyAsync(d_raw, raw, npx * 8, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && d_bg) e = hipMemcpyAsync(d_bg, background, npx * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && background) e = hipMemcpyAsync(d_bg, background, npx * 8, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "preprocess upload");
   (void)hipEventRecord(ev0, st);
+  if (median_bg) {  // no background given: compute_background (preprocess.py:255-266)
+    const int rc = median_background_dev(st, W, H, d_raw, cfg.filter_radius, cfg.median_passes,
+                                         d_bg);
+    if (rc != LM_OK) return rc;
+  }
   hipLaunchKernelGGL(bgsub_kernel, dim3(g), dim3(kThreads), 0, st, d_raw, d_bg, npx, d_v);
 
   // 2. threshold (thresholds.py:36-47, 92-110)

@@ -9,8 +9,10 @@ function.  OpenCV is not installed here: connectedComponentsWithStats with
 a 3x3 structure, which yields the same components (labels may be numbered
 differently; nothing downstream depends on the numbering).  scikit-image is not
 installed either, so "otsu" falls back to the adaptive noise floor exactly as
-the reference does without it (thresholds.py:92-110).  The background median
-filter (preprocess.py:43-64) is not restated: callers pass the background.
+the reference does without it (thresholds.py:92-110).  diplib is not
+installed either, so the background median filter is the reference's scipy
+branch (compute_background, preprocess.py:43-64), pinned on small frames by a
+brute-force window sort (median_brute).
 """
 from __future__ import annotations
 
@@ -19,6 +21,32 @@ import scipy.ndimage as ndi
 from scipy.spatial import cKDTree
 
 METHODS = ("adaptive", "percentile", "fixed", "otsu")
+
+
+def median_background(image, filter_radius=101, median_passes=1):
+    """compute_background without diplib (preprocess.py:43-64): zeros when
+    filter_radius <= 0 or median_passes <= 0 (:53-54), else
+    scipy.ndimage.median_filter(bg, size=filter_radius) once per pass
+    (:61-65)."""
+    if filter_radius <= 0 or median_passes <= 0:
+        return np.zeros_like(image, dtype=np.float64)
+    bg = image.astype(np.float64)
+    for _ in range(median_passes):
+        bg = ndi.median_filter(bg, size=filter_radius)
+    return bg
+
+
+def median_brute(image, size):
+    """One median pass by sorting every window: a size x size window whose
+    first row and column lie size // 2 before the pixel, the image extended by
+    numpy's 'symmetric' padding (repeated for windows wider than the image),
+    the element of rank size*size // 2.  For small frames only."""
+    lo = size // 2
+    a = np.asarray(image, dtype=np.float64)
+    p = np.pad(a, ((lo, size - 1 - lo), (lo, size - 1 - lo)), mode="symmetric")
+    win = np.lib.stride_tricks.sliding_window_view(p, (size, size))
+    k = (size * size) // 2
+    return np.sort(win.reshape(a.shape[0], a.shape[1], -1), axis=-1)[:, :, k]
 
 
 def noise_floor(image, k=4.0):
@@ -83,11 +111,14 @@ def gaussian_sigma(centres, pixel_size, distance, orient_spacing):
 
 
 def preprocess(raw, background=None, method="adaptive", fixed_value=0.0, percentile=90.0,
-               min_area=10, pixel_size=0.2, distance=0.513, orient_spacing=0.4):
-    """Steps 1, 3-6 of preprocess_image (preprocess.py:258-337), enhancement
-    off (its default).  Returns the float64 blurred image and a dict with the
-    threshold, sigma and component counts."""
-    bg = np.zeros_like(raw) if background is None else background
+               min_area=10, pixel_size=0.2, distance=0.513, orient_spacing=0.4,
+               filter_radius=0, median_passes=1):
+    """Steps 1, 3-6 of preprocess_image (preprocess.py:255-337), enhancement
+    off (its default); without a background it is computed from the raw
+    image (no background file: :255-266).  Returns the float64 blurred image
+    and a dict with the threshold, sigma and component counts."""
+    bg = (median_background(raw, filter_radius, median_passes) if background is None
+          else background)
     bg_sub = np.maximum(raw - bg, 0.0)
     t = threshold_value(bg_sub, method, fixed_value, percentile)
     thr = bg_sub.copy()

@@ -5,7 +5,9 @@ component counts and centres-derived sigma exact; the float64 blur within
 differ in the last bit); the uint8 image equal to quantizing the device's own
 blurred image on the host, and to the oracle's except at most one level at
 rounding boundaries.  The uint8 image left in device memory scores exactly
-like the host copy."""
+like the host copy.  The median background (lm_median_background) equals
+scipy.ndimage.median_filter, the reference's filter without diplib, pixel for
+pixel."""
 import os
 import sys
 
@@ -126,3 +128,84 @@ def test_preprocess_threshold_matches_reference_goldens(ctx, method):
         assert info["threshold"] == t, (name, info["threshold"], t)
         assert round(info["threshold"], 6) == want[name]["thresh"], name
         assert info["n_above"] == want[name]["nonzero"], name
+
+
+def _frame(kind, H, W, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "poisson":    # detector counts: few distinct values
+        return rng.poisson(80.0, (H, W)).astype(np.float64)
+    if kind == "wide":       # integers over most of the 16-bit range
+        return rng.integers(0, 60000, (H, W)).astype(np.float64)
+    if kind == "float":      # every value distinct: rank bits beyond 16
+        return rng.normal(100.0, 25.0, (H, W))
+    if kind == "signed":     # negatives and both zeros
+        img = rng.integers(-3, 4, (H, W)).astype(np.float64)
+        img[img == 0] = np.where(rng.random(np.count_nonzero(img == 0)) < 0.5, 0.0, -0.0)
+        return img
+    raise ValueError(kind)
+
+
+MEDIAN_CASES = [
+    # kind, H, W, filter size, passes
+    ("poisson", 200, 160, 31, 1),
+    ("wide", 256, 256, 25, 1),
+    ("float", 300, 400, 21, 1),     # 120000 distinct values: low rank bits resolved
+    ("float", 2048, 2048, 5, 1),    # 2^22 distinct values
+    ("poisson", 40, 24, 101, 2),    # window wider than the image
+    ("poisson", 97, 130, 10, 3),    # even size, three passes
+    ("signed", 64, 80, 7, 1),
+    ("wide", 50, 70, 1, 1),         # identity
+    ("poisson", 50, 70, 0, 1),      # zeros (preprocess.py:53-54)
+    ("poisson", 50, 70, 9, 0),
+]
+
+
+@pytest.mark.parametrize("case", MEDIAN_CASES,
+                         ids=[f"{c[0]}-{c[1]}x{c[2]}-s{c[3]}-p{c[4]}" for c in MEDIAN_CASES])
+def test_median_background_matches_oracle(ctx, case):
+    kind, H, W, size, passes = case
+    img = _frame(kind, H, W, seed=H * W + size)
+    _say(f"median background {kind} {H}x{W} size={size} passes={passes}")
+    bg, ms = api.median_background(ctx, img, size, passes)
+    want = po.median_background(img, size, passes)
+    np.testing.assert_array_equal(bg, want)
+
+
+def test_median_background_full_frame_sampled(ctx):
+    """A 2048 x 2048 detector frame with the reference's default filter (101,
+    one pass): every pixel is an element of the frame, and a sample of pixels
+    (all four edges and corners among them) equals the sort of its window."""
+    H = W = 2048
+    size = 101
+    img, _ = _raw(H, W, 3000, seed=5)
+    img = np.floor(img)
+    _say(f"median background {H}x{W} size={size}")
+    bg, ms = api.median_background(ctx, img, size, 1)
+    _say(f"median background {H}x{W} size={size}: {ms:.2f} ms")
+    assert np.isin(bg, img).all()
+    rng = np.random.default_rng(6)
+    ys = np.concatenate([rng.integers(0, H, 300), [0, 0, H - 1, H - 1], rng.integers(0, 60, 40),
+                         rng.integers(H - 60, H, 40), rng.integers(0, H, 80)])
+    xs = np.concatenate([rng.integers(0, W, 300), [0, W - 1, 0, W - 1], rng.integers(0, W, 40),
+                         rng.integers(0, W, 40), rng.integers(0, 60, 40),
+                         rng.integers(W - 60, W, 40)])
+    lo = size // 2
+    pad = np.pad(img, ((lo, size - 1 - lo), (lo, size - 1 - lo)), mode="symmetric")
+    k = size * size // 2
+    for y, x in zip(ys, xs):
+        win = pad[y:y + size, x:x + size].ravel()
+        assert bg[y, x] == np.partition(win, k)[k], (y, x)
+
+
+def test_preprocess_computes_the_median_background(ctx):
+    """No background given: the pipeline subtracts the median background of
+    the raw frame (preprocess.py:255-266) and then matches the oracle."""
+    raw, _ = _raw(256, 320, 60, seed=21)
+    _say("preprocess with a median background 256x320")
+    u8, scale, info, blurred = api.preprocess(ctx, raw, None, "adaptive", want_blurred=True,
+                                              filter_radius=31, median_passes=2)
+    want, winfo = po.preprocess(raw, None, "adaptive", filter_radius=31, median_passes=2)
+    assert info["threshold"] == winfo["threshold"]
+    assert info["n_components"] == winfo["n_components"]
+    assert info["n_kept"] == winfo["n_kept"]
+    np.testing.assert_allclose(blurred, want, rtol=1e-12, atol=1e-12 * max(1.0, want.max()))

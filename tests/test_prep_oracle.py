@@ -3,7 +3,8 @@ brute force and closed forms, so the device pipeline is checked against a
 restatement that is itself checked: 8-connectivity against a flood fill,
 centres of mass against explicit sums, the nearest-centre distance against
 all pairs, the blur's edge mode against an explicit padded correlation, the
-noise floor against a hand-computed case."""
+noise floor against a hand-computed case, the median background against a
+brute-force sort of every window and the properties an order statistic has."""
 import os
 import sys
 
@@ -107,3 +108,43 @@ def test_empty_and_all_small():
     raw[3, 3] = 100.0
     out, info = po.preprocess(raw, method="fixed", fixed_value=5.0, min_area=2)
     assert not out.any() and info["n_components"] == 1 and info["n_kept"] == 0
+
+
+# (H, W, size): odd and even windows, windows wider than the image (repeated
+# reflection), a single pixel
+MEDIAN_SHAPES = [(7, 5, 3), (9, 6, 4), (5, 4, 11), (3, 2, 12), (1, 1, 5), (13, 17, 6),
+                 (20, 11, 25), (16, 16, 1)]
+
+
+def test_median_background_equals_window_sort():
+    rng = np.random.default_rng(3)
+    for H, W, size in MEDIAN_SHAPES:
+        for img in (rng.integers(0, 9, (H, W)).astype(np.float64),  # many ties
+                    rng.normal(0.0, 1.0, (H, W))):                   # negatives, no ties
+            want = po.median_brute(img, size)
+            np.testing.assert_array_equal(po.median_background(img, size, 1), want)
+            twice = po.median_brute(want, size)
+            np.testing.assert_array_equal(po.median_background(img, size, 2), twice)
+
+
+def test_median_background_order_statistic_properties():
+    rng = np.random.default_rng(4)
+    img = rng.normal(100.0, 30.0, (24, 31))
+    bg = po.median_background(img, 7, 1)
+    # an element of its window, never an average of two
+    assert np.isin(bg, img).all()
+    # commutes with strictly increasing maps
+    np.testing.assert_array_equal(po.median_background(np.exp(img / 50.0), 7, 1),
+                                  np.exp(bg / 50.0))
+    np.testing.assert_array_equal(po.median_background(-img, 7, 1), -bg)
+    # constant frames, size 1, and the zero cases of preprocess.py:53-54
+    np.testing.assert_array_equal(po.median_background(np.full((9, 8), 2.5), 5, 3), 2.5)
+    np.testing.assert_array_equal(po.median_background(img, 1, 4), img)
+    assert not po.median_background(img, 0, 1).any()
+    assert not po.median_background(img, 5, 0).any()
+    # a step edge survives, an isolated spike does not
+    step = np.zeros((15, 15))
+    step[:, 8:] = 10.0
+    step[3, 3] = 1e6
+    np.testing.assert_array_equal(po.median_background(step, 3, 1), np.where(
+        np.arange(15)[None, :] >= 8, 10.0, 0.0) * np.ones((15, 1)))
