@@ -434,6 +434,43 @@ def prep_check(ctx, reps=5):
            "components": info["n_components"], "kept": info["n_kept"], "sigma": info["sigma"]}
     log(f"preprocess: {out['device_ms']:.2f} ms device, {out['wall_ms']:.1f} ms per call "
         f"({out['kept']}/{out['components']} components kept, sigma {out['sigma']:.2f})")
+    out["median_background"] = median_check(ctx, raw, reps)
+    return out
+
+
+def median_check(ctx, raw, reps=5, size=101):
+    """The median-filter background (compute_background's default size 101,
+    one pass) of the 2048x2048 frame as integer counts (a detector's frame)
+    and as float64 (every value distinct), the pipeline with that background
+    computed, and scipy.ndimage.median_filter (the reference's filter
+    without diplib) timed on a 160x160 crop on one host core, with the
+    device's result on that crop compared to scipy's."""
+    import scipy.ndimage as ndi
+    from lauematching_amd import api
+
+    counts = np.floor(raw)
+    out = {"filter_radius": size, "median_passes": 1, "frame": "2048x2048"}
+    for name, img in (("integer_counts", counts), ("float", raw)):
+        api.median_background(ctx, img, size, 1)
+        ms = [api.median_background(ctx, img, size, 1)[1] for _ in range(reps)]
+        out[f"{name}_device_ms"] = float(np.median(ms))
+    api.preprocess(ctx, counts, None, filter_radius=size)
+    pms = [api.preprocess(ctx, counts, None, filter_radius=size)[2]["ms"] for _ in range(reps)]
+    out["pipeline_with_median_device_ms"] = float(np.median(pms))
+    crop = counts[:160, :160].copy()
+    t0 = time.perf_counter()
+    want = ndi.median_filter(crop, size=size)
+    cpu_s = time.perf_counter() - t0
+    got, _ = api.median_background(ctx, crop, size, 1)
+    out["cpu_scipy"] = {"sample": "160x160 crop of the integer frame", "cores": 1,
+                        "ms": cpu_s * 1e3, "px_per_s": crop.size / cpu_s,
+                        "device_equal": bool(np.array_equal(got, want))}
+    out["device_px_per_s"] = counts.size / (out["integer_counts_device_ms"] * 1e-3)
+    log(f"median background {size}x{size}: {out['integer_counts_device_ms']:.2f} ms (integer "
+        f"counts), {out['float_device_ms']:.2f} ms (float), pipeline with it "
+        f"{out['pipeline_with_median_device_ms']:.2f} ms; scipy {crop.size / cpu_s:.3g} px/s "
+        f"on one core vs device {out['device_px_per_s']:.3g} px/s, crop equal "
+        f"{out['cpu_scipy']['device_equal']}")
     return out
 
 

@@ -93,10 +93,10 @@ __global__ void rank_kernel(const uint64_t *__restrict__ sk, const uint32_t *__r
 }
 
 __global__ void unrank_kernel(const uint32_t *__restrict__ rank, const double *__restrict__ table,
-                              uint64_t npx, double *__restrict__ out) {
+                              uint32_t D, uint64_t npx, double *__restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx;
        i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = table[rank[i]];
+    out[i] = table[min(rank[i], D - 1)];  // ranks are < D; the clamp keeps reads in the table
 }
 
 // scipy's 'reflect' extension: half-sample symmetric, period 2n
@@ -107,79 +107,171 @@ __device__ __forceinline__ int reflect(int i, int n) {
   return i < n ? i : p - 1 - i;
 }
 
-// Wave-uniform searches over counts h(0..n-1) for the bin m holding the
-// element of rank k: below = sum of h(j < m) on entry and exit, and on exit
-// below <= k < below + h(m).  Every loop ends: move_up stops at the last
-// bin, move_down at bin 0.
-template <class F>
-__device__ void move_up(const F &h, uint32_t n, uint32_t k, uint32_t &m, uint32_t &below) {
+// Inclusive prefix sum over the 64 lanes of a wavefront: DPP row shifts
+// within each row of 16 lanes, then row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
+}
+
+__device__ __forceinline__ uint32_t lane_value(uint32_t x, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, l);
+}
+
+// Wave-uniform search for the bin holding the element of rank k in counts
+// h(0..NB-1), grouped by 64.  On entry and on exit below = sum of h(j < m);
+// on exit below <= k < below + h(m).  From the current bin it scans the rest
+// of its group, then whole groups, then the bins of the group found.  With
+// GROUPS the group sums g(0..NB/64-1) are kept (64 groups per step), else a
+// group's sum is the scan of its bins.  Every loop ends at the first or last
+// group.
+template <bool GROUPS, class HF, class GF>
+__device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t &m,
+                     uint32_t &below) {
   const int lane = __lane_id();
-  while (m < n) {
-    const uint32_t bin = m + lane;
-    const uint32_t v = bin < n ? h(bin) : 0u;
-    uint32_t p = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t t = __shfl_up(p, d);
-      if (lane >= d) p += t;
+  const int NG = (int)(NB >> 6);
+  int grp;  // group holding the answer, below = count before its first bin
+  if (below > k) {
+    // down: bins m-1, m-2, ... of m's group, then groups before it
+    const int g0 = (int)(m >> 6) << 6;
+    const int bin = (int)m - 1 - lane;
+    const uint32_t v = bin >= g0 ? h((uint32_t)bin) : 0u;
+    const uint32_t p = wave_scan(v);
+    const uint64_t hit = __ballot(p >= below - k);
+    if (hit) {
+      const int l = __builtin_ctzll(hit);
+      m = (uint32_t)((int)m - 1 - l);
+      below -= lane_value(p, l);
+      return;
     }
+    below -= lane_value(p, 63);
+    int gg = (int)(m >> 6) - 1;
+    for (;;) {
+      if (gg < 0) {  // not reached while below counts the bins under m
+        m = 0;
+        below = 0;
+        return;
+      }
+      if (!GROUPS) {  // group gg's bins, descending
+        const uint32_t hv = h((uint32_t)(gg << 6) + 63 - lane);
+        const uint32_t hp = wave_scan(hv);
+        const uint64_t bhit = __ballot(hp >= below - k);
+        if (bhit) {
+          const int l = __builtin_ctzll(bhit);
+          m = (uint32_t)(gg << 6) + 63 - l;
+          below -= lane_value(hp, l);
+          return;
+        }
+        below -= lane_value(hp, 63);
+        --gg;
+        continue;
+      }
+      const int gi = gg - lane;
+      const uint32_t sv = gi >= 0 ? g((uint32_t)gi) : 0u;
+      const uint32_t sp = wave_scan(sv);
+      const uint64_t ghit = __ballot(sp >= below - k);
+      if (ghit) {
+        const int l = __builtin_ctzll(ghit);
+        grp = gg - l;
+        below -= lane_value(sp, l);
+        break;
+      }
+      below -= lane_value(sp, 63);
+      gg -= 64;
+    }
+  } else {
+    if (below + h(m) > k) return;
+    // up: bins m.. of m's group, then groups after it
+    const int g0 = (int)(m >> 6) << 6;
+    const int bin = g0 + lane;
+    const uint32_t v = bin >= (int)m ? h((uint32_t)bin) : 0u;
+    const uint32_t p = wave_scan(v);
     const uint64_t hit = __ballot(below + p > k);
     if (hit) {
       const int l = __builtin_ctzll(hit);
-      below += __shfl(p, l) - __shfl(v, l);
-      m += l;
+      m = (uint32_t)(g0 + l);
+      below += lane_value(p, l) - lane_value(v, l);
       return;
     }
-    below += __shfl(p, 63);
-    m += 64;
-  }
-  m = n - 1;  // not reached while the counts sum to more than k
-}
-
-template <class F>
-__device__ void move_down(const F &h, uint32_t k, uint32_t &m, uint32_t &below) {
-  const int lane = __lane_id();
-  while (m > 0) {
-    const int bin = (int)m - 64 + lane;
-    const uint32_t v = bin >= 0 ? h((uint32_t)bin) : 0u;
-    uint32_t s = v;  // sum of v over lanes >= this one
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t t = __shfl_down(s, d);
-      if (lane + d < 64) s += t;
+    below += lane_value(p, 63);
+    int gg = (int)(m >> 6) + 1;
+    for (;;) {
+      if (gg >= NG) {  // not reached while the counts sum to more than k
+        m = NB - 1;
+        return;
+      }
+      if (!GROUPS) {  // group gg's bins
+        const uint32_t hv = h((uint32_t)(gg << 6) + lane);
+        const uint32_t hp = wave_scan(hv);
+        const uint64_t bhit = __ballot(below + hp > k);
+        if (bhit) {
+          const int l = __builtin_ctzll(bhit);
+          m = (uint32_t)(gg << 6) + l;
+          below += lane_value(hp, l) - lane_value(hv, l);
+          return;
+        }
+        below += lane_value(hp, 63);
+        ++gg;
+        continue;
+      }
+      const int gi = gg + lane;
+      const uint32_t sv = gi < NG ? g((uint32_t)gi) : 0u;
+      const uint32_t sp = wave_scan(sv);
+      const uint64_t ghit = __ballot(below + sp > k);
+      if (ghit) {
+        const int l = __builtin_ctzll(ghit);
+        grp = gg + l;
+        below += lane_value(sp, l) - lane_value(sv, l);
+        break;
+      }
+      below += lane_value(sp, 63);
+      gg += 64;
     }
-    const uint64_t hit = __ballot(s >= below - k);
-    if (hit) {
-      const int l = 63 - __builtin_clzll(hit);
-      below -= __shfl(s, l);
-      m = (uint32_t)((int)m - 64 + l);
-      return;
-    }
-    below -= __shfl(s, 0);
-    m = m > 64 ? m - 64 : 0;
   }
-  below = 0;  // not reached while below counts the bins under m
+  // the bins of group grp
+  const uint32_t v = h((uint32_t)(grp << 6) + lane);
+  const uint32_t p = wave_scan(v);
+  const uint64_t hit = __ballot(below + p > k);
+  const int l = hit ? __builtin_ctzll(hit) : 63;
+  m = (uint32_t)(grp << 6) + l;
+  below += lane_value(p, l) - lane_value(v, l);
 }
 
 // One workgroup per (column, row segment).  NBITS: histogram bins 2^NBITS
-// over rank >> shift; FINE: resolve the low `shift` bits of the rank.  The
+// over rank >> shift; FINE: the low `shift` bits of the rank come from a
+// band of 2^15 full-resolution counts around the median, rebuilt from the
+// window only when the median's coarse bin leaves it.  The
 // rows a segment touches are reflected once into an LDS table; window scans
 // give each thread one column (of 128 or 256) and stride over rows; the
-// update loads run kDepth rows ahead of the histogram updates.
+// update loads run kDepth rows ahead of the histogram updates.  Every
+// wavefront tracks the median bin itself (the same search over the same
+// counts), so a row costs two barriers: after the updates and after the
+// searches.
 constexpr int kMaxSegRows = 1024;
 constexpr int kDepth = 4;
+constexpr uint32_t kBand = 1u << 15;
 
 template <int NBITS, bool FINE>
 __global__ void __launch_bounds__(kMedThreads)
 median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, int W, int H,
                      int s, int seg_rows, int shift) {
   constexpr uint32_t NB = 1u << NBITS;
+  constexpr bool GROUPS = NBITS >= 12;  // group sums pay once scans get long
+  // frames with few distinct values (integer counts) crowd into few groups:
+  // their group counts are summed per wavefront first
+  constexpr bool AGG = GROUPS && NBITS <= 14;
   __shared__ uint32_t hist[NB / 2];
-  __shared__ uint32_t fine[FINE ? 256 : 1];
-  __shared__ uint32_t part[kMedThreads];
+  __shared__ uint32_t grpc[GROUPS ? NB / 64 : 1];
+  // FINE: full-resolution counts of ranks band_lo .. band_lo + kBand - 1
+  // (16-bit, two per word), placed around the median's coarse bin
+  __shared__ uint32_t band[FINE ? kBand / 2 : 1];
   __shared__ int rowtab[kMaxSegRows + 256];
   __shared__ int delta[kMedThreads / 64];
-  __shared__ uint32_t st_m, st_below, st_fm;
 
   uint32_t b = blockIdx.x;
   if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);  // a band per XCD
@@ -187,18 +279,19 @@ median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out
   const int y0 = (int)(b / (uint32_t)W) * seg_rows;
   if (y0 >= H) return;  // an empty last segment (whole workgroup)
   const int y1 = min(H, y0 + seg_rows);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int lo = s >> 1;
   const uint32_t k = (uint32_t)(s * s) >> 1;
   const uint32_t fmask = (1u << shift) - 1u;
   const auto hcount = [&](uint32_t j) { return (hist[j >> 1] >> ((j & 1u) << 4)) & 0xffffu; };
-  const auto fcount = [&](uint32_t j) { return fine[j]; };
+  const auto gcount = [&](uint32_t j) { return grpc[j]; };
+  const auto bcount = [&](uint32_t j) { return (band[j >> 1] >> ((j & 1u) << 4)) & 0xffffu; };
 
   // rowtab[i] = image row of window row y0 - lo + i
   for (int i = tid; i < y1 - y0 + s; i += kMedThreads) rowtab[i] = reflect(y0 - lo + i, H);
   for (uint32_t i = tid; i < NB / 2; i += kMedThreads) hist[i] = 0;
-  if (FINE)
-    for (uint32_t i = tid; i < 256; i += kMedThreads) fine[i] = 0;
+  if (GROUPS)
+    for (uint32_t i = tid; i < NB / 64; i += kMedThreads) grpc[i] = 0;
   // window scans: this thread's column sdx and first row sdy
   const int cw = s <= 128 ? 128 : 256, rw = kMedThreads / cw;
   const int sdx = tid % cw, sdy = tid / cw;
@@ -210,29 +303,14 @@ median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out
     for (int dy = sdy; dy < s; dy += rw) {
       const uint32_t v = in[(size_t)rowtab[dy] * W + wcol] >> shift;
       atomicAdd(&hist[v >> 1], 1u << ((v & 1u) << 4));
+      if (GROUPS) atomicAdd(&grpc[v >> 6], 1u);
     }
   __syncthreads();
-  // its median bin: per-thread sums of NB / 256 bins, then bins within one
-  constexpr uint32_t CH = NB >= kMedThreads ? NB / kMedThreads : 1u;
-  {
-    uint32_t sum = 0;
-    if ((uint32_t)tid * CH < NB)
-      for (uint32_t j = 0; j < CH; ++j) sum += hcount(tid * CH + j);
-    part[tid] = sum;
-  }
-  __syncthreads();
-  if (wv == 0) {
-    uint32_t pm = 0, pb = 0;
-    move_up([&](uint32_t j) { return part[j]; }, (uint32_t)kMedThreads, k, pm, pb);
-    uint32_t m = pm * CH, below = pb;
-    move_up(hcount, NB, k, m, below);
-    if (lane == 0) {
-      st_m = m;
-      st_below = below;
-      st_fm = 0xffffffffu;
-    }
-  }
-  __syncthreads();
+  uint32_t m = 0, below = 0;
+  const uint32_t nbc = kBand >> shift;  // coarse bins the band covers
+  uint32_t c0 = 0, band_lo = 0x80000000u;  // no rank is in the band yet
+  bool band_ok = false;
+  seek<GROUPS>(hcount, gcount, NB, k, m, below);
 
   // this thread's updates per row: slots j = tid and tid + 256 of the s
   // removals (j < s) and s additions
@@ -253,65 +331,82 @@ median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out
       if (act[u]) v[u] = in[(size_t)(add[u] ? re : rl) * W + col[u]];
   };
   const auto update = [&](const uint32_t (&v)[2]) {
-    const uint32_t m = st_m, fm = st_fm;
     int c = 0;
+    // AGG: the group of the wave's first lane gets one atomic for all its
+    // lanes (one address per lane would serialize)
+    uint32_t gf[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      if (!act[u]) continue;
+      const uint32_t gl = act[u] ? (v[u] >> shift) >> 6 : 0xffffffffu;
+      gf[u] = AGG ? (uint32_t)__builtin_amdgcn_readfirstlane((int)gl) : 0xffffffffu;
+      if (AGG && gf[u] != 0xffffffffu) {
+        const int net = __popcll(__ballot(gl == gf[u] && add[u])) -
+                        __popcll(__ballot(gl == gf[u] && !add[u]));
+        if (lane == 0 && net) atomicAdd(&grpc[gf[u]], (uint32_t)net);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
       const uint32_t r = v[u], bin = r >> shift;
-      const uint32_t inc = 1u << ((bin & 1u) << 4);
-      atomicAdd(&hist[bin >> 1], add[u] ? inc : 0u - inc);
-      if (bin < m) c += add[u] ? 1 : -1;
-      if (FINE && bin == fm) atomicAdd(&fine[r & fmask], add[u] ? 1u : 0xffffffffu);
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
-    if (lane == 0) delta[wv] = c;
-    __syncthreads();
-  };
-  // the median of row y (its window counted, delta[] holding the change
-  // of the count below the previous median when y > y0)
-  const auto finish = [&](int y) {
-    if (wv == 0) {
-      uint32_t m = st_m;
-      uint32_t below = st_below;
-      if (y > y0) {
-        int dsum = 0;
-#pragma unroll
-        for (int w = 0; w < kMedThreads / 64; ++w) dsum += delta[w];
-        below = (uint32_t)((int)below + dsum);
-      }
-      if (below > k)
-        move_down(hcount, k, m, below);
-      else if (below + hcount(m) <= k)
-        move_up(hcount, NB, k, m, below);
-      if (lane == 0) {
-        st_m = m;
-        st_below = below;
-        if (!FINE) out[(size_t)y * W + x] = m;
-      }
-    }
-    __syncthreads();
-    if (FINE) {
-      const uint32_t m = st_m;
-      if (m != st_fm) {  // rebuild the low-bit counts of coarse bin m
-        for (uint32_t i = tid; i < 256; i += kMedThreads) fine[i] = 0;
-        __syncthreads();
-        if (scol)
-          for (int dy = sdy; dy < s; dy += rw) {
-            const uint32_t r = in[(size_t)rowtab[y - y0 + dy] * W + wcol];
-            if ((r >> shift) == m) atomicAdd(&fine[r & fmask], 1u);
+      if (act[u]) {
+        const uint32_t inc = 1u << ((bin & 1u) << 4);
+        atomicAdd(&hist[bin >> 1], add[u] ? inc : 0u - inc);
+        if (GROUPS && (bin >> 6) != gf[u]) atomicAdd(&grpc[bin >> 6], add[u] ? 1u : 0xffffffffu);
+        if (FINE) {
+          const uint32_t o = r - band_lo;
+          if (o < kBand) {
+            const uint32_t binc = 1u << ((o & 1u) << 4);
+            atomicAdd(&band[o >> 1], add[u] ? binc : 0u - binc);
           }
-        __syncthreads();
-        if (tid == 0) st_fm = m;
+        }
       }
-      if (wv == 0) {
-        uint32_t f = 0, fb = 0;
-        move_up(fcount, fmask + 1u, k - st_below, f, fb);
-        if (lane == 0) out[(size_t)y * W + x] = (m << shift) | f;
-      }
+      c += __popcll(__ballot(act[u] && add[u] && bin < m));
+      c -= __popcll(__ballot(act[u] && !add[u] && bin < m));
+    }
+    if (lane == 0) delta[tid >> 6] = c;
+    __syncthreads();
+    int dsum = 0;
+#pragma unroll
+    for (int w = 0; w < kMedThreads / 64; ++w) dsum += delta[w];
+    below = (uint32_t)((int)below + dsum);
+    seek<GROUPS>(hcount, gcount, NB, k, m, below);
+  };
+  // the output of row y (m and below current)
+  const auto finish = [&](int y) {
+    if (!FINE) {
+      if (tid == 0) out[(size_t)y * W + x] = m;
+      __syncthreads();
+      return;
+    }
+    if (!band_ok || m - c0 >= nbc) {  // move the band to centre coarse bin m
+      c0 = m > nbc / 2 ? m - nbc / 2 : 0;
+      band_lo = c0 << shift;
+      band_ok = true;
+      for (uint32_t i = tid; i < kBand / 2; i += kMedThreads) band[i] = 0;
+      __syncthreads();
+      if (scol)
+        for (int dy = sdy; dy < s; dy += rw) {
+          const uint32_t o = in[(size_t)rowtab[y - y0 + dy] * W + wcol] - band_lo;
+          if (o < kBand) atomicAdd(&band[o >> 1], 1u << ((o & 1u) << 4));
+        }
       __syncthreads();
     }
+    if (tid < 64) {  // the rank within coarse bin m: its 2^shift band counts
+      const uint32_t kk = k - below, off = (m - c0) << shift, nf = fmask + 1u;
+      uint32_t f = nf - 1, fb = 0;
+      for (uint32_t j0 = 0; j0 < nf; j0 += 64) {
+        const uint32_t v = j0 + lane < nf ? bcount(off + j0 + lane) : 0u;
+        const uint32_t p = wave_scan(v);
+        const uint64_t hit = __ballot(fb + p > kk);
+        if (hit) {
+          f = j0 + __builtin_ctzll(hit);
+          break;
+        }
+        fb += lane_value(p, 63);
+      }
+      if (tid == 0) out[(size_t)y * W + x] = (m << shift) | f;
+    }
+    __syncthreads();
   };
 
   uint32_t ring[kDepth][2];
@@ -395,7 +490,7 @@ int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int
   if (e != hipSuccess) return hip_fail(e, "median background ranks");
   int R = 1;  // bits of the largest rank D - 1 (at least 1)
   while (R < 32 && ((uint64_t)(D - 1) >> R)) ++R;
-  const int shift = std::max(0, R - 16);
+  const int shift = R > 16 ? R - 15 : 0;  // over 16 bits: 15 coarse bits and a band
   // row segments: enough workgroups to fill the chip, each several windows tall
   const int want = (4096 + W - 1) / W;
   const int max_seg = std::max(1, H / std::max(2 * size, 64));
@@ -404,7 +499,7 @@ int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int
   uint32_t *src = ra, *dst = rb;
   for (int p = 0; p < passes; ++p) {
     if (shift > 0)
-      launch_median<16, true>(st, src, dst, W, H, size, seg_rows, nseg, shift);
+      launch_median<15, true>(st, src, dst, W, H, size, seg_rows, nseg, shift);
     else if (R <= 8)
       launch_median<8, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
     else if (R <= 10)
@@ -417,7 +512,7 @@ int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int
       launch_median<16, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
     std::swap(src, dst);
   }
-  hipLaunchKernelGGL(unrank_kernel, dim3(g), dim3(kThreads), 0, st, src, table, npx, d_bg);
+  hipLaunchKernelGGL(unrank_kernel, dim3(g), dim3(kThreads), 0, st, src, table, D, npx, d_bg);
   e = hipStreamSynchronize(st);  // the buffers are freed on return
   return e == hipSuccess ? LM_OK : hip_fail(e, "median background");
 }

@@ -151,6 +151,7 @@ MEDIAN_CASES = [
     ("wide", 256, 256, 25, 1),
     ("float", 300, 400, 21, 1),     # 120000 distinct values: low rank bits resolved
     ("float", 2048, 2048, 5, 1),    # 2^22 distinct values
+    ("float", 4096, 4096, 3, 1),    # 2^24 pixels, the largest frame
     ("poisson", 40, 24, 101, 2),    # window wider than the image
     ("poisson", 97, 130, 10, 3),    # even size, three passes
     ("signed", 64, 80, 7, 1),
