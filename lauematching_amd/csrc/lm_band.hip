@@ -20,6 +20,7 @@
 #include <cstdio>
 
 #include "lm_internal.h"
+#include "lm_walk16.h"
 
 namespace lm {
 namespace {
@@ -527,65 +528,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
     }
     __syncthreads();
     if constexpr (S16) {
-      // sum mode: a row can pass only if its sum reaches MinIntensity and the
-      // spot count it bounds (each nonzero pixel adds at least 1) can reach
-      // MinNrSpots; such rows go to the candidate list for an exact recount
-      // (count mode: the counter is the exact spot count, tested against
-      // MinNrSpots; the sum is recounted)
-      const uint32_t sthr = count_mode ? 0u : (pk0 & 0x7fffffu);
-      const uint32_t nthr = (pk0 >> 23) > 1u ? (pk0 >> 23) : 1u;
-      const bool zok = pred->zero_ok[0] & 1u;
-      if (!zok) {
-        // 8 rows per 16-byte LDS read, one max test for the 8 (rows that
-        // pass are rare), and the words cleared for the next super-tile
-        const uint32_t thr = sthr > nthr ? sthr : nthr;
-        for (int i = tid; i < NCNT / 4; i += THREADS) {
-          const v4u q = ((v4u *)cnt)[i];
-          ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
-          uint32_t m = 0;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) m = max(m, max(q[k] & 0xffffu, q[k] >> 16));
-          if (m >= thr) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const uint32_t sum = (q[k] >> (16 * h)) & 0xffffu;
-                const uint64_t o = first + 8u * (uint32_t)i + 2u * k + h;
-                if (o < n && sum >= thr) {
-                  const uint32_t pos = atomicAdd(&counters[kCandSlot], 1u);
-                  if (pos < candCap) cands[pos] = o;
-                }
-              }
-          }
-        }
-        __syncthreads();
-        continue;
-      }
-      for (int rr = tid; rr < T; rr += THREADS) {
-        const uint64_t o = first + rr;
-        const uint32_t sum = (cnt[rr >> 1] >> ((rr & 1) << 4)) & 0xffffu;
-        const bool live = o < n;
-        if (zok) {  // rows with no spot pass only when both thresholds are <= 0
-          const uint64_t zm = __ballot(live && sum == 0);
-          if (zm && lane == 0) atomicAdd(&counters[1], (uint32_t)__popcll(zm));
-        }
-        const bool cand = live && sum >= nthr && sum >= sthr;
-        const uint64_t m = __ballot(cand);
-        if (m) {
-          const int leader = __ffsll((unsigned long long)m) - 1;
-          uint32_t b0 = 0;
-          if (lane == leader) b0 = atomicAdd(&counters[kCandSlot], (uint32_t)__popcll(m));
-          b0 = __shfl(b0, leader);
-          if (cand) {
-            const uint64_t pos = (uint64_t)b0 + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (pos < candCap) cands[pos] = o;
-          }
-        }
-      }
-      __syncthreads();
-      for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
-      __syncthreads();
+      sum16_epilogue<THREADS>(cnt, first, n, pred, pk0, count_mode, counters, cands, candCap);
       continue;
     }
     for (int rr = tid; rr < T; rr += THREADS) {

@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "lm_internal.h"
+#include "lm_walk16.h"
 
 namespace lm {
 namespace {
@@ -107,18 +108,6 @@ __device__ __forceinline__ int reflect(int i, int n) {
   return i < n ? i : p - 1 - i;
 }
 
-// Inclusive prefix sum over the 64 lanes of a wavefront: DPP row shifts
-// within each row of 16 lanes, then row broadcasts of lanes 15 and 31.
-__device__ __forceinline__ uint32_t wave_scan(uint32_t x) {
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return x;
-}
-
 __device__ __forceinline__ uint32_t lane_value(uint32_t x, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, l);
 }
@@ -141,7 +130,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
     const int g0 = (int)(m >> 6) << 6;
     const int bin = (int)m - 1 - lane;
     const uint32_t v = bin >= g0 ? h((uint32_t)bin) : 0u;
-    const uint32_t p = wave_scan(v);
+    const uint32_t p = wave_scan_dpp(v);
     const uint64_t hit = __ballot(p >= below - k);
     if (hit) {
       const int l = __builtin_ctzll(hit);
@@ -159,7 +148,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
       }
       if (!GROUPS) {  // group gg's bins, descending
         const uint32_t hv = h((uint32_t)(gg << 6) + 63 - lane);
-        const uint32_t hp = wave_scan(hv);
+        const uint32_t hp = wave_scan_dpp(hv);
         const uint64_t bhit = __ballot(hp >= below - k);
         if (bhit) {
           const int l = __builtin_ctzll(bhit);
@@ -173,7 +162,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
       }
       const int gi = gg - lane;
       const uint32_t sv = gi >= 0 ? g((uint32_t)gi) : 0u;
-      const uint32_t sp = wave_scan(sv);
+      const uint32_t sp = wave_scan_dpp(sv);
       const uint64_t ghit = __ballot(sp >= below - k);
       if (ghit) {
         const int l = __builtin_ctzll(ghit);
@@ -190,7 +179,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
     const int g0 = (int)(m >> 6) << 6;
     const int bin = g0 + lane;
     const uint32_t v = bin >= (int)m ? h((uint32_t)bin) : 0u;
-    const uint32_t p = wave_scan(v);
+    const uint32_t p = wave_scan_dpp(v);
     const uint64_t hit = __ballot(below + p > k);
     if (hit) {
       const int l = __builtin_ctzll(hit);
@@ -207,7 +196,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
       }
       if (!GROUPS) {  // group gg's bins
         const uint32_t hv = h((uint32_t)(gg << 6) + lane);
-        const uint32_t hp = wave_scan(hv);
+        const uint32_t hp = wave_scan_dpp(hv);
         const uint64_t bhit = __ballot(below + hp > k);
         if (bhit) {
           const int l = __builtin_ctzll(bhit);
@@ -221,7 +210,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
       }
       const int gi = gg + lane;
       const uint32_t sv = gi < NG ? g((uint32_t)gi) : 0u;
-      const uint32_t sp = wave_scan(sv);
+      const uint32_t sp = wave_scan_dpp(sv);
       const uint64_t ghit = __ballot(below + sp > k);
       if (ghit) {
         const int l = __builtin_ctzll(ghit);
@@ -235,7 +224,7 @@ __device__ void seek(const HF &h, const GF &g, uint32_t NB, uint32_t k, uint32_t
   }
   // the bins of group grp
   const uint32_t v = h((uint32_t)(grp << 6) + lane);
-  const uint32_t p = wave_scan(v);
+  const uint32_t p = wave_scan_dpp(v);
   const uint64_t hit = __ballot(below + p > k);
   const int l = hit ? __builtin_ctzll(hit) : 63;
   m = (uint32_t)(grp << 6) + l;
@@ -396,7 +385,7 @@ median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out
       uint32_t f = nf - 1, fb = 0;
       for (uint32_t j0 = 0; j0 < nf; j0 += 64) {
         const uint32_t v = j0 + lane < nf ? bcount(off + j0 + lane) : 0u;
-        const uint32_t p = wave_scan(v);
+        const uint32_t p = wave_scan_dpp(v);
         const uint64_t hit = __ballot(fb + p > kk);
         if (hit) {
           f = j0 + __builtin_ctzll(hit);
