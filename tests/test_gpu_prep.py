@@ -1,8 +1,9 @@
 """Image preprocessing on the device (lm_preprocess) against the CPU oracle
 (oracle/lm_prep_oracle.py, pinned in test_prep_oracle.py): threshold value,
 component counts and centres-derived sigma exact; the float64 blur within
-1e-12 relative (the Gaussian weights come from the host's exp, numpy's may
-differ in the last bit); the uint8 image equal to quantizing the device's own
+1e-12 relative (the Gaussian weights come from libm's exp; numpy's float64
+exp on AVX-512 hosts differs from it in the last bit for about 5% of the
+weight arguments); the uint8 image equal to quantizing the device's own
 blurred image on the host, and to the oracle's except at most one level at
 rounding boundaries.  The uint8 image left in device memory scores exactly
 like the host copy.  The median background (lm_median_background) equals
