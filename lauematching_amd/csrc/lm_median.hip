@@ -416,6 +416,104 @@ median_column_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out
   }
 }
 
+// The same running median with one wavefront per (column, row segment), for
+// histograms of at most 2^12 bins: every update, search and output of a row
+// is the wavefront's own, so rows need no barrier (LDS operations of one
+// wavefront complete in order), and a CU holds a dozen columns at once.
+// Lane l applies updates l, l + 64, ... of the 2 s per row.
+constexpr int kWaveSlots = (2 * kMaxMedianSize + 63) / 64;  // 8
+
+template <int NBITS>
+__global__ void __launch_bounds__(64)
+median_wave_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, int W, int H,
+                   int s, int seg_rows) {
+  constexpr uint32_t NB = 1u << NBITS;
+  static_assert(NBITS <= 12, "one wavefront per column for small histograms");
+  // no group sums: the search scans at most 64 groups of bins, and a median
+  // that moves far in one row is rare
+  constexpr bool GROUPS = false;
+  __shared__ uint32_t hist[NB / 2];
+  __shared__ uint32_t grpc[1];
+  __shared__ uint16_t rowtab[kMaxSegRows + 256];  // frames of at most 65535 rows
+  uint32_t b = blockIdx.x;
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);  // a band per XCD
+  const int x = (int)(b % (uint32_t)W);
+  const int y0 = (int)(b / (uint32_t)W) * seg_rows;
+  if (y0 >= H) return;
+  const int y1 = min(H, y0 + seg_rows);
+  const int lane = threadIdx.x;
+  const int lo = s >> 1;
+  const uint32_t k = (uint32_t)(s * s) >> 1;
+  const auto hcount = [&](uint32_t j) { return (hist[j >> 1] >> ((j & 1u) << 4)) & 0xffffu; };
+  const auto gcount = [&](uint32_t j) { return grpc[j]; };
+  const auto order = [] {  // LDS operations of the wavefront stay in program order
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  for (int i = lane; i < y1 - y0 + s; i += 64) rowtab[i] = (uint16_t)reflect(y0 - lo + i, H);
+  for (uint32_t i = lane; i < NB / 2; i += 64) hist[i] = 0;
+  order();
+  for (int dx0 = 0; dx0 < s; dx0 += 64) {
+    if (dx0 + lane >= s) continue;
+    const size_t col = (size_t)reflect(x - lo + dx0 + lane, W);
+    for (int dy = 0; dy < s; ++dy) {
+      const uint32_t v = in[(size_t)rowtab[dy] * W + col];
+      atomicAdd(&hist[v >> 1], 1u << ((v & 1u) << 4));
+    }
+  }
+  order();
+  uint32_t m = 0, below = 0;
+  seek<GROUPS>(hcount, gcount, NB, k, m, below);
+  if (lane == 0) out[(size_t)y0 * W + x] = m;
+
+  size_t col[kWaveSlots];
+  bool act[kWaveSlots], add[kWaveSlots];
+#pragma unroll
+  for (int u = 0; u < kWaveSlots; ++u) {
+    const int j = lane + 64 * u;
+    act[u] = j < 2 * s;
+    add[u] = j >= s;
+    col[u] = act[u] ? (size_t)reflect(x - lo + (add[u] ? j - s : j), W) : 0;
+  }
+  const int nslot = (2 * s + 63) / 64;  // slots any lane uses
+  const auto load_step = [&](int y, uint32_t (&v)[kWaveSlots]) {
+    const int rl = rowtab[y - 1 - y0], re = rowtab[y - 1 - y0 + s];
+#pragma unroll
+    for (int u = 0; u < kWaveSlots; ++u)
+      if (u < nslot && act[u]) v[u] = in[(size_t)(add[u] ? re : rl) * W + col[u]];
+  };
+  uint32_t ring[kDepth][kWaveSlots];
+#pragma unroll
+  for (int q = 0; q < kDepth; ++q)
+    if (y0 + 1 + q < y1) load_step(y0 + 1 + q, ring[q]);
+  for (int yb = y0 + 1; yb < y1; yb += kDepth) {
+#pragma unroll
+    for (int q = 0; q < kDepth; ++q) {
+      const int y = yb + q;
+      if (y >= y1) break;
+      int c = 0;
+#pragma unroll
+      for (int u = 0; u < kWaveSlots; ++u) {
+        if (u >= nslot) break;
+        const uint32_t bin = ring[q][u];
+        const bool a = act[u];
+        if (a) {
+          const uint32_t inc = 1u << ((bin & 1u) << 4);
+          atomicAdd(&hist[bin >> 1], add[u] ? inc : 0u - inc);
+        }
+        c += __popcll(__ballot(a && add[u] && bin < m));
+        c -= __popcll(__ballot(a && !add[u] && bin < m));
+      }
+      if (y + kDepth < y1) load_step(y + kDepth, ring[q]);
+      below = (uint32_t)((int)below + c);
+      order();
+      seek<GROUPS>(hcount, gcount, NB, k, m, below);
+      if (lane == 0) out[(size_t)y * W + x] = m;
+      order();
+    }
+  }
+}
+
 template <int NBITS, bool FINE>
 void launch_median(hipStream_t st, const uint32_t *in, uint32_t *out, int W, int H, int s,
                    int seg_rows, int nseg, int shift) {
@@ -485,9 +583,26 @@ int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int
   const int max_seg = std::max(1, H / std::max(2 * size, 64));
   const int nseg = std::max({1, std::min(want, max_seg), (H + kMaxSegRows - 1) / kMaxSegRows});
   const int seg_rows = (H + nseg - 1) / nseg;
+  // frames with at most 2^12 distinct values: one wavefront per column, with
+  // more segments (a CU holds a dozen such columns)
+  const bool wave = shift == 0 && R <= 12 && H <= 65535;
+  const int wseg = std::max(
+      {1, std::min((8192 + W - 1) / W, max_seg), (H + kMaxSegRows - 1) / kMaxSegRows});
+  const int wseg_rows = (H + wseg - 1) / wseg;
   uint32_t *src = ra, *dst = rb;
   for (int p = 0; p < passes; ++p) {
-    if (shift > 0)
+    if (wave) {
+      const dim3 grid((unsigned)(W * wseg));
+      if (R <= 8)
+        hipLaunchKernelGGL((median_wave_kernel<8>), grid, dim3(64), 0, st, src, dst, W, H, size,
+                           wseg_rows);
+      else if (R <= 10)
+        hipLaunchKernelGGL((median_wave_kernel<10>), grid, dim3(64), 0, st, src, dst, W, H, size,
+                           wseg_rows);
+      else
+        hipLaunchKernelGGL((median_wave_kernel<12>), grid, dim3(64), 0, st, src, dst, W, H, size,
+                           wseg_rows);
+    } else if (shift > 0)
       launch_median<15, true>(st, src, dst, W, H, size, seg_rows, nseg, shift);
     else if (R <= 8)
       launch_median<8, false>(st, src, dst, W, H, size, seg_rows, nseg, 0);
