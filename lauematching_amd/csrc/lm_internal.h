@@ -118,6 +118,10 @@ struct Scratch {
   uint64_t fwd_spill_cap = 0;
   Pred *pred = nullptr;          // device copy of the predicate
   Pred *pred_host = nullptr;     // pinned host copy
+  // the device copy holds *pred_host, uploaded on pred_stream (a call on the
+  // same stream with the same predicate skips the upload)
+  bool pred_current = false;
+  hipStream_t pred_stream = nullptr;
   // pinned host copies of the counters and of the first hit records of a
   // single-image call, fetched together after the launch sequence
   uint32_t *host_cnt = nullptr;

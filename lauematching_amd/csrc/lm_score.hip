@@ -1208,6 +1208,19 @@ int score_seq(lm_db *db, Scratch &sc, const void *d_img, bool f64, double scale,
 
 // The device predicate of a call: per image its thresholds, imScale and the
 // integer candidate tests the kernels apply; entries past nimg never pass.
+// Uploads the predicate unless the device copy already holds it and was
+// uploaded on this stream (stream order puts that upload before this call's
+// kernels): frames scored with the same thresholds and scale reuse it.
+static hipError_t upload_pred(Scratch &sc, const Pred &p, hipStream_t st) {
+  if (sc.pred_current && sc.pred_stream == st && memcmp(sc.pred_host, &p, sizeof(Pred)) == 0)
+    return hipSuccess;
+  memcpy(sc.pred_host, &p, sizeof(Pred));
+  const hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
+  sc.pred_current = e == hipSuccess;
+  sc.pred_stream = st;
+  return e;
+}
+
 static void build_pred(Pred &pred, int nimg, const double *im_scales, const int *min_spots,
                        const double *min_int) {
   memset(&pred, 0, sizeof pred);
@@ -1295,10 +1308,11 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   if (B > kMaxByteBatch && !nib) return halves();
   if (B >= 8 && B <= 64 && ((db->split_widths >> width_log2(B)) & 1) && nimg > B / 2)
     return halves(B / 2);
-  Pred &pred = *sc.pred_host;
-  build_pred(pred, nimg, im_scales, min_spots, min_int);
-  hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
+  Pred built;
+  build_pred(built, nimg, im_scales, min_spots, min_int);
+  hipError_t e = upload_pred(sc, built, st);
   if (e != hipSuccess) return hip_fail(e, "predicate upload");
+  const Pred &pred = *sc.pred_host;
   uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)B, 1u << 16);
   // kernel: 0 lane-per-orientation over the table, 1 single-image banded
   // copy, 2 batch banded copy, 3 4-bit bounds over the table, 4 batch
@@ -1642,10 +1656,11 @@ int score_images_fwd(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   int rc = scratch_init(sc);
   if (rc) return rc;
   const int B = batch_width(nimg);
-  Pred &pred = *sc.pred_host;
-  build_pred(pred, nimg, im_scales, min_spots, min_int);
-  hipError_t e = hipMemcpyAsync(sc.pred, sc.pred_host, sizeof(Pred), hipMemcpyHostToDevice, st);
+  Pred built;
+  build_pred(built, nimg, im_scales, min_spots, min_int);
+  hipError_t e = upload_pred(sc, built, st);
   if (e != hipSuccess) return hip_fail(e, "predicate upload");
+  const Pred &pred = *sc.pred_host;
   uint64_t capPer = std::max<uint64_t>(sc.hit_cap / (uint64_t)B, 1u << 16);
   if (B > 1) {
     rc = ensure_batch(sc, npx * B);
