@@ -821,3 +821,64 @@ def test_forward_without_room_for_kept_lists(ctx, guard):
     db.forward(api.LM_FWD_FP32_GUARD if guard else api.LM_FWD_FP64)
     np.testing.assert_array_equal(db.export_table(), want)
     db.close()
+
+
+def test_state_kept_between_calls_follows_the_stream(scored_case):
+    """A call reuses the predicate the previous call uploaded only on the
+    stream that uploaded it: single-image calls alternating between the
+    default stream and a side stream, thresholds changing and repeating, with
+    batch and on-the-fly calls and graph replays in between; every result
+    equals the oracle's."""
+    import torch
+
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img)
+    assert s == 1.0
+    perm = u8[np.random.default_rng(3).permutation(u8.size)]
+    dev = torch.from_numpy(np.stack([u8, perm])).to("cuda:0")
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    want = {}
+
+    def check_single(i, stream, ms, mi):
+        im = (u8, perm)[i]
+        key = (i, ms, mi)
+        if key not in want:
+            matched, ns, tot, nres = ob.compare(table, 64, im, p.NrPxX, 1.0, ms, mi)
+            want[key] = ob.collect(matched)
+        rows, mA = want[key]
+        h = db.score_dev(dev[i].data_ptr(), 1.0, ms, mi, stream)
+        np.testing.assert_array_equal(h.ids, rows)
+        np.testing.assert_array_equal(h.score, mA)
+
+    seq = [(0, 0, 5, 1000.0), (0, 0, 5, 1000.0), (1, side.cuda_stream, 5, 1000.0),
+           (0, 0, 5, 1000.0), (1, 0, 3, 500.0), (0, side.cuda_stream, 3, 500.0),
+           "batch", (0, 0, 3, 500.0), "fwd", (1, 0, 5, 1000.0), "graphs", (0, 0, 5, 1000.0),
+           (0, side.cuda_stream, 5, 1000.0), "nographs", (1, side.cuda_stream, 3, 500.0),
+           (1, 0, 3, 500.0)]
+    try:
+        for k, step in enumerate(seq):
+            _say(f"state between calls step {k}: {step}")
+            if step == "batch":
+                hs = db.score_batch_dev(dev.data_ptr(), 2, [1.0, 1.0], 4, 700.0)
+                for i, h in enumerate(hs):
+                    matched, ns, tot, nres = ob.compare(table, 64, (u8, perm)[i], p.NrPxX, 1.0,
+                                                        4, 700.0)
+                    rows, mA = ob.collect(matched)
+                    np.testing.assert_array_equal(h.ids, rows)
+            elif step == "fwd":
+                hs, _ = db.score_fwd_batch_dev(dev.data_ptr(), 2, [1.0, 1.0], 4, 700.0)
+                for i, h in enumerate(hs):
+                    matched, ns, tot, nres = ob.compare(table, 64, (u8, perm)[i], p.NrPxX, 1.0,
+                                                        4, 700.0)
+                    rows, mA = ob.collect(matched)
+                    np.testing.assert_array_equal(h.ids, rows)
+            elif step == "graphs":
+                db.set_use_graphs(True)
+            elif step == "nographs":
+                db.set_use_graphs(False)
+            else:
+                check_single(*step)
+            torch.cuda.synchronize()
+    finally:
+        db.set_use_graphs(False)
