@@ -46,6 +46,27 @@ unsigned blocks_for(uint64_t n) {
   return (unsigned)std::min<uint64_t>((n + kThreads - 1) / kThreads, 1u << 20);
 }
 
+// grid of the reductions: grid-stride loops, one atomic per block (one
+// address taking an atomic per wave of a 4M-pixel image serializes for
+// hundreds of microseconds)
+constexpr unsigned kReduceBlocks = 1024;
+unsigned reduce_blocks(uint64_t n) { return std::min(blocks_for(n), kReduceBlocks); }
+
+// sum / maximum over a block of kThreads (every thread gets the result)
+template <typename T, typename Op>
+__device__ T block_reduce(T v, Op op) {
+  __shared__ T part[kThreads / 64];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = op(v, __shfl_xor(v, d));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T r = part[0];
+#pragma unroll
+  for (int w = 1; w < kThreads / 64; ++w) r = op(r, part[w]);
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ uint64_t dbits(double x) {
   return (uint64_t)__double_as_longlong(x);
 }
@@ -60,20 +81,26 @@ __global__ void bgsub_kernel(const double *__restrict__ raw, const double *__res
   }
 }
 
-// 2. sort keys: mode 0 every pixel, 1 the nonzero pixels, 2 |v - med| of the
-// nonzero pixels (appended in any order; the sort orders them)
+// 2. sort keys of every pixel: mode 0 its value, 1 its value if nonzero,
+// 2 |v - med| if nonzero; zero pixels of modes 1 and 2 get the largest key,
+// so that after the sort the nonzero pixels' keys come first; *count gets
+// the number of nonzero pixels (one atomic per block)
 __global__ void keys_kernel(const double *__restrict__ v, uint64_t npx, int mode, double med,
                             uint64_t *__restrict__ keys, unsigned long long *__restrict__ count) {
+  unsigned long long c = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const double x = v[i];
     if (mode == 0) {
       keys[i] = dbits(x);
-    } else if (x > 0) {
-      const uint64_t pos = atomicAdd(count, 1ull);
-      keys[pos] = dbits(mode == 1 ? x : fabs(x - med));
+    } else {
+      c += x > 0;
+      keys[i] = x > 0 ? dbits(mode == 1 ? x : fabs(x - med)) : ~0ull;
     }
   }
+  if (mode == 0) return;  // (uniform: no block reduction)
+  c = block_reduce(c, [](unsigned long long a, unsigned long long b) { return a + b; });
+  if (threadIdx.x == 0 && c) atomicAdd(count, c);
 }
 
 // 3. threshold and uint16 cast (C conversion through a 64-bit integer)
@@ -87,9 +114,8 @@ __global__ void threshold_kernel(const double *__restrict__ v, uint64_t npx, dou
     c += x > t;
     u16[i] = x > t ? (uint16_t)(int64_t)x : (uint16_t)0;
   }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d);
-  if (__lane_id() == 0 && c) atomicAdd(above, c);
+  c = block_reduce(c, [](unsigned long long a, unsigned long long b) { return a + b; });
+  if (threadIdx.x == 0 && c) atomicAdd(above, c);
 }
 
 // 4. connected components
@@ -227,8 +253,8 @@ __global__ void max_kernel(const double *__restrict__ v, uint64_t npx,
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx;
        i += (uint64_t)gridDim.x * blockDim.x)
     m = std::max<unsigned long long>(m, dbits(v[i]));
-  for (int d = 32; d > 0; d >>= 1) m = std::max<unsigned long long>(m, __shfl_xor(m, d));
-  if ((threadIdx.x & 63) == 0) atomicMax(mx, m);
+  m = block_reduce(m, [](unsigned long long a, unsigned long long b) { return a > b ? a : b; });
+  if (threadIdx.x == 0) atomicMax(mx, m);
 }
 
 __global__ void quantize_kernel(const double *__restrict__ v, uint64_t npx,
@@ -292,11 +318,13 @@ hipError_t order_stats(hipStream_t st, Bufs &B, uint64_t *keys, uint64_t n, uint
   return e;
 }
 
-// median of n keys (numpy: the mean of the two middle values when n is even)
-hipError_t median_of(hipStream_t st, Bufs &B, uint64_t *keys, uint64_t n, double *med) {
+// median of the n smallest of nsort keys (numpy: the mean of the two middle
+// values when n is even)
+hipError_t median_of(hipStream_t st, Bufs &B, uint64_t *keys, uint64_t nsort, uint64_t n,
+                     double *med) {
   double a, b;
   const uint64_t h = n / 2;
-  const hipError_t e = order_stats(st, B, keys, n, n % 2 ? h : h - 1, h, &a, &b);
+  const hipError_t e = order_stats(st, B, keys, nsort, n % 2 ? h : h - 1, h, &a, &b);
   *med = n % 2 ? b : (a + b) / 2.0;
   return e;
 }
@@ -390,8 +418,8 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
     t = lerp_np(a, b, gamma);
   } else if (method == LM_THRESH_ADAPTIVE) {
     e = hipMemsetAsync(d_cnt, 0, 16, st);
-    hipLaunchKernelGGL(keys_kernel, dim3(g), dim3(kThreads), 0, st, d_v, npx, 1, 0.0, d_keys,
-                       d_cnt);
+    hipLaunchKernelGGL(keys_kernel, dim3(reduce_blocks(npx)), dim3(kThreads), 0, st, d_v, npx, 1,
+                       0.0, d_keys, d_cnt);
     unsigned long long nz = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&nz, d_cnt, 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -400,12 +428,12 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
       t = 1.0;
     } else {
       double med, m2;
-      e = median_of(st, B, d_keys, nz, &med);
+      e = median_of(st, B, d_keys, npx, nz, &med);
       if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, 16, st);
       if (e == hipSuccess) {
-        hipLaunchKernelGGL(keys_kernel, dim3(g), dim3(kThreads), 0, st, d_v, npx, 2, med,
-                           d_keys, d_cnt);
-        e = median_of(st, B, d_keys, nz, &m2);
+        hipLaunchKernelGGL(keys_kernel, dim3(reduce_blocks(npx)), dim3(kThreads), 0, st, d_v,
+                           npx, 2, med, d_keys, d_cnt);
+        e = median_of(st, B, d_keys, npx, nz, &m2);
       }
       if (e != hipSuccess) return hip_fail(e, "noise floor");
       const double mad = 1.4826 * m2;
@@ -418,8 +446,8 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   res.threshold = t;
   if (e == hipSuccess) e = hipMemsetAsync(d_cnt + 2, 0, 8, st);
   if (e != hipSuccess) return hip_fail(e, "threshold");
-  hipLaunchKernelGGL(threshold_kernel, dim3(g), dim3(kThreads), 0, st, d_v, npx, t, d_u16,
-                     d_cnt + 2);
+  hipLaunchKernelGGL(threshold_kernel, dim3(reduce_blocks(npx)), dim3(kThreads), 0, st, d_v, npx,
+                     t, d_u16, d_cnt + 2);
 
   // 4. components
   e = B.get(&d_p, npx);
@@ -526,7 +554,7 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   // 7. quantization
   e = hipMemsetAsync(d_cnt, 0, 16, st);
   if (e != hipSuccess) return hip_fail(e, "quantize");
-  hipLaunchKernelGGL(max_kernel, dim3(std::min(g, 4096u)), dim3(kThreads), 0, st, d_f, npx, d_cnt);
+  hipLaunchKernelGGL(max_kernel, dim3(reduce_blocks(npx)), dim3(kThreads), 0, st, d_f, npx, d_cnt);
   hipLaunchKernelGGL(quantize_kernel, dim3(g), dim3(kThreads), 0, st, d_f, npx, d_cnt, d_u8);
   (void)hipEventRecord(ev1, st);
   unsigned long long mxb = 0;
