@@ -271,6 +271,45 @@ void quantize(const T *img, uint64_t npx, uint8_t *out, double *im_scale) {
 
 using namespace lm;
 
+namespace lm {
+
+hipError_t pool_get(lm_ctx *ctx, DevPool &p, size_t slot, uint64_t bytes, void **out) {
+  if (slot >= p.ptr.size()) {
+    p.ptr.resize(slot + 1, nullptr);
+    p.cap.resize(slot + 1, 0);
+  }
+  bytes = std::max<uint64_t>((bytes + 255) & ~(uint64_t)255, 256);
+  if (p.cap[slot] < bytes) {
+    if (ctx->in_use - p.cap[slot] + bytes > ctx->budget) {
+      set_error("HBM budget exceeded by working buffers");
+      return hipErrorOutOfMemory;
+    }
+    if (p.ptr[slot]) (void)hipFree(p.ptr[slot]);  // the previous call has finished with it
+    ctx->in_use -= p.cap[slot];
+    p.ptr[slot] = nullptr;
+    p.cap[slot] = 0;
+    const hipError_t e = hipMalloc(&p.ptr[slot], bytes);
+    if (e != hipSuccess) {
+      p.ptr[slot] = nullptr;
+      return e;
+    }
+    p.cap[slot] = bytes;
+    ctx->in_use += bytes;
+  }
+  *out = p.ptr[slot];
+  return hipSuccess;
+}
+
+void pool_release(lm_ctx *ctx, DevPool &p) {
+  for (size_t i = 0; i < p.ptr.size(); ++i) {
+    if (p.ptr[i]) (void)hipFree(p.ptr[i]);
+    ctx->in_use -= p.cap[i];
+  }
+  p = DevPool{};
+}
+
+}  // namespace lm
+
 extern "C" {
 
 const char *lm_last_error(void) { return g_err.c_str(); }
@@ -317,6 +356,10 @@ int lm_ctx_create(int device, uint64_t budget, lm_ctx **out) {
 void lm_ctx_destroy(lm_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  pool_release(ctx, ctx->prep_pool);
+  pool_release(ctx, ctx->median_pool);
+  pool_release(ctx, ctx->io_pool);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -915,15 +958,16 @@ int lm_median_background(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const do
   hipStream_t st = ctx->stream;
   double *d_img = nullptr, *d_bg = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  hipError_t e = hipMalloc(&d_img, npx * 8);
-  if (e == hipSuccess) e = hipMalloc(&d_bg, npx * 8);
+  hipError_t e = pool_get(ctx, ctx->io_pool, 0, npx * 8, (void **)&d_img);
+  if (e == hipSuccess) e = pool_get(ctx, ctx->io_pool, 1, npx * 8, (void **)&d_bg);
   if (e == hipSuccess) e = hipEventCreate(&ev[0]);
   if (e == hipSuccess) e = hipEventCreate(&ev[1]);
   if (e == hipSuccess) e = hipMemcpyAsync(d_img, image, npx * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipEventRecord(ev[0], st);
   rc = e == hipSuccess ? LM_OK : hip_fail(e, "lm_median_background");
   if (rc == LM_OK)
-    rc = median_background_dev(st, nr_px_x, nr_px_y, d_img, filter_radius, median_passes, d_bg);
+    rc = median_background_dev(ctx, st, nr_px_x, nr_px_y, d_img, filter_radius, median_passes,
+                               d_bg);
   if (rc == LM_OK) {
     e = hipEventRecord(ev[1], st);
     if (e == hipSuccess)
@@ -936,8 +980,6 @@ int lm_median_background(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const do
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);
-  (void)hipFree(d_img);
-  (void)hipFree(d_bg);
   return rc;
 }
 

@@ -175,6 +175,15 @@ struct HostTable {
   uint64_t cap_tiles = 0, cap_entries = 0, bytes = 0;
 };
 
+// Device buffers a context keeps between calls of one kind (preprocessing,
+// median background): slot i holds the i-th buffer such a call asks for and
+// grows only when a call needs more, so repeated frames allocate nothing.
+// Callers hold the context's mutex; the bytes count against its HBM budget.
+struct DevPool {
+  std::vector<void *> ptr;
+  std::vector<uint64_t> cap;
+};
+
 }  // namespace lm
 
 struct lm_ctx {
@@ -183,6 +192,9 @@ struct lm_ctx {
   uint64_t budget = 0;
   uint64_t in_use = 0;
   std::mutex mu;
+  lm::DevPool prep_pool;    // lm_preprocess
+  lm::DevPool median_pool;  // the median background's ranks and sort storage
+  lm::DevPool io_pool;      // lm_median_background's image and result
 };
 
 struct lm_db {
@@ -343,12 +355,17 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
                const lm_prep_config &cfg, uint8_t *u8_host, uint8_t *d_u8, double *blurred,
                lm_prep_result &res);
 
+// lm_api.cpp: slot `slot` of pool p with at least `bytes` (ctx->mu held by
+// the caller); hipErrorOutOfMemory (message set) past the HBM budget.
+hipError_t pool_get(lm_ctx *ctx, DevPool &p, size_t slot, uint64_t bytes, void **out);
+void pool_release(lm_ctx *ctx, DevPool &p);
+
 // lm_median.hip: scipy.ndimage.median_filter(size=size), `passes` times, of
 // the W x H float64 device image d_img into d_bg (zeros when size <= 0 or
 // passes <= 0); synchronizes st before returning.
 constexpr int kMaxMedianSize = 255;
-int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int size,
-                          int passes, double *d_bg);
+int median_background_dev(lm_ctx *ctx, hipStream_t st, int W, int H, const double *d_img,
+                          int size, int passes, double *d_bg);
 
 // lm_merge.cpp
 void quat_from_matrix(const double m[9], double q[4]);

@@ -523,8 +523,8 @@ void launch_median(hipStream_t st, const uint32_t *in, uint32_t *out, int W, int
 
 }  // namespace
 
-int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int size,
-                          int passes, double *d_bg) {
+int median_background_dev(lm_ctx *ctx, hipStream_t st, int W, int H, const double *d_img,
+                          int size, int passes, double *d_bg) {
   const uint64_t npx = (uint64_t)W * H;
   if (W < 1 || H < 1 || npx > (1ull << 24) || size > kMaxMedianSize) {
     set_error("median background: image over 2^24 pixels or filter size over 255");
@@ -546,13 +546,10 @@ int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int
   if (e != hipSuccess) return hip_fail(e, "median background buffers");
   const auto up = [](uint64_t n) { return (n + 255) & ~(uint64_t)255; };
   const uint64_t b8 = up(npx * 8), b4 = up(npx * 4);
-  uint8_t *base = nullptr;
-  e = hipMalloc(&base, 3 * b8 + 5 * b4 + up(std::max(sort_bytes, scan_bytes)));
+  uint8_t *base = nullptr;  // kept by the context for the next call
+  e = pool_get(ctx, ctx->median_pool, 0, 3 * b8 + 5 * b4 + up(std::max(sort_bytes, scan_bytes)),
+               (void **)&base);
   if (e != hipSuccess) return hip_fail(e, "median background buffers");
-  struct Free {
-    void *p;
-    ~Free() { (void)hipFree(p); }
-  } free_base{base};
   uint64_t *keys = (uint64_t *)base, *skeys = (uint64_t *)(base + b8);
   double *table = (double *)(base + 2 * b8);
   uint32_t *idx = (uint32_t *)(base + 3 * b8), *sidx = (uint32_t *)(base + 3 * b8 + b4);
@@ -617,7 +614,7 @@ int median_background_dev(hipStream_t st, int W, int H, const double *d_img, int
     std::swap(src, dst);
   }
   hipLaunchKernelGGL(unrank_kernel, dim3(g), dim3(kThreads), 0, st, src, table, D, npx, d_bg);
-  e = hipStreamSynchronize(st);  // the buffers are freed on return
+  e = hipStreamSynchronize(st);  // the buffers are reused by the next call
   return e == hipSuccess ? LM_OK : hip_fail(e, "median background");
 }
 

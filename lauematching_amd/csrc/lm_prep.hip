@@ -281,17 +281,16 @@ double lerp_np(double a, double b, double t) {
   return t >= 0.5 ? b - d * (1 - t) : a + d * t;
 }
 
-// Device buffers of one call, freed on every exit.
+// Device buffers of one call: the i-th buffer a call asks for is slot i of
+// the context's preprocessing pool (kept and reused by the next call).
 struct Bufs {
-  std::vector<void *> ptrs;
-  ~Bufs() {
-    for (void *p : ptrs) (void)hipFree(p);
-  }
+  lm_ctx *ctx;
+  size_t next = 0;
   template <typename T>
   hipError_t get(T **p, uint64_t count) {
     void *q = nullptr;
-    const hipError_t e = hipMalloc(&q, std::max<uint64_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) ptrs.push_back(q);
+    const hipError_t e =
+        pool_get(ctx, ctx->prep_pool, next++, std::max<uint64_t>(count, 1) * sizeof(T), &q);
     *p = (T *)q;
     return e;
   }
@@ -353,7 +352,7 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   hipStream_t st = ctx->stream;
   const uint64_t npx = (uint64_t)W * H;
   std::memset(&res, 0, sizeof res);
-  Bufs B;
+  Bufs B{ctx};
   double *d_raw = nullptr, *d_bg = nullptr, *d_v = nullptr, *d_f = nullptr, *d_t = nullptr;
   uint16_t *d_u16 = nullptr;
   int32_t *d_p = nullptr, *d_cid = nullptr;
@@ -389,8 +388,8 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
   if (e != hipSuccess) return hip_fail(e, "preprocess upload");
   (void)hipEventRecord(ev0, st);
   if (median_bg) {  // no background given: compute_background (preprocess.py:255-266)
-    const int rc = median_background_dev(st, W, H, d_raw, cfg.filter_radius, cfg.median_passes,
-                                         d_bg);
+    const int rc = median_background_dev(ctx, st, W, H, d_raw, cfg.filter_radius,
+                                         cfg.median_passes, d_bg);
     if (rc != LM_OK) return rc;
   }
   hipLaunchKernelGGL(bgsub_kernel, dim3(g), dim3(kThreads), 0, st, d_raw, d_bg, npx, d_v);
