@@ -26,7 +26,16 @@ static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
 
+// set by pool_get when a buffer would pass the context's HBM budget: the
+// hipErrorOutOfMemory it returns is then reported as LM_ERR_BUDGET
+thread_local bool g_budget_fail = false;
+
 int hip_fail(hipError_t e, const char *what) {
+  if (e == hipErrorOutOfMemory && g_budget_fail) {
+    g_budget_fail = false;
+    g_err = std::string(what) + ": HBM budget exceeded by working buffers";
+    return LM_ERR_BUDGET;
+  }
   g_err = std::string(what) + ": " + hipGetErrorString(e);
   return LM_ERR_HIP;
 }
@@ -282,6 +291,7 @@ hipError_t pool_get(lm_ctx *ctx, DevPool &p, size_t slot, uint64_t bytes, void *
   if (p.cap[slot] < bytes) {
     if (ctx->in_use - p.cap[slot] + bytes > ctx->budget) {
       set_error("HBM budget exceeded by working buffers");
+      g_budget_fail = true;
       return hipErrorOutOfMemory;
     }
     if (p.ptr[slot]) (void)hipFree(p.ptr[slot]);  // the previous call has finished with it
@@ -939,8 +949,14 @@ int lm_preprocess(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *r
   int rc = check_ctx(ctx);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  return preprocess(ctx, nr_px_x, nr_px_y, raw, background, *cfg, u8_host, (uint8_t *)d_u8,
-                    blurred, *res);
+  const int r = preprocess(ctx, nr_px_x, nr_px_y, raw, background, *cfg, u8_host,
+                           (uint8_t *)d_u8, blurred, *res);
+  if (r == LM_ERR_BUDGET) {  // a frame too large for the budget keeps nothing
+    (void)hipStreamSynchronize(ctx->stream);
+    pool_release(ctx, ctx->prep_pool);
+    pool_release(ctx, ctx->median_pool);
+  }
+  return r;
 }
 
 int lm_median_background(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const double *image,
@@ -980,6 +996,11 @@ int lm_median_background(lm_ctx *ctx, int32_t nr_px_x, int32_t nr_px_y, const do
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);
+  if (rc == LM_ERR_BUDGET) {  // a frame too large for the budget keeps nothing
+    (void)hipStreamSynchronize(st);
+    pool_release(ctx, ctx->io_pool);
+    pool_release(ctx, ctx->median_pool);
+  }
   return rc;
 }
 

@@ -356,7 +356,8 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
                lm_prep_result &res);
 
 // lm_api.cpp: slot `slot` of pool p with at least `bytes` (ctx->mu held by
-// the caller); hipErrorOutOfMemory (message set) past the HBM budget.
+// the caller); past the HBM budget hipErrorOutOfMemory, which hip_fail then
+// reports as LM_ERR_BUDGET.
 hipError_t pool_get(lm_ctx *ctx, DevPool &p, size_t slot, uint64_t bytes, void **out);
 void pool_release(lm_ctx *ctx, DevPool &p);
 

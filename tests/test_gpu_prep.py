@@ -225,3 +225,28 @@ def test_preprocess_computes_the_median_background(ctx):
     assert info["n_components"] == winfo["n_components"]
     assert info["n_kept"] == winfo["n_kept"]
     np.testing.assert_allclose(blurred, want, rtol=1e-12, atol=1e-12 * max(1.0, want.max()))
+
+
+def test_working_buffers_count_against_the_budget():
+    """The buffers a context keeps for preprocessing and the median background
+    count against its HBM budget: a frame whose buffers do not fit fails with
+    LM_ERR_BUDGET, a smaller one runs and leaves them in use for the next."""
+    small = api.Context(0, hbm_budget=64 << 20)
+    try:
+        big, _ = _raw(2048, 2048, 100, seed=3)
+        _say("preprocess over a 64 MiB budget")
+        with pytest.raises(_lib.LaueMatchError) as err:
+            api.preprocess(small, big, None, "fixed", 100.0)
+        assert err.value.code == -4  # LM_ERR_BUDGET
+        with pytest.raises(_lib.LaueMatchError) as err:
+            api.median_background(small, big, 31, 1)
+        assert err.value.code == -4
+        raw, bg = _raw(256, 256, 20, seed=4)
+        u8, scale, info, blurred = api.preprocess(small, raw, bg, "adaptive", want_blurred=True)
+        want, winfo = po.preprocess(raw, bg, "adaptive")
+        assert info["threshold"] == winfo["threshold"]
+        assert small.bytes_in_use() > 0
+        np.testing.assert_array_equal(api.median_background(small, raw, 31, 1)[0],
+                                      po.median_background(raw, 31, 1))
+    finally:
+        small.close()
