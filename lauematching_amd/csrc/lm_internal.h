@@ -365,6 +365,11 @@ void pool_release(lm_ctx *ctx, DevPool &p);
 // the W x H float64 device image d_img into d_bg (zeros when size <= 0 or
 // passes <= 0); synchronizes st before returning.
 constexpr int kMaxMedianSize = 255;
+// Bits [lo, hi) in which n 64-bit device keys differ (hi = 0 when all are
+// equal), so a radix sort can skip the others; d_acc: 8 bytes of device
+// memory; synchronizes st.
+hipError_t key_bit_range(hipStream_t st, const uint64_t *keys, uint64_t n,
+                         unsigned long long *d_acc, int *lo, int *hi);
 int median_background_dev(lm_ctx *ctx, hipStream_t st, int W, int H, const double *d_img,
                           int size, int passes, double *d_bg);
 

@@ -100,6 +100,26 @@ __global__ void unrank_kernel(const uint32_t *__restrict__ rank, const double *_
     out[i] = table[min(rank[i], D - 1)];  // ranks are < D; the clamp keeps reads in the table
 }
 
+// OR of key ^ keys[0] over the keys (one atomic per block): the bits in
+// which they differ
+__global__ void key_diff_kernel(const uint64_t *__restrict__ keys, uint64_t n,
+                                unsigned long long *__restrict__ acc) {
+  __shared__ unsigned long long part[kThreads / 64];
+  const uint64_t k0 = keys[0];
+  unsigned long long d = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    d |= keys[i] ^ k0;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) d |= __shfl_xor(d, s);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) d |= part[w];
+    if (d) atomicOr(acc, d);
+  }
+}
+
 // scipy's 'reflect' extension: half-sample symmetric, period 2n
 __device__ __forceinline__ int reflect(int i, int n) {
   const int p = 2 * n;
@@ -523,6 +543,31 @@ void launch_median(hipStream_t st, const uint32_t *in, uint32_t *out, int W, int
 
 }  // namespace
 
+hipError_t key_bit_range(hipStream_t st, const uint64_t *keys, uint64_t n,
+                         unsigned long long *d_acc, int *lo, int *hi) {
+  *lo = 0;
+  *hi = 64;
+  if (n < 2) {
+    *hi = 0;
+    return hipSuccess;
+  }
+  hipError_t e = hipMemsetAsync(d_acc, 0, 8, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(key_diff_kernel, dim3(std::min(grid_for(n), 1024u)), dim3(kThreads), 0, st,
+                     keys, n, d_acc);
+  unsigned long long d = 0;
+  e = hipMemcpyAsync(&d, d_acc, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  if (!d) {
+    *hi = 0;  // all equal
+    return hipSuccess;
+  }
+  *lo = __builtin_ctzll(d);
+  *hi = 64 - __builtin_clzll(d);
+  return hipSuccess;
+}
+
 int median_background_dev(lm_ctx *ctx, hipStream_t st, int W, int H, const double *d_img,
                           int size, int passes, double *d_bg) {
   const uint64_t npx = (uint64_t)W * H;
@@ -547,8 +592,8 @@ int median_background_dev(lm_ctx *ctx, hipStream_t st, int W, int H, const doubl
   const auto up = [](uint64_t n) { return (n + 255) & ~(uint64_t)255; };
   const uint64_t b8 = up(npx * 8), b4 = up(npx * 4);
   uint8_t *base = nullptr;  // kept by the context for the next call
-  e = pool_get(ctx, ctx->median_pool, 0, 3 * b8 + 5 * b4 + up(std::max(sort_bytes, scan_bytes)),
-               (void **)&base);
+  e = pool_get(ctx, ctx->median_pool, 0,
+               3 * b8 + 5 * b4 + up(std::max(sort_bytes, scan_bytes)) + 256, (void **)&base);
   if (e != hipSuccess) return hip_fail(e, "median background buffers");
   uint64_t *keys = (uint64_t *)base, *skeys = (uint64_t *)(base + b8);
   double *table = (double *)(base + 2 * b8);
@@ -556,10 +601,18 @@ int median_background_dev(lm_ctx *ctx, hipStream_t st, int W, int H, const doubl
   uint32_t *scan = (uint32_t *)(base + 3 * b8 + 2 * b4);
   uint32_t *ra = (uint32_t *)(base + 3 * b8 + 3 * b4), *rb = (uint32_t *)(base + 3 * b8 + 4 * b4);
   void *tmp = base + 3 * b8 + 5 * b4;
+  unsigned long long *acc =
+      (unsigned long long *)(base + 3 * b8 + 5 * b4 + up(std::max(sort_bytes, scan_bytes)));
   const unsigned g = grid_for(npx);
   hipLaunchKernelGGL(key_kernel, dim3(g), dim3(kThreads), 0, st, d_img, npx, keys, idx);
-  e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, skeys, idx, sidx, (int)npx, 0, 64,
-                                         st);
+  // sort only the key bits that differ (integer counts leave the low
+  // mantissa bits zero: about half the radix passes)
+  int lo = 0, hi = 64;
+  e = key_bit_range(st, keys, npx, acc, &lo, &hi);
+  if (e == hipSuccess && hi == 0) lo = 0, hi = 1;  // all keys equal: one pass keeps them
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, skeys, idx, sidx, (int)npx, lo,
+                                           hi, st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(head_kernel, dim3(g), dim3(kThreads), 0, st, skeys, npx, ra);
     e = hipcub::DeviceScan::InclusiveSum(tmp, scan_bytes, ra, scan, (int)npx, st);

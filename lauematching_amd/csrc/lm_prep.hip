@@ -296,18 +296,24 @@ struct Bufs {
   }
 };
 
-// Order statistics k0 and k1 (0-based) of the keys, after sorting them.
+// Order statistics k0 and k1 (0-based) of the keys, after sorting them on
+// the bits in which they differ.
 hipError_t order_stats(hipStream_t st, Bufs &B, uint64_t *keys, uint64_t n, uint64_t k0,
                        uint64_t k1, double *v0, double *v1) {
   uint64_t *sorted = nullptr;
+  unsigned long long *acc = nullptr;
   hipError_t e = B.get(&sorted, n);
+  if (e == hipSuccess) e = B.get(&acc, 1);
+  int lo = 0, hi = 64;
+  if (e == hipSuccess) e = key_bit_range(st, keys, n, acc, &lo, &hi);
+  if (e == hipSuccess && hi == 0) lo = 0, hi = 1;  // all keys equal
   size_t tmp_bytes = 0;
   if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, sorted, (int)n, 0, 64, st);
+    e = hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, sorted, (int)n, lo, hi, st);
   void *tmp = nullptr;
   if (e == hipSuccess) e = B.get((uint8_t **)&tmp, tmp_bytes);
   if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, keys, sorted, (int)n, 0, 64, st);
+    e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, keys, sorted, (int)n, lo, hi, st);
   uint64_t b0 = 0, b1 = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&b0, sorted + k0, 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipMemcpyAsync(&b1, sorted + k1, 8, hipMemcpyDeviceToHost, st);
