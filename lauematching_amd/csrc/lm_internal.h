@@ -125,6 +125,7 @@ struct Scratch {
   // pinned host copies of the counters and of the first hit records of a
   // single-image call, fetched together after the launch sequence
   uint32_t *host_cnt = nullptr;
+  uint32_t *host_cnt_dev = nullptr;  // its device address (kernels write results there)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // around the compare kernel alone (eager single-image banded launches)
   hipEvent_t ew0 = nullptr, ew1 = nullptr;

@@ -818,6 +818,26 @@ __global__ void exact_hits_kernel(const uint16_t *__restrict__ counts,
   }
 }
 
+// A call's results for the host, written by the device straight into the
+// pinned buffer (its device address `out`): the counters and, for one image,
+// the first min(hits, cap) hit records and, with `exact`, their float64 sums.
+// One stream wait then replaces the copies.
+__global__ void publish_kernel(const uint32_t *__restrict__ counters,
+                               const uint64_t *__restrict__ hits, const double *__restrict__ vals,
+                               uint64_t cap, int single, int exact, uint32_t *__restrict__ out) {
+  for (int i = threadIdx.x; i < kCounterWords; i += blockDim.x) out[i] = counters[i];
+  if (single) {
+    const uint64_t k = min((uint64_t)counters[0], cap);
+    uint64_t *pre = (uint64_t *)(out + kCounterWords);
+    for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) pre[i] = hits[i];
+    if (exact) {
+      double *hv = (double *)(pre + kHitPrefix);
+      for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) hv[i] = vals[i];
+    }
+  }
+  __threadfence_system();
+}
+
 // Fill kernel for the launch counters (kept as a kernel so that a captured
 // launch sequence holds no memset node).
 __global__ void zero_kernel(uint32_t *__restrict__ p, int n) {
@@ -1061,6 +1081,12 @@ int scratch_init(Scratch &s) {
   if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipHostMalloc(&s.host_cnt, kCounterWords * 4 + kHitPrefix * 16, hipHostMallocDefault);
+  // (without a device address the results come back by copies)
+  if (e == hipSuccess && hipHostGetDevicePointer((void **)&s.host_cnt_dev, s.host_cnt, 0) !=
+                             hipSuccess) {
+    (void)hipGetLastError();
+    s.host_cnt_dev = nullptr;
+  }
   if (e == hipSuccess) e = hipEventCreate(&s.ev0);
   if (e == hipSuccess) e = hipEventCreate(&s.ev1);
   if (e == hipSuccess) e = hipEventCreate(&s.ew0);
@@ -1460,13 +1486,21 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
     if (rc) return rc;
     (void)hipEventRecord(sc.ev1, st);
     // the counters and, for one image, the first hit records in one wait
-    e = hipMemcpyAsync(sc.host_cnt, sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && nimg == 1)
-      e = hipMemcpyAsync(host_pre, sc.hits, std::min(capPer, kHitPrefix) * 8,
-                         hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && dev_exact)
-      e = hipMemcpyAsync(host_vals, sc.vals, std::min(capPer, kHitPrefix) * 8,
-                         hipMemcpyDeviceToHost, st);
+    if (sc.host_cnt_dev) {
+      hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(256), 0, st, sc.counters, sc.hits,
+                         sc.vals, std::min(capPer, kHitPrefix), nimg == 1 ? 1 : 0,
+                         dev_exact ? 1 : 0, sc.host_cnt_dev);
+      e = hipGetLastError();
+    } else {
+      e = hipMemcpyAsync(sc.host_cnt, sc.counters, kCounterWords * 4, hipMemcpyDeviceToHost,
+                         st);
+      if (e == hipSuccess && nimg == 1)
+        e = hipMemcpyAsync(host_pre, sc.hits, std::min(capPer, kHitPrefix) * 8,
+                           hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess && dev_exact)
+        e = hipMemcpyAsync(host_vals, sc.vals, std::min(capPer, kHitPrefix) * 8,
+                           hipMemcpyDeviceToHost, st);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "score kernel");
     memcpy(cnt.data(), sc.host_cnt, kCounterWords * 4);
