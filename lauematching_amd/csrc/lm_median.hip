@@ -12,28 +12,28 @@
 // sizes).  This file computes exactly that.
 //
 // 1. Dense ranks.  The float64 values are sorted (radix sort of an
-//    order-preserving 64-bit key, with the pixel index as payload); equal
-//    values share one rank, so the image becomes a uint32 rank image over
-//    D distinct values plus a table of those values.  The median of the
-//    ranks is the rank of the median, and a pass's output holds ranks of
-//    the same table, so every pass works on ranks and the table maps the
-//    last one back to float64.
-// 2. The running median.  One workgroup per output column (or column
-//    segment) slides the window down: per row it removes the leaving row's
-//    filter_radius ranks from a histogram in LDS, adds the entering row's,
-//    and moves the median bin up or down from where it was (Huang's running
-//    median), one wavefront scanning 64 bins per step with prefix sums.
-//    The histogram holds 16-bit counts, two per LDS word, over the rank's
-//    top bits: all of it when D <= 65536 (the integer-count frames of an
-//    X-ray detector), else its top 16 bits, and then the rank's low bits
-//    come from a second histogram of the window's elements in the current
-//    coarse bin, updated with the window and rebuilt from it when the
-//    median moves to another coarse bin.
+//    order-preserving 64-bit key, on the key bits that differ, with the
+//    pixel index as payload); equal values share one rank, so the image
+//    becomes a uint32 rank image over D distinct values plus a table of
+//    those values.  The median of the ranks is the rank of the median, and
+//    a pass's output holds ranks of the same table, so every pass works on
+//    ranks and the table maps the last one back to float64.
+// 2. The running median (Huang's): each column (or column segment) slides
+//    its window down; per row the leaving row's filter_radius ranks leave an
+//    LDS histogram of 16-bit counts (two per word), the entering row's
+//    enter, and the median bin moves from where it was by wavefront scans
+//    of 64 bins (DPP prefix sums; group sums of 64 bins from 4096 bins on).
+//    D <= 4096 (integer-count detector frames): one wavefront per column,
+//    no barrier (median_wave_kernel).  D <= 65536: one 256-thread workgroup
+//    per column, the histogram over all ranks.  D > 65536: the histogram
+//    over the rank's top 15 bits, and full-resolution counts of 32768 ranks
+//    around the median (the band) give the low bits; the band is rebuilt
+//    from the window only when the median's coarse bin leaves it.
 //
 // Limits: filter_radius <= 255 (a window of at most 65025 elements keeps
-// every count in 16 bits) and at most 2^24 pixels (so the low bits are at
-// most 8).  NaN values are not ordered (the reference's filters leave
-// them undefined as well).
+// every count in 16 bits) and at most 2^24 pixels (so ranks have at most 24
+// bits, 9 of them from the band).  NaN values are not ordered (the
+// reference's filters leave them undefined as well).
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
