@@ -835,7 +835,6 @@ __global__ void publish_kernel(const uint32_t *__restrict__ counters,
       for (uint64_t i = threadIdx.x; i < k; i += blockDim.x) hv[i] = vals[i];
     }
   }
-  __threadfence_system();
 }
 
 // Fill kernel for the launch counters (kept as a kernel so that a captured
@@ -1080,7 +1079,10 @@ int scratch_init(Scratch &s) {
   if (e == hipSuccess) e = hipMalloc(&s.pred, sizeof(Pred));
   if (e == hipSuccess) e = hipHostMalloc(&s.pred_host, sizeof(Pred), hipHostMallocDefault);
   if (e == hipSuccess)
-    e = hipHostMalloc(&s.host_cnt, kCounterWords * 4 + kHitPrefix * 16, hipHostMallocDefault);
+    // coherent: the device's writes (publish_kernel) reach host memory
+    // without passing through the device caches
+    e = hipHostMalloc(&s.host_cnt, kCounterWords * 4 + kHitPrefix * 16,
+                      hipHostMallocCoherent | hipHostMallocMapped);
   // (without a device address the results come back by copies)
   if (e == hipSuccess && hipHostGetDevicePointer((void **)&s.host_cnt_dev, s.host_cnt, 0) !=
                              hipSuccess) {
