@@ -201,8 +201,9 @@ uint64_t lm_db_table_spots(const lm_db *db);
 #define LM_OPT_BATCH_SUM16 10
 /* LM_OPT_BATCH_SPLIT: bit k set (k = 3..6) = a batch of width 2^k (more than
  * 2^(k-1) images) is scored as two calls of at most 2^(k-1) images, when the
- * narrower width's layout serves two calls faster than one (the autotuner
- * sets it from its timings). */
+ * narrower width's layout serves two calls faster than one; bit 2 = 2..4
+ * images (width 4) are scored one at a time (the autotuner sets both from
+ * its timings). */
 #define LM_OPT_BATCH_SPLIT 11
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);

@@ -200,7 +200,7 @@ class OrientationDB:
 
     def set_batch_split(self, mask: int) -> None:
         """Bit k set (k = 3..6): batches of width 2**k are scored as two calls
-        of at most 2**(k-1) images."""
+        of at most 2**(k-1) images; bit 2: 2..4 images one at a time."""
         check(lib().lm_db_set_option(self._h, 11, int(mask)))
 
     def get_option(self, option: int) -> int:

@@ -634,8 +634,8 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     return LM_OK;
   }
   if (option == LM_OPT_BATCH_SPLIT) {
-    if (value < 0 || (value & ~(int64_t)0x78)) {
-      set_error("LM_OPT_BATCH_SPLIT is a mask of bits 3..6 (batches of 8..64 images)");
+    if (value < 0 || (value & ~(int64_t)0x7c)) {
+      set_error("LM_OPT_BATCH_SPLIT is a mask of bits 2..6 (batches of 4..64 images)");
       return LM_ERR_INVALID;
     }
     db->split_widths = (uint32_t)value;

@@ -221,7 +221,7 @@ struct lm_db {
   uint32_t group_widths = 0xc0; // bit log2(B): table batches of width B use grouped lanes
   uint32_t nib_widths = 0x180;  // bit log2(B): batches of width B (128, 256) use 4-bit bounds
   uint32_t batch16_widths = 0;  // bit log2(B): the batch copy read with 16-bit sums
-  uint32_t split_widths = 0;    // bit log2(B): scored as two calls of B/2 images
+  uint32_t split_widths = 0;    // bit log2(B): scored as two calls of B/2 images (B = 4: singles)
   uint64_t generation = 0;      // bumped whenever the table or its banded copy changes
   lm::Scratch scratch;          // buffers of direct (non-serving) calls
   lm::HostTable htable;         // the table when it does not fit the budget

@@ -1336,6 +1336,26 @@ int score_images(lm_db *db, Scratch &sc, int nimg, const uint8_t *d_imgs,
   if (B > kMaxByteBatch && !nib) return halves();
   if (B >= 8 && B <= 64 && ((db->split_widths >> width_log2(B)) & 1) && nimg > B / 2)
     return halves(B / 2);
+  // 2..4 images (width 4) one at a time, when the autotuner found single
+  // calls faster than the width-4 launch
+  if (B == 4 && ((db->split_widths >> 2) & 1)) {
+    double k = 0, w = 0;
+    for (int b = 0; b < nimg; ++b) {
+      const int r = score_images(db, sc, 1, d_imgs + (uint64_t)b * npx, im_scales + b,
+                                 min_spots + b, min_int + b, st, outs + b);
+      if (r) {
+        for (int c = 0; c < b; ++c) lm_hits_free(&outs[c]);
+        return r;
+      }
+      k += outs[b].kernel_ms;
+      w += outs[b].walk_ms;
+    }
+    for (int b = 0; b < nimg; ++b) {
+      outs[b].kernel_ms = k;
+      outs[b].walk_ms = w;
+    }
+    return LM_OK;
+  }
   Pred built;
   build_pred(built, nimg, im_scales, min_spots, min_int);
   hipError_t e = upload_pred(sc, built, st);

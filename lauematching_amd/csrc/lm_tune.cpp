@@ -231,6 +231,11 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   // a width served faster by two calls of half the images (the narrower
   // width's layout staying in L2 where the wider one's does not)
   uint32_t split = 0;
+  // width 4 (2..4 images) against four single-image calls
+  if (nw > 0 && rep->widths[0] == 4 && 4.0 * best1 < wms[0]) {
+    wms[0] = 4.0 * best1;
+    split |= 1u << 2;
+  }
   for (int w = 1; w < nw; ++w) {
     const int B = rep->widths[w];
     if (B < 8 || B > 64 || rep->widths[w - 1] * 2 != B) continue;

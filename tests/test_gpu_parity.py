@@ -641,10 +641,11 @@ def test_batch_sum16_copy_matches_oracle(scored_case, tb, nimg, low):
             np.testing.assert_array_equal(gg[i].nspots, ns[rows.astype(np.int64)])
 
 
-@pytest.mark.parametrize("nimg", [13, 40])
+@pytest.mark.parametrize("nimg", [2, 4, 13, 40])
 def test_batch_split_matches_oracle(scored_case, nimg):
-    """A width scored as two calls of half the width (LM_OPT_BATCH_SPLIT):
-    every image bit-exact against the oracle, per-image thresholds kept."""
+    """A width scored as two calls of half the width (LM_OPT_BATCH_SPLIT), and
+    2..4 images scored one at a time: every image bit-exact against the
+    oracle, per-image thresholds kept."""
     p, hk, O, grains, img, table, db = scored_case
     rng = np.random.default_rng(500 + nimg)
     base, _ = api.quantize_u8(img)
@@ -652,7 +653,7 @@ def test_batch_split_matches_oracle(scored_case, nimg):
     msp = [int(v) for v in rng.integers(3, 7, nimg)]
     mint = [float(v) for v in rng.uniform(300, 900, nimg)]
     _say(f"batch split nimg={nimg}")
-    db.set_batch_split(0x78)
+    db.set_batch_split(0x7C)
     try:
         got = db.score_batch_each(imgs, [1.0] * nimg, msp, mint)
     finally:
