@@ -444,16 +444,25 @@ def median_check(ctx, raw, reps=5, size=101):
     and as float64 (every value distinct), the pipeline with that background
     computed, and scipy.ndimage.median_filter (the reference's filter
     without diplib) timed on a 160x160 crop on one host core, with the
-    device's result on that crop compared to scipy's."""
+    device's result on that crop compared to scipy's; both full frames are
+    compared with the C running-median oracle (pinned against scipy)."""
     import scipy.ndimage as ndi
     from lauematching_amd import api
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
 
     counts = np.floor(raw)
     out = {"filter_radius": size, "median_passes": 1, "frame": "2048x2048"}
     for name, img in (("integer_counts", counts), ("float", raw)):
-        api.median_background(ctx, img, size, 1)
+        got, _ = api.median_background(ctx, img, size, 1)
         ms = [api.median_background(ctx, img, size, 1)[1] for _ in range(reps)]
         out[f"{name}_device_ms"] = float(np.median(ms))
+        t0 = time.perf_counter()
+        want = ob.median_background(img, size, 1)
+        out[f"{name}_oracle"] = {"equal_full_frame": bool(np.array_equal(got, want)),
+                                 "distinct_values": int(np.unique(img).size),
+                                 "oracle_s": time.perf_counter() - t0,
+                                 "oracle_threads": len(os.sched_getaffinity(0))}
     api.preprocess(ctx, counts, None, filter_radius=size)
     pms = [api.preprocess(ctx, counts, None, filter_radius=size)[2]["ms"] for _ in range(reps)]
     out["pipeline_with_median_device_ms"] = float(np.median(pms))
@@ -470,7 +479,8 @@ def median_check(ctx, raw, reps=5, size=101):
         f"counts), {out['float_device_ms']:.2f} ms (float), pipeline with it "
         f"{out['pipeline_with_median_device_ms']:.2f} ms; scipy {crop.size / cpu_s:.3g} px/s "
         f"on one core vs device {out['device_px_per_s']:.3g} px/s, crop equal "
-        f"{out['cpu_scipy']['device_equal']}")
+        f"{out['cpu_scipy']['device_equal']}, full frames equal to the oracle "
+        f"{out['integer_counts_oracle']['equal_full_frame']}/{out['float_oracle']['equal_full_frame']}")
     return out
 
 

@@ -7,7 +7,9 @@
 // pixels whose first row and column lie filter_radius // 2 before the output
 // pixel, the image extended by half-sample reflection (scipy's 'reflect',
 // ... c b a | a b c ... | c b a ..., periodic with period 2n when the window
-// is wider than the image), and as output the element of 0-based rank
+// is wider than the image: scipy's result while filter_radius < 8n on each
+// axis, past which scipy reads outside its line buffers), and as output the
+// element of 0-based rank
 // (filter_radius^2) // 2 of the window's values (no averaging for even
 // sizes).  This file computes exactly that.
 //

@@ -93,6 +93,13 @@ int lmo_merge(const double *orients, const uint64_t *rows, const double *scores,
               double maxAngle, int sg, int nthreads, uint64_t *best_row, double *best_score,
               int32_t *cluster_size);
 
+/* One pass of scipy.ndimage.median_filter(size=s), mode 'reflect', as
+ * compute_background calls it (laue_index/preprocess.py:59-63), on a frame of
+ * dense ranks rk[H][W] (0..D-1, increasing with the value): out = the rank
+ * of each window's element of 0-based order s*s/2.  s <= 255.  Returns 0,
+ * -1 on bad arguments, -2 when a buffer could not be allocated. */
+int lmo_median_ranks(const uint32_t *rk, int W, int H, int s, uint32_t D, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,8 @@ def lib():
                                   ctypes.POINTER(i32), vp, ctypes.POINTER(dbl)]
         L.lmo_calc_recip.argtypes = [vp, i32, vp]
         L.lmo_detector_rotT.argtypes = [vp, vp]
+        L.lmo_median_ranks.restype = i32
+        L.lmo_median_ranks.argtypes = [vp, i32, i32, i32, ctypes.c_uint32, vp]
         _LIB = L
     return _LIB
 
@@ -171,3 +173,23 @@ def merge(orients, rows, scores, max_angle, sg=225, threads=8):
     k = _merge_lib().lmo_merge(_p(o), _p(r), _p(s), nr, max_angle, sg, threads, _p(best),
                                _p(bs), _p(cs))
     return best[:k], bs[:k], cs[:k]
+
+
+def median_background(image, size, passes=1):
+    """compute_background's scipy branch (laue_index/preprocess.py:59-63)
+    through the C oracle: the frame's values ranked densely (np.unique),
+    lmo_median_ranks once per pass, ranks mapped back to values; zeros for
+    size <= 0 or passes <= 0 (:53-54)."""
+    a = np.asarray(image, dtype=np.float64)
+    if size <= 0 or passes <= 0:
+        return np.zeros_like(a)
+    H, W = a.shape
+    vals, inv = np.unique(a, return_inverse=True)
+    rk = np.ascontiguousarray(inv.reshape(H, W), dtype=np.uint32)
+    out = np.empty_like(rk)
+    for _ in range(passes):
+        rc = lib().lmo_median_ranks(_p(rk), W, H, int(size), int(vals.size), _p(out))
+        if rc:
+            raise RuntimeError(f"lmo_median_ranks failed ({rc})")
+        rk, out = out, rk
+    return vals[rk]

@@ -8,7 +8,8 @@ blurred image on the host, and to the oracle's except at most one level at
 rounding boundaries.  The uint8 image left in device memory scores exactly
 like the host copy.  The median background (lm_median_background) equals
 scipy.ndimage.median_filter, the reference's filter without diplib, pixel for
-pixel."""
+pixel, and on full detector frames the C running-median oracle (pinned
+against scipy in test_prep_oracle.py)."""
 import os
 import sys
 
@@ -20,6 +21,7 @@ from lauematching_amd import _lib, api, synth
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                 "oracle"))
 import lm_prep_oracle as po  # noqa: E402
+import oracle_bind as ob  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -211,6 +213,26 @@ def test_median_background_full_frame_sampled(ctx):
     for y, x in zip(ys, xs):
         win = pad[y:y + size, x:x + size].ravel()
         assert bg[y, x] == np.partition(win, k)[k], (y, x)
+
+
+@pytest.mark.parametrize("kind,n,size,passes", [("counts", 2048, 101, 1), ("float", 2048, 101, 1),
+                                                ("float", 4096, 31, 2)])
+def test_median_background_full_frame_equals_oracle(ctx, kind, n, size, passes):
+    """Detector frames at the reference's default filter (101, one pass) and
+    the largest frame (2^24 pixels) twice at 31: every pixel equal to the C
+    running-median oracle's.  Integer counts (a few thousand distinct values:
+    one wavefront per column) and float64 (every value distinct: 22- and
+    24-bit ranks, the coarse walk with its rank band)."""
+    img, _ = _raw(n, n, 3000 * (n // 2048) ** 2, seed=7 + n + size)
+    if kind == "counts":
+        img = np.floor(img)
+    else:
+        img += np.random.default_rng(n).random(img.shape)
+    _say(f"median background {kind} {n}x{n} size={size} passes={passes}")
+    bg, ms = api.median_background(ctx, img, size, passes)
+    _say(f"median background {kind} {n}x{n}: {ms:.2f} ms; oracle")
+    want = ob.median_background(img, size, passes)
+    np.testing.assert_array_equal(bg, want)
 
 
 def test_preprocess_computes_the_median_background(ctx):

@@ -4,7 +4,8 @@ restatement that is itself checked: 8-connectivity against a flood fill,
 centres of mass against explicit sums, the nearest-centre distance against
 all pairs, the blur's edge mode against an explicit padded correlation, the
 noise floor against a hand-computed case, the median background against a
-brute-force sort of every window and the properties an order statistic has."""
+brute-force sort of every window and the properties an order statistic has,
+and the C running-median oracle against scipy's median_filter."""
 import os
 import sys
 
@@ -14,6 +15,7 @@ import scipy.ndimage as ndi
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                 "oracle"))
 import lm_prep_oracle as po  # noqa: E402
+import oracle_bind as ob  # noqa: E402
 
 
 def _flood(mask):
@@ -148,3 +150,67 @@ def test_median_background_order_statistic_properties():
     step[3, 3] = 1e6
     np.testing.assert_array_equal(po.median_background(step, 3, 1), np.where(
         np.arange(15)[None, :] >= 8, 10.0, 0.0) * np.ones((15, 1)))
+
+
+def _median_frames(rng, H, W):
+    """Frames with many ties, Poisson counts, distinct negatives, and signed
+    integers holding both zeros."""
+    signed = np.round(rng.normal(0.0, 1.5, (H, W)))
+    z = signed == 0
+    signed[z] = np.where(rng.random(int(z.sum())) < 0.5, 0.0, -0.0)
+    return (rng.integers(0, 9, (H, W)).astype(np.float64),
+            rng.poisson(50.0, (H, W)).astype(np.float64),
+            rng.normal(0.0, 1.0, (H, W)), signed)
+
+
+def test_median_c_oracle_equals_scipy():
+    """lmo_median_ranks (the running-histogram oracle the device median is
+    checked against on full frames) equals scipy.ndimage.median_filter, the
+    reference's own call, pass by pass: the window-sort shapes, windows wider
+    than the frame up to 255 pixels, even sizes, several passes, and a frame
+    of 75000 distinct values (the coarse counts move across many bins).
+    scipy mirrors with period 2n only while size < 8n on each axis (from
+    size 8n on, its line buffers are read past the extension: on a 2 x 50
+    frame, size 16 already differs); windows that wide are checked against
+    the sort of every periodically mirrored window instead."""
+    rng = np.random.default_rng(21)
+    cases = [(H, W, s, 1) for H, W, s in MEDIAN_SHAPES] + [
+        (60, 45, 31, 1), (33, 70, 101, 2), (40, 32, 255, 1), (64, 96, 10, 3), (250, 300, 21, 1)]
+    for H, W, size, passes in cases:
+        assert size < 8 * min(H, W)
+        for img in _median_frames(rng, H, W):
+            want = po.median_background(img, size, passes)
+            np.testing.assert_array_equal(ob.median_background(img, size, passes), want,
+                                          err_msg=f"{H}x{W} size {size} passes {passes}")
+    for H, W, size in ((2, 2, 17), (3, 5, 31), (2, 50, 16), (30, 7, 101)):
+        for img in _median_frames(rng, H, W):
+            np.testing.assert_array_equal(ob.median_background(img, size, 1),
+                                          po.median_brute(img, size),
+                                          err_msg=f"{H}x{W} size {size}")
+    assert not ob.median_background(np.ones((5, 5)), 0, 1).any()
+    assert not ob.median_background(np.ones((5, 5)), 3, 0).any()
+
+
+def test_median_c_oracle_thread_count_invariant():
+    """Rows are independent: one thread and eight give the same frame."""
+    rng = np.random.default_rng(22)
+    img = rng.normal(0.0, 1.0, (97, 131))
+    vals, inv = np.unique(img, return_inverse=True)
+    rk = np.ascontiguousarray(inv.reshape(img.shape), dtype=np.uint32)
+    outs = []
+    for threads in ("1", "8"):
+        code = ("import sys, numpy as np; sys.path.insert(0, %r); import oracle_bind as ob; "
+                "rk = np.load(sys.argv[1]); out = np.empty_like(rk); "
+                "assert ob.lib().lmo_median_ranks(ob._p(rk), rk.shape[1], rk.shape[0], 23, "
+                "%d, ob._p(out)) == 0; np.save(sys.argv[2], out)"
+                % (os.path.dirname(os.path.abspath(__file__)), vals.size))
+        import subprocess
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            np.save(os.path.join(d, "rk.npy"), rk)
+            subprocess.run([sys.executable, "-c", code, os.path.join(d, "rk.npy"),
+                            os.path.join(d, "out.npy")], check=True,
+                           env={**os.environ, "OMP_NUM_THREADS": threads})
+            outs.append(np.load(os.path.join(d, "out.npy")))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(vals[outs[0]], po.median_background(img, 23, 1))
