@@ -242,10 +242,13 @@ int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_
                    int32_t reps, lm_tune_report *out);
 
 /* Build the resident spot table on the device (replaces the DoFwd=1 forward
- * simulation).  mode = LM_FWD_FP64 or LM_FWD_FP32_GUARD; stats may be NULL. */
+ * simulation: src/LaueMatchingCPU.c:591-667, in the GPU programs
+ * src/LaueMatchingGPU.cu:463-617 and src/LaueMatchingGPUStream.cu:836-952).
+ * mode = LM_FWD_FP64 or LM_FWD_FP32_GUARD; stats may be NULL. */
 int lm_db_forward(lm_db *db, int32_t mode, lm_fwd_stats *stats);
 
-/* Replace the table from ForwardFile records: count records of (1+2M) uint16
+/* Replace the table from ForwardFile records (the DoFwd=0 read,
+ * src/LaueMatchingCPU.c:446-486, 548-576): count records of (1+2M) uint16
  * (word 0 = spot count, then x,y pairs), orientation-major, covering rows
  * [0, count) with count == lm_db_size(db).  Counts > M or pixels outside the
  * detector are rejected with LM_ERR_INVALID. */
@@ -258,7 +261,7 @@ int lm_db_import_table(lm_db *db, const uint16_t *records, uint64_t count);
 int lm_db_table_checksum(lm_db *db, uint64_t *sum);
 
 /* Write rows [first, first+count) as ForwardFile records (zero padded),
- * byte-identical to the reference writer. */
+ * byte-identical to the reference writer (src/LaueMatchingCPU.c:693-726). */
 int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out);
 
 /* ---- scoring -------------------------------------------------------------------- */
@@ -266,7 +269,10 @@ int lm_db_export_table(lm_db *db, uint64_t first, uint64_t count, uint16_t *out)
 /* Score a host uint8 image (nrPxX x nrPxY of the DB geometry, row-major,
  * X fastest) with the reference predicate nspots >= min_nr_spots &&
  * totint >= min_intensity, totint = sequential float64 sum of raw*im_scale
- * in table order.  Results in ascending orientation order. */
+ * in table order (the DoFwd=0 compare, src/LaueMatchingCPU.c:668-689; the
+ * GPU programs' compare kernel, src/LaueMatchingGPU.cu:45-81, 722-775).
+ * Results in ascending orientation order, as the reference collects them
+ * (src/LaueMatchingCPU.c:772-788). */
 int lm_score(lm_db *db, const uint8_t *img_u8, double im_scale, int32_t min_nr_spots,
              double min_intensity, lm_hits *out);
 
@@ -300,7 +306,10 @@ int lm_score_dev_noalloc(lm_db *db, const uint8_t *d_img_u8, double im_scale,
                          uint64_t *count, uint64_t *nr_results);
 
 /* Score nimg (1..256) host images against the DB in one pass over the table.
- * outs[i] receives image i's hits, identical to lm_score on that image. */
+ * outs[i] receives image i's hits, identical to lm_score on that image.  No
+ * reference counterpart: the reference programs score one image per pass
+ * over the table (src/LaueMatchingGPU.cu:660-749,
+ * src/LaueMatchingGPUStream.cu:1167-1257). */
 int lm_score_batch(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
                    const double *im_scales, int32_t min_nr_spots, double min_intensity,
                    lm_hits *outs);
@@ -314,7 +323,7 @@ int lm_score_batch_each(lm_db *db, int32_t nimg, const uint8_t *const *imgs_u8,
                         const double *min_intensity, lm_hits *outs);
 
 /* Device-resident batch: d_imgs points to nimg device images laid out back to
- * back (nimg * nrPxX * nrPxY bytes). */
+ * back (nimg * nrPxX * nrPxY bytes).  No reference counterpart. */
 int lm_score_batch_dev(lm_db *db, int32_t nimg, const uint8_t *d_imgs, const double *im_scales,
                        int32_t min_nr_spots, double min_intensity, void *stream, lm_hits *outs);
 
