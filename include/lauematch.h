@@ -205,6 +205,12 @@ uint64_t lm_db_table_spots(const lm_db *db);
  * images (width 4) are scored one at a time (the autotuner sets both from
  * its timings). */
 #define LM_OPT_BATCH_SPLIT 11
+/* LM_OPT_BANDED_SPLIT: the LDS-staged walk's work items are pieces of the
+ * entries of one image chunk: 2^value balanced pieces per chunk (0..3), or -1
+ * (default) for the fewest that bring the mean piece to at most 512 entries
+ * (small detectors put thousands of a super-tile's entries in each chunk).
+ * Reading it gives the split of the current copy. */
+#define LM_OPT_BANDED_SPLIT 12
 int lm_db_set_option(lm_db *db, int32_t option, int64_t value);
 int lm_db_get_option(lm_db *db, int32_t option, int64_t *value);
 
@@ -220,7 +226,8 @@ typedef struct lm_tune_report {
   int32_t widths[7];          /* batch widths timed (4 .. max_batch <= 256)      */
   double ms_single_table;     /* one image: orientation-major table               */
   double ms_single[4][8];     /* one image: banded 2^13 / 2^14 / 2^15 / 2^16 x
-                                 loads 0..7 (0 = not available)                  */
+                                 loads 0..7, the LDS walks at their best split
+                                 (0 = not available)                             */
   double ms_lane[7];          /* per width: orientation-major table               */
   double ms_grouped[7];       /* per width: table, grouped lanes (64, 128 only)   */
   double ms_banded[7][5];     /* per width: batch copy 2^9..2^13 (0 = no fit)     */
@@ -237,6 +244,9 @@ typedef struct lm_tune_report {
                                  sums (0 = no fit)                               */
   uint32_t sum16_mask;        /* chosen LM_OPT_BATCH_SUM16                        */
   uint32_t split_mask;        /* chosen LM_OPT_BATCH_SPLIT                        */
+  int32_t banded_split;       /* chosen LM_OPT_BANDED_SPLIT (-1: no LDS walk kept) */
+  double ms_split[4];         /* the kept single-image layout with 2^0..2^3 pieces
+                                 per chunk (0 = not timed)                      */
 } lm_tune_report;
 int lm_db_autotune(lm_db *db, const uint8_t *const *images, int32_t nimg, int32_t max_batch,
                    int32_t reps, lm_tune_report *out);

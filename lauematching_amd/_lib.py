@@ -84,7 +84,8 @@ class LmTuneReport(ctypes.Structure):
                 ("grouped_mask", ctypes.c_uint32), ("nibble_mask", ctypes.c_uint32),
                 ("batch_tile_bits_w", ctypes.c_int32 * 7),
                 ("ms_banded16", (ctypes.c_double * 5) * 7), ("sum16_mask", ctypes.c_uint32),
-                ("split_mask", ctypes.c_uint32)]
+                ("split_mask", ctypes.c_uint32), ("banded_split", ctypes.c_int32),
+                ("ms_split", ctypes.c_double * 4)]
 
 
 _lib = None

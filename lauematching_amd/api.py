@@ -214,6 +214,12 @@ class OrientationDB:
         memory, streamed to the device in chunks by every scoring call."""
         return self.get_option(8) == 1
 
+    def set_banded_split(self, bits: int) -> None:
+        """LDS-staged walk: each image chunk's entries in 2**bits balanced
+        pieces (0..3), or -1 for the fewest giving pieces of at most 512
+        entries on average."""
+        check(lib().lm_db_set_option(self._h, 12, int(bits)))
+
     def banded_layout(self):
         """(super-tile bits, walk) of the single-image layout in use."""
         return self.get_option(1), self.get_option(5)
@@ -252,7 +258,8 @@ class OrientationDB:
             "banded_tile_bits": r.banded_tile_bits, "batch_tile_bits": r.batch_tile_bits,
             "batch_widths_mask": r.batch_widths_mask, "grouped_mask": r.grouped_mask,
             "nibble_mask": r.nibble_mask, "sum16_mask": r.sum16_mask,
-            "split_mask": r.split_mask,
+            "split_mask": r.split_mask, "banded_split": r.banded_split,
+            "ms_split": {1 << k: r.ms_split[k] for k in range(4) if r.ms_split[k] > 0},
             "batch_tile_bits_per_width": {B: r.batch_tile_bits_w[i] for i, B in enumerate(widths)},
         }
 

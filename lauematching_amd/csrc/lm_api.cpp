@@ -602,6 +602,15 @@ int lm_db_set_option(lm_db *db, int32_t option, int64_t value) {
     ++db->generation;
     return LM_OK;
   }
+  if (option == LM_OPT_BANDED_SPLIT) {
+    if (value < -1 || value > kMaxSplitBits) {
+      set_error("LM_OPT_BANDED_SPLIT must be -1 (from the mean chunk) or 0..3");
+      return LM_ERR_INVALID;
+    }
+    db->banded_split = (int)value;
+    if (db->table.ready) return build_banded(db);
+    return LM_OK;
+  }
   if (option == LM_OPT_BANDED_LOADS) {
     if (value < 0 || value > 7) {
       set_error("LM_OPT_BANDED_LOADS must be 0..7");
@@ -667,6 +676,9 @@ int lm_db_get_option(lm_db *db, int32_t option, int64_t *value) {
     case LM_OPT_BATCH_WIDTHS: *value = db->batch_widths; return LM_OK;
     case LM_OPT_USE_GRAPHS: *value = db->scratch.use_graphs ? 1 : 0; return LM_OK;
     case LM_OPT_BANDED_LOADS: *value = db->banded_loads; return LM_OK;
+    case LM_OPT_BANDED_SPLIT:  // the copy's split when it has one, else the setting
+      *value = db->banded.ready && db->banded.chunk_off ? db->banded.ks : db->banded_split;
+      return LM_OK;
     case LM_OPT_TABLE_ON_HOST: *value = db->htable.ready ? 1 : 0; return LM_OK;
     case LM_OPT_BATCH_GROUPED: *value = db->group_widths; return LM_OK;
     case LM_OPT_BATCH_NIBBLE: *value = db->nib_widths; return LM_OK;

@@ -39,12 +39,13 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 
 // One workgroup per super-tile: counting sort of the super-tile's entries by
 // pixel bucket (two traversals of the orientation-major table: histogram,
-// then scatter).
+// then scatter).  With chunk offsets, each 2^cb-pixel chunk's entry range is
+// split into 2^ks pieces of balanced size (the LDS walk's work items).
 __global__ void __launch_bounds__(kBuildThreads)
 band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restrict__ tileOff,
                   const uint32_t *__restrict__ entries, uint64_t n, int tb, int pb, int bb,
                   int nbuckets, int nbands, uint32_t *__restrict__ bentries,
-                  uint32_t *__restrict__ bandOff, int cb, int nchunks,
+                  uint32_t *__restrict__ bandOff, int cb, int nchunks, int ks,
                   uint32_t *__restrict__ chunkOff) {
   __shared__ uint32_t hist[kMaxBuckets];
   __shared__ uint32_t wsum[kBuildThreads / 64];
@@ -109,10 +110,20 @@ band_build_kernel(const uint16_t *__restrict__ counts, const uint64_t *__restric
         bo[j] = (j == nbands || bk >= nbuckets) ? run : hist[bk];
       }
       if (chunkOff) {
-        uint32_t *co = chunkOff + t * (uint64_t)(nchunks + 1);
+        const uint64_t npieces = (uint64_t)nchunks << ks;
+        uint32_t *co = chunkOff + t * (npieces + 1);
+        auto start = [&](int c) -> uint32_t {
+          const int64_t bk = (int64_t)c << (cb - bb);
+          return (c == nchunks || bk >= nbuckets) ? run : hist[bk];
+        };
         for (int j = tid; j <= nchunks; j += kBuildThreads) {
-          const int64_t bk = (int64_t)j << (cb - bb);
-          co[j] = (j == nchunks || bk >= nbuckets) ? run : hist[bk];
+          if (j == nchunks) {
+            co[npieces] = run;
+            continue;
+          }
+          const uint32_t s0 = start(j), len = start(j + 1) - s0;
+          for (int p = 0; p < (1 << ks); ++p)
+            co[((uint64_t)j << ks) + p] = s0 + (uint32_t)(((uint64_t)len * (uint32_t)p) >> ks);
         }
       }
       __syncthreads();
@@ -339,7 +350,7 @@ score_band_v4_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__re
 template <int TB, int CB, int EMAX, int D, int THREADS, bool DENSE, bool E16>
 __global__ void __launch_bounds__(THREADS)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
-                      const uint32_t *__restrict__ chunkOff, int nchunks, uint64_t n,
+                      const uint32_t *__restrict__ chunkOff, int nchunks, int ks, uint64_t n,
                       uint64_t nsuper, const uint8_t *__restrict__ img, uint64_t npx,
                       const Pred *__restrict__ pred, uint64_t *__restrict__ hits, uint64_t cap,
                       uint32_t *__restrict__ counters, uint64_t *__restrict__ cands,
@@ -366,9 +377,11 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
       __builtin_amdgcn_make_buffer_rsrc((void *)img, (short)0, (int)(uint32_t)npx, kRawBufferWord3);
   // image chunks with no nonzero pixel (chunkNz[c] == 0, from
   // chunk_nz_kernel on this image): their spots add nothing, so their entry
-  // ranges are taken as empty and never loaded; flags past nchunks read 0
+  // ranges are taken as empty and never loaded; flags past the last chunk
+  // read 0.  The work items are pieces: chunk c's entries are pieces
+  // c << ks .. (c << ks) + 2^ks - 1 (nchunks counts pieces here)
   const __amdgpu_buffer_rsrc_t frs = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)chunkNz, (short)0, (int)((uint32_t)nchunks * 4u), kRawBufferWord3);
+      (void *)chunkNz, (short)0, (int)(((uint32_t)nchunks >> ks) * 4u), kRawBufferWord3);
   // 16-bit counters: the sum of each row's pixels, or, when fewer than a
   // quarter of the image's pixels are nonzero and MinNrSpots >= 2, its count
   // of nonzero pixels (the selective test for sparse images)
@@ -417,7 +430,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
       vb = coff(cl);
       const uint32_t e1 = coff(cl + 1);
       const uint32_t nz = __builtin_amdgcn_raw_buffer_load_b32(
-          frs, (uint32_t)(cl < nchunks ? cl : nchunks) * 4u, 0, 0);
+          frs, ((uint32_t)(cl < nchunks ? cl : nchunks) >> ks) * 4u, 0, 0);
       ve = nz ? e1 : vb;
     };
     uint32_t cur_b, cur_e, nxt_b, nxt_e;
@@ -430,7 +443,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
         end[d] = in ? (uint32_t)__builtin_amdgcn_readlane((int)ve, jj) : total;
         // a chunk with no entries to walk (none, or an all-zero image chunk)
         // needs no pixels: its loads read chunk 0 again, which stays cached
-        const uint32_t ib = beg[d] != end[d] ? (uint32_t)(w + NW * j) << CB : 0u;
+        const uint32_t ib = beg[d] != end[d] ? ((uint32_t)(w + NW * j) >> ks) << CB : 0u;
 #pragma unroll
         for (int v = 0; v < V; ++v)
           pim[d][v] = __builtin_amdgcn_raw_buffer_load_b128(irs, ib + 16u * (v * 64 + lane), 0, 0);
@@ -946,7 +959,19 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
   const int cb = wave_chunk_bits(tb);
   chunks = chunks && bb <= cb;
   b.nchunks = chunks ? (int)((npx + (1ull << cb) - 1) >> cb) : 0;
-  const uint64_t cbytes = chunks ? b.nsuper * (uint64_t)(b.nchunks + 1) * 4 : 0;
+  // pieces per chunk: as set, or the fewest that bring the mean piece to at
+  // most 512 entries (the walk's fast loads hold 8 or 12 entries per lane;
+  // a small detector puts thousands of entries in each chunk)
+  b.ks = 0;
+  if (chunks && db->banded_split >= 0) {
+    b.ks = db->banded_split;
+  } else if (chunks) {
+    const double mean = (double)t.total / (double)std::max<uint64_t>(db->n, 1) *
+                        (double)(1ull << tb) / (double)b.nchunks;
+    while (b.ks < kMaxSplitBits && mean > 512.0 * (double)(1 << b.ks)) ++b.ks;
+  }
+  const uint64_t cbytes =
+      chunks ? b.nsuper * (((uint64_t)b.nchunks << b.ks) + 1) * 4 : 0;
   int rc = reserve_bytes(db, ebytes + obytes + cbytes);
   if (rc) {
     b = Banded{};
@@ -961,7 +986,7 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
   if (e == hipSuccess) {
     hipLaunchKernelGGL(band_build_kernel, dim3((unsigned)b.nsuper), dim3(kBuildThreads), 0, st,
                        t.counts, t.tile_off, t.entries, db->n, tb, b.pb, bb, nbuckets, b.nbands,
-                       b.entries, b.band_off, cb, b.nchunks, b.chunk_off);
+                       b.entries, b.band_off, cb, b.nchunks, b.ks, b.chunk_off);
     e = hipGetLastError();
   }
   if (e == hipSuccess && chunks) {
@@ -970,7 +995,7 @@ int build_one(lm_db *db, Banded &b, int tb, bool chunks) {
     if (e == hipSuccess) e = hipMemsetAsync(d_max, 0, 4, st);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(chunk_max_kernel, dim3(1024), dim3(256), 0, st, b.chunk_off, b.nsuper,
-                         b.nchunks, d_max);
+                         b.nchunks << b.ks, d_max);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&b.max_chunk, d_max, 4, hipMemcpyDeviceToHost, st);
@@ -1090,12 +1115,14 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
   if (b.max_chunk + slack <= (uint32_t)(EM)*64u)                                                 \
     hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, false, kE16>), \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
-                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters,  \
+                       b.nchunks << b.ks, b.ks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, \
+                       sc.counters,                                                               \
                        sc.cands, sc.cand_cap, sc.chunk_nz);                                       \
   else                                                                                            \
     hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, true, kE16>),  \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
-                       b.nchunks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, sc.counters,  \
+                       b.nchunks << b.ks, b.ks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, \
+                       sc.counters,                                                               \
                        sc.cands, sc.cand_cap, sc.chunk_nz)
 #define LM_LDS_ALL(TB, E1, E2)               \
   switch (L) {                               \

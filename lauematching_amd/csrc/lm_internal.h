@@ -45,17 +45,20 @@ struct Table {
 struct Banded {
   uint32_t *entries = nullptr;   // same length as Table::entries
   uint32_t *band_off = nullptr;  // nsuper * (nbands + 1)
-  uint32_t *chunk_off = nullptr; // nsuper * (nchunks + 1): offsets of 2^wave_chunk_bits(tb)-pixel chunks
+  uint32_t *chunk_off = nullptr; // nsuper * ((nchunks << ks) + 1): offsets of the pieces of
+                                 // the 2^wave_chunk_bits(tb)-pixel chunks
   uint64_t nsuper = 0;
   int tb = 0;
   int pb = 0;
   int nbands = 0;
   int nchunks = 0;
-  uint32_t max_chunk = 0;        // largest number of entries in one chunk
+  int ks = 0;                    // each chunk's entries split into 2^ks balanced pieces
+  uint32_t max_chunk = 0;        // largest number of entries in one piece
   uint64_t bytes = 0;
   bool ready = false;
 };
 constexpr int kBucketBits = 10;
+constexpr int kMaxSplitBits = 3;  // at most 8 pieces per chunk of the LDS walk
 // LDS-staged walk: pixels per wave chunk (log2) for a super-tile of 2^tb rows;
 // 16 chunk slots and 2^tb row counters fill at most 160 KiB of LDS.
 constexpr int wave_chunk_bits(int tb) { return tb >= 15 ? 11 : 12; }
@@ -214,6 +217,7 @@ struct lm_db {
   lm::Banded banded_bk[5];      // batch copies of 2^9..2^13-row super-tiles (B x T LDS counters)
   int banded_tb = 16;           // super-tile bits for the banded copy (0 = no copy)
   int banded_loads = 3;         // single-image walk (LM_OPT_BANDED_LOADS)
+  int banded_split = -1;        // log2 pieces per chunk of the LDS walk (-1 = from the mean)
   int batch_tb = 0;             // LM_OPT_BATCH_TILE_BITS: the copy every width takes by default
   int8_t width_tb[9] = {};      // per width 2^k: super-tile bits of its batch copy (0 = none)
   uint32_t batch_widths = 0x7f; // bit log2(B): batches of width B use the batch copy

@@ -61,7 +61,8 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     if (e != hipSuccess) return hip_fail(e, "autotune staging");
   }
   if (hipStreamSynchronize(st) != hipSuccess) return hip_fail(hipGetLastError(), "autotune");
-  const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads;
+  const int saved_tb = db->banded_tb, saved_btb = db->batch_tb, saved_loads = db->banded_loads,
+            saved_bsplit = db->banded_split;
   const uint32_t saved_mask = db->batch_widths, saved_group = db->group_widths,
                  saved_nib = db->nib_widths, saved_s16 = db->batch16_widths,
                  saved_split = db->split_widths;
@@ -83,6 +84,7 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
     db->nib_widths = saved_nib;
     db->batch16_widths = saved_s16;
     db->split_widths = saved_split;
+    db->banded_split = saved_bsplit;
     const int r = build_banded(db);
     return code ? code : r;
   };
@@ -90,35 +92,52 @@ int autotune(lm_db *db, const uint8_t *const *imgs, int nimg, int max_batch, int
   db->batch_tb = 0;
   set_all_widths(0);
   db->banded_tb = 0;
+  db->banded_split = -1;
   rc = build_banded(db);
   if (rc) return restore(rc);
   rc = time_batch(db, d, 1, reps, &rep->ms_single_table);
   if (rc) return restore(rc);
   double best1 = rep->ms_single_table;
-  int best_tb = 0, best_loads = 0;
+  int best_tb = 0, best_loads = 0, best_split = -1;
   // (2^16 rows keep 16-bit sums: only the LDS-staged walks, only when a
-  // row's sum fits)
+  // row's sum fits).  The LDS walks are timed with every split of the image
+  // chunks' entries into 1..8 pieces (ms_single keeps each walk's best);
+  // the gather walks read no chunk offsets and are timed once.
   const int max_tb = db->max_spots <= kSum16MaxSpots ? kSum16TileBits : 15;
+  double split_ms[4][8][kMaxSplitBits + 1] = {};
   for (int i = 0; 13 + i <= max_tb; ++i) {
     db->banded_tb = 13 + i;
-    rc = build_banded(db);
-    if (rc) return restore(rc);
-    for (int loads = 0; loads < 8; ++loads) {
-      const bool lds_walk = loads == 3 || loads >= 5;
-      if (lds_walk && (!db->banded.chunk_off || db->banded_tb == 13)) continue;
-      if (!lds_walk && db->banded_tb == kSum16TileBits) continue;
-      db->banded_loads = loads;
-      ++db->generation;
-      rc = time_batch(db, d, 1, reps, &rep->ms_single[i][loads]);
+    for (int ks = 0; ks <= (db->banded_tb >= 14 ? kMaxSplitBits : 0); ++ks) {
+      db->banded_split = ks;
+      rc = build_banded(db);
       if (rc) return restore(rc);
-      // a later layout must be faster by more than the timing noise (1%)
-      if (rep->ms_single[i][loads] < 0.99 * best1) {
-        best1 = rep->ms_single[i][loads];
-        best_tb = 13 + i;
-        best_loads = loads;
+      for (int loads = 0; loads < 8; ++loads) {
+        const bool lds_walk = loads == 3 || loads >= 5;
+        if (lds_walk && (!db->banded.chunk_off || db->banded_tb == 13)) continue;
+        if (!lds_walk && (db->banded_tb == kSum16TileBits || ks > 0)) continue;
+        db->banded_loads = loads;
+        ++db->generation;
+        double ms = 0;
+        rc = time_batch(db, d, 1, reps, &ms);
+        if (rc) return restore(rc);
+        split_ms[i][loads][ks] = ms;
+        double &cell = rep->ms_single[i][loads];
+        if (cell == 0 || ms < cell) cell = ms;
+        // a later layout must be faster by more than the timing noise (1%)
+        if (ms < 0.99 * best1) {
+          best1 = ms;
+          best_tb = 13 + i;
+          best_loads = loads;
+          best_split = lds_walk ? ks : -1;
+        }
       }
     }
   }
+  if (best_tb)
+    for (int ks = 0; ks <= kMaxSplitBits; ++ks)
+      rep->ms_split[ks] = split_ms[best_tb - 13][best_loads][ks];
+  db->banded_split = best_split;
+  rep->banded_split = best_split;
   // batch widths 4..max_batch
   int nw = 0;
   for (int B = 4; B <= std::min(max_batch, kMaxBatch); B <<= 1) rep->widths[nw++] = B;

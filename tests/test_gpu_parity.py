@@ -396,6 +396,49 @@ def test_score_other_detectors(ctx, geo):
         np.testing.assert_array_equal(hits.score, mA)
 
 
+def test_chunk_pieces_match_oracle(ctx):
+    """The LDS walk's work items are pieces of one image chunk's entries
+    (LM_OPT_BANDED_SPLIT): a 1024 x 1024 detector (four times the entries per
+    chunk of a 2048 x 2048 one) over 2^14..2^16-row super-tiles (the last one
+    partial), both LDS walks, every split and the default, on a dense frame
+    and on a frame whose rows are mostly zero (whole chunks skipped): every
+    result bit-exact against the oracle."""
+    p = synth.baseline_params(MaxNrLaueSpots=64, NrPxX=1024, NrPxY=1024, PxX=0.0004, PxY=0.0004)
+    hk = synth.fcc_hkls(8, "box")
+    n = 150000
+    O = synth.random_orientations(n, seed=41)
+    img = synth.synth_image(p, hk, O[[7, 77777, 149999]], seed=9)
+    u8, s = api.quantize_u8(img)
+    sparse = u8.copy().reshape(1024, 1024)
+    sparse[np.arange(1024) % 5 != 0] = 0
+    sparse = sparse.ravel()
+    db = _db(ctx, p, O, hk)
+    db.forward()
+    table = db.export_table()
+    want = {}
+    for name, im, ms, mi in (("dense", u8, 5, 1000.0), ("sparse", sparse, 2, 60.0)):
+        matched, ns, tot, nres = ob.compare(table, 64, im, p.NrPxX, s, ms, mi)
+        want[name] = (nres,) + ob.collect(matched)
+    try:
+        for bits in (16, 15, 14):
+            db.set_banded_tile_bits(bits)
+            for loads in (3, 5):
+                db.set_banded_loads(loads)
+                for split in (-1, 0, 1, 2, 3):
+                    db.set_banded_split(split)
+                    if split >= 0:
+                        assert db.get_option(12) == split
+                    for name, im, ms, mi in (("dense", u8, 5, 1000.0), ("sparse", sparse, 2, 60.0)):
+                        _say(f"pieces tb={bits} loads={loads} split={split} {name}")
+                        hits = db.score(im, s, ms, mi)
+                        nres, rows, mA = want[name]
+                        assert hits.nr_results == nres, (bits, loads, split, name)
+                        np.testing.assert_array_equal(hits.ids, rows)
+                        np.testing.assert_array_equal(hits.score, mA)
+    finally:
+        db.close()
+
+
 @pytest.mark.parametrize("factor", [0.37, 3.3])
 def test_score_matches_oracle_nonunit_scale(scored_case, factor):
     p, hk, O, grains, img, table, db = scored_case
