@@ -352,9 +352,10 @@ __global__ void __launch_bounds__(THREADS)
 score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__restrict__ tileOff,
                       const uint32_t *__restrict__ chunkOff, int nchunks, int ks, uint64_t n,
                       uint64_t nsuper, const uint8_t *__restrict__ img, uint64_t npx,
-                      const Pred *__restrict__ pred, uint64_t *__restrict__ hits, uint64_t cap,
-                      uint32_t *__restrict__ counters, uint64_t *__restrict__ cands,
-                      uint64_t candCap, const uint32_t *__restrict__ chunkNz) {
+                      uint32_t spots256, const Pred *__restrict__ pred,
+                      uint64_t *__restrict__ hits, uint64_t cap, uint32_t *__restrict__ counters,
+                      uint64_t *__restrict__ cands, uint64_t candCap,
+                      const uint32_t *__restrict__ chunkNz) {
   typedef uint32_t v4u __attribute__((ext_vector_type(4)));
   constexpr int T = 1 << TB;
   constexpr int NW = THREADS / 64;
@@ -368,6 +369,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   constexpr int NCNT = S16 ? T / 2 : T;
   static_assert(NCNT * 4 + NW * C <= 163840, "LDS");
   constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
+  constexpr int kNonTemporal = 2;              // buffer load aux bit 1: nt
   __shared__ __attribute__((aligned(16))) uint32_t cnt[NCNT];
   __shared__ v4u slot[NW][C / 16];
   const int tid = threadIdx.x, lane = __lane_id();
@@ -382,11 +384,14 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
   // c << ks .. (c << ks) + 2^ks - 1 (nchunks counts pieces here)
   const __amdgpu_buffer_rsrc_t frs = __builtin_amdgcn_make_buffer_rsrc(
       (void *)chunkNz, (short)0, (int)(((uint32_t)nchunks >> ks) * 4u), kRawBufferWord3);
-  // 16-bit counters: the sum of each row's pixels, or, when fewer than a
-  // quarter of the image's pixels are nonzero and MinNrSpots >= 2, its count
-  // of nonzero pixels (the selective test for sparse images)
+  // 16-bit counters: the sum of each row's pixels, or its count of nonzero
+  // pixels when that count is the selective test: MinNrSpots >= 2 and a row
+  // expects at most MinNrSpots / 4 nonzero pixels (nonzero fraction x mean
+  // spots per row, spots256 = 256 x that mean), as on sparse images
   const uint32_t pk0 = pred->pk_thr[0];
-  const bool count_mode = S16 && (pk0 >> 23) >= 2u && (uint64_t)counters[kNzSlot] * 4 < npx;
+  const uint32_t kmin = pk0 >> 23;
+  const bool count_mode = S16 && kmin >= 2u &&
+                          (uint64_t)counters[kNzSlot] * 4u * spots256 <= npx * kmin * 256u;
   // counters start at zero; each epilogue clears the words it reads, so the
   // next super-tile starts without a clearing pass
   for (int i = tid; i < NCNT / 4; i += THREADS) ((v4u *)cnt)[i] = v4u{0u, 0u, 0u, 0u};
@@ -451,7 +456,8 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
           // lane l holds entries a + 4l .. a + 4l + 3 (+ 256 per load), a the
           // chunk's first entry rounded down to a 16-byte boundary; the
           // descriptor ends at the chunk's last entry, so loads past it read
-          // zeros instead of the next chunk's entries
+          // zeros instead of the next chunk's entries.  Entries are read once:
+          // non-temporal loads, so the L2 keeps the image chunks
           const uint32_t a = (beg[d] + mis) & ~3u;
           const __amdgpu_buffer_rsrc_t crs16 = __builtin_amdgcn_make_buffer_rsrc(
               (void *)(bentries + (e0 - mis) + a), (short)0, (int)((end[d] + mis - a) * 4u),
@@ -459,7 +465,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
 #pragma unroll
           for (int q = 0; q < EMAX / 4; ++q) {
             const v4u v = __builtin_amdgcn_raw_buffer_load_b128(crs16, (q * 256 + 4 * lane) * 4u,
-                                                                0, 0);
+                                                                0, kNonTemporal);
             pe[d][4 * q + 0] = v.x;
             pe[d][4 * q + 1] = v.y;
             pe[d][4 * q + 2] = v.z;
@@ -469,7 +475,8 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
 #pragma unroll
           for (int k = 0; k < EMAX; ++k)
             pe[d][k] =
-                __builtin_amdgcn_raw_buffer_load_b32(ers, (beg[d] + k * 64 + lane) * 4u, 0, 0);
+                __builtin_amdgcn_raw_buffer_load_b32(ers, (beg[d] + k * 64 + lane) * 4u, 0,
+                                                     kNonTemporal);
         }
         // keep the sets' loads in issue order, so the wait before a set
         // counts only the later sets
@@ -510,7 +517,7 @@ score_band_lds_kernel(const uint32_t *__restrict__ bentries, const uint64_t *__r
           if (raw[k]) add(pe[d][k], raw[k]);
         if constexpr (DENSE) {
           for (uint32_t i = a + EMAX * 64 + lane; i < cend + mis; i += 64) {
-            const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, 0);
+            const uint32_t e = __builtin_amdgcn_raw_buffer_load_b32(ers, i * 4u, 0, kNonTemporal);
             const uint32_t r = mine[e & (C - 1)];
             if (r) add(e, r);
           }
@@ -1111,19 +1118,20 @@ int launch_banded(lm_db *db, Scratch &sc, const uint8_t *img, uint64_t cap, hipS
     // fits the fast loads when it has at most EM * 64 - 3 entries
     constexpr bool kE16 = true;
     const uint32_t slack = kE16 ? 3u : 0u;
+    // mean spots per row (x 256, rounded up): picks sums or counts per image
+    const uint64_t rows = std::max<uint64_t>(db->n, 1);
+    const uint32_t spots256 = (uint32_t)((t.total * 256u + rows - 1) / rows);
 #define LM_LDS(TB, EM, DD, TH)                                                                   \
   if (b.max_chunk + slack <= (uint32_t)(EM)*64u)                                                 \
     hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, false, kE16>), \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
-                       b.nchunks << b.ks, b.ks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, \
-                       sc.counters,                                                               \
-                       sc.cands, sc.cand_cap, sc.chunk_nz);                                       \
+                       b.nchunks << b.ks, b.ks, db->n, b.nsuper, img, npx, spots256, sc.pred,     \
+                       sc.hits, cap, sc.counters, sc.cands, sc.cand_cap, sc.chunk_nz);           \
   else                                                                                            \
     hipLaunchKernelGGL((score_band_lds_kernel<TB, wave_chunk_bits(TB), EM, DD, TH, true, kE16>),  \
                        dim3(grid), dim3(TH), 0, st, b.entries, t.tile_off, b.chunk_off,           \
-                       b.nchunks << b.ks, b.ks, db->n, b.nsuper, img, npx, sc.pred, sc.hits, cap, \
-                       sc.counters,                                                               \
-                       sc.cands, sc.cand_cap, sc.chunk_nz)
+                       b.nchunks << b.ks, b.ks, db->n, b.nsuper, img, npx, spots256, sc.pred,     \
+                       sc.hits, cap, sc.counters, sc.cands, sc.cand_cap, sc.chunk_nz)
 #define LM_LDS_ALL(TB, E1, E2)               \
   switch (L) {                               \
     case 3: LM_LDS(TB, E1, 2, 1024); break;  \

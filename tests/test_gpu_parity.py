@@ -439,6 +439,33 @@ def test_chunk_pieces_match_oracle(ctx):
         db.close()
 
 
+@pytest.mark.parametrize("keep", [0.002, 0.05, 0.2, 0.3])
+def test_sum16_sums_or_counts_match_oracle(scored_case, keep):
+    """The 16-bit walk counts nonzero pixels instead of summing them only
+    where a row expects at most MinNrSpots / 4 of them: frames with 0.2% to
+    30% of their pixels kept (the rest zeroed) at MinNrSpots 2..6, bit-exact
+    against the oracle either way."""
+    p, hk, O, grains, img, table, db = scored_case
+    u8, s = api.quantize_u8(img)
+    rng = np.random.default_rng(int(keep * 1000))
+    frame = np.where(rng.random(u8.size) < keep, u8, 0).astype(np.uint8)
+    was = db.banded_layout()
+    db.set_banded_tile_bits(16)
+    db.set_banded_loads(3)
+    try:
+        for ms, mi in ((2, 20.0), (4, 100.0), (6, 300.0)):
+            _say(f"sums or counts keep={keep} min_sp={ms}")
+            hits = db.score(frame, s, ms, mi)
+            matched, ns, tot, nres = ob.compare(table, 64, frame, p.NrPxX, s, ms, mi)
+            rows, mA = ob.collect(matched)
+            assert hits.nr_results == nres, (keep, ms)
+            np.testing.assert_array_equal(hits.ids, rows)
+            np.testing.assert_array_equal(hits.score, mA)
+    finally:
+        db.set_banded_tile_bits(was[0])
+        db.set_banded_loads(was[1])
+
+
 @pytest.mark.parametrize("factor", [0.37, 3.3])
 def test_score_matches_oracle_nonunit_scale(scored_case, factor):
     p, hk, O, grains, img, table, db = scored_case
