@@ -96,6 +96,8 @@ typedef struct lm_hits {
 /* Bind device `device`; hbm_budget_bytes = 0 means 90% of free device memory.
  * No reference counterpart (the reference selects device 0 implicitly). */
 int lm_ctx_create(int device, uint64_t hbm_budget_bytes, lm_ctx **out);
+/* Releases the caller's context; DBs and servers made on it keep it (its
+ * stream and budget) until the last of them is destroyed, in any order. */
 void lm_ctx_destroy(lm_ctx *ctx);
 /* Bytes of device memory currently held by the context's DBs and buffers. */
 uint64_t lm_ctx_bytes_in_use(const lm_ctx *ctx);

@@ -310,6 +310,19 @@ hipError_t pool_get(lm_ctx *ctx, DevPool &p, size_t slot, uint64_t bytes, void *
   return hipSuccess;
 }
 
+void ctx_ref(lm_ctx *ctx) { ctx->refs.fetch_add(1); }
+
+void ctx_unref(lm_ctx *ctx) {
+  if (ctx->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  pool_release(ctx, ctx->prep_pool);
+  pool_release(ctx, ctx->median_pool);
+  pool_release(ctx, ctx->io_pool);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 void pool_release(lm_ctx *ctx, DevPool &p) {
   for (size_t i = 0; i < p.ptr.size(); ++i) {
     if (p.ptr[i]) (void)hipFree(p.ptr[i]);
@@ -364,14 +377,7 @@ int lm_ctx_create(int device, uint64_t budget, lm_ctx **out) {
 }
 
 void lm_ctx_destroy(lm_ctx *ctx) {
-  if (!ctx) return;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  pool_release(ctx, ctx->prep_pool);
-  pool_release(ctx, ctx->median_pool);
-  pool_release(ctx, ctx->io_pool);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  if (ctx) ctx_unref(ctx);  // DBs and servers still made on it keep it until they go
 }
 
 uint64_t lm_ctx_bytes_in_use(const lm_ctx *ctx) { return ctx ? ctx->in_use : 0; }
@@ -504,6 +510,7 @@ int lm_db_create(lm_ctx *ctx, const double *orients, uint64_t n, const int32_t *
     delete db;
     return rc;
   }
+  ctx_ref(ctx);
   *out = db;
   return LM_OK;
 }
@@ -522,7 +529,9 @@ void lm_db_destroy(lm_db *db) {
     std::lock_guard<std::mutex> lk(db->ctx->mu);
     db->ctx->in_use -= db->n * 72 + (uint64_t)db->nhkls * 12 + 64;
   }
+  lm_ctx *ctx = db->ctx;
   delete db;
+  ctx_unref(ctx);
 }
 
 uint64_t lm_db_size(const lm_db *db) { return db ? db->n : 0; }

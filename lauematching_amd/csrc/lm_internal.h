@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -192,6 +193,9 @@ struct DevPool {
 
 struct lm_ctx {
   int device = 0;
+  // the caller's reference plus one per DB and server made on it: the
+  // context is released when the last of them is destroyed, in any order
+  std::atomic<int> refs{1};
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
   uint64_t in_use = 0;
@@ -364,6 +368,8 @@ int preprocess(lm_ctx *ctx, int W, int H, const double *raw, const double *backg
 // the caller); past the HBM budget hipErrorOutOfMemory, which hip_fail then
 // reports as LM_ERR_BUDGET.
 hipError_t pool_get(lm_ctx *ctx, DevPool &p, size_t slot, uint64_t bytes, void **out);
+void ctx_ref(lm_ctx *ctx);
+void ctx_unref(lm_ctx *ctx);  // releases the context at the last reference
 void pool_release(lm_ctx *ctx, DevPool &p);
 
 // lm_median.hip: scipy.ndimage.median_filter(size=size), `passes` times, of

@@ -239,6 +239,7 @@ int lm_server_create(lm_ctx *ctx, const lm_server_config *cfg, lm_server **out) 
   if (s->cfg.max_batch > kMaxBatch) s->cfg.max_batch = kMaxBatch;
   if (s->cfg.workers <= 0) s->cfg.workers = 2;
   if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2 * s->cfg.workers * s->cfg.max_batch;
+  ctx_ref(ctx);
   *out = s;
   return LM_OK;
 }
@@ -435,7 +436,9 @@ void lm_server_destroy(lm_server *s) {
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     s->ctx->in_use -= s->reserved;
   }
+  lm_ctx *ctx = s->ctx;
   delete s;
+  ctx_unref(ctx);
 }
 
 }  // extern "C"

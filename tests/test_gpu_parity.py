@@ -466,6 +466,30 @@ def test_sum16_sums_or_counts_match_oracle(scored_case, keep):
         db.set_banded_loads(was[1])
 
 
+def test_context_released_before_its_db():
+    """A DB keeps its context: releasing the context first (as a garbage
+    collector may) leaves the DB scoring exactly, and destroying the DB
+    afterwards releases both."""
+    p = synth.baseline_params(MaxNrLaueSpots=64)
+    hk = synth.fcc_hkls(6, "box")
+    O = synth.random_orientations(5000, seed=51)
+    img = synth.synth_image(p, hk, O[[10, 4000]], seed=3)
+    u8, s = api.quantize_u8(img)
+    table, _, _ = ob.forward(O, hk, ob.geom(p), 64)
+    c2 = api.Context(0)
+    db = _db(c2, p, O, hk)
+    db.forward()
+    c2.close()
+    _say("score after the context's release")
+    hits = db.score(u8, s, 4, 200.0)
+    matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, s, 4, 200.0)
+    rows, mA = ob.collect(matched)
+    assert hits.nr_results == nres
+    np.testing.assert_array_equal(hits.ids, rows)
+    np.testing.assert_array_equal(hits.score, mA)
+    db.close()
+
+
 @pytest.mark.parametrize("factor", [0.37, 3.3])
 def test_score_matches_oracle_nonunit_scale(scored_case, factor):
     p, hk, O, grains, img, table, db = scored_case
