@@ -777,6 +777,14 @@ def test_autotuner_picks_timed_layouts_and_stays_exact(scored_case):
         chosen = "table" if tb == 0 else f"banded{tb}_loads{rep['banded_loads']}"
         # the kept layout is within 1% of the fastest (a later one must win by more)
         assert singles[chosen] <= min(singles.values()) / 0.99
+        # an LDS walk is kept with one of its timed splits of the chunks into pieces
+        split = rep["banded_split"]
+        if tb >= 14 and rep["banded_loads"] in (3, 5, 6, 7):
+            assert sorted(rep["ms_split"]) == [1, 2, 4, 8] and 0 <= split <= 3
+            assert rep["ms_split"][1 << split] <= min(rep["ms_split"].values()) / 0.99
+            assert db.get_option(12) == split
+        else:
+            assert split == -1
         u8 = samples[0]
         one = db.score(u8, 1.0, 5, 500.0)
         matched, ns, tot, nres = ob.compare(table, 64, u8, p.NrPxX, 1.0, 5, 500.0)
